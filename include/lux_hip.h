@@ -1,0 +1,202 @@
+/*
+ * lux_hip.h -- C ABI of the MI355X (gfx950) hot path of LUX: LBVH build/refit,
+ * closest-hit / any-hit traversal and AABB overlap queries.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an
+ * int status (LUX_OK == 0). On failure the thread-local string returned by
+ * lux_hip_last_error() says what went wrong.
+ *
+ * Two families of calls exist for every operation:
+ *   - host-array calls (no suffix): copy inputs in, run, copy results out and
+ *     synchronise, exactly like the reference C++ functions they replace;
+ *   - "_device" calls: operands are device pointers already resident in HBM, the
+ *     work is enqueued on `stream` (NULL = the library stream) and nothing is
+ *     synchronised; results land in device memory.
+ *
+ * Record layouts are byte-identical to the reference's POD types so that arrays
+ * can be handed over without repacking:
+ *   lux_aabb      == AABB         src/core/aabb.cuh:10-13      (24 B)
+ *   lux_ray       == Ray          src/core/ray.cuh:10-15       (36 B)
+ *   lux_triangle  == Triangle     src/geometry/triangle.h:10-12 (36 B)
+ *   lux_hit       == GpuRayHit    src/bvh/gpu_bvh.cuh:20-23    (8 B)
+ *   lux_pair      == PrimitivePair src/bvh/bvh_overlap.cuh:10-13 (8 B)
+ *   lux_bvh_node  == GpuBvhNode   src/bvh/bvh_device.cuh:12-17 (32 B)
+ *   lux_bvh_view  == GpuBvhView   src/bvh/bvh_device.cuh:19-23
+ *
+ * Node layout contract (lux_bvh_view): nodes are stored in RIGHT-FIRST preorder.
+ * A walker that tests node i, descends with i+1 on an internal hit and jumps to
+ * nodes[i].escape on a miss or after a leaf visits nodes in exactly the order of
+ * the host LBVH stack traversal (src/bvh/cpu_bvh_traversal.h:37-78, which pops
+ * the right child first), so closest-hit ties resolve like the host oracle.
+ */
+#ifndef LUX_HIP_H
+#define LUX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    LUX_OK = 0,
+    LUX_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative count, bad id ...      */
+    LUX_ERR_OVERFLOW = 2,         /* a fixed-capacity buffer/queue overflowed       */
+    LUX_ERR_MISMATCH = 3,         /* refit input does not match the built topology  */
+    LUX_ERR_HIP = 4,              /* a HIP runtime call failed                      */
+    LUX_ERR_NOT_INITIALIZED = 5,  /* lux_hip_init() has not run                     */
+    LUX_ERR_OUT_OF_MEMORY = 6,
+    LUX_ERR_DEVICE_CHECK = 7      /* a kernel's own bounds/termination check fired  */
+};
+
+/* ---- POD records (layouts as listed above) ----------------------------- */
+typedef struct lux_vec3 { float x, y, z; } lux_vec3;
+typedef struct lux_aabb { lux_vec3 p_min, p_max; } lux_aabb;
+typedef struct lux_ray {
+    lux_vec3 origin, direction;
+    float t_min, t_max, time;
+} lux_ray;
+typedef struct lux_triangle { lux_vec3 v0, v1, v2; } lux_triangle;
+typedef struct lux_hit { int32_t primitive_id; float t; } lux_hit;
+typedef struct lux_pair { int32_t a, b; } lux_pair;
+typedef struct lux_bvh_node {
+    lux_aabb bounds;
+    int32_t primitive_id; /* -1 = internal node, >= 0 = leaf primitive id      */
+    int32_t escape;       /* next node after a miss / finished subtree, -1 = end */
+} lux_bvh_node;
+typedef struct lux_bvh_view {
+    const lux_bvh_node* nodes; /* device pointer, valid while the handle lives */
+    int32_t node_count;
+    lux_aabb root_bounds;
+} lux_bvh_view;
+
+typedef struct lux_hip_bvh lux_hip_bvh; /* opaque owner of one LBVH (GpuBvh) */
+typedef void* lux_stream;               /* a hipStream_t; NULL = library stream */
+
+/* ---- library lifetime --------------------------------------------------- */
+/* Selects `device`, creates the library stream. Replaces the implicit device-0
+ * context of the reference (src/core/cuda_check.cuh:7-27 error convention).   */
+int lux_hip_init(int device);
+int lux_hip_shutdown(void);
+/* Thread-local text of the last failure ("" if none). */
+const char* lux_hip_last_error(void);
+/* Library version, e.g. 0x000100 for 0.1.0. */
+int lux_hip_version(void);
+
+/* ---- device memory helpers (for callers without a GPU framework) ------- */
+int lux_hip_malloc(void** device_ptr, size_t bytes);
+int lux_hip_free(void* device_ptr);
+int lux_hip_memcpy_htod(void* device_dst, const void* host_src, size_t bytes);
+int lux_hip_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
+int lux_hip_synchronize(void);
+
+/* ---- LBVH ------------------------------------------------------------- */
+/* GpuBvh::GpuBvh / ~GpuBvh  (src/bvh/gpu_bvh.cuh:25-47). */
+int lux_hip_bvh_create(lux_hip_bvh** out);
+int lux_hip_bvh_destroy(lux_hip_bvh* bvh);
+
+/* build_gpu_lbvh (src/bvh/gpu_bvh.cuh:50, src/bvh/gpu_bvh.cu:535-595).
+ * boxes[i] / primitive_ids[i] are BvhBuildPrimitive::bounds / ::primitive_id
+ * (src/bvh/bvh_build_input.h:15-23); Morton codes use the box centroid exactly as
+ * the GPU reference does (src/bvh/gpu_bvh.cu:143). n == 0 leaves the handle
+ * invalid with primitive_count 0, like the reference. Ids must be >= 0. */
+int lux_hip_bvh_build(lux_hip_bvh* bvh, const lux_aabb* boxes,
+                      const int32_t* primitive_ids, int32_t n);
+int lux_hip_bvh_build_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes,
+                             const int32_t* d_primitive_ids, int32_t n,
+                             lux_stream stream);
+
+/* refit_gpu_lbvh (src/bvh/gpu_bvh.cuh:51, src/bvh/gpu_bvh.cu:597-613): same
+ * topology, new boxes. Returns LUX_ERR_MISMATCH (the reference returns false)
+ * when n differs from the built count or primitive_ids differ element-wise. */
+int lux_hip_bvh_refit(lux_hip_bvh* bvh, const lux_aabb* boxes,
+                      const int32_t* primitive_ids, int32_t n);
+/* Device-resident refit; the ids are taken to be unchanged. */
+int lux_hip_bvh_refit_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes, int32_t n,
+                             lux_stream stream);
+/* Refit straight from triangle vertices: box of input primitive i is the
+ * bounds of d_triangles[i] (triangle_bounds, src/geometry/triangle.h:22-28).
+ * Fuses build-input preparation (src/scene/scene_prepare.h:420-439) into the
+ * bottom-up pass. */
+int lux_hip_bvh_refit_triangles_device(lux_hip_bvh* bvh,
+                                       const lux_triangle* d_triangles, int32_t n,
+                                       lux_stream stream);
+
+/* GpuBvh::is_valid / primitive_count / root_bounds (src/bvh/gpu_bvh.cuh:35-47).
+ * Synchronises with outstanding device work on the handle. */
+int lux_hip_bvh_info(lux_hip_bvh* bvh, int32_t* valid, int32_t* primitive_count,
+                     lux_aabb* root_bounds);
+/* GpuBvh::device_view (src/bvh/gpu_bvh.cuh:38). */
+int lux_hip_bvh_view(lux_hip_bvh* bvh, lux_bvh_view* out);
+/* Copies the stackless node array (2n-1 records) to the host. */
+int lux_hip_bvh_download_nodes(lux_hip_bvh* bvh, lux_bvh_node* nodes,
+                               int32_t capacity);
+/* Radix-tree topology for bit-exact checks and for download_bvh_for_validation
+ * (src/bvh/gpu_bvh.cu:615-665). Any pointer may be NULL. Sizes: morton_sorted,
+ * sorted_indices, leaf_parent: n; left, right, parent: n-1 (bit 31 of left/right
+ * = leaf child, bit 31 of parent/leaf_parent = "I am the right child", as
+ * src/bvh/gpu_bvh.cu:165-179); node_bounds: 2n-1 in host-LBVH order (internal
+ * nodes 0..n-2, then leaf i at n-1+i, src/bvh/cpu_lbvh_builder.h:245-257). */
+int lux_hip_bvh_download_topology(lux_hip_bvh* bvh, uint32_t* morton_sorted,
+                                  uint32_t* sorted_indices, uint32_t* left,
+                                  uint32_t* right, uint32_t* parent,
+                                  uint32_t* leaf_parent, lux_aabb* node_bounds);
+
+/* ---- queries ----------------------------------------------------------- */
+/* intersect_gpu_bvh_stackless (src/bvh/gpu_bvh.cuh:57-59, gpu_bvh.cu:893-915):
+ * closest hit per ray; hits[i] = {primitive_id, t} or {-1, +INF}. triangles is
+ * indexed by primitive id; triangle_count bounds every leaf id. */
+int lux_hip_trace_closest(lux_hip_bvh* bvh, const lux_triangle* triangles,
+                          int32_t triangle_count, const lux_ray* rays,
+                          int32_t ray_count, lux_hit* hits);
+int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_triangles,
+                                 int32_t triangle_count, const lux_ray* d_rays,
+                                 int32_t ray_count, lux_hit* d_hits,
+                                 lux_stream stream);
+/* Any-hit occlusion: occluded[i] = 1 iff some triangle is hit with
+ * t_min < t < t_max. The reference has no early-exit any-hit; its shadow path
+ * repeats the closest-hit walk (src/integrator/wavefront_kernels.cu:28-83), so
+ * occluded[i] == (closest-hit found a hit) is the exact contract. */
+int lux_hip_trace_any(lux_hip_bvh* bvh, const lux_triangle* triangles,
+                      int32_t triangle_count, const lux_ray* rays, int32_t ray_count,
+                      uint8_t* occluded);
+int lux_hip_trace_any_device(lux_hip_bvh* bvh, const lux_triangle* d_triangles,
+                             int32_t triangle_count, const lux_ray* d_rays,
+                             int32_t ray_count, uint8_t* d_occluded,
+                             lux_stream stream);
+
+/* query_aabb_overlaps_self (src/bvh/bvh_overlap.cuh:21, bvh_overlap.cu:186-215):
+ * all pairs (a, b), a < b, of primitives whose boxes overlap (inclusive,
+ * AABB::overlaps src/core/aabb.cuh:68-72). *found_count is the exact total;
+ * at most max_pairs are written (truncated iff *found_count > max_pairs).
+ * Pair order is unspecified. max_pairs <= 0 returns found_count 0. */
+int lux_hip_overlap_self(lux_hip_bvh* bvh, int32_t max_pairs, lux_pair* pairs,
+                         int32_t* found_count);
+int lux_hip_overlap_self_device(lux_hip_bvh* bvh, int32_t max_pairs,
+                                lux_pair* d_pairs, int32_t* d_found_count,
+                                lux_stream stream);
+/* query_aabb_overlaps_cross (src/bvh/bvh_overlap.cuh:22-23, bvh_overlap.cu:217-251):
+ * pairs (left primitive id, right primitive id); right boxes are visited in
+ * the right tree's Morton order. */
+int lux_hip_overlap_cross(lux_hip_bvh* left, lux_hip_bvh* right, int32_t max_pairs,
+                          lux_pair* pairs, int32_t* found_count);
+int lux_hip_overlap_cross_device(lux_hip_bvh* left, lux_hip_bvh* right,
+                                 int32_t max_pairs, lux_pair* d_pairs,
+                                 int32_t* d_found_count, lux_stream stream);
+
+/* ---- measurement --------------------------------------------------------
+ * No reference counterpart (the reference only prints wall-clock stage times,
+ * src/app/render_runner.cpp:274-286). While a probe is open, every launch of
+ * the kernel named `kernel_name` is bracketed by HIP events recorded on the
+ * stream it is launched on; lux_hip_probe_end synchronises and returns the
+ * summed kernel time and the number of launches. */
+int lux_hip_probe_begin(const char* kernel_name);
+int lux_hip_probe_end(double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUX_HIP_H */

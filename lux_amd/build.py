@@ -1,0 +1,65 @@
+"""Build steps: the gfx950 HIP library, the CPU oracle and (where the reference
+sources exist) the reference shim. Run from anywhere; outputs are in-tree."""
+
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+from . import LIB_PATH, PACKAGE_DIR, REPO_ROOT
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if Path("/opt/rocm/bin/hipcc").exists() else "hipcc")
+GPU_ARCH = "gfx950"
+REFERENCE_SRC = Path("/root/reference/src")
+
+HIP_SOURCES = ["lbvh.hip", "query.hip", "abi.hip"]
+# -ffp-contract=off keeps the reference's exact float operation order (no FMA
+# contraction) in box, triangle and Morton arithmetic; packed FP32 ops are off.
+HIP_FLAGS = [
+    "-O3", f"--offload-arch={GPU_ARCH}", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+ORACLE_SRC = REPO_ROOT / "oracle" / "lbvh_oracle.cpp"
+ORACLE_LIB = REPO_ROOT / "oracle" / "_build" / "liblux_oracle.so"
+
+
+def _run(cmd, cwd=REPO_ROOT):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(map(str, cmd))}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build_hip():
+    srcs = [str(PACKAGE_DIR / "csrc" / s) for s in HIP_SOURCES]
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [HIPCC, *HIP_FLAGS, f"-I{REPO_ROOT / 'include'}", f"-I{PACKAGE_DIR / 'csrc'}", *srcs, "-o", str(tmp)]
+    _run(cmd)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def build_oracle():
+    ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off", str(ORACLE_SRC), "-o",
+           str(ORACLE_LIB)]
+    _run(cmd)
+    return ORACLE_LIB
+
+
+def build_reference_shim():
+    """oracle/_ref/libluxref.so from the reference's own host sources (only where
+    they are mounted; the GPU box uses the prebuilt file)."""
+    if not REFERENCE_SRC.exists() or shutil.which("make") is None:
+        return None
+    _run(["make", "-s", "-B", "-f", "oracle/ref.mk", f"REF_SRC={REFERENCE_SRC}"])
+    return REPO_ROOT / "oracle" / "_ref" / "libluxref.so"
+
+
+def build_all():
+    out = [build_hip(), build_oracle()]
+    ref = build_reference_shim()
+    if ref:
+        out.append(ref)
+    return out

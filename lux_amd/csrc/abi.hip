@@ -1,0 +1,638 @@
+// abi.hip -- the extern "C" boundary (include/lux_hip.h) over the gfx950
+// kernels: handle ownership, host<->device staging for the drop-in calls,
+// status codes + thread-local error text, and the kernel-time probe.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lux_runtime.hpp"
+
+struct lux_hip_bvh {
+    lux::BvhDev dev;
+    bool valid = false;              // GpuBvh::is_valid
+    bool host_ids_known = false;     // host copy of the build ids is current
+    std::vector<int32_t> host_ids;   // for refit's id check (src/bvh/gpu_bvh.cu:601-605)
+    hipStream_t last_stream = nullptr;
+};
+
+namespace lux {
+namespace {
+
+thread_local std::string g_last_error;
+bool g_initialized = false;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+
+struct Probe {
+    std::mutex mu;
+    bool open = false;
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t pending = nullptr;
+} g_probe;
+
+hipEvent_t probe_event() {
+    if (!g_probe.pool.empty()) {
+        hipEvent_t e = g_probe.pool.back();
+        g_probe.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const Error& e) {
+        return fail(e.status, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(LUX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(LUX_ERR_HIP, e.what());
+    }
+}
+
+void require_init() {
+    if (!g_initialized) throw Error(LUX_ERR_NOT_INITIALIZED, "lux_hip_init() has not been called");
+}
+
+hipStream_t pick(lux_stream s) { return s ? static_cast<hipStream_t>(s) : g_stream; }
+
+// Reads and clears the device-side check bits of a handle (caller synchronised).
+void check_device_flags(lux_hip_bvh* h) {
+    if (!h->dev.info) return;
+    DevInfo info;
+    hip_check(hipMemcpy(&info, h->dev.info, sizeof(DevInfo), hipMemcpyDeviceToHost), "read device info");
+    if (info.error == 0) return;
+    uint32_t zero = 0;
+    hip_check(hipMemcpy(reinterpret_cast<char*>(h->dev.info) + offsetof(DevInfo, error), &zero,
+                        sizeof(zero), hipMemcpyHostToDevice),
+              "clear device info");
+    std::string what;
+    if (info.error & kErrNegativeId) what += " negative primitive id;";
+    if (info.error & kErrPrimRange) what += " leaf primitive id outside the triangle array;";
+    if (info.error & kErrSpinLimit) what += " radix-sort look-back spin limit;";
+    if (info.error & kErrWalkOrder) what += " stackless walk did not advance;";
+    if (info.error & kErrClimbLimit) what += " bottom-up pass exceeded n levels;";
+    if (info.error & kErrNegativeId) h->valid = false;
+    throw Error(LUX_ERR_DEVICE_CHECK, "device check failed:" + what);
+}
+
+void sync_handle(lux_hip_bvh* h) {
+    hip_check(hipStreamSynchronize(h->last_stream ? h->last_stream : g_stream), "synchronize");
+    check_device_flags(h);
+}
+
+void require_handle(lux_hip_bvh* h) {
+    if (!h) throw Error(LUX_ERR_INVALID_ARGUMENT, "null lux_hip_bvh handle");
+}
+
+template <typename T>
+struct DeviceBuffer {
+    T* p = nullptr;
+    explicit DeviceBuffer(size_t count) {
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e != hipSuccess) throw Error(LUX_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    }
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
+
+}  // namespace
+
+hipStream_t library_stream() { return g_stream; }
+
+void probe_pre_launch(const char* name, hipStream_t s) {
+    if (!g_probe.open || g_probe.name != name) return;
+    std::lock_guard<std::mutex> lock(g_probe.mu);
+    g_probe.pending = probe_event();
+    hip_check(hipEventRecord(g_probe.pending, s), "hipEventRecord");
+}
+
+void probe_post_launch(const char* name, hipStream_t s) {
+    if (!g_probe.open || g_probe.name != name || !g_probe.pending) return;
+    std::lock_guard<std::mutex> lock(g_probe.mu);
+    hipEvent_t stop = probe_event();
+    hip_check(hipEventRecord(stop, s), "hipEventRecord");
+    g_probe.events.emplace_back(g_probe.pending, stop);
+    g_probe.pending = nullptr;
+}
+
+}  // namespace lux
+
+using namespace lux;
+
+extern "C" {
+
+int lux_hip_version(void) { return 0x000100; }
+
+const char* lux_hip_last_error(void) { return g_last_error.c_str(); }
+
+int lux_hip_init(int device) {
+    return guarded([&] {
+        if (g_initialized) {
+            if (device != g_device) throw Error(LUX_ERR_INVALID_ARGUMENT, "already initialised on another device");
+            return int(LUX_OK);
+        }
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "device index out of range");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
+        g_device = device;
+        g_initialized = true;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_shutdown(void) {
+    return guarded([&] {
+        if (!g_initialized) return int(LUX_OK);
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        {
+            std::lock_guard<std::mutex> lock(g_probe.mu);
+            for (auto& pr : g_probe.events) {
+                (void)hipEventDestroy(pr.first);
+                (void)hipEventDestroy(pr.second);
+            }
+            for (auto e : g_probe.pool) (void)hipEventDestroy(e);
+            g_probe.events.clear();
+            g_probe.pool.clear();
+            g_probe.open = false;
+        }
+        (void)hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+        g_initialized = false;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        require_init();
+        if (!ptr) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output pointer");
+        *ptr = nullptr;
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+        if (e != hipSuccess) throw Error(LUX_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_free(void* ptr) {
+    return guarded([&] {
+        require_init();
+        if (ptr) hip_check(hipFree(ptr), "hipFree");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        require_init();
+        if (bytes && (!dst || !src)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null pointer");
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream), "hipMemcpy H2D");
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        require_init();
+        if (bytes && (!dst || !src)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null pointer");
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_synchronize(void) {
+    return guarded([&] {
+        require_init();
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_create(lux_hip_bvh** out) {
+    return guarded([&] {
+        require_init();
+        if (!out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output pointer");
+        *out = new lux_hip_bvh();
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_destroy(lux_hip_bvh* bvh) {
+    return guarded([&] {
+        if (!bvh) return int(LUX_OK);
+        if (g_initialized) (void)hipStreamSynchronize(bvh->last_stream ? bvh->last_stream : g_stream);
+        bvh_release(bvh->dev);
+        delete bvh;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_build(lux_hip_bvh* bvh, const lux_aabb* boxes, const int32_t* ids, int32_t n) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative primitive count");
+        if (n > 0 && (!boxes || !ids)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null input arrays");
+        if (n >= (1 << 29)) throw Error(LUX_ERR_INVALID_ARGUMENT, "primitive count exceeds 2^29");
+        for (int32_t i = 0; i < n; ++i)
+            if (ids[i] < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "primitive ids must be >= 0");
+        bvh->valid = false;
+        bvh->host_ids_known = false;
+        if (n == 0) {  // build_gpu_lbvh of an empty list: invalid GpuBvh, count 0
+            bvh->dev.n = 0;
+            bvh->host_ids.clear();
+            bvh->host_ids_known = true;
+            return int(LUX_OK);
+        }
+        bvh_allocate(bvh->dev, n);
+        hip_check(hipMemcpyAsync(bvh->dev.boxes, boxes, sizeof(lux_aabb) * n, hipMemcpyHostToDevice, g_stream),
+                  "H2D boxes");
+        hip_check(hipMemcpyAsync(bvh->dev.ids, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream),
+                  "H2D ids");
+        bvh->last_stream = g_stream;
+        lbvh_build(bvh->dev, bvh->dev.boxes, bvh->dev.ids, n, g_stream);
+        sync_handle(bvh);
+        bvh->host_ids.assign(ids, ids + n);
+        bvh->host_ids_known = true;
+        bvh->valid = true;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_build_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes, const int32_t* d_ids, int32_t n,
+                             lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative primitive count");
+        if (n > 0 && (!d_boxes || !d_ids)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null input arrays");
+        if (n >= (1 << 29)) throw Error(LUX_ERR_INVALID_ARGUMENT, "primitive count exceeds 2^29");
+        bvh->valid = false;
+        bvh->host_ids_known = false;
+        if (n == 0) {
+            bvh->dev.n = 0;
+            return int(LUX_OK);
+        }
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        bvh_allocate(bvh->dev, n);
+        lbvh_build(bvh->dev, d_boxes, d_ids, n, s);
+        bvh->valid = true;  // negative ids are reported by lux_hip_bvh_info
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_refit(lux_hip_bvh* bvh, const lux_aabb* boxes, const int32_t* ids, int32_t n) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || (n > 0 && (!boxes || !ids))) throw Error(LUX_ERR_INVALID_ARGUMENT, "bad refit input");
+        if (n != (bvh->valid ? bvh->dev.n : 0))
+            throw Error(LUX_ERR_MISMATCH, "refit primitive count does not match the built BVH");
+        if (!bvh->valid) return int(LUX_OK);  // empty == empty (src/bvh/gpu_bvh.cu:599)
+        if (!bvh->host_ids_known) {
+            sync_handle(bvh);
+            bvh->host_ids.resize(n);
+            hip_check(hipMemcpy(bvh->host_ids.data(), bvh->dev.ids, sizeof(int32_t) * n, hipMemcpyDeviceToHost),
+                      "D2H ids");
+            bvh->host_ids_known = true;
+        }
+        for (int32_t i = 0; i < n; ++i)
+            if (bvh->host_ids[i] != ids[i])
+                throw Error(LUX_ERR_MISMATCH, "refit primitive ids differ from the built BVH");
+        hipStream_t s = g_stream;
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        hip_check(hipMemcpyAsync(bvh->dev.boxes, boxes, sizeof(lux_aabb) * n, hipMemcpyHostToDevice, s), "H2D boxes");
+        lbvh_refit_boxes(bvh->dev, bvh->dev.boxes, s);
+        sync_handle(bvh);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_refit_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes, int32_t n, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n != (bvh->valid ? bvh->dev.n : 0))
+            throw Error(LUX_ERR_MISMATCH, "refit primitive count does not match the built BVH");
+        if (!bvh->valid) return int(LUX_OK);
+        if (!d_boxes) throw Error(LUX_ERR_INVALID_ARGUMENT, "null boxes");
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        lbvh_refit_boxes(bvh->dev, d_boxes, s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_refit_triangles_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t n,
+                                       lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n != (bvh->valid ? bvh->dev.n : 0))
+            throw Error(LUX_ERR_MISMATCH, "refit primitive count does not match the built BVH");
+        if (!bvh->valid) return int(LUX_OK);
+        if (!d_tris) throw Error(LUX_ERR_INVALID_ARGUMENT, "null triangles");
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        lbvh_refit_triangles(bvh->dev, d_tris, s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_info(lux_hip_bvh* bvh, int32_t* valid, int32_t* count, lux_aabb* root) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (bvh->valid) sync_handle(bvh);
+        if (valid) *valid = bvh->valid ? 1 : 0;
+        if (count) *count = bvh->valid ? bvh->dev.n : 0;
+        if (root) {
+            lux_aabb empty;
+            empty.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
+            empty.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
+            *root = empty;
+            if (bvh->valid)
+                hip_check(hipMemcpy(root, &bvh->dev.info->root, sizeof(lux_aabb), hipMemcpyDeviceToHost), "D2H root");
+        }
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_view(lux_hip_bvh* bvh, lux_bvh_view* out) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (!out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output");
+        lux_bvh_view v;
+        std::memset(&v, 0, sizeof(v));
+        if (bvh->valid) {
+            sync_handle(bvh);
+            v.nodes = bvh->dev.nodes;
+            v.node_count = 2 * bvh->dev.n - 1;
+            hip_check(hipMemcpy(&v.root_bounds, &bvh->dev.info->root, sizeof(lux_aabb), hipMemcpyDeviceToHost),
+                      "D2H root");
+        }
+        *out = v;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_download_nodes(lux_hip_bvh* bvh, lux_bvh_node* nodes, int32_t capacity) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (!bvh->valid) return int(LUX_OK);
+        int total = 2 * bvh->dev.n - 1;
+        if (!nodes || capacity < total) throw Error(LUX_ERR_INVALID_ARGUMENT, "node buffer too small");
+        sync_handle(bvh);
+        hip_check(hipMemcpy(nodes, bvh->dev.nodes, sizeof(lux_bvh_node) * total, hipMemcpyDeviceToHost), "D2H nodes");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_download_topology(lux_hip_bvh* bvh, uint32_t* morton_sorted, uint32_t* sorted_indices,
+                                  uint32_t* left, uint32_t* right, uint32_t* parent, uint32_t* leaf_parent,
+                                  lux_aabb* node_bounds) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (!bvh->valid) return int(LUX_OK);
+        sync_handle(bvh);
+        const BvhDev& d = bvh->dev;
+        int n = d.n;
+        auto d2h = [](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H topology");
+        };
+        d2h(morton_sorted, d.keys[0], 4 * size_t(n));
+        d2h(sorted_indices, d.vals[0], 4 * size_t(n));
+        if (n > 1) {
+            d2h(left, d.left, 4 * size_t(n - 1));
+            d2h(right, d.right, 4 * size_t(n - 1));
+            d2h(parent, d.parent, 4 * size_t(n - 1));
+            d2h(leaf_parent, d.leaf_parent, 4 * size_t(n));
+        } else if (leaf_parent) {
+            leaf_parent[0] = 0;
+        }
+        if (node_bounds) {
+            int total = 2 * n - 1;
+            std::vector<lux_bvh_node> nodes(total);
+            std::vector<int32_t> pre_i(std::max(n - 1, 1)), pre_l(n);
+            d2h(nodes.data(), d.nodes, sizeof(lux_bvh_node) * total);
+            d2h(pre_l.data(), d.pre_leaf, 4 * size_t(n));
+            if (n > 1) d2h(pre_i.data(), d.pre_internal, 4 * size_t(n - 1));
+            for (int i = 0; i < n - 1; ++i) node_bounds[i] = nodes[pre_i[i]].bounds;
+            for (int j = 0; j < n; ++j) node_bounds[n - 1 + j] = nodes[pre_l[j]].bounds;
+        }
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_trace_closest(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tri_count, const lux_ray* rays,
+                          int32_t n, lux_hit* hits) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > 0 && (!rays || !hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
+        for (int32_t i = 0; i < n; ++i) {
+            hits[i].primitive_id = -1;
+            hits[i].t = INFINITY;
+        }
+        if (!bvh->valid || n == 0) return int(LUX_OK);
+        if (tri_count > 0 && !tris) throw Error(LUX_ERR_INVALID_ARGUMENT, "null triangle array");
+        sync_handle(bvh);
+        DeviceBuffer<lux_triangle> dt(tri_count);
+        DeviceBuffer<lux_ray> dr(n);
+        DeviceBuffer<lux_hit> dh(n);
+        hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
+        hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
+        bvh->last_stream = g_stream;
+        trace_closest(bvh->dev, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        hip_check(hipMemcpyAsync(hits, dh.p, sizeof(lux_hit) * n, hipMemcpyDeviceToHost, g_stream), "D2H hits");
+        sync_handle(bvh);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t tri_count,
+                                 const lux_ray* d_rays, int32_t n, lux_hit* d_hits, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > 0 && (!d_rays || !d_hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
+        if (!bvh->valid) throw Error(LUX_ERR_INVALID_ARGUMENT, "BVH is not valid");
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        trace_closest(bvh->dev, d_tris, tri_count, d_rays, n, d_hits, s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_trace_any(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tri_count, const lux_ray* rays,
+                      int32_t n, uint8_t* occluded) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > 0 && (!rays || !occluded)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/output arrays");
+        std::memset(occluded, 0, size_t(n));
+        if (!bvh->valid || n == 0) return int(LUX_OK);
+        if (tri_count > 0 && !tris) throw Error(LUX_ERR_INVALID_ARGUMENT, "null triangle array");
+        sync_handle(bvh);
+        DeviceBuffer<lux_triangle> dt(tri_count);
+        DeviceBuffer<lux_ray> dr(n);
+        DeviceBuffer<uint8_t> dh(n);
+        hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
+        hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
+        bvh->last_stream = g_stream;
+        trace_any(bvh->dev, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        hip_check(hipMemcpyAsync(occluded, dh.p, size_t(n), hipMemcpyDeviceToHost, g_stream), "D2H occluded");
+        sync_handle(bvh);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_trace_any_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t tri_count,
+                             const lux_ray* d_rays, int32_t n, uint8_t* d_occluded, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > 0 && (!d_rays || !d_occluded)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/output arrays");
+        if (!bvh->valid) throw Error(LUX_ERR_INVALID_ARGUMENT, "BVH is not valid");
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        trace_any(bvh->dev, d_tris, tri_count, d_rays, n, d_occluded, s);
+        return int(LUX_OK);
+    });
+}
+
+static int overlap_host(lux_hip_bvh* tree, lux_hip_bvh* queries, bool self_mode, int32_t max_pairs,
+                        lux_pair* pairs, int32_t* found) {
+    require_init();
+    require_handle(tree);
+    require_handle(queries);
+    if (!found) throw Error(LUX_ERR_INVALID_ARGUMENT, "null found_count");
+    *found = 0;
+    if (!tree->valid || !queries->valid || max_pairs <= 0) return LUX_OK;
+    if (!pairs) throw Error(LUX_ERR_INVALID_ARGUMENT, "null pair buffer");
+    sync_handle(tree);
+    if (queries != tree) sync_handle(queries);
+    DeviceBuffer<lux_pair> dp{size_t(max_pairs)};
+    tree->last_stream = g_stream;
+    overlap_query(tree->dev, queries->dev, self_mode, max_pairs, dp.p, tree->dev.counter, g_stream);
+    int32_t count = 0;
+    hip_check(hipMemcpyAsync(&count, tree->dev.counter, sizeof(int32_t), hipMemcpyDeviceToHost, g_stream), "D2H count");
+    hip_check(hipStreamSynchronize(g_stream), "synchronize");
+    int32_t copy = count < max_pairs ? count : max_pairs;
+    if (copy > 0) hip_check(hipMemcpy(pairs, dp.p, sizeof(lux_pair) * copy, hipMemcpyDeviceToHost), "D2H pairs");
+    check_device_flags(tree);
+    *found = count;
+    return LUX_OK;
+}
+
+int lux_hip_overlap_self(lux_hip_bvh* bvh, int32_t max_pairs, lux_pair* pairs, int32_t* found) {
+    return guarded([&] { return overlap_host(bvh, bvh, true, max_pairs, pairs, found); });
+}
+
+int lux_hip_overlap_cross(lux_hip_bvh* left, lux_hip_bvh* right, int32_t max_pairs, lux_pair* pairs,
+                          int32_t* found) {
+    return guarded([&] { return overlap_host(left, right, false, max_pairs, pairs, found); });
+}
+
+static int overlap_device(lux_hip_bvh* tree, lux_hip_bvh* queries, bool self_mode, int32_t max_pairs,
+                          lux_pair* d_pairs, int32_t* d_found, lux_stream stream) {
+    require_init();
+    require_handle(tree);
+    require_handle(queries);
+    if (!d_found) throw Error(LUX_ERR_INVALID_ARGUMENT, "null found_count");
+    if (max_pairs > 0 && !d_pairs) throw Error(LUX_ERR_INVALID_ARGUMENT, "null pair buffer");
+    hipStream_t s = pick(stream);
+    for (lux_hip_bvh* h : {tree, queries})
+        if (h->last_stream && h->last_stream != s) hip_check(hipStreamSynchronize(h->last_stream), "synchronize");
+    tree->last_stream = s;
+    queries->last_stream = s;
+    BvhDev none;
+    overlap_query(tree->valid ? tree->dev : none, queries->valid ? queries->dev : none, self_mode, max_pairs,
+                  d_pairs, d_found, s);
+    return LUX_OK;
+}
+
+int lux_hip_overlap_self_device(lux_hip_bvh* bvh, int32_t max_pairs, lux_pair* d_pairs, int32_t* d_found,
+                                lux_stream stream) {
+    return guarded([&] { return overlap_device(bvh, bvh, true, max_pairs, d_pairs, d_found, stream); });
+}
+
+int lux_hip_overlap_cross_device(lux_hip_bvh* left, lux_hip_bvh* right, int32_t max_pairs, lux_pair* d_pairs,
+                                 int32_t* d_found, lux_stream stream) {
+    return guarded([&] { return overlap_device(left, right, false, max_pairs, d_pairs, d_found, stream); });
+}
+
+int lux_hip_probe_begin(const char* kernel_name) {
+    return guarded([&] {
+        require_init();
+        if (!kernel_name) throw Error(LUX_ERR_INVALID_ARGUMENT, "null kernel name");
+        std::lock_guard<std::mutex> lock(g_probe.mu);
+        for (auto& pr : g_probe.events) {
+            g_probe.pool.push_back(pr.first);
+            g_probe.pool.push_back(pr.second);
+        }
+        g_probe.events.clear();
+        g_probe.name = kernel_name;
+        g_probe.open = true;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_probe_end(double* total_ms, int64_t* launches) {
+    return guarded([&] {
+        require_init();
+        std::lock_guard<std::mutex> lock(g_probe.mu);
+        g_probe.open = false;
+        double sum = 0;
+        for (auto& pr : g_probe.events) {
+            hip_check(hipEventSynchronize(pr.second), "hipEventSynchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime");
+            sum += ms;
+        }
+        if (total_ms) *total_ms = sum;
+        if (launches) *launches = int64_t(g_probe.events.size());
+        return int(LUX_OK);
+    });
+}
+
+}  // extern "C"

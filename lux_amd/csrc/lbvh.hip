@@ -1,0 +1,874 @@
+// lbvh.hip -- LBVH construction and refit for gfx950.
+//
+// Build pipeline (one stream, no host round trip):
+//   root_partials  grid-stride min/max of the input boxes (+ id range) into
+//                  per-block partials; block 0 clears the histogram and the
+//                  frontier counter
+//   morton_hist    every block folds the partials into the root box, then
+//                  30-bit Morton codes, identity payload and the four 8-bit
+//                  digit histograms
+//   sort_pass x4   stable LSD radix pass with decoupled look-back (one read and
+//                  one write of key+value per pass)
+//   karras         radix tree over the sorted codes (merged 64-bit keys); also
+//                  lists the "frontier": nodes whose leaf range lies inside one
+//                  leaf block while their parent's range does not
+//   top_layout     right-first preorder positions of the nodes above the
+//                  frontier (one thread per frontier node)
+//   local          one workgroup per block of kLocalLeaves sorted leaves: places
+//                  the leaves and the nodes inside the block, then unions boxes
+//                  bottom-up inside the block with LDS arrival counters
+//   top_climb      one workgroup unions the boxes of the nodes above the frontier
+// Refit runs only local (bounds phase) + top_climb.
+//
+// Only the few thousand nodes above the frontier need cross-workgroup hand-off,
+// and a single workgroup handles them, so no agent-scope release/acquire (an L2
+// write-back + invalidate on this chip) is issued per node.
+//
+// Algorithms follow the reference: Morton quantisation
+// src/bvh/cpu_lbvh_builder.h:37-61 (device twin src/bvh/gpu_bvh.cu:135-149),
+// stable key order src/bvh/cpu_lbvh_builder.h:212-216, Karras 2012 range/split
+// with merged keys :63-131, internal box = union of children :139-159.
+#include "lux_runtime.hpp"
+
+namespace lux {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kBlock * kSortItems;  // 2048 keys per tile
+constexpr int kRadix = 256;
+constexpr int kPasses = 4;                      // 4 x 8 bits cover the 30-bit codes
+constexpr int kReduceBlocksMax = 1024;
+constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kTopThreads = 1024;
+
+// ---------------------------------------------------------------------------
+// block-wide helpers (256 threads = 4 waves of 64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= uint32_t(off)) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_tmp /*4 words*/) {
+    uint32_t incl = wave_incl_scan(x);
+    uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += s_tmp[w];
+    __syncthreads();
+    return incl - x + add;
+}
+
+struct BoxIds {
+    float lx, ly, lz, hx, hy, hz;
+    int32_t imin, imax;
+};
+
+__device__ __forceinline__ void fold(BoxIds& a, const BoxIds& b) {
+    a.lx = fminf(a.lx, b.lx);
+    a.ly = fminf(a.ly, b.ly);
+    a.lz = fminf(a.lz, b.lz);
+    a.hx = fmaxf(a.hx, b.hx);
+    a.hy = fmaxf(a.hy, b.hy);
+    a.hz = fmaxf(a.hz, b.hz);
+    a.imin = min(a.imin, b.imin);
+    a.imax = max(a.imax, b.imax);
+}
+
+__device__ __forceinline__ BoxIds boxids_empty() {
+    return BoxIds{INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, INT32_MAX, INT32_MIN};
+}
+
+// Block-wide fold of one BoxIds per thread; result valid in thread 0.
+__device__ __forceinline__ BoxIds block_fold(BoxIds v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        BoxIds o;
+        o.lx = __shfl_xor(v.lx, off, 64);
+        o.ly = __shfl_xor(v.ly, off, 64);
+        o.lz = __shfl_xor(v.lz, off, 64);
+        o.hx = __shfl_xor(v.hx, off, 64);
+        o.hy = __shfl_xor(v.hy, off, 64);
+        o.hz = __shfl_xor(v.hz, off, 64);
+        o.imin = __shfl_xor(v.imin, off, 64);
+        o.imax = __shfl_xor(v.imax, off, 64);
+        fold(v, o);
+    }
+    __shared__ BoxIds s_w[kBlock / 64];
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kBlock / 64; ++w) fold(v, s_w[w]);
+    __syncthreads();
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// 1. root bounds partials (+ optional copy of the input into the handle)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_root_partials(const lux_aabb* __restrict__ in_boxes,
+                                                          const int32_t* __restrict__ in_ids,
+                                                          lux_aabb* __restrict__ boxes,
+                                                          int32_t* __restrict__ ids, int copy, int n,
+                                                          BoxIds* __restrict__ partials,
+                                                          uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ frontier_count,
+                                                          DevInfo* __restrict__ info) {
+    BoxIds acc = boxids_empty();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        lux_aabb b = in_boxes[i];
+        int32_t id = in_ids[i];
+        if (copy) {
+            boxes[i] = b;
+            ids[i] = id;
+        }
+        BoxIds v{b.p_min.x, b.p_min.y, b.p_min.z, b.p_max.x, b.p_max.y, b.p_max.z, id, id};
+        fold(acc, v);
+    }
+    acc = block_fold(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+    if (blockIdx.x == 0) {
+        for (int w = threadIdx.x; w < kPasses * kRadix; w += kBlock) hist[w] = 0;
+        if (threadIdx.x == 0) {
+            *frontier_count = 0;
+            info->max_depth = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Morton codes + digit histograms
+// ---------------------------------------------------------------------------
+// Spread the 10 low bits of v to every third bit (bit k -> bit 3k).
+__device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quantize10(float value) {
+    value = fmaxf(0.0f, fminf(1023.0f, value * 1024.0f));
+    return uint32_t(value);
+}
+
+__device__ __forceinline__ uint32_t morton30(const lux_aabb& root, const lux_aabb& b) {
+    // centroid = (p_min + p_max) * 0.5, normalized_offset against the root box
+    float cx = (b.p_min.x + b.p_max.x) * 0.5f;
+    float cy = (b.p_min.y + b.p_max.y) * 0.5f;
+    float cz = (b.p_min.z + b.p_max.z) * 0.5f;
+    float dx = root.p_max.x - root.p_min.x;
+    float dy = root.p_max.y - root.p_min.y;
+    float dz = root.p_max.z - root.p_min.z;
+    float ox = dx > 0 ? (cx - root.p_min.x) / dx : 0.5f;
+    float oy = dy > 0 ? (cy - root.p_min.y) / dy : 0.5f;
+    float oz = dz > 0 ? (cz - root.p_min.z) / dz : 0.5f;
+    return (spread_bits3(quantize10(ox)) << 2) | (spread_bits3(quantize10(oy)) << 1) |
+           spread_bits3(quantize10(oz));
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton_hist(const lux_aabb* __restrict__ boxes, int n,
+                                                        const BoxIds* __restrict__ partials,
+                                                        int num_partials, DevInfo* __restrict__ info,
+                                                        uint32_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals,
+                                                        uint32_t* hist) {
+    __shared__ uint32_t s_hist[kPasses * kRadix];
+    __shared__ lux_aabb s_root;
+    for (int w = threadIdx.x; w < kPasses * kRadix; w += kBlock) s_hist[w] = 0;
+    // Every block folds the (at most 1024) partials itself: exact min/max, so all
+    // blocks get the same root box without a cross-block hand-off.
+    BoxIds acc = boxids_empty();
+    for (int i = threadIdx.x; i < num_partials; i += kBlock) fold(acc, partials[i]);
+    acc = block_fold(acc);
+    if (threadIdx.x == 0) {
+        lux_aabb r;
+        r.p_min = lux_vec3{acc.lx, acc.ly, acc.lz};
+        r.p_max = lux_vec3{acc.hx, acc.hy, acc.hz};
+        s_root = r;
+        if (blockIdx.x == 0) {
+            info->root = r;
+            info->id_min = acc.imin;
+            info->id_max = acc.imax;
+            if (acc.imin < 0) info->error |= kErrNegativeId;
+        }
+    }
+    __syncthreads();
+    const lux_aabb root = s_root;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        uint32_t c = morton30(root, boxes[i]);
+        keys[i] = c;
+        vals[i] = uint32_t(i);
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) atomicAdd(&s_hist[p * kRadix + ((c >> (8 * p)) & 0xffu)], 1u);
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < kPasses * kRadix; w += kBlock) {
+        uint32_t c = s_hist[w];
+        if (c) __hip_atomic_fetch_add(&hist[w], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. one stable LSD radix pass (8-bit digit) with decoupled look-back
+// ---------------------------------------------------------------------------
+// Look-back word: [epoch:32][flag:2][count:30]. flag 1 = tile aggregate only,
+// flag 2 = inclusive prefix over tiles 0..t. Words from an older pass carry an
+// older epoch and read as "not ready", so the array is never cleared.
+constexpr uint64_t kFlagAggregate = 1;
+constexpr uint64_t kFlagInclusive = 2;
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint64_t flag, uint32_t count) {
+    return (uint64_t(epoch) << 32) | (flag << 30) | uint64_t(count & 0x3fffffffu);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint32_t* __restrict__ keys_in,
+                                                      const uint32_t* __restrict__ vals_in,
+                                                      uint32_t* __restrict__ keys_out,
+                                                      uint32_t* __restrict__ vals_out, int n,
+                                                      int shift, const uint32_t* __restrict__ hist,
+                                                      uint64_t* status, uint32_t* ticket,
+                                                      uint32_t epoch, int num_tiles,
+                                                      DevInfo* info) {
+    __shared__ uint32_t s_wave_cnt[4][kRadix];
+    __shared__ uint32_t s_tile_start[kRadix];
+    __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint32_t s_ticket;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t lane = lane_id();
+
+    if (tid == 0) s_ticket = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int w = tid; w < 4 * kRadix; w += kBlock) (&s_wave_cnt[0][0])[w] = 0;
+    __syncthreads();
+    const uint32_t tile = s_ticket;
+    // Tickets are handed out in start order, so a tile only ever waits on tiles
+    // that are already running. The last ticket resets the counter for the next
+    // launch on this stream.
+    if (tid == 0 && tile == uint32_t(num_tiles - 1))
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    const int tile_base = int(tile) * kSortTile;
+    uint32_t k[kSortItems], v[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        int g = tile_base + int(wave) * (64 * kSortItems) + r * 64 + int(lane);
+        bool valid = g < n;
+        k[r] = valid ? keys_in[g] : 0u;
+        v[r] = valid ? vals_in[g] : 0u;
+    }
+    // Rank each key among equal digits: earlier rounds of this wave via the
+    // running per-wave counter, earlier lanes of this round via ballot matching.
+    volatile uint32_t* wave_cnt = &s_wave_cnt[wave][0];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        int g = tile_base + int(wave) * (64 * kSortItems) + r * 64 + int(lane);
+        bool valid = g < n;
+        uint32_t dig = (k[r] >> shift) & 0xffu;
+        uint64_t peers = ballot64(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (dig >> b) & 1u;
+            uint64_t m = ballot64(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t prior = valid ? wave_cnt[dig] : 0u;
+        rank[r] = prior + uint32_t(__popcll(peers & lanemask_lt()));
+        bool last_of_group = valid && ((peers >> lane) == 1ull);
+        __builtin_amdgcn_wave_barrier();
+        if (last_of_group) wave_cnt[dig] = prior + uint32_t(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Per digit (thread = digit): wave offsets and the tile total.
+    const uint32_t dgt = tid;
+    uint32_t c0 = s_wave_cnt[0][dgt], c1 = s_wave_cnt[1][dgt], c2 = s_wave_cnt[2][dgt],
+             c3 = s_wave_cnt[3][dgt];
+    uint32_t total = c0 + c1 + c2 + c3;
+    uint64_t* my_status = status + size_t(tile) * kRadix + dgt;
+    __hip_atomic_store(my_status, pack_status(epoch, tile == 0 ? kFlagInclusive : kFlagAggregate, total),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_wave_cnt[0][dgt] = 0;
+    s_wave_cnt[1][dgt] = c0;
+    s_wave_cnt[2][dgt] = c0 + c1;
+    s_wave_cnt[3][dgt] = c0 + c1 + c2;
+
+    uint32_t tile_start = block_excl_scan(total, s_tmp);
+    uint32_t global_start = block_excl_scan(hist[dgt], s_tmp);
+
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int j = int(tile) - 1;
+        uint32_t spins = 0;
+        while (j >= 0) {
+            uint64_t s = __hip_atomic_load(status + size_t(j) * kRadix + dgt, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t flag = (s >> 30) & 3u;
+            if (uint32_t(s >> 32) != epoch || flag == 0) {
+                if (++spins > kSpinLimit) {
+                    set_error(info, kErrSpinLimit);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += uint32_t(s & 0x3fffffffu);
+            if (flag == kFlagInclusive) break;
+            --j;
+        }
+        __hip_atomic_store(my_status, pack_status(epoch, kFlagInclusive, excl + total),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_tile_start[dgt] = tile_start;
+    s_base[dgt] = global_start + excl;
+    __syncthreads();
+
+    // Local scatter into LDS in digit order, then coalesced runs to HBM.
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        int g = tile_base + int(wave) * (64 * kSortItems) + r * 64 + int(lane);
+        if (g < n) {
+            uint32_t dig = (k[r] >> shift) & 0xffu;
+            uint32_t pos = s_tile_start[dig] + s_wave_cnt[wave][dig] + rank[r];
+            s_keys[pos] = k[r];
+            s_vals[pos] = v[r];
+        }
+    }
+    __syncthreads();
+    int tile_count = min(kSortTile, n - tile_base);
+    for (int i = tid; i < tile_count; i += kBlock) {
+        uint32_t key = s_keys[i];
+        uint32_t dig = (key >> shift) & 0xffu;
+        uint32_t out = s_base[dig] + (uint32_t(i) - s_tile_start[dig]);
+        if (out >= uint32_t(n)) {  // only after a failed look-back
+            set_error(info, kErrSpinLimit);
+            continue;
+        }
+        keys_out[out] = key;
+        vals_out[out] = s_vals[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. Karras radix tree (+ frontier list)
+// ---------------------------------------------------------------------------
+// Leaves are grouped into blocks of kLocalLeaves consecutive sorted leaves. A
+// node is LOCAL when its leaf range lies inside one block; a local node whose
+// parent is not local is a FRONTIER node. Every internal node is either local
+// (handled entirely by the workgroup of its block) or above the frontier.
+constexpr int kLocalLeaves = 1024;
+constexpr int kLocalThreads = 256;
+
+__device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ codes, int a, int b) {
+    uint64_t ka = (uint64_t(codes[a]) << 32) | uint64_t(uint32_t(a));
+    uint64_t kb = (uint64_t(codes[b]) << 32) | uint64_t(uint32_t(b));
+    return __clzll(ka ^ kb);  // a != b, so the keys differ
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ codes, int n,
+                                                   uint32_t* __restrict__ left,
+                                                   uint32_t* __restrict__ right, uint32_t* parent,
+                                                   uint32_t* leaf_parent,
+                                                   uint32_t* __restrict__ first_out,
+                                                   uint32_t* __restrict__ last_out,
+                                                   uint32_t* __restrict__ arrive,
+                                                   uint32_t* __restrict__ frontier,
+                                                   uint32_t* frontier_count) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int first, last;
+    if (i == 0) {
+        first = 0;
+        last = n - 1;
+        parent[0] = 0;
+    } else {
+        int dl = prefix_len(codes, i, i - 1);
+        int dr = prefix_len(codes, i, i + 1);
+        int dir = dr > dl ? 1 : -1;
+        int dmin = min(dl, dr);
+        int lmax = 2;
+        for (;;) {
+            int j = i + dir * lmax;
+            if (j < 0 || j >= n || prefix_len(codes, i, j) <= dmin) break;
+            lmax <<= 1;
+        }
+        int len = 0;
+        for (int step = lmax >> 1; step > 0; step >>= 1) {
+            int j = i + (len + step) * dir;
+            if (j < 0 || j >= n) continue;
+            if (prefix_len(codes, i, j) > dmin) len += step;
+        }
+        int other = i + len * dir;
+        first = dir < 0 ? other : i;
+        last = dir < 0 ? i : other;
+    }
+    int dnode = prefix_len(codes, first, last);
+    int split = first, stride = last - first;
+    do {
+        stride = (stride + 1) >> 1;
+        int mid = split + stride;
+        if (mid < last && prefix_len(codes, first, mid) > dnode) split = mid;
+    } while (stride > 1);
+
+    uint32_t lchild, rchild;
+    if (first == split) {
+        lchild = kLeafBit | uint32_t(split);
+        leaf_parent[split] = uint32_t(i);
+    } else {
+        lchild = uint32_t(split);
+        parent[split] = uint32_t(i);
+    }
+    if (last == split + 1) {
+        rchild = kLeafBit | uint32_t(split + 1);
+        leaf_parent[split + 1] = uint32_t(i) | kLeafBit;
+    } else {
+        rchild = uint32_t(split + 1);
+        parent[split + 1] = uint32_t(i) | kLeafBit;
+    }
+    left[i] = lchild;
+    right[i] = rchild;
+    first_out[i] = uint32_t(first);
+    last_out[i] = uint32_t(last);
+    arrive[i] = 0;
+    if (first / kLocalLeaves != last / kLocalLeaves) {
+        if (first / kLocalLeaves == split / kLocalLeaves)
+            frontier[__hip_atomic_fetch_add(frontier_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = lchild;
+        if ((split + 1) / kLocalLeaves == last / kLocalLeaves)
+            frontier[__hip_atomic_fetch_add(frontier_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = rchild;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. right-first preorder layout
+// ---------------------------------------------------------------------------
+// In right-first preorder a node X is preceded by its ancestors and by the right
+// subtrees of the ancestors whose LEFT subtree holds X; those subtrees hold the
+// n-1-last(X) leaves right of X. Hence
+//     pre(X) = 2 (n - 1 - last(X)) + R(X),
+// R(X) = number of ancestors whose right subtree holds X (right steps on the
+// path to the root). Descending a right child adds one, so the nodes on the
+// "right chain" above X (reached from X by right steps only) sit at pre(X) - 1,
+// pre(X) - 2, ... Every internal node is on the right chain of exactly one leaf
+// (its last leaf) and -- if it is above the frontier -- of exactly one frontier
+// node. Subtree size = 2 (last - first + 1) - 1 and escape = pre + size.
+//
+// top_layout: one thread per frontier node F walks to the root (only nodes above
+// the frontier), stores R(F) (| depth << 16) and places F's right chain.
+__device__ __forceinline__ void put_node(lux_bvh_node* nodes, int pos, int32_t prim, int total, int size) {
+    int32_t* rec = reinterpret_cast<int32_t*>(nodes) + size_t(pos) * 8;
+    int esc = pos + size;
+    rec[6] = prim;
+    rec[7] = esc < total ? esc : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_top_layout(int n, const uint32_t* __restrict__ frontier,
+                                                       const uint32_t* __restrict__ frontier_count,
+                                                       const uint32_t* __restrict__ parent,
+                                                       const uint32_t* __restrict__ leaf_parent,
+                                                       const uint32_t* __restrict__ first_of,
+                                                       const uint32_t* __restrict__ last_of,
+                                                       uint32_t* __restrict__ rtop_int,
+                                                       uint32_t* __restrict__ rtop_leaf,
+                                                       lux_bvh_node* __restrict__ nodes,
+                                                       int32_t* __restrict__ pre_internal, DevInfo* info) {
+    int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= int(*frontier_count)) return;
+    const int total = 2 * n - 1;
+    uint32_t F = frontier[f];
+    bool is_leaf = F & kLeafBit;
+    uint32_t idx = F & kIndexMask;
+    int last_f = is_leaf ? int(idx) : int(last_of[idx]);
+    uint32_t pinfo = is_leaf ? leaf_parent[idx] : parent[idx];
+    uint32_t all_right = pinfo >> 31;
+    int rights = int(all_right);
+    int chain = 0;
+    uint32_t depth = 1;
+    uint32_t p = pinfo & kIndexMask;
+    for (int guard = 0;; ++guard) {
+        chain += int(all_right);
+        if (p == 0) break;
+        if (guard > n) {
+            set_error(info, kErrClimbLimit);
+            return;
+        }
+        uint32_t pp = parent[p];
+        uint32_t r = pp >> 31;
+        rights += int(r);
+        all_right &= r;
+        p = pp & kIndexMask;
+        ++depth;
+    }
+    uint32_t packed = uint32_t(rights) | (depth << 16);
+    if (is_leaf)
+        rtop_leaf[idx] = packed;
+    else
+        rtop_int[idx] = packed;
+    int pos = 2 * (n - 1 - last_f) + rights;
+    if (pos - chain < 0 || pos >= total) {
+        set_error(info, kErrClimbLimit);
+        return;
+    }
+    p = pinfo & kIndexMask;
+    for (int c = 0; c < chain; ++c) {
+        --pos;
+        put_node(nodes, pos, -1, total, 2 * (last_f - int(first_of[p]) + 1) - 1);
+        pre_internal[p] = pos;
+        if (c + 1 < chain) p = parent[p] & kIndexMask;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6. local pass: one workgroup per leaf block
+// ---------------------------------------------------------------------------
+// kMode 0: build (layout, then boxes from the handle's input copy)
+// kMode 1: refit from boxes        kMode 2: refit from triangles
+//
+// Bounds: every local internal node gets exactly two arrivals from threads of
+// this workgroup. The arrival that reads 0 from the LDS counter leaves its box in
+// the node's LDS slot and stops; the other unions both slots, writes the node and
+// climbs on. Climbing stops at the frontier; the frontier box is in the node
+// array for top_climb.
+template <int kMode>
+__global__ __launch_bounds__(kLocalThreads) void k_local(
+    int n, const uint32_t* __restrict__ sorted_input, const int32_t* __restrict__ ids,
+    const lux_aabb* __restrict__ src_boxes, const lux_triangle* __restrict__ src_tris,
+    lux_aabb* own_boxes, int write_back, const uint32_t* __restrict__ parent,
+    const uint32_t* __restrict__ leaf_parent, const uint32_t* __restrict__ first_of,
+    const uint32_t* __restrict__ last_of, const uint32_t* __restrict__ rtop_int,
+    const uint32_t* __restrict__ rtop_leaf, lux_bvh_node* nodes, int32_t* pre_internal,
+    int32_t* pre_leaf, DevInfo* info) {
+    __shared__ uint32_t s_arrive[kLocalLeaves];
+    __shared__ lux_aabb s_slot[kLocalLeaves][2];
+    const int lo = blockIdx.x * kLocalLeaves;
+    const int hi = min(lo + kLocalLeaves, n) - 1;
+    const int total = 2 * n - 1;
+    for (int t = threadIdx.x; t < kLocalLeaves; t += kLocalThreads) s_arrive[t] = 0;
+    auto is_local = [&](uint32_t p) { return int(first_of[p]) >= lo && int(last_of[p]) <= hi; };
+
+    if (kMode == 0) {
+        uint32_t max_depth = 0;
+        for (int j = lo + int(threadIdx.x); j <= hi; j += kLocalThreads) {
+            uint32_t lp = leaf_parent[j];
+            uint32_t side = lp >> 31;
+            uint32_t p = lp & kIndexMask;
+            int rights = 0, chain = 0;
+            uint32_t all_right = 1, steps = 0;
+            bool reached_root = false, f_is_leaf = true;
+            uint32_t f = uint32_t(j);
+            bool bad = false;
+            for (int guard = 0;; ++guard) {
+                if (!is_local(p)) break;
+                if (guard > kLocalLeaves) {
+                    bad = true;
+                    break;
+                }
+                rights += int(side);
+                all_right &= side;
+                chain += int(all_right);
+                ++steps;
+                f_is_leaf = false;
+                f = p;
+                if (p == 0) {
+                    reached_root = true;
+                    break;
+                }
+                uint32_t pp = parent[p];
+                side = pp >> 31;
+                p = pp & kIndexMask;
+            }
+            uint32_t top = reached_root ? 0u : (f_is_leaf ? rtop_leaf[f] : rtop_int[f]);
+            int pos = 2 * (n - 1 - j) + rights + int(top & 0xffffu);
+            max_depth = max(max_depth, steps + (top >> 16));
+            if (bad || pos - chain < 0 || pos >= total) {  // no early return: a barrier follows
+                set_error(info, kErrClimbLimit);
+                continue;
+            }
+            put_node(nodes, pos, ids[sorted_input[j]], total, 1);
+            pre_leaf[j] = pos;
+            uint32_t c_node = lp & kIndexMask;
+            for (int c = 0; c < chain; ++c) {
+                --pos;
+                put_node(nodes, pos, -1, total, 2 * (j - int(first_of[c_node]) + 1) - 1);
+                pre_internal[c_node] = pos;
+                if (c + 1 < chain) c_node = parent[c_node] & kIndexMask;
+            }
+        }
+        __hip_atomic_fetch_max(&info->max_depth, max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();  // layout of this block's nodes and the LDS clear are visible
+
+    for (int j = lo + int(threadIdx.x); j <= hi; j += kLocalThreads) {
+        uint32_t in = sorted_input[j];
+        lux_aabb box = kMode == 2 ? tri_box(src_tris[in]) : src_boxes[in];
+        if (write_back) own_boxes[in] = box;
+        int pre = pre_leaf[j];
+        if (uint32_t(pre) >= uint32_t(total)) {
+            set_error(info, kErrClimbLimit);
+            return;
+        }
+        nodes[pre].bounds = box;
+        uint32_t lp = leaf_parent[j];
+        for (int guard = 0;; ++guard) {
+            uint32_t side = lp >> 31;
+            uint32_t p = lp & kIndexMask;
+            if (!is_local(p)) break;
+            uint32_t slot = p - uint32_t(lo);
+            if (slot >= uint32_t(kLocalLeaves) || guard > kLocalLeaves) {
+                set_error(info, kErrClimbLimit);
+                return;
+            }
+            s_slot[slot][side] = box;
+            uint32_t old = __hip_atomic_fetch_add(&s_arrive[slot], 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            box = box_join(s_slot[slot][0], s_slot[slot][1]);
+            pre = pre_internal[p];
+            if (uint32_t(pre) >= uint32_t(total)) {
+                set_error(info, kErrClimbLimit);
+                return;
+            }
+            nodes[pre].bounds = box;
+            if (p == 0) {
+                info->root = box;
+                break;
+            }
+            lp = parent[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7. top climb: one workgroup unions the boxes above the frontier
+// ---------------------------------------------------------------------------
+// Same two-arrival protocol on global counters (never reset: each pass adds two
+// per node, so the parity restarts every pass); workgroup scope suffices because
+// every arrival comes from this one workgroup.
+__global__ __launch_bounds__(kTopThreads) void k_top_climb(int n, const uint32_t* __restrict__ frontier,
+                                                           const uint32_t* __restrict__ frontier_count,
+                                                           const uint32_t* __restrict__ parent,
+                                                           const uint32_t* __restrict__ leaf_parent,
+                                                           const int32_t* __restrict__ pre_internal,
+                                                           const int32_t* __restrict__ pre_leaf,
+                                                           uint32_t* arrive, lux_aabb* slots,
+                                                           lux_bvh_node* nodes, DevInfo* info) {
+    const uint32_t count = *frontier_count;
+    const int total = 2 * n - 1;
+    for (uint32_t f = threadIdx.x; f < count; f += kTopThreads) {
+        uint32_t F = frontier[f];
+        bool is_leaf = F & kLeafBit;
+        uint32_t idx = F & kIndexMask;
+        int pre = is_leaf ? pre_leaf[idx] : pre_internal[idx];
+        if (uint32_t(pre) >= uint32_t(total)) {
+            set_error(info, kErrClimbLimit);
+            continue;
+        }
+        lux_aabb box = nodes[pre].bounds;
+        uint32_t pinfo = is_leaf ? leaf_parent[idx] : parent[idx];
+        for (int guard = 0;; ++guard) {
+            if (guard > n) {
+                set_error(info, kErrClimbLimit);
+                break;
+            }
+            uint32_t side = pinfo >> 31;
+            uint32_t p = pinfo & kIndexMask;
+            slots[2 * size_t(p) + side] = box;
+            uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if ((old & 1u) == 0) break;
+            lux_aabb sib = slots[2 * size_t(p) + (side ^ 1u)];
+            box = side ? box_join(sib, box) : box_join(box, sib);
+            pre = pre_internal[p];
+            if (uint32_t(pre) >= uint32_t(total)) {
+                set_error(info, kErrClimbLimit);
+                break;
+            }
+            nodes[pre].bounds = box;
+            if (p == 0) {
+                info->root = box;
+                break;
+            }
+            pinfo = parent[p];
+        }
+    }
+}
+
+__global__ void k_single_leaf(const lux_aabb* __restrict__ src_boxes,
+                              const lux_triangle* __restrict__ src_tris, const int32_t* ids,
+                              lux_aabb* own_boxes, int write_back, lux_bvh_node* nodes,
+                              int32_t* pre_leaf, uint32_t* sorted_keys, uint32_t* sorted_input,
+                              DevInfo* info, int layout) {
+    lux_aabb b = src_tris ? tri_box(src_tris[0]) : src_boxes[0];
+    if (write_back) own_boxes[0] = b;
+    nodes[0].bounds = b;
+    if (layout) {
+        nodes[0].primitive_id = ids[0];
+        nodes[0].escape = -1;
+        pre_leaf[0] = 0;
+        sorted_keys[0] = 0;
+        sorted_input[0] = 0;
+    }
+    info->root = b;
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) throw Error(LUX_ERR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return static_cast<T*>(p);
+}
+
+int reduce_blocks(int n) { return max(1, min(kReduceBlocksMax, div_up(n, kBlock * 4))); }
+int sort_tiles(int n) { return max(1, div_up(n, kSortTile)); }
+
+}  // namespace
+
+void bvh_release(BvhDev& d) {
+    void* ptrs[] = {d.boxes,     d.ids,         d.keys[0],      d.keys[1],    d.vals[0],    d.vals[1],
+                    d.left,      d.right,       d.parent,       d.first,      d.last,       d.leaf_parent,
+                    d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
+                    d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
+                    d.rtop_int,  d.rtop_leaf};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    BvhDev empty;
+    d = empty;
+}
+
+void bvh_allocate(BvhDev& d, int n) {
+    if (n <= d.capacity && d.info) {
+        d.n = n;
+        return;
+    }
+    bvh_release(d);
+    int cap = max(n, 1);
+    d.capacity = cap;
+    d.n = n;
+    d.boxes = dalloc<lux_aabb>(cap);
+    d.ids = dalloc<int32_t>(cap);
+    for (int b = 0; b < 2; ++b) {
+        d.keys[b] = dalloc<uint32_t>(cap);
+        d.vals[b] = dalloc<uint32_t>(cap);
+    }
+    d.left = dalloc<uint32_t>(cap);
+    d.right = dalloc<uint32_t>(cap);
+    d.parent = dalloc<uint32_t>(cap);
+    d.first = dalloc<uint32_t>(cap);
+    d.last = dalloc<uint32_t>(cap);
+    d.leaf_parent = dalloc<uint32_t>(cap);
+    d.arrive = dalloc<uint32_t>(cap);
+    d.slots = dalloc<lux_aabb>(2 * size_t(cap));
+    d.pre_internal = dalloc<int32_t>(cap);
+    d.pre_leaf = dalloc<int32_t>(cap);
+    d.nodes = dalloc<lux_bvh_node>(2 * size_t(cap));
+    d.frontier = dalloc<uint32_t>(2 * size_t(cap));
+    d.rtop_int = dalloc<uint32_t>(cap);
+    d.rtop_leaf = dalloc<uint32_t>(cap);
+    d.hist = dalloc<uint32_t>(kPasses * kRadix);
+    d.status_words = sort_tiles(cap) * kRadix;
+    d.status = dalloc<uint64_t>(d.status_words);
+    d.tickets = dalloc<uint32_t>(4);
+    d.partials = dalloc<uint8_t>(sizeof(BoxIds) * kReduceBlocksMax);
+    d.info = dalloc<DevInfo>(1);
+    d.counter = dalloc<int32_t>(4);
+    // One-time clears at allocation: look-back words (epoch 0 is never used),
+    // tickets, arrival counters (even = idle), info, frontier count.
+    hip_check(hipMemset(d.status, 0, sizeof(uint64_t) * d.status_words), "hipMemset status");
+    hip_check(hipMemset(d.tickets, 0, sizeof(uint32_t) * 4), "hipMemset tickets");
+    hip_check(hipMemset(d.arrive, 0, sizeof(uint32_t) * cap), "hipMemset arrive");
+    hip_check(hipMemset(d.info, 0, sizeof(DevInfo)), "hipMemset info");
+    hip_check(hipMemset(d.counter, 0, sizeof(int32_t) * 4), "hipMemset counter");
+    d.epoch = 0;
+}
+
+void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s) {
+    bvh_allocate(d, n);
+    if (n <= 0) return;
+    int copy = (in_boxes != d.boxes || in_ids != d.ids) ? 1 : 0;
+    int rb = reduce_blocks(n);
+    uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
+    BoxIds* partials = reinterpret_cast<BoxIds*>(d.partials);
+    LUX_LAUNCH("lbvh_root_partials", k_root_partials, rb, kBlock, 0, s, in_boxes, in_ids, d.boxes, d.ids, copy,
+               n, partials, d.hist, frontier_count, d.info);
+    if (n == 1) {
+        LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, (const lux_aabb*)d.boxes,
+                   (const lux_triangle*)nullptr, (const int32_t*)d.ids, d.boxes, 0, d.nodes, d.pre_leaf,
+                   d.keys[0], d.vals[0], d.info, 1);
+        // root box from the one leaf (the reduction result is identical)
+        return;
+    }
+    int mh_blocks = max(1, min(1024, div_up(n, kBlock * 16)));
+    LUX_LAUNCH("lbvh_morton_hist", k_morton_hist, mh_blocks, kBlock, 0, s, (const lux_aabb*)d.boxes, n,
+               (const BoxIds*)partials, rb, d.info, d.keys[0], d.vals[0], d.hist);
+    int tiles = sort_tiles(n);
+    for (int p = 0; p < kPasses; ++p) {
+        uint32_t epoch = ++d.epoch;
+        int src = p & 1, dst = src ^ 1;
+        LUX_LAUNCH("lbvh_sort_pass", k_sort_pass, tiles, kBlock, 0, s, (const uint32_t*)d.keys[src],
+                   (const uint32_t*)d.vals[src], d.keys[dst], d.vals[dst], n, 8 * p,
+                   (const uint32_t*)(d.hist + p * kRadix), d.status, d.tickets, epoch, tiles, d.info);
+    }
+    LUX_LAUNCH("lbvh_karras", k_karras, div_up(n - 1, kBlock), kBlock, 0, s, (const uint32_t*)d.keys[0], n,
+               d.left, d.right, d.parent, d.leaf_parent, d.first, d.last, d.arrive, d.frontier, frontier_count);
+    LUX_LAUNCH("lbvh_top_layout", k_top_layout, div_up(2 * n, kBlock), kBlock, 0, s, n,
+               (const uint32_t*)d.frontier, (const uint32_t*)frontier_count, (const uint32_t*)d.parent,
+               (const uint32_t*)d.leaf_parent, (const uint32_t*)d.first, (const uint32_t*)d.last, d.rtop_int,
+               d.rtop_leaf, d.nodes, d.pre_internal, d.info);
+    LUX_LAUNCH("lbvh_local", k_local<0>, div_up(n, kLocalLeaves), kLocalThreads, 0, s, n,
+               (const uint32_t*)d.vals[0], (const int32_t*)d.ids, (const lux_aabb*)d.boxes,
+               (const lux_triangle*)nullptr, d.boxes, 0, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
+               (const uint32_t*)d.first, (const uint32_t*)d.last, (const uint32_t*)d.rtop_int,
+               (const uint32_t*)d.rtop_leaf, d.nodes, d.pre_internal, d.pre_leaf, d.info);
+    LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
+               (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
+               (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+}
+
+namespace {
+template <int kMode>
+void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris, hipStream_t s) {
+    int n = d.n;
+    int wb = (kMode == 2 || in_boxes != d.boxes) ? 1 : 0;
+    uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
+    if (n == 1) {
+        LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, in_boxes, tris, (const int32_t*)d.ids, d.boxes,
+                   wb, d.nodes, d.pre_leaf, d.keys[0], d.vals[0], d.info, 0);
+        return;
+    }
+    LUX_LAUNCH(kMode == 2 ? "lbvh_local_tris" : "lbvh_local_boxes", k_local<kMode>, div_up(n, kLocalLeaves),
+               kLocalThreads, 0, s, n, (const uint32_t*)d.vals[0], (const int32_t*)d.ids, in_boxes, tris, d.boxes,
+               wb, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent, (const uint32_t*)d.first,
+               (const uint32_t*)d.last, (const uint32_t*)d.rtop_int, (const uint32_t*)d.rtop_leaf, d.nodes,
+               d.pre_internal, d.pre_leaf, d.info);
+    LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
+               (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
+               (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+}
+}  // namespace
+
+void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s) {
+    if (d.n <= 0) return;
+    refit_launch<1>(d, in_boxes, nullptr, s);
+}
+
+void lbvh_refit_triangles(BvhDev& d, const lux_triangle* tris, hipStream_t s) {
+    if (d.n <= 0) return;
+    refit_launch<2>(d, nullptr, tris, s);
+}
+
+}  // namespace lux

@@ -1,0 +1,115 @@
+// lux_runtime.hpp -- host-side runtime pieces shared by the kernel translation
+// units and the C ABI: error state, checked launches with the event probe, and
+// the device-buffer set that one LBVH owns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lux_common.hpp"
+
+namespace lux {
+
+// Thrown inside the library, converted to a status code at the ABI boundary.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+// "HIP error after <label>: <msg>" mirrors check_cuda_or_throw
+// (src/core/cuda_check.cuh:7-27).
+inline void hip_check(hipError_t e, const char* label) {
+    if (e != hipSuccess)
+        throw Error(LUX_ERR_HIP, std::string("HIP error after ") + label + ": " + hipGetErrorString(e));
+}
+
+// ---- kernel-time probe ------------------------------------------------------
+void probe_pre_launch(const char* name, hipStream_t s);
+void probe_post_launch(const char* name, hipStream_t s);
+
+hipStream_t library_stream();
+
+#define LUX_LAUNCH(NAME, KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)                        \
+    do {                                                                                  \
+        ::lux::probe_pre_launch(NAME, STREAM);                                            \
+        hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), SHMEM, STREAM, __VA_ARGS__);  \
+        ::lux::hip_check(hipGetLastError(), NAME);                                        \
+        ::lux::probe_post_launch(NAME, STREAM);                                           \
+    } while (0)
+
+inline int div_up(int64_t a, int64_t b) { return int((a + b - 1) / b); }
+
+// ---- LBVH device state ------------------------------------------------------
+// Layout in HBM (n = primitive count, all arrays sized to `capacity`):
+//   boxes[n], ids[n]              copy of the build input (input order)
+//   keys[2][n], vals[2][n]        radix-sort ping-pong; after the build keys[0]
+//                                 holds the sorted Morton codes, vals[0] the
+//                                 sorted-position -> input-index permutation
+//   left/right/parent[n-1]        Karras radix tree (bit 31 flags as in the ABI)
+//   first/last[n-1]               leaf range of each internal node
+//   leaf_parent[n]
+//   frontier[<=2n], counter[1]    nodes whose range fits one leaf block but
+//                                 whose parent's does not
+//   rtop_int[n-1], rtop_leaf[n]   right steps (| depth << 16) from a frontier
+//                                 node to the root
+//   arrive[n-1]                   arrival counters above the frontier (parity)
+//   slots[2(n-1)]                 child boxes handed from first to second arriver
+//   pre_internal[n-1], pre_leaf[n] node -> position in the stackless array
+//   nodes[2n-1]                   32-byte stackless nodes, right-first preorder
+//   sort scratch: hist[4*256], status[tiles*256] (epoch-tagged look-back words),
+//                 tickets[1], partials for the root reduction
+struct BvhDev {
+    int n = 0;
+    int capacity = 0;
+    lux_aabb* boxes = nullptr;
+    int32_t* ids = nullptr;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* left = nullptr;
+    uint32_t* right = nullptr;
+    uint32_t* parent = nullptr;
+    uint32_t* first = nullptr;
+    uint32_t* last = nullptr;
+    uint32_t* leaf_parent = nullptr;
+    uint32_t* frontier = nullptr;
+    uint32_t* rtop_int = nullptr;
+    uint32_t* rtop_leaf = nullptr;
+    uint32_t* arrive = nullptr;
+    lux_aabb* slots = nullptr;
+    int32_t* pre_internal = nullptr;
+    int32_t* pre_leaf = nullptr;
+    lux_bvh_node* nodes = nullptr;
+    uint32_t* hist = nullptr;
+    uint64_t* status = nullptr;
+    uint32_t* tickets = nullptr;
+    uint8_t* partials = nullptr;
+    DevInfo* info = nullptr;
+    // counter[0]: overlap pair counter of host-array queries; counter[1]: frontier size
+    int32_t* counter = nullptr;
+    uint32_t epoch = 0;
+    int status_words = 0;
+};
+
+void bvh_allocate(BvhDev& d, int n);
+void bvh_release(BvhDev& d);
+
+// Enqueue a full build of n primitives; in_boxes/in_ids are device arrays
+// (may alias d.boxes/d.ids).
+void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s);
+// Enqueue a refit with new boxes (input order) or triangles (input order).
+void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);
+void lbvh_refit_triangles(BvhDev& d, const lux_triangle* tris, hipStream_t s);
+
+// Queries (all enqueue only).
+void trace_closest(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+                   int n, lux_hit* hits, hipStream_t s);
+void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+               uint8_t* occluded, hipStream_t s);
+void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
+                   lux_pair* pairs, int32_t* counter, hipStream_t s);
+
+}  // namespace lux

@@ -1,0 +1,216 @@
+// query.hip -- closest-hit, any-hit and AABB-overlap queries over the
+// right-first preorder stackless LBVH.
+//
+// Walk rule (src/bvh/bvh_device.cuh:59-87): test node i; on a miss or after a
+// leaf jump to nodes[i].escape, on an internal hit step to i+1. With the nodes in
+// right-first preorder this visits nodes in the pop order of the host stack
+// traversal (src/bvh/cpu_bvh_traversal.h:37-78), so the shrinking t_max and the
+// winner of exact-t ties are those of the host LBVH.
+//
+// Each walk also checks that it moves forward (escape and i+1 are always greater
+// than i in preorder), so a damaged node array ends the walk instead of looping.
+#include "lux_runtime.hpp"
+
+namespace lux {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct RayIn {
+    f3 o, d;
+    float t_min, t_max;
+};
+
+__device__ __forceinline__ RayIn load_ray(const lux_ray* rays, int i) {
+    const float* p = reinterpret_cast<const float*>(rays + i);
+    RayIn r;
+    r.o = make3(p[0], p[1], p[2]);
+    r.d = make3(p[3], p[4], p[5]);
+    r.t_min = p[6];
+    r.t_max = p[7];
+    return r;
+}
+
+__device__ __forceinline__ void load_tri(const lux_triangle* tris, int i, f3& v0, f3& v1, f3& v2) {
+    const float* p = reinterpret_cast<const float*>(tris + i);
+    v0 = make3(p[0], p[1], p[2]);
+    v1 = make3(p[3], p[4], p[5]);
+    v2 = make3(p[6], p[7], p[8]);
+}
+
+// kAny = false: closest hit; kAny = true: stop at the first accepted triangle.
+template <bool kAny>
+__device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                    const lux_triangle* __restrict__ tris, int tri_count,
+                                    const RayIn& ray, float& t_best, DevInfo* info) {
+    SlabRay sr = make_slab_ray(ray.o, ray.d, ray.t_min);
+    float t_max = ray.t_max;
+    int best = -1;
+    int k = 0;
+    while (uint32_t(k) < uint32_t(node_count)) {
+        float4 a, b;
+        load_node(nodes, k, a, b);
+        int prim = __float_as_int(b.z);
+        int esc = __float_as_int(b.w);
+        int next;
+        if (!slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)) {
+            next = esc;
+        } else if (prim >= 0) {
+            next = esc;
+            if (uint32_t(prim) >= uint32_t(tri_count)) {
+                set_error(info, kErrPrimRange);
+            } else {
+                f3 v0, v1, v2;
+                load_tri(tris, prim, v0, v1, v2);
+                float t, u, v;
+                if (tri_hit(ray.o, ray.d, ray.t_min, t_max, v0, v1, v2, t, u, v)) {
+                    best = prim;
+                    t_max = t;
+                    if (kAny) break;
+                }
+            }
+        } else {
+            next = k + 1;
+        }
+        if (next >= 0 && next <= k) {
+            set_error(info, kErrWalkOrder);
+            break;
+        }
+        k = next;
+    }
+    t_best = t_max;
+    return best;
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __restrict__ nodes,
+                                                          int node_count,
+                                                          const lux_triangle* __restrict__ tris,
+                                                          int tri_count,
+                                                          const lux_ray* __restrict__ rays, int n,
+                                                          lux_hit* __restrict__ hits, DevInfo* info) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    RayIn r = load_ray(rays, i);
+    float t;
+    int id = walk<false>(nodes, node_count, tris, tri_count, r, t, info);
+    lux_hit h;
+    h.primitive_id = id;
+    h.t = id >= 0 ? t : INFINITY;
+    hits[i] = h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __restrict__ nodes,
+                                                      int node_count,
+                                                      const lux_triangle* __restrict__ tris,
+                                                      int tri_count, const lux_ray* __restrict__ rays,
+                                                      int n, uint8_t* __restrict__ occluded,
+                                                      DevInfo* info) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    RayIn r = load_ray(rays, i);
+    float t;
+    int id = walk<true>(nodes, node_count, tris, tri_count, r, t, info);
+    occluded[i] = id >= 0 ? 1 : 0;
+}
+
+__global__ void k_zero_counter(int32_t* counter) { *counter = 0; }
+
+// Overlap: one query box per lane walks the tree; every loop step is one node
+// visit per live lane, and pairs found in that step are appended with a single
+// wave-aggregated atomic. Queries run in the query tree's Morton order for
+// coherent walks. The counter keeps counting past max_pairs so the caller gets
+// the exact total (src/bvh/bvh_overlap.cu:209-214).
+__global__ __launch_bounds__(kBlock) void k_overlap(const lux_bvh_node* __restrict__ nodes,
+                                                    int node_count,
+                                                    const lux_aabb* __restrict__ qboxes,
+                                                    const int32_t* __restrict__ qids,
+                                                    const uint32_t* __restrict__ qorder, int nq,
+                                                    int self_mode, lux_pair* __restrict__ pairs,
+                                                    int max_pairs, int32_t* counter, DevInfo* info) {
+    int q = blockIdx.x * kBlock + threadIdx.x;
+    bool active = q < nq;
+    lux_aabb qb = box_empty();
+    int qid = -1;
+    if (active) {
+        uint32_t src = qorder[q];
+        qb = qboxes[src];
+        qid = qids[src];
+    }
+    int k = active ? 0 : -1;
+    const uint32_t lane = lane_id();
+    for (;;) {
+        bool live = uint32_t(k) < uint32_t(node_count);
+        if (ballot64(live) == 0) break;
+        bool emit = false;
+        int other = -1;
+        if (live) {
+            float4 a, b;
+            load_node(nodes, k, a, b);
+            int prim = __float_as_int(b.z);
+            int esc = __float_as_int(b.w);
+            int next;
+            if (!box_overlap(a.x, a.y, a.z, a.w, b.x, b.y, qb)) {
+                next = esc;
+            } else if (prim >= 0) {
+                emit = self_mode ? prim > qid : true;
+                other = prim;
+                next = esc;
+            } else {
+                next = k + 1;
+            }
+            if (next >= 0 && next <= k) {
+                set_error(info, kErrWalkOrder);
+                next = -1;
+            }
+            k = next;
+        }
+        uint64_t m = ballot64(emit);
+        if (m) {
+            uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+            int base = 0;
+            if (lane == leader)
+                base = __hip_atomic_fetch_add(counter, int(__popcll(m)), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            base = __shfl(base, int(leader), 64);
+            if (emit) {
+                int idx = base + int(__popcll(m & lanemask_lt()));
+                if (idx < max_pairs) {
+                    lux_pair pr;
+                    pr.a = self_mode ? qid : other;
+                    pr.b = self_mode ? other : qid;
+                    pairs[idx] = pr;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void trace_closest(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+                   int n, lux_hit* hits, hipStream_t s) {
+    if (n <= 0) return;
+    int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    LUX_LAUNCH("trace_closest", k_trace_closest, div_up(n, kBlock), kBlock, 0, s,
+               (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, hits, d.info);
+}
+
+void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+               uint8_t* occluded, hipStream_t s) {
+    if (n <= 0) return;
+    int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
+               node_count, tris, tri_count, rays, n, occluded, d.info);
+}
+
+void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
+                   lux_pair* pairs, int32_t* counter, hipStream_t s) {
+    LUX_LAUNCH("overlap_zero", k_zero_counter, 1, 1, 0, s, counter);
+    if (tree.n <= 0 || queries.n <= 0 || max_pairs <= 0) return;
+    LUX_LAUNCH(self_mode ? "overlap_self" : "overlap_cross", k_overlap, div_up(queries.n, kBlock), kBlock,
+               0, s, (const lux_bvh_node*)tree.nodes, 2 * tree.n - 1, (const lux_aabb*)queries.boxes,
+               (const int32_t*)queries.ids, (const uint32_t*)queries.vals[0], queries.n,
+               self_mode ? 1 : 0, pairs, max_pairs, counter, tree.info);
+}
+
+}  // namespace lux

@@ -1,0 +1,267 @@
+"""CPU oracle for the LUX hot path -- TEST INFRASTRUCTURE ONLY.
+
+``liblux_oracle.so`` (built from oracle/lbvh_oracle.cpp) restates the reference's
+host LBVH, host stack traversal, brute-force intersection and overlap rules.
+``oracle/_ref/libluxref.so`` (built by oracle/ref.mk from the reference's own
+sources when they are present) is the reference itself behind a small shim.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+package, always as the checker; the product path (lux_amd) never imports it.
+"""
+
+import ctypes
+from ctypes import POINTER, c_float, c_int32, c_int64, c_uint32, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+ORACLE_LIB = ORACLE_DIR / "_build" / "liblux_oracle.so"
+REF_LIB = ORACLE_DIR / "_ref" / "libluxref.so"
+
+_lib = None
+_ref = None
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return c_void_p(a.ctypes.data)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            raise ImportError(f"{ORACLE_LIB} missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(str(ORACLE_LIB))
+        L.orc_bvh_build.restype = c_void_p
+        L.orc_bvh_build.argtypes = [c_void_p, c_void_p, c_int32]
+        L.orc_bvh_free.argtypes = [c_void_p]
+        L.orc_bvh_refit.argtypes = [c_void_p, c_void_p]
+        L.orc_bvh_count.restype = c_int32
+        L.orc_bvh_count.argtypes = [c_void_p]
+        L.orc_bvh_topology.argtypes = [c_void_p] + [c_void_p] * 10
+        L.orc_morton_codes.argtypes = [c_void_p, c_int32, c_void_p, c_void_p]
+        L.orc_trace_closest.restype = c_int32
+        L.orc_trace_closest.argtypes = [c_void_p, c_void_p, c_void_p, c_int32] + [c_void_p] * 5
+        L.orc_trace_stackless.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
+        L.orc_brute_closest.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]
+        L.orc_tri_t_fp64.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]
+        L.orc_overlap_self_brute.restype = c_int64
+        L.orc_overlap_self_brute.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_int64]
+        L.orc_overlap_cross_brute.restype = c_int64
+        L.orc_overlap_cross_brute.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32,
+                                              c_void_p, c_int64]
+        L.orc_overlap_tree.restype = c_int64
+        L.orc_overlap_tree.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_int64]
+        L.orc_triangle_bounds.argtypes = [c_void_p, c_int32, c_void_p]
+        _lib = L
+    return _lib
+
+
+NODE_DTYPE = np.dtype([("p_min", "<f4", 3), ("p_max", "<f4", 3), ("primitive_id", "<i4"), ("escape", "<i4")])
+
+
+class OracleBvh:
+    """Host LBVH (build_cpu_lbvh_tree restated) + right-first stackless layout."""
+
+    def __init__(self, boxes, ids=None):
+        self.boxes = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        n = len(self.boxes)
+        self.ids = np.arange(n, dtype=np.int32) if ids is None else np.ascontiguousarray(ids, np.int32)
+        self.n = n
+        self.h = lib().orc_bvh_build(_p(self.boxes), _p(self.ids), n)
+        m = max(n - 1, 0)
+        self.morton_sorted = np.zeros(n, np.uint32)
+        self.perm = np.zeros(n, np.uint32)
+        self.leaf_ids = np.zeros(n, np.int32)
+        self.left = np.zeros(m, np.uint32)
+        self.right = np.zeros(m, np.uint32)
+        self.parent = np.zeros(m, np.uint32)
+        self.leaf_parent = np.zeros(n, np.uint32)
+        self.node_bounds = np.zeros((max(2 * n - 1, 0), 6), np.float32)
+        self.root = np.zeros(6, np.float32)
+        self.nodes = np.zeros(max(2 * n - 1, 0), NODE_DTYPE)
+        if n:
+            lib().orc_bvh_topology(self.h, _p(self.morton_sorted), _p(self.perm), _p(self.leaf_ids),
+                                   _p(self.left), _p(self.right), _p(self.parent), _p(self.leaf_parent),
+                                   _p(self.node_bounds), _p(self.root), _p(self.nodes))
+
+    def refit(self, boxes):
+        """Same topology, new input boxes (refit_gpu_lbvh restated)."""
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        assert len(b) == self.n
+        self.boxes = b
+        lib().orc_bvh_refit(self.h, _p(b))
+        if self.n:
+            lib().orc_bvh_topology(self.h, None, None, None, None, None, None, None, _p(self.node_bounds),
+                                   _p(self.root), _p(self.nodes))
+        return self
+
+    def __del__(self):
+        try:
+            lib().orc_bvh_free(self.h)
+        except Exception:
+            pass
+
+    def host_children(self):
+        """left/right as host node indices (internal i, leaf j -> n-1+j)."""
+        n = self.n
+
+        def conv(c):
+            c = c.astype(np.int64)
+            leaf = (c & 0x80000000) != 0
+            return np.where(leaf, (c & 0x7FFFFFFF) + n - 1, c).astype(np.int32)
+
+        return conv(self.left), conv(self.right)
+
+    def trace_closest(self, tris, rays, with_stats=False):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        k = len(r)
+        ids = np.zeros(k, np.int32)
+        th = np.zeros(k, np.float32)
+        u = np.zeros(k, np.float32)
+        v = np.zeros(k, np.float32)
+        stats = np.zeros(2, np.int64)
+        ov = lib().orc_trace_closest(self.h, _p(t), _p(r), k, _p(ids), _p(th), _p(u), _p(v), _p(stats))
+        out = {"ids": ids, "t": th, "u": u, "v": v, "overflows": int(ov)}
+        if with_stats:
+            out["nodes_visited"] = int(stats[0])
+            out["tri_tests"] = int(stats[1])
+        return out
+
+    def trace_stackless(self, tris, rays):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        ids = np.zeros(len(r), np.int32)
+        th = np.zeros(len(r), np.float32)
+        lib().orc_trace_stackless(self.h, _p(t), _p(r), len(r), _p(ids), _p(th))
+        return ids, th
+
+    def overlap_tree(self, qboxes, qids, self_mode, cap=1 << 22):
+        q = np.ascontiguousarray(qboxes, np.float32).reshape(-1, 6)
+        qi = np.ascontiguousarray(qids, np.int32)
+        pairs = np.zeros((cap, 2), np.int32)
+        count = lib().orc_overlap_tree(self.h, _p(q), _p(qi), len(q), int(self_mode), _p(pairs), cap)
+        return pairs[: min(count, cap)], int(count)
+
+
+def morton_codes(boxes, root):
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    r = np.ascontiguousarray(root, np.float32)
+    out = np.zeros(len(b), np.uint32)
+    lib().orc_morton_codes(_p(b), len(b), _p(r), _p(out))
+    return out
+
+
+def brute_closest(tris, rays):
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+    ids = np.zeros(len(r), np.int32)
+    th = np.zeros(len(r), np.float32)
+    lib().orc_brute_closest(_p(t), len(t), _p(r), len(r), _p(ids), _p(th))
+    return ids, th
+
+
+def tri_t_fp64(tris, tri_ids, rays):
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+    ti = np.ascontiguousarray(tri_ids, np.int32)
+    out = np.zeros(len(r), np.float64)
+    lib().orc_tri_t_fp64(_p(t), _p(ti), _p(r), len(r), _p(out))
+    return out
+
+
+def overlap_self_brute(boxes, ids=None, cap=1 << 22):
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    i = np.arange(len(b), dtype=np.int32) if ids is None else np.ascontiguousarray(ids, np.int32)
+    pairs = np.zeros((cap, 2), np.int32)
+    count = lib().orc_overlap_self_brute(_p(b), _p(i), len(b), _p(pairs), cap)
+    return pairs[: min(count, cap)], int(count)
+
+
+def overlap_cross_brute(lboxes, lids, rboxes, rids, cap=1 << 22):
+    lb = np.ascontiguousarray(lboxes, np.float32).reshape(-1, 6)
+    rb = np.ascontiguousarray(rboxes, np.float32).reshape(-1, 6)
+    li = np.ascontiguousarray(lids, np.int32)
+    ri = np.ascontiguousarray(rids, np.int32)
+    pairs = np.zeros((cap, 2), np.int32)
+    count = lib().orc_overlap_cross_brute(_p(lb), _p(li), len(lb), _p(rb), _p(ri), len(rb), _p(pairs), cap)
+    return pairs[: min(count, cap)], int(count)
+
+
+def triangle_bounds(tris):
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    out = np.zeros((len(t), 6), np.float32)
+    lib().orc_triangle_bounds(_p(t), len(t), _p(out))
+    return out
+
+
+# ---- the reference's own host code (oracle/_ref) -------------------------------
+def ref_available():
+    return REF_LIB.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not REF_LIB.exists():
+            raise ImportError(f"{REF_LIB} missing (built only where the reference sources exist)")
+        L = ctypes.CDLL(str(REF_LIB))
+        L.ref_lbvh_build.restype = c_void_p
+        L.ref_lbvh_build.argtypes = [c_void_p, c_void_p, c_int32]
+        L.ref_lbvh_free.argtypes = [c_void_p]
+        L.ref_lbvh_node_count.restype = c_int32
+        L.ref_lbvh_node_count.argtypes = [c_void_p]
+        L.ref_lbvh_export.argtypes = [c_void_p] + [c_void_p] * 6
+        L.ref_morton_codes.argtypes = [c_void_p, c_int32, c_void_p, c_void_p]
+        L.ref_trace_closest.restype = c_int32
+        L.ref_trace_closest.argtypes = [c_void_p, c_void_p, c_void_p, c_int32] + [c_void_p] * 4
+        L.ref_brute_closest.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]
+        L.ref_triangle_bounds.argtypes = [c_void_p, c_int32, c_void_p]
+        L.ref_overlap_self_brute.restype = c_int64
+        L.ref_overlap_self_brute.argtypes = [c_void_p, c_int32, c_void_p, c_int64]
+        L.ref_overlap_cross_brute.restype = c_int64
+        L.ref_overlap_cross_brute.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int64]
+        _ref = L
+    return _ref
+
+
+class RefLbvh:
+    """build_cpu_lbvh_tree + intersect_cpu_bvh_tree of the reference itself."""
+
+    def __init__(self, boxes, ids=None):
+        self.boxes = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        n = len(self.boxes)
+        self.ids = np.arange(n, dtype=np.int32) if ids is None else np.ascontiguousarray(ids, np.int32)
+        self.h = ref().ref_lbvh_build(_p(self.boxes), _p(self.ids), n)
+        m = ref().ref_lbvh_node_count(self.h)
+        self.bounds = np.zeros((m, 6), np.float32)
+        self.left = np.zeros(m, np.int32)
+        self.right = np.zeros(m, np.int32)
+        self.first = np.zeros(m, np.int32)
+        self.count = np.zeros(m, np.int32)
+        self.leaf_ids = np.zeros(n, np.int32)
+        if m:
+            ref().ref_lbvh_export(self.h, _p(self.bounds), _p(self.left), _p(self.right), _p(self.first),
+                                  _p(self.count), _p(self.leaf_ids))
+
+    def __del__(self):
+        try:
+            ref().ref_lbvh_free(self.h)
+        except Exception:
+            pass
+
+    def trace_closest(self, tris, rays):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        k = len(r)
+        ids = np.zeros(k, np.int32)
+        th = np.zeros(k, np.float32)
+        u = np.zeros(k, np.float32)
+        v = np.zeros(k, np.float32)
+        ov = ref().ref_trace_closest(self.h, _p(t), _p(r), k, _p(ids), _p(th), _p(u), _p(v))
+        return {"ids": ids, "t": th, "u": u, "v": v, "overflows": int(ov)}

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP LBVH build, refit, closest/any-hit traversal and overlap
+queries against the golden vectors (reference host code) and the CPU oracle.
+Integer/index results must match bit-exactly; FP32 hit distances and boxes too
+(same operation order, no contraction)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import LBVH_CASES, load_golden
+from lux_amd import abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+LEAF = 0x80000000
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def print_case(*what):
+    print("case:", *what, flush=True)
+
+
+@pytest.mark.parametrize("case", LBVH_CASES)
+def test_build_topology_bit_exact(gpu, case):
+    print_case("build", case)
+    g = load_golden(f"lbvh_{case}")
+    n = len(g["boxes"])
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    valid, count, root = bvh.info()
+    assert valid and count == n
+    assert np.array_equal(bits(root), bits(g["root"]))
+    topo = bvh.topology()
+    o = oracle.OracleBvh(g["boxes"], g["ids"])
+    if n > 1:
+        assert np.array_equal(topo["morton_sorted"], g["morton_input"][topo["sorted_indices"]])
+        assert np.array_equal(topo["sorted_indices"], o.perm)
+        assert np.array_equal(topo["left"], o.left)
+        assert np.array_equal(topo["right"], o.right)
+        assert np.array_equal(topo["parent"][1:], o.parent[1:])
+        assert np.array_equal(topo["leaf_parent"], o.leaf_parent)
+    assert np.array_equal(g["ids"][topo["sorted_indices"]], g["leaf_ids"])
+    assert np.array_equal(bits(topo["node_bounds"]), bits(g["node_bounds"]))
+    nodes = bvh.nodes()
+    assert np.array_equal(nodes.view(np.uint32), o.nodes.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", LBVH_CASES)
+def test_closest_hit_bit_exact(gpu, case):
+    print_case("closest", case)
+    g = load_golden(f"lbvh_{case}")
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, g["tri_by_id"], g["rays"])
+    assert np.array_equal(hits["primitive_id"], g["hit_ids"])
+    assert np.array_equal(bits(hits["t"]), bits(g["hit_t"]))
+    occ = gpu.trace_any(bvh, g["tri_by_id"], g["rays"])
+    assert np.array_equal(occ.astype(bool), g["hit_ids"] >= 0)
+
+
+@pytest.mark.parametrize("case", ["soup2000", "clustered1500", "sparse_ids777", "tiny3", "tiny1"])
+def test_refit_matches_oracle_refit(gpu, case):
+    print_case("refit", case)
+    g = load_golden(f"lbvh_{case}")
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    moved = W.jitter(g["tris"], sigma=2e-3, seed=99)
+    new_boxes = W.triangle_boxes(moved)
+    assert gpu.refit_gpu_lbvh(bvh, new_boxes, g["ids"])
+    o = oracle.OracleBvh(g["boxes"], g["ids"]).refit(new_boxes)
+    topo = bvh.topology()
+    assert np.array_equal(topo["sorted_indices"], o.perm)
+    assert np.array_equal(bits(topo["node_bounds"]), bits(o.node_bounds))
+    assert np.array_equal(bvh.nodes().view(np.uint32), o.nodes.view(np.uint32))
+    assert np.array_equal(bits(bvh.root_bounds), bits(o.root))
+    tri_by_id = np.zeros_like(g["tri_by_id"])
+    tri_by_id[g["ids"]] = moved
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, g["rays"])
+    exp = o.trace_closest(tri_by_id, g["rays"])
+    assert np.array_equal(hits["primitive_id"], exp["ids"])
+    assert np.array_equal(bits(hits["t"]), bits(exp["t"]))
+
+
+def test_refit_mismatch_returns_false(gpu):
+    print_case("refit-mismatch")
+    g = load_golden("lbvh_soup2000")
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    assert not gpu.refit_gpu_lbvh(bvh, g["boxes"][:-1], g["ids"][:-1])
+    bad_ids = g["ids"].copy()
+    bad_ids[[3, 4]] = bad_ids[[4, 3]]
+    assert not gpu.refit_gpu_lbvh(bvh, g["boxes"], bad_ids)
+    assert gpu.refit_gpu_lbvh(bvh, g["boxes"], g["ids"])
+
+
+def test_empty_input_gives_invalid_bvh(gpu):
+    print_case("empty")
+    bvh = gpu.build_gpu_lbvh(np.zeros((0, 6), np.float32))
+    assert bvh.info()[:2] == (False, 0)
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, np.zeros((0, 9), np.float32), W.rays(5))
+    assert np.all(hits["primitive_id"] == -1) and np.all(np.isinf(hits["t"]))
+    assert gpu.refit_gpu_lbvh(bvh, np.zeros((0, 6), np.float32))
+    res = gpu.query_aabb_overlaps_self(bvh)
+    assert res.found_count == 0 and not res.truncated and len(res.pairs) == 0
+
+
+def test_negative_ids_rejected(gpu):
+    print_case("negative-ids")
+    g = load_golden("lbvh_tiny7")
+    ids = g["ids"].copy()
+    ids[2] = -4
+    with pytest.raises(abi.LuxError) as e:
+        gpu.build_gpu_lbvh(g["boxes"], ids)
+    assert e.value.status == abi.LUX_ERR_INVALID_ARGUMENT
+
+
+def test_leaf_id_outside_triangle_array_is_reported(gpu):
+    print_case("prim-range")
+    g = load_golden("lbvh_soup2000")
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    with pytest.raises(abi.LuxError) as e:
+        gpu.intersect_gpu_bvh_stackless(bvh, g["tri_by_id"][:100], g["rays"])
+    assert e.value.status == abi.LUX_ERR_DEVICE_CHECK
+    # the handle stays usable
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, g["tri_by_id"], g["rays"])
+    assert np.array_equal(hits["primitive_id"], g["hit_ids"])
+
+
+def test_overlap_self_and_cross_sets(gpu):
+    print_case("overlap")
+    g = load_golden("overlap_dense2000")
+    left = gpu.build_gpu_lbvh(g["boxes"])
+    right = gpu.build_gpu_lbvh(g["qboxes"])
+    s = gpu.query_aabb_overlaps_self(left)
+    assert not s.truncated and s.found_count == len(g["self_pairs"])
+    assert set(map(tuple, s.pairs)) == set(map(tuple, g["self_pairs"]))
+    assert np.all(s.pairs[:, 0] < s.pairs[:, 1])
+    c = gpu.query_aabb_overlaps_cross(left, right)
+    assert c.found_count == len(g["cross_pairs"])
+    assert set(map(tuple, c.pairs)) == set(map(tuple, g["cross_pairs"]))
+    # truncation: exact total, at most max_pairs written, all of them genuine
+    t = gpu.query_aabb_overlaps_self(left, max_pairs=100)
+    assert t.truncated and t.found_count == len(g["self_pairs"]) and len(t.pairs) == 100
+    assert set(map(tuple, t.pairs)) <= set(map(tuple, g["self_pairs"]))
+    z = gpu.query_aabb_overlaps_self(left, max_pairs=0)
+    assert z.found_count == 0 and len(z.pairs) == 0
+
+
+def test_repeated_builds_reuse_handle(gpu):
+    """Builds of shrinking and growing sizes on one handle (buffer reuse, epoch-
+    tagged look-back words, ticket reset) stay bit-exact."""
+    print_case("rebuilds")
+    bvh = gpu.GpuBvh()
+    for n, seed in [(5000, 1), (1000, 2), (70000, 3), (3, 4), (70000, 5)]:
+        boxes = W.triangle_boxes(W.soup(n, seed=seed))
+        bvh.build(boxes, np.arange(n, dtype=np.int32))
+        o = oracle.OracleBvh(boxes)
+        assert np.array_equal(bvh.nodes().view(np.uint32), o.nodes.view(np.uint32)), (n, seed)
+
+
+def test_full_size_microbench_inputs(gpu):
+    """1M-triangle soup, 4M rays: the tree equals the oracle's bit for bit; a
+    65536-ray sample matches the oracle traversal; all rays satisfy any-hit ==
+    (closest found); refit from jittered vertices equals a refit from boxes."""
+    print_case("full-size 1M")
+    inp = W.c2_inputs(n_tris=1 << 20, n_rays=1 << 22, n_queries=100_000)
+    tris, boxes, rays = inp["tris"], inp["boxes"], inp["rays"]
+    bvh = gpu.build_gpu_lbvh(boxes)
+    o = oracle.OracleBvh(boxes)
+    assert np.array_equal(bvh.nodes().view(np.uint32), o.nodes.view(np.uint32))
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, tris, rays)
+    sample = np.random.default_rng(0).choice(len(rays), 1 << 16, replace=False)
+    exp = o.trace_closest(tris, rays[sample])
+    assert exp["overflows"] == 0
+    assert np.array_equal(hits["primitive_id"][sample], exp["ids"])
+    assert np.array_equal(bits(hits["t"][sample]), bits(exp["t"]))
+    occ = gpu.trace_any(bvh, tris, rays)
+    assert np.array_equal(occ.astype(bool), hits["primitive_id"] >= 0)
+    # refit from vertices (device entry point) == refit from host boxes
+    d_tris = gpu.DeviceArray.from_numpy(inp["tris_jittered"])
+    bvh.refit_triangles_device(d_tris, len(tris))
+    a = bvh.nodes()
+    bvh2 = gpu.build_gpu_lbvh(boxes)
+    assert gpu.refit_gpu_lbvh(bvh2, W.triangle_boxes(inp["tris_jittered"]))
+    assert np.array_equal(a.view(np.uint32), bvh2.nodes().view(np.uint32))
+    # the refitted tree equals the oracle refit of the same topology
+    o.refit(W.triangle_boxes(inp["tris_jittered"]))
+    assert np.array_equal(a.view(np.uint32), o.nodes.view(np.uint32))
+    # cross overlap of the (refitted) 1M tree with the 100k query boxes vs the
+    # oracle tree walk, as pair sets
+    q = gpu.build_gpu_lbvh(inp["query_boxes"])
+    c = gpu.query_aabb_overlaps_cross(bvh, q, max_pairs=1 << 22)
+    qids = np.arange(100_000, dtype=np.int32)
+    cp, cc = o.overlap_tree(inp["query_boxes"], qids, False)
+    assert c.found_count == cc and not c.truncated
+    key = lambda p: np.sort(p[:, 0].astype(np.int64) * (1 << 20) + p[:, 1])  # noqa: E731
+    assert np.array_equal(key(c.pairs), key(cp))
+    s = gpu.query_aabb_overlaps_self(bvh, max_pairs=1 << 22)
+    sp, sc = o.overlap_tree(o.boxes, np.arange(len(boxes), dtype=np.int32), True)
+    assert s.found_count == sc and np.array_equal(key(s.pairs), key(sp))

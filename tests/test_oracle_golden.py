@@ -1,0 +1,191 @@
+"""CPU: the oracle restatement against the golden vectors produced from the
+reference's own host code, plus the mathematical checks that pin each oracle
+function independently of those vectors."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import LBVH_CASES, load_golden
+from lux_amd import workloads as W
+
+LEAF = 0x80000000
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("case", LBVH_CASES)
+def test_oracle_matches_reference_vectors(case):
+    g = load_golden(f"lbvh_{case}")
+    n = len(g["boxes"])
+    assert np.array_equal(bits(oracle.triangle_bounds(g["tris"])), bits(g["boxes"]))
+    o = oracle.OracleBvh(g["boxes"], g["ids"])
+    assert np.array_equal(bits(o.root), bits(g["root"]))
+    if n > 1:
+        assert np.array_equal(oracle.morton_codes(g["boxes"], g["root"]), g["morton_input"])
+        assert np.array_equal(o.morton_sorted, g["morton_input"][o.perm])
+    assert np.array_equal(o.leaf_ids, g["leaf_ids"])
+    left, right = o.host_children()
+    assert np.array_equal(left, g["node_left"][: n - 1])
+    assert np.array_equal(right, g["node_right"][: n - 1])
+    assert np.array_equal(bits(o.node_bounds), bits(g["node_bounds"]))
+    hit = o.trace_closest(g["tri_by_id"], g["rays"])
+    assert hit["overflows"] == int(g["hit_overflows"]) == 0
+    assert np.array_equal(hit["ids"], g["hit_ids"])
+    assert np.array_equal(bits(hit["t"]), bits(g["hit_t"]))
+    found = g["hit_ids"] >= 0
+    assert np.array_equal(bits(hit["u"][found]), bits(g["hit_u"][found]))
+    assert np.array_equal(bits(hit["v"][found]), bits(g["hit_v"][found]))
+    bi, bt = oracle.brute_closest(g["tri_by_id"], g["rays"])
+    assert np.array_equal(bi, g["brute_ids"])
+    assert np.array_equal(bits(bt), bits(g["brute_t"]))
+
+
+@pytest.mark.parametrize("case", LBVH_CASES)
+def test_right_first_stackless_walk_equals_host_stack_walk(case):
+    g = load_golden(f"lbvh_{case}")
+    o = oracle.OracleBvh(g["boxes"], g["ids"])
+    ids, t = o.trace_stackless(g["tri_by_id"], g["rays"])
+    assert np.array_equal(ids, g["hit_ids"])
+    assert np.array_equal(bits(t), bits(g["hit_t"]))
+
+
+def test_closest_hit_equals_brute_force_up_to_exact_ties():
+    g = load_golden("lbvh_clustered1500")
+    o = oracle.OracleBvh(g["boxes"])
+    hit = o.trace_closest(g["tris"], g["rays"])
+    bi, bt = oracle.brute_closest(g["tris"], g["rays"])
+    assert np.array_equal(bits(hit["t"]), bits(bt))
+    differ = hit["ids"] != bi
+    assert differ.any(), "fixture must contain exact-t ties"
+    # every disagreement is an exact tie: both triangles give the same t
+    for k in np.nonzero(differ)[0]:
+        r = g["rays"][k : k + 1]
+        _, ta = oracle.brute_closest(g["tris"][hit["ids"][k] : hit["ids"][k] + 1], r)
+        _, tb = oracle.brute_closest(g["tris"][bi[k] : bi[k] + 1], r)
+        assert ta[0] == tb[0] == bt[k]
+
+
+def test_tie_winner_depends_on_visit_order():
+    """On 64 identical triangles every hit is a tie; the winner is the first leaf
+    in right-first order, not the lowest index a brute force would keep."""
+    g = load_golden("lbvh_identical64")
+    hit = g["hit_ids"][g["hit_ids"] >= 0]
+    assert len(hit) > 400 and np.all(hit == hit[0]) and hit[0] != 0
+    o = oracle.OracleBvh(g["boxes"])
+    first_leaf = o.nodes["primitive_id"][o.nodes["primitive_id"] >= 0][0]
+    assert hit[0] == first_leaf
+
+
+def spread_ref(v):
+    return sum(((v >> k) & 1) << (3 * k) for k in range(10))
+
+
+def test_morton_closed_form_and_corners():
+    rng = np.random.default_rng(0)
+    boxes = W.triangle_boxes(W.soup(300, seed=3))
+    root = np.concatenate([boxes[:, :3].min(0), boxes[:, 3:].max(0)]).astype(np.float32)
+    codes = oracle.morton_codes(boxes, root)
+    c = ((boxes[:, :3] + boxes[:, 3:]) * np.float32(0.5)).astype(np.float32)
+    off = ((c - root[:3]) / (root[3:] - root[:3])).astype(np.float32)
+    q = np.clip(off * np.float32(1024), 0, 1023).astype(np.uint32)
+    expect = np.array([(spread_ref(int(x)) << 2) | (spread_ref(int(y)) << 1) | spread_ref(int(z)) for x, y, z in q],
+                      np.uint32)
+    assert np.array_equal(codes, expect)
+    # centroid at root.min -> 0, at root.max -> all 30 bits set, degenerate axis -> 0.5 -> 512
+    unit = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    corner = np.array([[0, 0, 0, 0, 0, 0], [1, 1, 1, 1, 1, 1]], np.float32)
+    assert list(oracle.morton_codes(corner, unit)) == [0, 0x3FFFFFFF]
+    flat = np.array([0, 0, 0.5, 1, 1, 0.5], np.float32)
+    mid = np.array([[0, 0, 0.5, 0, 0, 0.5]], np.float32)
+    assert oracle.morton_codes(mid, flat)[0] == spread_ref(512)
+    del rng
+
+
+@pytest.mark.parametrize("maker", [lambda: W.soup(3000, seed=4), lambda: W.clustered_soup(2500, seed=8)])
+def test_karras_tree_invariants(maker):
+    tris = maker()
+    boxes = W.triangle_boxes(tris)
+    o = oracle.OracleBvh(boxes)
+    n = len(boxes)
+    # stable ascending permutation
+    assert sorted(o.perm.tolist()) == list(range(n))
+    keys = oracle.morton_codes(boxes, o.root)[o.perm]
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    eq = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(o.perm.astype(np.int64))[eq] > 0), "stable: equal keys keep input order"
+    # parent/child links are mutual and every node is reached once
+    for i in range(1, n - 1):
+        p = o.parent[i]
+        ch = o.right[p & 0x7FFFFFFF] if p & LEAF else o.left[p & 0x7FFFFFFF]
+        assert ch == i
+    for j in range(n):
+        p = o.leaf_parent[j]
+        ch = o.right[p & 0x7FFFFFFF] if p & LEAF else o.left[p & 0x7FFFFFFF]
+        assert ch == (LEAF | j)
+    # internal box == union of children, root == union of all inputs
+    left, right = o.host_children()
+    nb = o.node_bounds
+    u = np.concatenate([np.minimum(nb[left, :3], nb[right, :3]), np.maximum(nb[left, 3:], nb[right, 3:])], 1)
+    assert np.array_equal(bits(nb[: n - 1]), bits(u))
+    assert np.array_equal(bits(nb[0]), bits(o.root))
+    assert np.array_equal(bits(o.root), bits(np.concatenate([boxes[:, :3].min(0), boxes[:, 3:].max(0)])))
+
+
+def test_stackless_layout_escape_is_preorder_plus_size():
+    o = oracle.OracleBvh(W.triangle_boxes(W.soup(1000, seed=5)))
+    nodes = o.nodes
+    total = len(nodes)
+    # subtree sizes from the layout itself: an internal node's subtree is
+    # [k, escape) and must hold (size+1)/2 leaves
+    for k in range(total):
+        esc = nodes["escape"][k] if nodes["escape"][k] >= 0 else total
+        size = esc - k
+        leaves = int((nodes["primitive_id"][k:esc] >= 0).sum())
+        assert size == 2 * leaves - 1
+        if nodes["primitive_id"][k] >= 0:
+            assert size == 1
+    assert nodes["escape"][0] == -1
+
+
+def test_overlap_vectors():
+    g = load_golden("overlap_dense2000")
+    boxes, q = g["boxes"], g["qboxes"]
+    ids = np.arange(len(boxes), dtype=np.int32)
+    sp, sc = oracle.overlap_self_brute(boxes, ids)
+    assert sc == len(g["self_pairs"])
+    assert set(map(tuple, sp)) == set(map(tuple, g["self_pairs"]))
+    o = oracle.OracleBvh(boxes)
+    tp, tc = o.overlap_tree(boxes, ids, True)
+    assert tc == sc and set(map(tuple, tp)) == set(map(tuple, g["self_pairs"]))
+    qids = np.arange(len(q), dtype=np.int32)
+    cp, cc = o.overlap_tree(q, qids, False)
+    assert cc == len(g["cross_pairs"])
+    assert set(map(tuple, cp)) == set(map(tuple, g["cross_pairs"]))
+
+
+def test_fp64_error_statement_for_hit_distance():
+    """FP32 hit t against an FP64 Moller-Trumbore of the same triangle: the
+    relative error stays small (stated bound 1e-4 relative)."""
+    g = load_golden("lbvh_soup2000")
+    found = g["hit_ids"] >= 0
+    t64 = oracle.tri_t_fp64(g["tri_by_id"], g["hit_ids"], g["rays"])
+    rel = np.abs(g["hit_t"][found].astype(np.float64) - t64[found]) / np.maximum(np.abs(t64[found]), 1e-30)
+    assert np.nanmax(rel) < 1e-4
+
+
+def test_oracle_refit_equals_rebuild_boxes_on_same_topology():
+    tris = W.soup(1500, seed=17)
+    boxes = W.triangle_boxes(tris)
+    moved = W.triangle_boxes(W.jitter(tris, sigma=1e-3, seed=18))
+    o = oracle.OracleBvh(boxes).refit(moved)
+    n = len(boxes)
+    nb = o.node_bounds
+    assert np.array_equal(bits(nb[n - 1 :]), bits(moved[o.perm]))
+    left, right = o.host_children()
+    u = np.concatenate([np.minimum(nb[left, :3], nb[right, :3]), np.maximum(nb[left, 3:], nb[right, 3:])], 1)
+    assert np.array_equal(bits(nb[: n - 1]), bits(u))
+    k = o.nodes
+    assert np.array_equal(bits(k["p_min"][0]), bits(moved[:, :3].min(0)))
