@@ -91,6 +91,10 @@ int lux_hip_free(void* device_ptr);
 int lux_hip_memcpy_htod(void* device_dst, const void* host_src, size_t bytes);
 int lux_hip_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
 int lux_hip_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (16-byte vector loads and
+ * stores; bytes must be a multiple of 16), enqueued on `stream`. Also the
+ * reference workload for calibrating HBM byte counters. */
+int lux_hip_copy_device(void* d_dst, const void* d_src, size_t bytes, lux_stream stream);
 
 /* ---- LBVH ------------------------------------------------------------- */
 /* GpuBvh::GpuBvh / ~GpuBvh  (src/bvh/gpu_bvh.cuh:25-47). */
@@ -194,6 +198,13 @@ int lux_hip_overlap_cross_device(lux_hip_bvh* left, lux_hip_bvh* right,
  * summed kernel time and the number of launches. */
 int lux_hip_probe_begin(const char* kernel_name);
 int lux_hip_probe_end(double* total_ms, int64_t* launches);
+/* Runs the closest-hit and any-hit walks of the given device rays once more,
+ * counting work: stats = {closest node visits, closest triangle tests, any-hit
+ * node visits, any-hit triangle tests}. Synchronous; no reference counterpart.
+ * Used to state the bytes a traversal touches (32 B/node, 36 B/triangle). */
+int lux_hip_trace_stats_device(lux_hip_bvh* bvh, const lux_triangle* d_triangles,
+                               int32_t triangle_count, const lux_ray* d_rays,
+                               int32_t ray_count, uint64_t* stats);
 
 #ifdef __cplusplus
 }

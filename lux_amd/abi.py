@@ -69,6 +69,7 @@ _SIGNATURES = {
     "lux_hip_memcpy_htod": [_P, _P, c_size_t],
     "lux_hip_memcpy_dtoh": [_P, _P, c_size_t],
     "lux_hip_synchronize": [],
+    "lux_hip_copy_device": [_P, _P, c_size_t, _P],
     "lux_hip_bvh_create": [POINTER(c_void_p)],
     "lux_hip_bvh_destroy": [_P],
     "lux_hip_bvh_build": [_P, _P, _P, c_int32],
@@ -90,6 +91,7 @@ _SIGNATURES = {
     "lux_hip_overlap_cross_device": [_P, _P, c_int32, _P, _P, _P],
     "lux_hip_probe_begin": [c_char_p],
     "lux_hip_probe_end": [POINTER(c_double), POINTER(c_int64)],
+    "lux_hip_trace_stats_device": [_P, _P, c_int32, _P, c_int32, _P],
 }
 _RESTYPES = {"lux_hip_last_error": c_char_p}
 

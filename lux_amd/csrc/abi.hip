@@ -233,6 +233,18 @@ int lux_hip_synchronize(void) {
     });
 }
 
+int lux_hip_copy_device(void* dst, const void* src, size_t bytes, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        if (bytes % 16 != 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "copy size must be a multiple of 16 bytes");
+        if (bytes && (!dst || !src)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null pointer");
+        if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16 != 0)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "copy pointers must be 16-byte aligned");
+        copy_device(dst, src, bytes, pick(stream));
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_bvh_create(lux_hip_bvh** out) {
     return guarded([&] {
         require_init();
@@ -599,6 +611,27 @@ int lux_hip_overlap_self_device(lux_hip_bvh* bvh, int32_t max_pairs, lux_pair* d
 int lux_hip_overlap_cross_device(lux_hip_bvh* left, lux_hip_bvh* right, int32_t max_pairs, lux_pair* d_pairs,
                                  int32_t* d_found, lux_stream stream) {
     return guarded([&] { return overlap_device(left, right, false, max_pairs, d_pairs, d_found, stream); });
+}
+
+int lux_hip_trace_stats_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t tri_count,
+                               const lux_ray* d_rays, int32_t n, uint64_t* stats) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0 || !stats) throw Error(LUX_ERR_INVALID_ARGUMENT, "bad stats arguments");
+        if (n > 0 && !d_rays) throw Error(LUX_ERR_INVALID_ARGUMENT, "null rays");
+        for (int k = 0; k < 4; ++k) stats[k] = 0;
+        if (!bvh->valid) return int(LUX_OK);
+        sync_handle(bvh);
+        DeviceBuffer<unsigned long long> ds(4);
+        bvh->last_stream = g_stream;
+        trace_stats(bvh->dev, d_tris, tri_count, d_rays, n, ds.p, g_stream);
+        unsigned long long h[4];
+        hip_check(hipMemcpyAsync(h, ds.p, sizeof(h), hipMemcpyDeviceToHost, g_stream), "D2H stats");
+        sync_handle(bvh);
+        for (int k = 0; k < 4; ++k) stats[k] = h[k];
+        return int(LUX_OK);
+    });
 }
 
 int lux_hip_probe_begin(const char* kernel_name) {

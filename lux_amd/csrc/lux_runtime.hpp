@@ -109,6 +109,9 @@ void trace_closest(const BvhDev& d, const lux_triangle* tris, int tri_count, con
                    int n, lux_hit* hits, hipStream_t s);
 void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                uint8_t* occluded, hipStream_t s);
+void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+                 unsigned long long* d_stats, hipStream_t s);
+void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
                    lux_pair* pairs, int32_t* counter, hipStream_t s);
 

@@ -9,6 +9,8 @@
 //
 // Each walk also checks that it moves forward (escape and i+1 are always greater
 // than i in preorder), so a damaged node array ends the walk instead of looping.
+#include <algorithm>
+
 #include "lux_runtime.hpp"
 
 namespace lux {
@@ -39,10 +41,12 @@ __device__ __forceinline__ void load_tri(const lux_triangle* tris, int i, f3& v0
 }
 
 // kAny = false: closest hit; kAny = true: stop at the first accepted triangle.
+// visits/tests count node loads and triangle tests (only read by the stats kernel).
 template <bool kAny>
 __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int node_count,
                                     const lux_triangle* __restrict__ tris, int tri_count,
-                                    const RayIn& ray, float& t_best, DevInfo* info) {
+                                    const RayIn& ray, float& t_best, DevInfo* info,
+                                    uint32_t& visits, uint32_t& tests) {
     SlabRay sr = make_slab_ray(ray.o, ray.d, ray.t_min);
     float t_max = ray.t_max;
     int best = -1;
@@ -50,6 +54,7 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
     while (uint32_t(k) < uint32_t(node_count)) {
         float4 a, b;
         load_node(nodes, k, a, b);
+        ++visits;
         int prim = __float_as_int(b.z);
         int esc = __float_as_int(b.w);
         int next;
@@ -62,6 +67,7 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
             } else {
                 f3 v0, v1, v2;
                 load_tri(tris, prim, v0, v1, v2);
+                ++tests;
                 float t, u, v;
                 if (tri_hit(ray.o, ray.d, ray.t_min, t_max, v0, v1, v2, t, u, v)) {
                     best = prim;
@@ -92,7 +98,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __
     if (i >= n) return;
     RayIn r = load_ray(rays, i);
     float t;
-    int id = walk<false>(nodes, node_count, tris, tri_count, r, t, info);
+    uint32_t visits = 0, tests = 0;
+    int id = walk<false>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
     lux_hit h;
     h.primitive_id = id;
     h.t = id >= 0 ? t : INFINITY;
@@ -109,11 +116,47 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __rest
     if (i >= n) return;
     RayIn r = load_ray(rays, i);
     float t;
-    int id = walk<true>(nodes, node_count, tris, tri_count, r, t, info);
+    uint32_t visits = 0, tests = 0;
+    int id = walk<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
     occluded[i] = id >= 0 ? 1 : 0;
 }
 
+// Same walks, counting node visits and triangle tests per ray (summed per wave,
+// one atomic per wave): stats[0..1] closest-hit, stats[2..3] any-hit. Used to
+// state the bytes a traversal actually touches (32 B per node, 36 B per triangle).
+__global__ __launch_bounds__(kBlock) void k_trace_stats(const lux_bvh_node* __restrict__ nodes,
+                                                        int node_count,
+                                                        const lux_triangle* __restrict__ tris,
+                                                        int tri_count, const lux_ray* __restrict__ rays,
+                                                        int n, unsigned long long* stats, DevInfo* info) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t v0 = 0, t0 = 0, v1 = 0, t1 = 0;
+    if (i < n) {
+        RayIn r = load_ray(rays, i);
+        float t;
+        walk<false>(nodes, node_count, tris, tri_count, r, t, info, v0, t0);
+        walk<true>(nodes, node_count, tris, tri_count, r, t, info, v1, t1);
+    }
+    unsigned long long c[4] = {v0, t0, v1, t1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        unsigned long long x = c[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane_id() == 0) atomicAdd(&stats[k], x);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+
 __global__ void k_zero_counter(int32_t* counter) { *counter = 0; }
+
+__global__ __launch_bounds__(kBlock) void k_copy16(const int4* __restrict__ src, int4* __restrict__ dst,
+                                                   size_t count) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < count; i += size_t(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
 
 // Overlap: one query box per lane walks the tree; every loop step is one node
 // visit per live lane, and pairs found in that step are appended with a single
@@ -201,6 +244,21 @@ void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const l
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
     LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
                node_count, tris, tri_count, rays, n, occluded, d.info);
+}
+
+void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+                 unsigned long long* d_stats, hipStream_t s) {
+    LUX_LAUNCH("trace_stats_zero", k_zero_u64, 1, 64, 0, s, d_stats, 4);
+    if (n <= 0 || d.n <= 0) return;
+    LUX_LAUNCH("trace_stats", k_trace_stats, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
+               2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
+}
+
+void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    size_t count = bytes / 16;
+    if (count == 0) return;
+    int grid = int(std::min<size_t>((count + kBlock - 1) / kBlock, 4096));
+    LUX_LAUNCH("copy16", k_copy16, grid, kBlock, 0, s, static_cast<const int4*>(src), static_cast<int4*>(dst), count);
 }
 
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
