@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-ORACLE_SRC = REPO_ROOT / "oracle" / "lbvh_oracle.cpp"
+ORACLE_SRCS = [REPO_ROOT / "oracle" / "lbvh_oracle.cpp", REPO_ROOT / "oracle" / "pt_oracle.cpp"]
 ORACLE_LIB = REPO_ROOT / "oracle" / "_build" / "liblux_oracle.so"
 
 
@@ -42,8 +42,8 @@ def build_hip():
 
 def build_oracle():
     ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
-    cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off", str(ORACLE_SRC), "-o",
-           str(ORACLE_LIB)]
+    cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+           *map(str, ORACLE_SRCS), "-o", str(ORACLE_LIB)]
     _run(cmd)
     return ORACLE_LIB
 

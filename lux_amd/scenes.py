@@ -1,0 +1,238 @@
+"""Scene descriptions for the wavefront path tracer, as flat numpy arrays.
+
+A scene is a dict with
+  vertices      (V, 3) float32
+  indices       (T, 3) uint32
+  normals       (V, 3) float32 (all zero = flat shading)
+  uvs           (V, 2) float32
+  material_ids  (T,)   int32
+  materials     (M, 11) float32 rows {type, albedo rgb, emission rgb,
+                two_sided, roughness, metallic, eta}
+  camera        (8,)   float32 {origin xyz, look_at xyz, vertical fov deg, aspect}
+Material types follow MaterialType (src/material/material.h:6-13):
+0 Lambert, 1 ThinLambert, 2 GGX, 3 Dielectric, 4 Interface, 5 Transparent.
+No coordinate is -0.0 (every array goes through `+ 0.0`).
+"""
+
+import numpy as np
+
+LAMBERT, THIN_LAMBERT, GGX, DIELECTRIC, INTERFACE, TRANSPARENT = range(6)
+
+
+def material(kind, albedo=(0.8, 0.8, 0.8), emission=(0, 0, 0), two_sided=0, roughness=0.5, metallic=0.0,
+             eta=1.5):
+    return [kind, *albedo, *emission, two_sided, roughness, metallic, eta]
+
+
+def _pack(tris, mat_ids, materials, camera, normals=None):
+    tris = np.asarray(tris, np.float32).reshape(-1, 9)
+    t = len(tris)
+    verts = tris.reshape(-1, 3) + np.float32(0.0)
+    return {
+        "vertices": np.ascontiguousarray(verts, np.float32),
+        "indices": np.arange(3 * t, dtype=np.uint32).reshape(t, 3),
+        "normals": np.zeros_like(verts) if normals is None else normals,
+        "uvs": np.zeros((3 * t, 2), np.float32),
+        "material_ids": np.asarray(mat_ids, np.int32),
+        "materials": np.asarray(materials, np.float32),
+        "camera": np.asarray(camera, np.float32),
+    }
+
+
+def cornell(width=256, height=256):
+    """The built-in Cornell box (make_cornell_box, src/scene/cornell_box.h:31-56)
+    with its CLI camera (src/app/render_runner.cpp:85-90): 36 triangles, one
+    quad area light (15, 14, 12)."""
+    f = np.float32
+    materials = [
+        material(LAMBERT, (0.78, 0.78, 0.74)),
+        material(LAMBERT, (0.65, 0.05, 0.05)),
+        material(LAMBERT, (0.12, 0.45, 0.15)),
+        material(LAMBERT, (0, 0, 0), emission=(15.0, 14.0, 12.0)),
+    ]
+    white, red, green, light = 0, 1, 2, 3
+    tris, mats = [], []
+
+    def quad(p0, p1, p2, p3, m):
+        tris.append([*p0, *p1, *p2])
+        tris.append([*p0, *p2, *p3])
+        mats.extend([m, m])
+
+    def box(lo, hi, m):
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        p000, p001, p010, p011 = (x0, y0, z0), (x0, y0, z1), (x0, y1, z0), (x0, y1, z1)
+        p100, p101, p110, p111 = (x1, y0, z0), (x1, y0, z1), (x1, y1, z0), (x1, y1, z1)
+        quad(p000, p100, p110, p010, m)
+        quad(p101, p001, p011, p111, m)
+        quad(p001, p000, p010, p011, m)
+        quad(p100, p101, p111, p110, m)
+        quad(p010, p110, p111, p011, m)
+        quad(p001, p101, p100, p000, m)
+
+    quad((-1, 0, 2), (1, 0, 2), (1, 0, 0), (-1, 0, 0), white)
+    quad((-1, 2, 0), (1, 2, 0), (1, 2, 2), (-1, 2, 2), white)
+    quad((-1, 0, 0), (1, 0, 0), (1, 2, 0), (-1, 2, 0), white)
+    quad((-1, 0, 2), (-1, 0, 0), (-1, 2, 0), (-1, 2, 2), red)
+    quad((1, 0, 0), (1, 0, 2), (1, 2, 2), (1, 2, 0), green)
+    box((f(-0.65), 0, f(0.35)), (f(-0.15), f(0.8), f(0.9)), white)
+    box((f(0.25), 0, f(0.75)), (f(0.75), f(1.2), f(1.35)), white)
+    quad((f(-0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(1.35)),
+         (f(-0.35), f(1.99), f(1.35)), light)
+    aspect = np.float32(width) / np.float32(height)
+    return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
+
+
+# ---- seeded displaced icosphere -----------------------------------------------
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+    return v, f
+
+
+def geodesic_sphere(freq):
+    """Unit geodesic icosphere of frequency `freq`: 20 freq^2 triangles,
+    10 freq^2 + 2 shared vertices."""
+    base, faces = _icosahedron()
+    n = freq
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    keep = ii + jj <= n
+    gi, gj = ii[keep], jj[keep]  # grid points of one face
+    local = -np.ones((n + 1, n + 1), np.int64)
+    local[gi, gj] = np.arange(len(gi))
+    # triangles of one face in local grid numbering
+    up_i, up_j = np.nonzero((ii + jj) <= n - 1)
+    up = np.stack([local[up_i, up_j], local[up_i + 1, up_j], local[up_i, up_j + 1]], 1)
+    dn_i, dn_j = np.nonzero((ii + jj) <= n - 2)
+    dn = np.stack([local[dn_i + 1, dn_j], local[dn_i + 1, dn_j + 1], local[dn_i, dn_j + 1]], 1)
+    tri_local = np.concatenate([up, dn])
+    pts, tris = [], []
+    for fi, (a, b, c) in enumerate(faces):
+        A, B, C = base[a], base[b], base[c]
+        p = A + (B - A) * (gi / n)[:, None] + (C - A) * (gj / n)[:, None]
+        pts.append(p / np.linalg.norm(p, axis=1, keepdims=True))
+        tris.append(tri_local + fi * len(gi))
+    pts = np.concatenate(pts)
+    tris = np.concatenate(tris)
+    key = np.round(pts * 1e9).astype(np.int64)
+    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    return pts[first], inverse.reshape(-1)[tris]
+
+
+def _lattice_hash(ix, iy, iz, seed):
+    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
+         ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ np.uint64(seed))
+    h ^= h >> np.uint64(13)
+    h *= np.uint64(0x5BD1E995)
+    h &= np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    return (h & np.uint64(0xFFFFFF)).astype(np.float64) / float(0xFFFFFF) * 2.0 - 1.0
+
+
+def value_noise(p, seed, octaves=3, base_freq=4.0, lacunarity=2.0, gain=0.5):
+    """Trilinear value noise with smoothstep weights, summed over octaves and
+    normalised to [-1, 1]."""
+    total = np.zeros(len(p))
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for o in range(octaves):
+        q = p * freq + 1000.0
+        i0 = np.floor(q).astype(np.int64)
+        f = q - i0
+        w = f * f * (3 - 2 * f)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    v = _lattice_hash(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + o)
+                    wx = w[:, 0] if dx else 1 - w[:, 0]
+                    wy = w[:, 1] if dy else 1 - w[:, 1]
+                    wz = w[:, 2] if dz else 1 - w[:, 2]
+                    acc += v * wx * wy * wz
+        total += amp * acc
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return total / norm
+
+
+def displaced_icosphere(freq=208, seed=0x4C5559, width=1024, height=1024, palette=48, amplitude=0.05):
+    """Seeded icosphere (20 freq^2 triangles; freq 208 -> 865 280) with radial
+    displacement amplitude * noise(p), smooth vertex normals, a seeded-hash mix of
+    Lambert / GGX (roughness U(0.05, 0.5)) / dielectric (IOR 1.5) materials and
+    one 1x1 quad area light at y = 2.5 facing down, emission (15, 14, 12);
+    camera (0, 0, 3.5) -> origin, vertical fov 40 degrees."""
+    pts, tris = geodesic_sphere(freq)
+    r = 1.0 + amplitude * value_noise(pts, seed)
+    verts = pts * r[:, None]
+    v0, v1, v2 = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    fn = np.cross(v1 - v0, v2 - v0)
+    # make faces point outwards
+    flip = np.einsum("ij,ij->i", fn, v0 + v1 + v2) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    v0, v1, v2 = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    fn = np.cross(v1 - v0, v2 - v0)
+    vn = np.zeros_like(verts)
+    for k in range(3):
+        np.add.at(vn, tris[:, k], fn)
+    vn /= np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-30)
+
+    rng = np.random.default_rng(seed)
+    mats = []
+    for k in range(palette):
+        kind = k % 3
+        if kind == 0:
+            mats.append(material(LAMBERT, tuple(rng.uniform(0.2, 0.8, 3))))
+        elif kind == 1:
+            mats.append(material(GGX, tuple(rng.uniform(0.2, 0.8, 3)), roughness=rng.uniform(0.05, 0.5)))
+        else:
+            mats.append(material(DIELECTRIC, (1.0, 1.0, 1.0), roughness=0.0, eta=1.5))
+    light_mat = len(mats)
+    mats.append(material(LAMBERT, (0, 0, 0), emission=(15.0, 14.0, 12.0)))
+    h = (np.arange(len(tris), dtype=np.uint64) * np.uint64(0x9E3779B1) ^ np.uint64(seed)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x7FEB352D)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    mat_ids = (h % np.uint64(palette)).astype(np.int32)
+
+    # light quad at y = 2.5, normal (0, -1, 0)
+    lq = np.array([[-0.5, 2.5, -0.5], [0.5, 2.5, -0.5], [0.5, 2.5, 0.5], [-0.5, 2.5, 0.5]], np.float64)
+    n_v = len(verts)
+    verts = np.concatenate([verts, lq])
+    vn = np.concatenate([vn, np.zeros((4, 3))])
+    tris = np.concatenate([tris, np.array([[n_v, n_v + 1, n_v + 2], [n_v, n_v + 2, n_v + 3]])])
+    mat_ids = np.concatenate([mat_ids, [light_mat, light_mat]]).astype(np.int32)
+    aspect = np.float32(width) / np.float32(height)
+    return {
+        "vertices": np.ascontiguousarray(verts.astype(np.float32) + np.float32(0.0)),
+        "indices": np.ascontiguousarray(tris.astype(np.uint32)),
+        "normals": np.ascontiguousarray(vn.astype(np.float32) + np.float32(0.0)),
+        "uvs": np.zeros((len(verts), 2), np.float32),
+        "material_ids": mat_ids,
+        "materials": np.asarray(mats, np.float32),
+        "camera": np.asarray([0, 0, 3.5, 0, 0, 0, 40.0, aspect], np.float32),
+    }
+
+
+def icosphere_freq_for_triangles(t):
+    return max(7, int(round((t / 20.0) ** 0.5)))
+
+
+def batched_mix(count=64, seed=0x4C555A):
+    """The multi-scene mix: triangle counts log-uniform in [1e3, 5e5], image sides
+    in {128, 256, 512, 1024}, spp log-uniform in [16, 256], depth 6. Returns scene
+    specs (generate each with displaced_icosphere)."""
+    specs = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + i)
+        t = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        side = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(round(float(np.exp(rng.uniform(np.log(16), np.log(256))))))
+        specs.append({"seed": seed + i, "freq": icosphere_freq_for_triangles(t), "width": side, "height": side,
+                      "spp": spp, "depth": 6})
+    return specs

@@ -265,3 +265,152 @@ class RefLbvh:
         v = np.zeros(k, np.float32)
         ov = ref().ref_trace_closest(self.h, _p(t), _p(r), k, _p(ids), _p(th), _p(u), _p(v))
         return {"ids": ids, "t": th, "u": u, "v": v, "overflows": int(ov)}
+
+
+# ---- the reference's own CPU path tracer (oracle/_ref) --------------------------
+def _ref_pt_lib():
+    L = ref()
+    if not hasattr(L, "_pt_ready"):
+        L.ref_pt_create.restype = c_void_p
+        L.ref_pt_create.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                    c_int32, c_void_p]
+        L.ref_pt_free.argtypes = [c_void_p]
+        L.ref_pt_camera.argtypes = [c_void_p, c_void_p]
+        L.ref_pt_lights.restype = c_int32
+        L.ref_pt_lights.argtypes = [c_void_p] + [c_void_p] * 8
+        L.ref_pt_render.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
+                                    c_void_p]
+        L.ref_pt_samples.argtypes = [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
+                                     c_void_p, c_void_p, c_int32, c_void_p]
+        L._pt_ready = True
+    return L
+
+
+def _scene_args(scene):
+    v = np.ascontiguousarray(scene["vertices"], np.float32)
+    i = np.ascontiguousarray(scene["indices"], np.uint32)
+    n = np.ascontiguousarray(scene["normals"], np.float32)
+    uv = np.ascontiguousarray(scene["uvs"], np.float32)
+    m = np.ascontiguousarray(scene["material_ids"], np.int32)
+    mats = np.ascontiguousarray(scene["materials"], np.float32)
+    cam = np.ascontiguousarray(scene["camera"], np.float32)
+    return (v, i, n, uv, m, mats, cam)
+
+
+class RefPathTracer:
+    """render_cpu_path_tracer / trace_path of the reference on a scene dict."""
+
+    def __init__(self, scene):
+        L = _ref_pt_lib()
+        self._keep = _scene_args(scene)
+        v, i, n, uv, m, mats, cam = self._keep
+        self.h = L.ref_pt_create(_p(v), len(v), _p(i), _p(n), _p(uv), _p(m), len(m), _p(mats), len(mats), _p(cam))
+
+    def __del__(self):
+        try:
+            _ref_pt_lib().ref_pt_free(self.h)
+        except Exception:
+            pass
+
+    def camera(self):
+        out = np.zeros(20, np.float32)
+        _ref_pt_lib().ref_pt_camera(self.h, _p(out))
+        return out
+
+    def lights(self):
+        L = _ref_pt_lib()
+        n = L.ref_pt_lights(self.h, None, None, None, None, None, None, None, None)
+        t = len(self._keep[4])
+        out = {"prim": np.zeros(n, np.int32), "area": np.zeros(n, np.float32), "cdf": np.zeros(n, np.float32),
+               "prob": np.zeros(n, np.float32), "alias": np.zeros(n, np.int32), "pmf": np.zeros(n, np.float32),
+               "primitive_light_index": np.zeros(t, np.int32), "total_weight": np.zeros(1, np.float32)}
+        L.ref_pt_lights(self.h, _p(out["prim"]), _p(out["area"]), _p(out["cdf"]), _p(out["prob"]), _p(out["alias"]),
+                        _p(out["pmf"]), _p(out["primitive_light_index"]), _p(out["total_weight"]))
+        return out
+
+    def render(self, width, height, spp, depth, seed=0, frame=0, sample_offset=0):
+        film = np.zeros((height, width, 3), np.float32)
+        _ref_pt_lib().ref_pt_render(self.h, width, height, spp, depth, seed, frame, sample_offset, _p(film))
+        return film
+
+    def samples(self, width, height, depth, pixels, samples, seed=0, frame=0, sample_offset=0):
+        px = np.ascontiguousarray(pixels, np.int32)
+        sm = np.ascontiguousarray(samples, np.int32)
+        out = np.zeros((len(px), 3), np.float32)
+        _ref_pt_lib().ref_pt_samples(self.h, width, height, depth, seed, frame, sample_offset, _p(px), _p(sm), len(px),
+                                     _p(out))
+        return out
+
+
+# ---- path-tracer restatement (pt_oracle.cpp) ----------------------------------
+def _pt_lib():
+    L = lib()
+    if not hasattr(L, "_pt_ready"):
+        L.orc_pt_create.restype = c_void_p
+        L.orc_pt_create.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32,
+                                    c_void_p]
+        L.orc_pt_free.argtypes = [c_void_p]
+        L.orc_pt_camera.argtypes = [c_void_p, c_void_p]
+        L.orc_pt_lights.restype = c_int32
+        L.orc_pt_lights.argtypes = [c_void_p] + [c_void_p] * 8
+        L.orc_pt_render.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
+                                    c_int32, c_int32, c_int32, c_void_p]
+        L.orc_pt_samples.argtypes = [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
+                                     c_void_p, c_void_p, c_int32, c_void_p]
+        L._pt_ready = True
+    return L
+
+
+class OraclePathTracer:
+    """The CPU restatement of the reference path tracer on a scene dict."""
+
+    def __init__(self, scene):
+        L = _pt_lib()
+        self._keep = _scene_args(scene)
+        v, i, n, uv, m, mats, cam = self._keep
+        self.h = L.orc_pt_create(_p(v), len(v), _p(i), _p(n), _p(m), len(m), _p(mats), len(mats), _p(cam))
+
+    def __del__(self):
+        try:
+            _pt_lib().orc_pt_free(self.h)
+        except Exception:
+            pass
+
+    def camera(self):
+        out = np.zeros(20, np.float32)
+        _pt_lib().orc_pt_camera(self.h, _p(out))
+        return out
+
+    def lights(self):
+        L = _pt_lib()
+        n = L.orc_pt_lights(self.h, None, None, None, None, None, None, None, None)
+        t = len(self._keep[4])
+        out = {"prim": np.zeros(n, np.int32), "area": np.zeros(n, np.float32), "cdf": np.zeros(n, np.float32),
+               "prob": np.zeros(n, np.float32), "alias": np.zeros(n, np.int32), "pmf": np.zeros(n, np.float32),
+               "primitive_light_index": np.zeros(t, np.int32), "total_weight": np.zeros(1, np.float32)}
+        L.orc_pt_lights(self.h, _p(out["prim"]), _p(out["area"]), _p(out["cdf"]), _p(out["prob"]), _p(out["alias"]),
+                        _p(out["pmf"]), _p(out["primitive_light_index"]), _p(out["total_weight"]))
+        return out
+
+    def render(self, width, height, spp, depth, seed=0, frame=0, sample_offset=0, rows=None, threads=1):
+        film = np.zeros((height, width, 3), np.float32)
+        r0, r1 = (0, height) if rows is None else rows
+        _pt_lib().orc_pt_render(self.h, width, height, spp, depth, seed, frame, sample_offset, r0, r1, threads,
+                                _p(film))
+        return film
+
+    def samples(self, width, height, depth, pixels, samples, seed=0, frame=0, sample_offset=0):
+        px = np.ascontiguousarray(pixels, np.int32)
+        sm = np.ascontiguousarray(samples, np.int32)
+        out = np.zeros((len(px), 3), np.float32)
+        _pt_lib().orc_pt_samples(self.h, width, height, depth, seed, frame, sample_offset, _p(px), _p(sm), len(px),
+                                 _p(out))
+        return out
+
+
+def pcg32(seed, seq, n):
+    L = lib()
+    L.orc_pcg32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, c_int32, c_void_p]
+    out = np.zeros(n, np.uint32)
+    L.orc_pcg32(seed, seq, n, _p(out))
+    return out

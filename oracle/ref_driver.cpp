@@ -167,3 +167,171 @@ int64_t ref_overlap_cross_brute(const float* lboxes, int32_t nl, const float* rb
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Path tracer: the reference's own CPU path tracer (render_cpu_path_tracer /
+// trace_path, src/integrator/cpu_path_tracer.cpp) on a scene given as arrays.
+// The scene is one mesh + one identity instance, built through the reference's
+// own scene API (add_scene_mesh / add_scene_instance / PreparedScene).
+// ---------------------------------------------------------------------------
+#include <unistd.h>
+
+#include <cstdio>
+#include <memory>
+
+#include "camera/pinhole_camera.h"
+#include "integrator/cpu_path_tracer.h"
+#include "integrator/render_context.h"
+#include "scene/prepared_scene.h"
+#include "scene/scene_build.h"
+#include "scene/scene_edit.h"
+#include "scene/scene_light_sampler.h"
+
+extern "C" {
+
+struct ref_pt {
+    std::unique_ptr<PreparedScene> prepared;
+};
+
+// materials: m x 11 floats {type, albedo rgb, emission rgb, two_sided, roughness,
+// metallic, eta}; camera: {origin xyz, look_at xyz, vertical fov degrees, aspect}
+ref_pt* ref_pt_create(const float* vertices, int32_t nv, const uint32_t* indices, const float* normals,
+                      const float* uvs, const int32_t* material_ids, int32_t nt, const float* materials,
+                      int32_t nm, const float* camera) {
+    Scene scene;
+    for (int m = 0; m < nm; ++m) {
+        const float* p = materials + 11 * m;
+        Material mat(static_cast<MaterialType>(int(p[0])), vec3(p[1], p[2], p[3]), vec3(p[4], p[5], p[6]), p[8],
+                     p[9], p[10],
+                     p[7] != 0 ? EmissionSidedness::DoubleSided : EmissionSidedness::SingleSided);
+        scene.materials.push_back(mat);
+    }
+    SceneMesh mesh;
+    mesh.vertices.resize(nv);
+    mesh.normals.assign(nv, vec3(0));
+    mesh.uvs.assign(nv, vec2(0));
+    for (int i = 0; i < nv; ++i) {
+        mesh.vertices[i] = vec3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]);
+        if (normals) mesh.normals[i] = vec3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+        if (uvs) mesh.uvs[i] = vec2(uvs[2 * i], uvs[2 * i + 1]);
+    }
+    mesh.indices.resize(nt);
+    mesh.material_ids.resize(nt);
+    mesh.boundary_edges.assign(nt, 0);
+    for (int t = 0; t < nt; ++t) {
+        mesh.indices[t] = TriangleIndices{indices[3 * t], indices[3 * t + 1], indices[3 * t + 2]};
+        mesh.material_ids[t] = material_ids[t];
+    }
+    int mesh_id = add_scene_mesh(scene, std::move(mesh));
+    add_scene_instance(scene, mesh_id, Transform{});
+    scene.camera = make_pinhole_camera(vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]),
+                                       camera[6], camera[7]);
+    ref_pt* r = new ref_pt();
+    r->prepared = std::make_unique<PreparedScene>(std::move(scene));
+    r->prepared->prepare_host_bvh();
+    return r;
+}
+
+void ref_pt_free(ref_pt* r) { delete r; }
+
+// {origin, lower_left, horizontal, vertical, lens_u, lens_v} xyz, lens_radius, focal_distance
+void ref_pt_camera(const ref_pt* r, float* out) {
+    const PerspectiveCamera& c = r->prepared->host_scene().camera.perspective;
+    const vec3* v[6] = {&c.origin, &c.lower_left, &c.horizontal, &c.vertical, &c.lens_u, &c.lens_v};
+    for (int k = 0; k < 6; ++k) {
+        out[3 * k] = v[k]->x;
+        out[3 * k + 1] = v[k]->y;
+        out[3 * k + 2] = v[k]->z;
+    }
+    out[18] = c.lens_radius;
+    out[19] = c.focal_distance;
+}
+
+// Light selection table: per light {primitive ordinal, area, power cdf, alias
+// probability, alias index, pmf}; returns the light count (arrays sized by the
+// caller, call with null first). primitive_light_index: one int per triangle.
+int32_t ref_pt_lights(const ref_pt* r, int32_t* prim, float* area, float* cdf, float* prob, int32_t* alias,
+                      float* pmf, int32_t* primitive_light_index, float* total_weight) {
+    const Scene& scene = r->prepared->host_scene();
+    const PreparedLightSampling& ls = r->prepared->light_sampling();
+    const LightSelectionTable& sel = ls.light_selection;
+    int count = int(sel.lights.size());
+    if (!prim) return count;
+    for (int i = 0; i < count; ++i) {
+        prim[i] = scene_primitive_index(scene, sel.lights[i].area.primitive_ref);
+        area[i] = sel.lights[i].area.area;
+        cdf[i] = sel.power_distribution.cdf[i];
+        prob[i] = sel.power_distribution.alias_table[i].probability;
+        alias[i] = sel.power_distribution.alias_table[i].alias;
+        pmf[i] = sel.power_distribution.alias_table[i].pmf;
+    }
+    *total_weight = sel.power_distribution.total_weight;
+    const auto& lookup = ls.surface_lookup.primitive_light_indices;
+    for (size_t t = 0; t < lookup.size(); ++t) primitive_light_index[t] = lookup[t];
+    return count;
+}
+
+static RenderSettings make_settings(int32_t depth, uint64_t seed, int32_t frame, int32_t sample_offset) {
+    RenderSettings s;
+    s.max_depth = depth;
+    s.seed = seed;
+    s.frame_index = frame;
+    s.sample_offset = sample_offset;
+    s.sampler_kind = SamplerKind::Independent;
+    s.light_sampler_kind = LightSamplerKind::Power;
+    return s;
+}
+
+// render_cpu_path_tracer itself; film_sum receives the per-pixel SUM over spp
+// (Film::pixels, row-major, y = 0 at the top). Its row printout is silenced.
+void ref_pt_render(const ref_pt* r, int32_t w, int32_t h, int32_t spp, int32_t depth, uint64_t seed,
+                   int32_t frame, int32_t sample_offset, float* film_sum) {
+    Film film(w, h, spp);
+    RenderSettings s = make_settings(depth, seed, frame, sample_offset);
+    std::fflush(stdout);
+    int saved = dup(1);
+    FILE* devnull = std::fopen("/dev/null", "w");
+    if (devnull) dup2(fileno(devnull), 1);
+    render_cpu_path_tracer(r->prepared->host_scene(), r->prepared->host_bvh(), r->prepared->light_sampling(),
+                           film, s, nullptr);
+    std::fflush(stdout);
+    if (saved >= 0) {
+        dup2(saved, 1);
+        close(saved);
+    }
+    if (devnull) std::fclose(devnull);
+    for (int i = 0; i < w * h; ++i) {
+        film_sum[3 * i] = film.pixels[i].x;
+        film_sum[3 * i + 1] = film.pixels[i].y;
+        film_sum[3 * i + 2] = film.pixels[i].z;
+    }
+}
+
+// trace_path radiance of individual (pixel, sample) pairs, with the camera
+// sample drawn exactly as render_cpu_path_tracer draws it.
+void ref_pt_samples(const ref_pt* r, int32_t w, int32_t h, int32_t depth, uint64_t seed, int32_t frame,
+                    int32_t sample_offset, const int32_t* pixels, const int32_t* samples, int32_t n, float* out) {
+    const Scene& scene = r->prepared->host_scene();
+    RenderSettings s = make_settings(depth, seed, frame, sample_offset);
+    SceneLightSampler lights =
+        make_scene_light_sampler(scene, r->prepared->light_sampling(), s.light_sampler_kind);
+    for (int i = 0; i < n; ++i) {
+        int pixel = pixels[i];
+        int x = pixel % w, y = pixel / w;
+        SamplerState sampler = make_render_sampler(s, pixel, samples[i]);
+        Float jx = sampler_get_1d(sampler);
+        Float jy = sampler_get_1d(sampler);
+        CameraSample cs;
+        cs.p_film = vec2((Float(x) + jx) / Float(w), (Float(h - 1 - y) + jy) / Float(h));
+        cs.p_lens = sampler_get_2d(sampler);
+        cs.time = sampler_get_1d(sampler);
+        CameraRay cr = generate_camera_ray(scene.camera, cs);
+        vec3 c(0);
+        if (cr.valid) c = trace_path(scene, r->prepared->host_bvh(), lights, cr.ray, sampler, depth);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
+}
+
+}  // extern "C"

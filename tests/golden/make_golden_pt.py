@@ -1,0 +1,62 @@
+"""Writes path-tracer golden vectors under tests/golden/ from the REFERENCE's own
+CPU path tracer (oracle/_ref/libluxref.so: render_cpu_path_tracer, trace_path,
+make_pinhole_camera, prepare_scene_light_sampling), on scenes given as arrays.
+
+    python tests/golden/make_golden_pt.py
+
+Cases:
+  pt_cornell_32   Cornell box 32x32, 4 spp, depth 4: film + every (pixel, sample)
+  pt_cornell_c1   Cornell box 256x256, 64 spp, depth 4, seed 0: the C1 oracle image
+  pt_icosphere    displaced icosphere (freq 12, 2882 tris, Lambert/GGX/dielectric,
+                  smooth normals) 40x40, 8 spp, depth 8, seed 7: film + 2048 samples
+Each stores the scene arrays, the camera and light tables the reference derived.
+"""
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+import oracle  # noqa: E402
+from lux_amd import scenes  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def case(name, scene, w, h, spp, depth, seed, sample_pixels=None):
+    r = oracle.RefPathTracer(scene)
+    film = r.render(w, h, spp, depth, seed=seed)
+    data = {k: scene[k] for k in ("vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera")}
+    data.update(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
+                seed=np.uint64(seed), film_sum=film, camera_vectors=r.camera())
+    for k, v in r.lights().items():
+        data["light_" + k] = v
+    if sample_pixels is not None:
+        px, sm = sample_pixels
+        data["sample_pixels"] = px.astype(np.int32)
+        data["sample_index"] = sm.astype(np.int32)
+        data["sample_radiance"] = r.samples(w, h, depth, px, sm, seed=seed)
+    np.savez_compressed(OUT / f"{name}.npz", **data)
+    print(f"{name}: {w}x{h} spp={spp} depth={depth} tris={len(scene['indices'])} mean={film.mean() / spp:.5f}")
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref/libluxref.so missing: run `make -f oracle/ref.mk` where the reference exists")
+    w = h = 32
+    px = np.repeat(np.arange(w * h), 4)
+    sm = np.tile(np.arange(4), w * h)
+    case("pt_cornell_32", scenes.cornell(w, h), w, h, 4, 4, 0, (px, sm))
+    case("pt_cornell_c1", scenes.cornell(256, 256), 256, 256, 64, 4, 0)
+    ico = scenes.displaced_icosphere(freq=12, width=40, height=40)
+    rng = np.random.default_rng(5)
+    px = rng.integers(0, 40 * 40, 2048)
+    sm = rng.integers(0, 8, 2048)
+    case("pt_icosphere", ico, 40, 40, 8, 8, 7, (px, sm))
+
+
+if __name__ == "__main__":
+    main()
