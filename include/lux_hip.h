@@ -190,6 +190,81 @@ int lux_hip_overlap_cross_device(lux_hip_bvh* left, lux_hip_bvh* right,
                                  int32_t max_pairs, lux_pair* d_pairs,
                                  int32_t* d_found_count, lux_stream stream);
 
+/* ---- wavefront path tracer --------------------------------------------- */
+/* Material record (the fields of Material, src/material/material.h:24-35;
+ * constant emission only). type follows MaterialType (:6-13). */
+enum {
+    LUX_MAT_LAMBERT = 0,
+    LUX_MAT_THIN_LAMBERT = 1,
+    LUX_MAT_GGX = 2,
+    LUX_MAT_DIELECTRIC = 3,
+    LUX_MAT_INTERFACE = 4,
+    LUX_MAT_TRANSPARENT = 5
+};
+typedef struct lux_material {
+    int32_t type;
+    lux_vec3 albedo;
+    lux_vec3 emission;
+    int32_t emission_two_sided; /* EmissionSidedness::DoubleSided */
+    float roughness, metallic, eta;
+} lux_material;
+
+/* One mesh under an identity instance (SceneMesh + SceneInstance,
+ * src/scene/scene_data.h:35-58) and a pinhole camera given as
+ * make_pinhole_camera's arguments {origin xyz, look_at xyz, vertical fov in
+ * degrees, aspect} (src/camera/pinhole_camera.h:13-17). normals may be NULL
+ * (flat shading). Emissive triangles become power-sampled area lights
+ * (prepare_scene_light_sampling, src/scene/scene_prepare.h:268-387). */
+typedef struct lux_scene_desc {
+    const lux_vec3* vertices;
+    int32_t vertex_count;
+    const uint32_t* indices; /* 3 per triangle */
+    const lux_vec3* normals; /* per vertex or NULL */
+    const int32_t* material_ids;
+    int32_t triangle_count;
+    const lux_material* materials;
+    int32_t material_count;
+    float camera[8];
+} lux_scene_desc;
+
+enum { LUX_RENDER_WAVEFRONT = 0, LUX_RENDER_PLAIN = 1 };
+/* RenderSettings (src/integrator/render_context.h:16-24) + film size. mode
+ * PLAIN reproduces the reference schedule (batch_size paths per batch, one
+ * sample per batch, full-grid launches, a host sync per batch); WAVEFRONT packs
+ * several samples per batch and never waits on the host mid-render. batch_size
+ * 0 picks the mode's default. */
+typedef struct lux_render_settings {
+    int32_t width, height, spp, max_depth;
+    uint64_t seed;
+    int32_t frame_index, sample_offset;
+    int32_t batch_size;
+    int32_t mode;
+} lux_render_settings;
+
+typedef struct lux_hip_scene lux_hip_scene;
+
+/* PreparedScene(scene) + prepare_gpu() (src/scene/prepared_scene.cu:19-23,
+ * :61-79): light tables, camera, device arrays and the LBVH over the
+ * triangles. Synchronous. */
+int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out);
+int lux_hip_scene_destroy(lux_hip_scene* scene);
+/* Camera vectors {origin, lower_left, horizontal, vertical} (12 floats). */
+int lux_hip_scene_camera(lux_hip_scene* scene, float* out12);
+/* Light table: count, and (if the arrays are non-NULL, sized by count) each
+ * light's triangle, area, alias probability, alias index and pmf;
+ * light_of_prim gets one entry per triangle (-1 = not an emitter). */
+int lux_hip_scene_lights(lux_hip_scene* scene, int32_t* count, int32_t* prim, float* area, float* prob,
+                         int32_t* alias, float* pmf, int32_t* light_of_prim);
+/* The LBVH of the scene's triangles (for the traversal entry points). */
+int lux_hip_scene_bvh(lux_hip_scene* scene, lux_hip_bvh** out_borrowed);
+
+/* WavefrontPathTracer::render (src/integrator/wavefront_pt.cu:51-84): the film
+ * receives per-pixel SUMS over spp, row-major with y = 0 at the top, like
+ * Film::pixels (src/film/film.h:15-67). Host film: synchronous. */
+int lux_hip_render(lux_hip_scene* scene, const lux_render_settings* settings, float* film_sum);
+int lux_hip_render_device(lux_hip_scene* scene, const lux_render_settings* settings, float* d_film_sum,
+                          lux_stream stream);
+
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart (the reference only prints wall-clock stage times,
  * src/app/render_runner.cpp:274-286). While a probe is open, every launch of

@@ -92,6 +92,13 @@ _SIGNATURES = {
     "lux_hip_probe_begin": [c_char_p],
     "lux_hip_probe_end": [POINTER(c_double), POINTER(c_int64)],
     "lux_hip_trace_stats_device": [_P, _P, c_int32, _P, c_int32, _P],
+    "lux_hip_scene_create": [_P, POINTER(c_void_p)],
+    "lux_hip_scene_destroy": [_P],
+    "lux_hip_scene_camera": [_P, _P],
+    "lux_hip_scene_lights": [_P, POINTER(c_int32), _P, _P, _P, _P, _P, _P],
+    "lux_hip_scene_bvh": [_P, POINTER(c_void_p)],
+    "lux_hip_render": [_P, _P, _P],
+    "lux_hip_render_device": [_P, _P, _P, _P],
 }
 _RESTYPES = {"lux_hip_last_error": c_char_p}
 

@@ -669,3 +669,137 @@ int lux_hip_probe_end(double* total_ms, int64_t* launches) {
 }
 
 }  // extern "C"
+
+// ---- wavefront path tracer ------------------------------------------------------
+#include "pt_runtime.hpp"
+
+struct lux_hip_scene {
+    lux::pt::SceneDev dev;
+    lux::pt::PathBuf paths;
+    lux_hip_bvh bvh_alias;  // borrowed view of dev.bvh for the traversal entry points
+};
+
+namespace {
+void check_settings(const lux_render_settings* s) {
+    if (!s) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "null render settings");
+    if (s->width <= 0 || s->height <= 0 || s->spp <= 0)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "WavefrontPathTracer requires a non-empty Film");
+    if (s->max_depth < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "WavefrontPathTracer requires max_depth >= 0");
+    if (int64_t(s->width) * s->height >= (int64_t(1) << 31))
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "film too large");
+    if (s->mode != LUX_RENDER_PLAIN && s->mode != LUX_RENDER_WAVEFRONT)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "unknown render mode");
+    if (s->batch_size < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "negative batch size");
+}
+
+void scene_sync(lux_hip_scene* sc) {
+    lux::hip_check(hipStreamSynchronize(lux::library_stream()), "synchronize");
+    lux::DevInfo info;
+    lux::hip_check(hipMemcpy(&info, sc->dev.bvh.info, sizeof(info), hipMemcpyDeviceToHost), "read device info");
+    if (info.error) {
+        uint32_t zero = 0;
+        lux::hip_check(hipMemcpy(reinterpret_cast<char*>(sc->dev.bvh.info) + offsetof(lux::DevInfo, error), &zero,
+                                 4, hipMemcpyHostToDevice),
+                       "clear device info");
+        throw lux::Error(LUX_ERR_DEVICE_CHECK, "device check failed during render");
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out) {
+    return guarded([&] {
+        require_init();
+        if (!desc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto* sc = new lux_hip_scene();
+        try {
+            pt::scene_upload(sc->dev, *desc, g_stream);
+        } catch (...) {
+            pt::scene_release(sc->dev);
+            delete sc;
+            throw;
+        }
+        *out = sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_destroy(lux_hip_scene* sc) {
+    return guarded([&] {
+        if (!sc) return int(LUX_OK);
+        if (g_initialized) (void)hipStreamSynchronize(g_stream);
+        pt::pathbuf_release(sc->paths);
+        pt::scene_release(sc->dev);
+        sc->bvh_alias.dev = BvhDev{};
+        delete sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_camera(lux_hip_scene* sc, float* out) {
+    return guarded([&] {
+        if (!sc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        std::memcpy(out, &sc->dev.camera, sizeof(float) * 12);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float* area, float* prob, int32_t* alias,
+                         float* pmf, int32_t* light_of_prim) {
+    return guarded([&] {
+        if (!sc || !count) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const auto& L = sc->dev.host_lights;
+        *count = int32_t(L.size());
+        for (size_t i = 0; i < L.size(); ++i) {
+            if (prim) prim[i] = L[i].prim;
+            if (area) area[i] = L[i].area;
+            if (prob) prob[i] = L[i].prob;
+            if (alias) alias[i] = L[i].alias;
+            if (pmf) pmf[i] = L[i].pmf;
+        }
+        if (light_of_prim)
+            std::memcpy(light_of_prim, sc->dev.host_light_of_prim.data(), 4 * sc->dev.host_light_of_prim.size());
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_bvh(lux_hip_scene* sc, lux_hip_bvh** out) {
+    return guarded([&] {
+        if (!sc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        // The alias shares the scene's device buffers; it must not be destroyed.
+        sc->bvh_alias.dev = sc->dev.bvh;
+        sc->bvh_alias.valid = sc->dev.bvh.n > 0;
+        sc->bvh_alias.host_ids_known = false;
+        sc->bvh_alias.last_stream = g_stream;
+        *out = &sc->bvh_alias;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_render(lux_hip_scene* sc, const lux_render_settings* s, float* film) {
+    return guarded([&] {
+        require_init();
+        if (!sc || !film) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        check_settings(s);
+        size_t bytes = sizeof(float) * 3 * size_t(s->width) * s->height;
+        DeviceBuffer<float> d_film{3 * size_t(s->width) * s->height};
+        pt::render(sc->dev, sc->paths, *s, d_film.p, g_stream, nullptr);
+        hip_check(hipMemcpyAsync(film, d_film.p, bytes, hipMemcpyDeviceToHost, g_stream), "D2H film");
+        scene_sync(sc);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_render_device(lux_hip_scene* sc, const lux_render_settings* s, float* d_film, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        if (!sc || !d_film) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        check_settings(s);
+        pt::render(sc->dev, sc->paths, *s, d_film, pick(stream), nullptr);
+        return int(LUX_OK);
+    });
+}
+
+}  // extern "C"
