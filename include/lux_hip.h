@@ -264,6 +264,100 @@ int lux_hip_scene_bvh(lux_hip_scene* scene, lux_hip_bvh** out_borrowed);
 int lux_hip_render(lux_hip_scene* scene, const lux_render_settings* settings, float* film_sum);
 int lux_hip_render_device(lux_hip_scene* scene, const lux_render_settings* settings, float* d_film_sum,
                           lux_stream stream);
+/* Rays traced by the scene's last render: extension (closest-hit) rays and
+ * shadow rays, summed from every bounce's queue sizes. No reference
+ * counterpart (the reference keeps no ray count); synchronises `stream`. */
+int lux_hip_render_rays(lux_hip_scene* scene, lux_stream stream, uint64_t* extension_rays, uint64_t* shadow_rays);
+
+/* Triangle and vertex counts and the HBM the scene holds (mesh + LBVH). */
+int lux_hip_scene_info(lux_hip_scene* scene, int32_t* triangle_count, int32_t* vertex_count, uint64_t* device_bytes);
+
+/* Moves the scene's vertices (same count, same triangles): the triangles are
+ * regathered on the device and the LBVH is refit with its topology kept, as
+ * refit_gpu_lbvh does for a primitive set with unchanged ids
+ * (src/bvh/gpu_bvh.cu:597-613); emitter areas and the power-alias light table
+ * are rebuilt. Renders enqueued earlier read the old geometry. Synchronous. */
+int lux_hip_scene_update_vertices(lux_hip_scene* scene, const lux_vec3* vertices, int32_t vertex_count);
+
+/* ---- multi-scene batches ------------------------------------------------
+ * Batched WavefrontPathTracer::render over n (scene, settings, film) jobs:
+ * the jobs' paths share one wavefront loop over a multi-scene ray queue (per
+ * path: its scene's BVH root, mesh, lights and camera), each batch's bounce
+ * loop replayed as one captured hipGraph. Each film equals the film of the
+ * same job rendered alone, bit for bit (per-path arithmetic is independent
+ * of the batch, and every pixel's samples are summed in sample order).
+ * mode LUX_RENDER_PLAIN renders the jobs one after another on the
+ * reference's schedule (the comparison baseline). No reference counterpart
+ * (the reference renders one scene per call). */
+int lux_hip_batch_render(lux_hip_scene* const* scenes, const lux_render_settings* settings, float* const* films,
+                         int32_t n, int32_t mode);
+int lux_hip_batch_render_device(lux_hip_scene* const* scenes, const lux_render_settings* settings,
+                                float* const* d_films, int32_t n, int32_t mode, lux_stream stream);
+/* Rays traced by the last batch (extension, shadow); synchronises `stream`. */
+int lux_hip_batch_rays(lux_stream stream, uint64_t* extension_rays, uint64_t* shadow_rays);
+
+/* Launch geometry of the wavefront schedule: path slots per batch, workgroups
+ * of the grid-stride kernels, graph replay on/off. Applies to renders started
+ * after the call. No reference counterpart (the reference's capacity is
+ * RenderSettings::batch_size, src/integrator/render_context.h:16-24). */
+typedef struct lux_render_tuning {
+    int32_t capacity;
+    int32_t grid;
+    int32_t use_graph;
+} lux_render_tuning;
+int lux_hip_set_render_tuning(const lux_render_tuning* tuning);
+int lux_hip_get_render_tuning(lux_render_tuning* tuning);
+
+/* ---- request server ------------------------------------------------------
+ * Continuous batching: render requests are admitted in arrival order while
+ * their HBM footprint (resident scenes + films + path buffer) fits the
+ * budget, join the running wavefront loop in slices of slice_spp samples
+ * (round robin) while earlier requests are still on the GPU, and complete
+ * when the batch holding their last path finishes. A request carrying new
+ * vertex positions waits until its scene has no unfinished request, then the
+ * scene is refit (lux_hip_scene_update_vertices) before its first batch.
+ * Every film equals the film of the same render run alone. Scenes must
+ * outlive the server. Not thread-safe. No reference counterpart (the
+ * reference renders one request per process run, src/app/render_runner.cpp). */
+typedef struct lux_serve_config {
+    uint64_t hbm_budget_bytes;
+    int32_t slice_spp;     /* samples per request per turn; 0 = no limit */
+    int32_t max_inflight;  /* batches enqueued ahead of the GPU (1..64) */
+} lux_serve_config;
+
+typedef struct lux_serve_stats {
+    uint64_t submitted, admitted, completed, refits;
+    uint64_t batches, paths;
+    uint64_t resident_bytes, peak_bytes;
+    uint64_t extension_rays, shadow_rays;
+} lux_serve_stats;
+
+typedef struct lux_hip_server lux_hip_server;
+
+int lux_hip_server_create(const lux_serve_config* config, lux_hip_server** out);
+int lux_hip_server_destroy(lux_hip_server* server);
+/* Queues a request (new_vertices may be NULL; otherwise vertex_count must be
+ * the scene's and the array is copied). Returns immediately. */
+int lux_hip_server_submit(lux_hip_server* server, lux_hip_scene* scene, const lux_render_settings* settings,
+                          const lux_vec3* new_vertices, int32_t vertex_count, uint64_t* request_id);
+/* One scheduling round (retire, admit, enqueue); with block != 0 it waits for
+ * the oldest batch on the GPU when no new batch could be enqueued. Returns up
+ * to max_ids completed request ids. */
+int lux_hip_server_poll(lux_hip_server* server, int32_t block, uint64_t* done_ids, int32_t max_ids,
+                        int32_t* n_done);
+/* Film sums (W*H*3, may be NULL) and arrival-to-completion latency of a
+ * completed request; releases its film. */
+int lux_hip_server_result(lux_hip_server* server, uint64_t request_id, float* film_sum, double* latency_s);
+int lux_hip_server_stats(lux_hip_server* server, lux_serve_stats* stats);
+/* Replays an arrival trace in real time on the server: request i arrives
+ * arrival_s[i] seconds after the call (sorted), renders settings[i] of
+ * scenes[i] (after moving the vertices to new_vertices[i] if new_vertices and
+ * new_vertices[i] are non-NULL). films (may be NULL) receive the film sums;
+ * latency_s[i] = completion - arrival; wall_s = total time. */
+int lux_hip_server_run(lux_hip_server* server, int32_t n, lux_hip_scene* const* scenes,
+                       const lux_render_settings* settings, const double* arrival_s,
+                       const lux_vec3* const* new_vertices, const int32_t* vertex_counts, float* const* films,
+                       double* latency_s, double* wall_s);
 
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart (the reference only prints wall-clock stage times,

@@ -1,7 +1,7 @@
 """ctypes binding of include/lux_hip.h (the C ABI of liblux_hip.so)."""
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint32, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -99,6 +99,21 @@ _SIGNATURES = {
     "lux_hip_scene_bvh": [_P, POINTER(c_void_p)],
     "lux_hip_render": [_P, _P, _P],
     "lux_hip_render_device": [_P, _P, _P, _P],
+    "lux_hip_render_rays": [_P, _P, POINTER(c_uint64), POINTER(c_uint64)],
+    "lux_hip_scene_info": [_P, POINTER(c_int32), POINTER(c_int32), POINTER(c_uint64)],
+    "lux_hip_scene_update_vertices": [_P, _P, c_int32],
+    "lux_hip_batch_render": [_P, _P, _P, c_int32, c_int32],
+    "lux_hip_batch_render_device": [_P, _P, _P, c_int32, c_int32, _P],
+    "lux_hip_batch_rays": [_P, POINTER(c_uint64), POINTER(c_uint64)],
+    "lux_hip_set_render_tuning": [_P],
+    "lux_hip_get_render_tuning": [_P],
+    "lux_hip_server_create": [_P, POINTER(c_void_p)],
+    "lux_hip_server_destroy": [_P],
+    "lux_hip_server_submit": [_P, _P, _P, _P, c_int32, POINTER(c_uint64)],
+    "lux_hip_server_poll": [_P, c_int32, _P, c_int32, POINTER(c_int32)],
+    "lux_hip_server_result": [_P, c_uint64, _P, POINTER(c_double)],
+    "lux_hip_server_stats": [_P, _P],
+    "lux_hip_server_run": [_P, c_int32, _P, _P, _P, _P, _P, _P, _P, POINTER(c_double)],
 }
 _RESTYPES = {"lux_hip_last_error": c_char_p}
 

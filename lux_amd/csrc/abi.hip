@@ -139,6 +139,9 @@ void probe_post_launch(const char* name, hipStream_t s) {
 
 using namespace lux;
 
+// Frees the path tracer's shared engines (defined with the path tracer entry points).
+static void release_render_engines();
+
 extern "C" {
 
 int lux_hip_version(void) { return 0x000100; }
@@ -178,6 +181,7 @@ int lux_hip_shutdown(void) {
             g_probe.pool.clear();
             g_probe.open = false;
         }
+        release_render_engines();
         (void)hipStreamDestroy(g_stream);
         g_stream = nullptr;
         g_initialized = false;
@@ -671,15 +675,26 @@ int lux_hip_probe_end(double* total_ms, int64_t* launches) {
 }  // extern "C"
 
 // ---- wavefront path tracer ------------------------------------------------------
+#include <memory>
+
 #include "pt_runtime.hpp"
+#include "serve.hpp"
+
+struct lux_hip_server {
+    std::unique_ptr<lux::pt::Server> server;
+    std::vector<uint64_t> ready;  // completed ids not yet returned by poll
+};
 
 struct lux_hip_scene {
     lux::pt::SceneDev dev;
-    lux::pt::PathBuf paths;
+    std::unique_ptr<lux::pt::Renderer> renderer;  // path buffers of this scene's single renders
     lux_hip_bvh bvh_alias;  // borrowed view of dev.bvh for the traversal entry points
 };
 
 namespace {
+lux::pt::Tuning g_tuning;
+std::unique_ptr<lux::pt::Renderer> g_batch;  // engine of the multi-scene batch entry points
+
 void check_settings(const lux_render_settings* s) {
     if (!s) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "null render settings");
     if (s->width <= 0 || s->height <= 0 || s->spp <= 0)
@@ -687,13 +702,15 @@ void check_settings(const lux_render_settings* s) {
     if (s->max_depth < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "WavefrontPathTracer requires max_depth >= 0");
     if (int64_t(s->width) * s->height >= (int64_t(1) << 31))
         throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "film too large");
+    if (int64_t(s->spp) + s->sample_offset >= (int64_t(1) << 31) || s->sample_offset < 0)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "sample range out of bounds");
     if (s->mode != LUX_RENDER_PLAIN && s->mode != LUX_RENDER_WAVEFRONT)
         throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "unknown render mode");
     if (s->batch_size < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "negative batch size");
 }
 
-void scene_sync(lux_hip_scene* sc) {
-    lux::hip_check(hipStreamSynchronize(lux::library_stream()), "synchronize");
+// Reads and clears the device check bits of a scene (caller synchronised).
+void scene_check(lux_hip_scene* sc) {
     lux::DevInfo info;
     lux::hip_check(hipMemcpy(&info, sc->dev.bvh.info, sizeof(info), hipMemcpyDeviceToHost), "read device info");
     if (info.error) {
@@ -704,7 +721,58 @@ void scene_sync(lux_hip_scene* sc) {
         throw lux::Error(LUX_ERR_DEVICE_CHECK, "device check failed during render");
     }
 }
+
+lux::pt::Renderer& scene_renderer(lux_hip_scene* sc) {
+    if (!sc->renderer) sc->renderer = std::make_unique<lux::pt::Renderer>();
+    sc->renderer->set_tuning(g_tuning);
+    return *sc->renderer;
+}
+
+lux::pt::Renderer& batch_renderer() {
+    if (!g_batch) g_batch = std::make_unique<lux::pt::Renderer>();
+    g_batch->set_tuning(g_tuning);
+    return *g_batch;
+}
+
+void render_one(lux_hip_scene* sc, const lux_render_settings* s, float* d_film, hipStream_t stream) {
+    lux::pt::Renderer& r = scene_renderer(sc);
+    lux::pt::Request rq;
+    rq.sv = lux::pt::scene_view(sc->dev);
+    rq.rs = *s;
+    rq.film = d_film;
+    std::vector<lux::pt::Request*> reqs{&rq};
+    r.reset_ray_counts(stream);
+    r.render_all(reqs, s->mode, stream, nullptr);
+}
+
+void check_batch_args(lux_hip_scene* const* scenes, const lux_render_settings* settings, int32_t n, int mode) {
+    if (n < 0 || (n > 0 && (!scenes || !settings))) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "bad batch arguments");
+    if (mode != LUX_RENDER_PLAIN && mode != LUX_RENDER_WAVEFRONT)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "unknown render mode");
+    for (int32_t i = 0; i < n; ++i) {
+        if (!scenes[i]) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "null scene in batch");
+        check_settings(&settings[i]);
+    }
+}
+
+void render_batch(lux_hip_scene* const* scenes, const lux_render_settings* settings, float* const* d_films,
+                  int32_t n, int mode, hipStream_t stream) {
+    lux::pt::Renderer& r = batch_renderer();
+    std::vector<lux::pt::Request> reqs(n);
+    std::vector<lux::pt::Request*> ptrs(n);
+    for (int32_t i = 0; i < n; ++i) {
+        reqs[i].sv = lux::pt::scene_view(scenes[i]->dev);
+        reqs[i].rs = settings[i];
+        reqs[i].film = d_films[i];
+        reqs[i].tag = uint64_t(i);
+        ptrs[i] = &reqs[i];
+    }
+    r.reset_ray_counts(stream);
+    r.render_all(ptrs, mode, stream, nullptr);
+}
 }  // namespace
+
+static void release_render_engines() { g_batch.reset(); }
 
 extern "C" {
 
@@ -729,11 +797,31 @@ int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out) {
 int lux_hip_scene_destroy(lux_hip_scene* sc) {
     return guarded([&] {
         if (!sc) return int(LUX_OK);
-        if (g_initialized) (void)hipStreamSynchronize(g_stream);
-        pt::pathbuf_release(sc->paths);
+        if (g_initialized) (void)hipDeviceSynchronize();
+        sc->renderer.reset();
         pt::scene_release(sc->dev);
         sc->bvh_alias.dev = BvhDev{};
         delete sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_update_vertices(lux_hip_scene* sc, const lux_vec3* vertices, int32_t vertex_count) {
+    return guarded([&] {
+        require_init();
+        if (!sc || !vertices) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        pt::scene_update_vertices(sc->dev, vertices, vertex_count, g_stream);
+        scene_check(sc);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_info(lux_hip_scene* sc, int32_t* triangle_count, int32_t* vertex_count, uint64_t* device_bytes) {
+    return guarded([&] {
+        if (!sc) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        if (triangle_count) *triangle_count = sc->dev.tri_count;
+        if (vertex_count) *vertex_count = sc->dev.vertex_count;
+        if (device_bytes) *device_bytes = sc->dev.device_bytes;
         return int(LUX_OK);
     });
 }
@@ -785,9 +873,10 @@ int lux_hip_render(lux_hip_scene* sc, const lux_render_settings* s, float* film)
         check_settings(s);
         size_t bytes = sizeof(float) * 3 * size_t(s->width) * s->height;
         DeviceBuffer<float> d_film{3 * size_t(s->width) * s->height};
-        pt::render(sc->dev, sc->paths, *s, d_film.p, g_stream, nullptr);
+        render_one(sc, s, d_film.p, g_stream);
         hip_check(hipMemcpyAsync(film, d_film.p, bytes, hipMemcpyDeviceToHost, g_stream), "D2H film");
-        scene_sync(sc);
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        scene_check(sc);
         return int(LUX_OK);
     });
 }
@@ -797,7 +886,194 @@ int lux_hip_render_device(lux_hip_scene* sc, const lux_render_settings* s, float
         require_init();
         if (!sc || !d_film) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
         check_settings(s);
-        pt::render(sc->dev, sc->paths, *s, d_film, pick(stream), nullptr);
+        render_one(sc, s, d_film, pick(stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_render_rays(lux_hip_scene* sc, lux_stream stream, uint64_t* extension_rays, uint64_t* shadow_rays) {
+    return guarded([&] {
+        require_init();
+        if (!sc || !extension_rays || !shadow_rays) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        if (!sc->renderer) {
+            *extension_rays = *shadow_rays = 0;
+            return int(LUX_OK);
+        }
+        sc->renderer->ray_counts(pick(stream), extension_rays, shadow_rays);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_batch_render(lux_hip_scene* const* scenes, const lux_render_settings* settings, float* const* films,
+                         int32_t n, int32_t mode) {
+    return guarded([&] {
+        require_init();
+        check_batch_args(scenes, settings, n, mode);
+        if (!films && n > 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "null films");
+        std::vector<std::unique_ptr<DeviceBuffer<float>>> bufs;
+        std::vector<float*> d_films(n);
+        for (int32_t i = 0; i < n; ++i) {
+            if (!films[i]) throw Error(LUX_ERR_INVALID_ARGUMENT, "null film");
+            bufs.push_back(std::make_unique<DeviceBuffer<float>>(3 * size_t(settings[i].width) * settings[i].height));
+            d_films[i] = bufs.back()->p;
+        }
+        render_batch(scenes, settings, d_films.data(), n, mode, g_stream);
+        for (int32_t i = 0; i < n; ++i)
+            hip_check(hipMemcpyAsync(films[i], d_films[i], sizeof(float) * 3 * size_t(settings[i].width) *
+                                                               settings[i].height,
+                                     hipMemcpyDeviceToHost, g_stream),
+                      "D2H film");
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        for (int32_t i = 0; i < n; ++i) scene_check(scenes[i]);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_batch_render_device(lux_hip_scene* const* scenes, const lux_render_settings* settings,
+                                float* const* d_films, int32_t n, int32_t mode, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        check_batch_args(scenes, settings, n, mode);
+        if (!d_films && n > 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "null films");
+        for (int32_t i = 0; i < n; ++i)
+            if (!d_films[i]) throw Error(LUX_ERR_INVALID_ARGUMENT, "null film");
+        render_batch(scenes, settings, d_films, n, mode, pick(stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_batch_rays(lux_stream stream, uint64_t* extension_rays, uint64_t* shadow_rays) {
+    return guarded([&] {
+        require_init();
+        if (!extension_rays || !shadow_rays) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        batch_renderer().ray_counts(pick(stream), extension_rays, shadow_rays);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_create(const lux_serve_config* cfg, lux_hip_server** out) {
+    return guarded([&] {
+        require_init();
+        if (!cfg || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        pt::ServeConfig c;
+        c.hbm_budget = cfg->hbm_budget_bytes;
+        c.slice_spp = cfg->slice_spp;
+        c.max_inflight = cfg->max_inflight;
+        auto* sv = new lux_hip_server();
+        try {
+            sv->server = std::make_unique<pt::Server>(c, g_tuning, g_stream);
+        } catch (...) {
+            delete sv;
+            throw;
+        }
+        *out = sv;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_destroy(lux_hip_server* sv) {
+    return guarded([&] {
+        delete sv;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_submit(lux_hip_server* sv, lux_hip_scene* scene, const lux_render_settings* settings,
+                          const lux_vec3* new_vertices, int32_t vertex_count, uint64_t* request_id) {
+    return guarded([&] {
+        require_init();
+        if (!sv || !scene || !request_id) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        check_settings(settings);
+        *request_id = sv->server->submit(&scene->dev, *settings, new_vertices, vertex_count, sv->server->now());
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_poll(lux_hip_server* sv, int32_t block, uint64_t* done_ids, int32_t max_ids, int32_t* n_done) {
+    return guarded([&] {
+        require_init();
+        if (!sv || !n_done || max_ids < 0 || (max_ids > 0 && !done_ids))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "bad poll arguments");
+        sv->server->progress(block != 0, &sv->ready);
+        int32_t n = std::min<int32_t>(max_ids, int32_t(sv->ready.size()));
+        for (int32_t i = 0; i < n; ++i) done_ids[i] = sv->ready[i];
+        sv->ready.erase(sv->ready.begin(), sv->ready.begin() + n);
+        *n_done = n;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_result(lux_hip_server* sv, uint64_t request_id, float* film_sum, double* latency_s) {
+    return guarded([&] {
+        require_init();
+        if (!sv) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        double a = 0, ad = 0, f = 0;
+        sv->server->take_result(request_id, film_sum, &a, &ad, &f);
+        if (latency_s) *latency_s = f - a;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_stats(lux_hip_server* sv, lux_serve_stats* out) {
+    return guarded([&] {
+        require_init();
+        if (!sv || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        pt::ServeStats st = sv->server->stats();
+        out->submitted = st.submitted;
+        out->admitted = st.admitted;
+        out->completed = st.completed;
+        out->refits = st.refits;
+        out->batches = st.batches;
+        out->paths = st.paths;
+        out->resident_bytes = st.resident_bytes;
+        out->peak_bytes = st.peak_bytes;
+        sv->server->ray_counts(&out->extension_rays, &out->shadow_rays);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_server_run(lux_hip_server* sv, int32_t n, lux_hip_scene* const* scenes,
+                       const lux_render_settings* settings, const double* arrival_s,
+                       const lux_vec3* const* new_vertices, const int32_t* vertex_counts, float* const* films,
+                       double* latency_s, double* wall_s) {
+    return guarded([&] {
+        require_init();
+        if (!sv || n < 0 || (n > 0 && (!scenes || !settings || !arrival_s || !latency_s)))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "bad trace arguments");
+        if (new_vertices && !vertex_counts) throw Error(LUX_ERR_INVALID_ARGUMENT, "vertex counts missing");
+        std::vector<pt::SceneDev*> devs(n);
+        std::vector<int> counts(n, 0);
+        for (int32_t i = 0; i < n; ++i) {
+            if (!scenes[i]) throw Error(LUX_ERR_INVALID_ARGUMENT, "null scene in trace");
+            check_settings(&settings[i]);
+            devs[i] = &scenes[i]->dev;
+            if (new_vertices) counts[i] = vertex_counts[i];
+        }
+        pt::run_trace(*sv->server, n, devs.data(), settings, arrival_s, new_vertices, counts.data(), films, latency_s,
+                      wall_s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_set_render_tuning(const lux_render_tuning* t) {
+    return guarded([&] {
+        if (!t) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        if (t->capacity <= 0 || t->capacity > (1 << 28) || t->grid <= 0 || t->grid > (1 << 20))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "tuning out of range");
+        g_tuning.capacity = t->capacity;
+        g_tuning.grid = t->grid;
+        g_tuning.use_graph = t->use_graph ? 1 : 0;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_get_render_tuning(lux_render_tuning* t) {
+    return guarded([&] {
+        if (!t) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        t->capacity = g_tuning.capacity;
+        t->grid = g_tuning.grid;
+        t->use_graph = g_tuning.use_graph;
         return int(LUX_OK);
     });
 }

@@ -1,24 +1,33 @@
-// pt.hip -- wavefront path tracer for gfx950.
+// pt.hip -- wavefront path tracer for gfx950: kernels and the batch engine.
 //
 // Stages per bounce (src/integrator/wavefront_scheduler.cu:73-131 is the
 // reference's schedule; the transport rules are those of trace_path,
 // src/integrator/cpu_path_tracer.cpp:403-498):
-//   intersect  closest hit of every live path (right-first stackless LBVH walk)
+//   trace      closest hit of every live path (right-first stackless LBVH walk)
+//              and, in the same launch, the shadow rays of the previous bounce
 //   shade      emission + MIS, pass-through, termination, next-event estimation
 //              (power-alias light choice, area sample -> shadow queue), BSDF
 //              sample, Russian roulette -> next ray queue
-//   shadow     shadow rays through pass-through surfaces; adds the NEE term
 // then one flush per batch adds each path's radiance to its pixel, samples of a
 // pixel in sample order (so the film sum is the same float sum as the CPU's).
+// Tracing bounce b's shadow rays with bounce b+1's extension rays keeps the
+// order of the radiance additions (emission of b, NEE of b, emission of b+1)
+// and hides the long tail of a small shadow launch behind the extension rays.
+//
+// A batch is a list of segments, each a contiguous range of one request's
+// sample-major paths with its own scene (BVH root, mesh, lights, camera), so
+// one launch sequence serves many scenes. The device reads the batch from a
+// descriptor, which lets the whole bounce loop be captured once as a hipGraph
+// and replayed per batch after a small descriptor copy.
 //
 // Queues are path-index lists appended with one atomic per wave (ballot +
 // popcount). Two schedules share the kernels:
-//   plain      the reference's: 65 536-path batches, one sample per pixel per
-//              batch, full-grid launches, one shade launch per material type,
-//              a host synchronisation + counter read-back after every batch
-//   wavefront  large multi-sample batches, grid-stride kernels sized to the
-//              chip, one shade launch per bounce, no host round trip until the
-//              render ends
+//   plain      the reference's: 65 536-path single-sample batches, full-grid
+//              launches, one shade launch per material type, a separate shadow
+//              launch, a host synchronisation + counter read-back per batch
+//   wavefront  multi-sample, multi-request batches of up to `capacity` paths,
+//              grid-stride kernels sized to the chip, one shade launch per
+//              bounce, no host round trip until the caller synchronises
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,10 +41,12 @@ namespace pt {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kPlainBatch = 65536;  // RenderSettings::batch_size default (src/integrator/render_context.h:16-24)
 
 struct Queues {
     uint32_t* q[2];
-    uint32_t* counts;  // [0], [1] ray queues, [2] shadow queue
+    uint32_t* counts;          // [0], [1] ray queues, [2], [3] shadow queues by bounce parity
+    unsigned long long* rays;  // [0] extension rays, [1] shadow rays
 };
 
 // Path state, one slot per path of the batch (float4 records: one 16-byte load).
@@ -44,7 +55,7 @@ struct Paths {
     float4* L;      // radiance xyz, previous BSDF pdf
     float4* prev;   // previous vertex xyz, bits: flags | measure << 8 | depth << 16
     float4* ro;     // ray origin xyz, bits: pixel
-    float4* rd;     // ray direction xyz, bits: sample index
+    float4* rd;     // ray direction xyz, bits: segment of the batch
     ulonglong2* rng;
     float4* hit;    // bits: prim, t, u, v
     // shadow records, one per path at most per bounce
@@ -70,13 +81,16 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
 // skips one leaf's triangle test (the source primitive of a shadow ray).
 __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
                                        int ignore, float& t_hit, float& u_hit, float& v_hit) {
+    const lux_bvh_node* nodes = sv.nodes;
+    const lux_triangle* soup = sv.soup;
+    const int node_count = sv.node_count, tri_count = sv.tri_count;
     f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
     SlabRay sr = make_slab_ray(O, D, t_min);
     int best = -1;
     int k = 0;
-    while (uint32_t(k) < uint32_t(sv.node_count)) {
+    while (uint32_t(k) < uint32_t(node_count)) {
         float4 a, b;
-        load_node(sv.nodes, k, a, b);
+        load_node(nodes, k, a, b);
         int prim = __float_as_int(b.z);
         int esc = __float_as_int(b.w);
         int next;
@@ -85,10 +99,10 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
         } else if (prim >= 0) {
             next = esc;
             if (prim != ignore) {
-                if (uint32_t(prim) >= uint32_t(sv.tri_count)) {
+                if (uint32_t(prim) >= uint32_t(tri_count)) {
                     set_error(sv.info, kErrPrimRange);
                 } else {
-                    const float* p = reinterpret_cast<const float*>(sv.soup + prim);
+                    const float* p = reinterpret_cast<const float*>(soup + prim);
                     f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
                     float t, u, v;
                     if (tri_hit(O, D, t_min, t_max, v0, v1, v2, t, u, v)) {
@@ -115,23 +129,42 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
 __device__ __forceinline__ vec xyz(const float4& f) { return mk(f.x, f.y, f.z); }
 __device__ __forceinline__ float4 pack(const vec& v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-// ---- generate ------------------------------------------------------------------
-// Batch path j: local pixel j % npix (pixel = pix0 + local), sample s0 + j / npix.
-// Camera sample dimensions as render_cpu_path_tracer (src/integrator/cpu_path_tracer.cpp:532-542).
-__global__ __launch_bounds__(kBlock) void k_generate(SceneView sv, Paths ps, Queues qs, int width, int height,
-                                                     int pix0, int npix, int s0, int nsub, uint64_t seed, int frame,
-                                                     int sample_offset) {
-    const int total = npix * nsub;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        qs.counts[0] = uint32_t(total);
-        qs.counts[2] = 0;
+// Segment holding batch slot `slot` (segments are sorted by slot_begin and
+// cover [0, count) without gaps).
+__device__ __forceinline__ uint32_t find_segment(const BatchDesc* d, uint32_t slot) {
+    uint32_t lo = 0, hi = d->nseg;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d->seg[mid].slot_begin <= slot) lo = mid; else hi = mid;
     }
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
-        int local = j % npix;
-        int sub = j / npix;
-        int pixel = pix0 + local;
-        int sample = s0 + sub;
-        Rng rng = make_rng(seed, frame, pixel, sample + sample_offset);
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t segment_of_path(const Paths& ps, uint32_t path) {
+    return __float_as_uint(reinterpret_cast<const float*>(ps.rd + path)[3]);
+}
+
+// ---- generate ------------------------------------------------------------------
+// Batch slot j of segment s is request path r = req_begin + (j - slot_begin):
+// sample r / npix, pixel r % npix. Camera sample dimensions as
+// render_cpu_path_tracer (src/integrator/cpu_path_tracer.cpp:532-542).
+__global__ __launch_bounds__(kBlock) void k_generate(const BatchDesc* __restrict__ desc, Paths ps, Queues qs) {
+    const uint32_t total = desc->count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        qs.counts[0] = total;
+        qs.counts[1] = 0;
+        qs.counts[2] = 0;
+        qs.counts[3] = 0;
+    }
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
+        const uint32_t si = find_segment(desc, j);
+        const Segment& sg = desc->seg[si];
+        const int width = sg.width, height = sg.height;
+        const uint64_t npix = uint64_t(width) * uint64_t(height);
+        const uint64_t r = sg.req_begin + (j - sg.slot_begin);
+        const int sample = int(r / npix);
+        const int pixel = int(r - uint64_t(sample) * npix);
+        Rng rng = make_rng(sg.seed, sg.frame, pixel, sample + sg.sample_offset);
         int x = pixel % width, y = pixel / width;
         float jx = rng.get1();
         float jy = rng.get1();
@@ -140,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(SceneView sv, Paths ps, Que
         rng.get1();  // lens u
         rng.get1();  // lens v
         rng.get1();  // time
-        const DevCamera& c = sv.cam;
+        const DevCamera& c = sg.sv.cam;
         vec target = mk(c.lx, c.ly, c.lz) + u * mk(c.hx, c.hy, c.hz) + v * mk(c.vx, c.vy, c.vz);
         vec origin = mk(c.ox, c.oy, c.oz);
         vec dir = target - origin;
@@ -150,34 +183,86 @@ __global__ __launch_bounds__(kBlock) void k_generate(SceneView sv, Paths ps, Que
         ps.L[j] = make_float4(0, 0, 0, 0);
         ps.prev[j] = make_float4(0, 0, 0, __int_as_float(0));
         ps.ro[j] = pack(origin, __int_as_float(pixel));
-        ps.rd[j] = pack(dir, __int_as_float(sample));
+        ps.rd[j] = pack(dir, __uint_as_float(si));
         ps.rng[j] = make_ulonglong2(rng.state, rng.inc);
-        qs.q[0][j] = uint32_t(j);
+        qs.q[0][j] = j;
     }
 }
 
-// ---- intersect -----------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_intersect(SceneView sv, Paths ps, Queues qs, int in) {
-    const uint32_t count = qs.counts[in];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        qs.counts[in ^ 1] = 0;  // next-bounce queue; the previous bounce no longer reads it
-        qs.counts[2] = 0;       // shadow queue of this bounce
+// ---- shadow rays -------------------------------------------------------------------
+// trace_shadow (src/integrator/cpu_path_tracer.cpp:27-72): repeated closest hits
+// through pass-through surfaces; radiance += T * (contribution * Tr * mis).
+__device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, const Paths& ps, uint32_t i) {
+    float4 so = ps.so[i], sd = ps.sd[i], sc = ps.sc[i], st = ps.st[i];
+    const uint32_t path = __float_as_uint(sc.w);
+    const SceneView& sv = desc->seg[segment_of_path(ps, path)].sv;
+    vec o = xyz(so), d = xyz(sd);
+    float t_max = so.w, t_min = kEps;
+    int ignore = __float_as_int(st.w);
+    vec tr = splat(1);
+    for (int guard = 0; guard < 1 << 16; ++guard) {
+        float t = 0, u = 0, v = 0;
+        int prim = closest(sv, o, d, t_min, t_max, ignore, t, u, v);
+        if (prim < 0) break;
+        const DevMaterial& m = sv.mats[sv.mat_id[prim]];
+        if (!passthrough(m)) {
+            tr = splat(0);
+            break;
+        }
+        tr = tr * (m.type == kTransparent ? albedo_of(m) : splat(1));
+        if (max_component(tr) <= 0) break;
+        float adv = t + kEps;
+        if (adv >= t_max) break;
+        o = o + d * adv;
+        t_max -= adv;
+        t_min = kEps;
+        ignore = -1;
     }
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        uint32_t path = qs.q[in][i];
-        float4 o = ps.ro[path], d = ps.rd[path];
-        float t = INFINITY, u = 0, v = 0;
-        int prim = closest(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
-        ps.hit[path] = make_float4(__int_as_float(prim), t, u, v);
+    if (max_component(tr) > 0) {
+        float4 Lq = ps.L[path];
+        vec add = xyz(st) * (xyz(sc) * tr * sd.w);
+        ps.L[path] = pack(xyz(Lq) + add, Lq.w);
+    }
+}
+
+// ---- trace -----------------------------------------------------------------------
+// Extension rays of queue `ext_in` (-1: none) and then the shadow records
+// counted by counts[sh_in] (-1: none), in one grid-stride pass. Block 0 clears
+// the counters the following shade launch appends to (zero_a, zero_b; never
+// the ones read here) and adds the queue sizes to the ray totals.
+__global__ __launch_bounds__(kBlock) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
+                                                  int sh_in, int zero_a, int zero_b) {
+    const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
+    const uint32_t n_sh = sh_in >= 0 ? qs.counts[sh_in] : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (zero_a >= 0) qs.counts[zero_a] = 0;
+        if (zero_b >= 0) qs.counts[zero_b] = 0;
+        qs.rays[0] += n_ext;
+        qs.rays[1] += n_sh;
+    }
+    const uint32_t total = n_ext + n_sh;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < total; i += gridDim.x * kBlock) {
+        if (i < n_ext) {
+            const uint32_t path = qs.q[ext_in][i];
+            const float4 o = ps.ro[path], d = ps.rd[path];
+            const SceneView& sv = desc->seg[__float_as_uint(d.w)].sv;
+            float t = INFINITY, u = 0, v = 0;
+            int prim = closest(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
+            ps.hit[path] = make_float4(__int_as_float(prim), t, u, v);
+        } else {
+            shadow_ray(desc, ps, i - n_ext);
+        }
     }
 }
 
 // ---- shade -----------------------------------------------------------------------
-// One bounce of trace_path for every live path. `filter` >= 0 restricts the
-// launch to hits of that material type (misses go with filter 0).
-__global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Paths ps, Queues qs, int in, int bounce,
-                                                  int max_depth, int filter) {
+// One bounce of trace_path for every live path of queue `in`. `filter` >= 0
+// restricts the launch to hits of that material type (misses go with filter 0).
+// Next rays go to queue in ^ 1, shadow records to counts[2 + (bounce & 1)].
+__global__ __launch_bounds__(kBlock) void k_shade(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int in,
+                                                  int bounce, int filter) {
     const uint32_t count = qs.counts[in];
+    uint32_t* const sh_count = qs.counts + 2 + (bounce & 1);
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t rounds = (count + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {  // uniform trip count: ballots see the whole wave
@@ -187,13 +272,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Paths ps, Queues
         bool push_ray = false, push_shadow = false;
         float4 hit = live ? ps.hit[path] : make_float4(__int_as_float(-1), 0, 0, 0);
         int prim = __float_as_int(hit.x);
+        const Segment* sg = live ? &desc->seg[segment_of_path(ps, path)] : nullptr;
         const DevMaterial* mp = nullptr;
-        if (live && prim >= 0) mp = &sv.mats[sv.mat_id[prim]];
+        if (live && prim >= 0) mp = &sg->sv.mats[sg->sv.mat_id[prim]];
         if (live && filter >= 0) live = (prim < 0) ? filter == 0 : mp->type == filter;
         // NEE record and next ray, filled below
         vec sh_o = splat(0), sh_d = splat(0), sh_c = splat(0), sh_T = splat(0);
         float sh_tmax = 0, sh_mis = 0;
         if (live && prim >= 0) {
+            const SceneView& sv = sg->sv;
+            const int max_depth = sg->max_depth;
             const DevMaterial m = *mp;
             float4 Tq = ps.T[path], Lq = ps.L[path], pq = ps.prev[path], oq = ps.ro[path], dq = ps.rd[path];
             ulonglong2 rs = ps.rng[path];
@@ -349,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Paths ps, Queues
         }
         uint32_t slot = wave_append(&qs.counts[in ^ 1], push_ray);
         if (push_ray) qs.q[in ^ 1][slot] = path;
-        uint32_t sslot = wave_append(&qs.counts[2], push_shadow);
+        uint32_t sslot = wave_append(sh_count, push_shadow);
         if (push_shadow) {
             ps.so[sslot] = pack(sh_o, sh_tmax);
             ps.sd[sslot] = pack(sh_d, sh_mis);
@@ -359,63 +447,40 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Paths ps, Queues
     }
 }
 
-// ---- shadow ------------------------------------------------------------------------
-// trace_shadow (src/integrator/cpu_path_tracer.cpp:27-72): repeated closest hits
-// through pass-through surfaces; radiance += T * (contribution * Tr * mis).
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Paths ps, Queues qs) {
-    const uint32_t count = qs.counts[2];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        float4 so = ps.so[i], sd = ps.sd[i], sc = ps.sc[i], st = ps.st[i];
-        vec o = xyz(so), d = xyz(sd);
-        float t_max = so.w, t_min = kEps;
-        int ignore = __float_as_int(st.w);
-        vec tr = splat(1);
-        for (int guard = 0; guard < 1 << 16; ++guard) {
-            float t = 0, u = 0, v = 0;
-            int prim = closest(sv, o, d, t_min, t_max, ignore, t, u, v);
-            if (prim < 0) break;
-            const DevMaterial& m = sv.mats[sv.mat_id[prim]];
-            if (!passthrough(m)) {
-                tr = splat(0);
-                break;
-            }
-            tr = tr * (m.type == kTransparent ? albedo_of(m) : splat(1));
-            if (max_component(tr) <= 0) break;
-            float adv = t + kEps;
-            if (adv >= t_max) break;
-            o = o + d * adv;
-            t_max -= adv;
-            t_min = kEps;
-            ignore = -1;
-        }
-        if (max_component(tr) > 0) {
-            uint32_t path = __float_as_uint(sc.w);
-            float4 Lq = ps.L[path];
-            vec add = xyz(st) * (xyz(sc) * tr * sd.w);
-            ps.L[path] = pack(xyz(Lq) + add, Lq.w);
-        }
-    }
-}
-
 // ---- flush ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_flush(Paths ps, float* film, int pix0, int npix, int nsub) {
-    for (int local = blockIdx.x * kBlock + threadIdx.x; local < npix; local += gridDim.x * kBlock) {
-        float* f = film + 3 * size_t(pix0 + local);
-        float r = f[0], g = f[1], b = f[2];
-        for (int sub = 0; sub < nsub; ++sub) {
-            float4 L = ps.L[size_t(sub) * npix + local];
-            r += L.x;
-            g += L.y;
-            b += L.z;
+// The first slot of each (segment, pixel) sums that pixel's samples of the
+// segment in sample order onto the film; a request's sample 0 overwrites the
+// film (0 + x == x for the non-negative radiance sums), later samples add.
+__global__ __launch_bounds__(kBlock) void k_flush(const BatchDesc* __restrict__ desc, Paths ps) {
+    const uint32_t total = desc->count;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
+        const Segment& sg = desc->seg[find_segment(desc, j)];
+        const uint32_t local = j - sg.slot_begin;
+        const uint64_t npix = uint64_t(sg.width) * uint64_t(sg.height);
+        if (local >= npix) continue;
+        const uint64_t r = sg.req_begin + local;
+        const uint64_t pixel = r % npix;
+        float* f = sg.film + 3 * pixel;
+        float a = 0, b = 0, c = 0;
+        if (r >= npix) {
+            a = f[0];
+            b = f[1];
+            c = f[2];
         }
-        f[0] = r;
-        f[1] = g;
-        f[2] = b;
+        for (uint64_t k = local; k < sg.count; k += npix) {
+            float4 L = ps.L[sg.slot_begin + k];
+            a += L.x;
+            b += L.y;
+            c += L.z;
+        }
+        f[0] = a;
+        f[1] = b;
+        f[2] = c;
     }
 }
 
-__global__ void k_zero_film(float* film, size_t n) {
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) film[i] = 0;
+__global__ void k_zero_rays(unsigned long long* rays) {
+    if (threadIdx.x < 2) rays[threadIdx.x] = 0;
 }
 
 template <typename T>
@@ -426,275 +491,6 @@ T* dalloc(size_t count) {
     return static_cast<T*>(p);
 }
 
-}  // namespace
-
-// ---- host: scene preparation ------------------------------------------------------
-namespace host {
-
-struct V {
-    float x, y, z;
-};
-inline V mkv(float x, float y, float z) { return V{x, y, z}; }
-inline V vsub(V a, V b) { return mkv(a.x - b.x, a.y - b.y, a.z - b.z); }
-inline V vadd(V a, V b) { return mkv(a.x + b.x, a.y + b.y, a.z + b.z); }
-inline V vmul(V a, float s) { return mkv(a.x * s, a.y * s, a.z * s); }
-inline float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline V vcross(V a, V b) { return mkv(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-inline V vnorm(V v) {
-    float len = sqrtf(vdot(v, v));
-    if (!(len > 0)) return v;
-    float inv = 1 / len;
-    return vmul(v, inv);
-}
-
-// make_perspective_camera with lens radius 0 and focal distance 1
-// (src/camera/camera.h:58-109, src/camera/pinhole_camera.h:13-17).
-DevCamera make_camera(const float* c) {
-    V origin = mkv(c[0], c[1], c[2]), look = mkv(c[3], c[4], c[5]);
-    float vfov = c[6], aspect = c[7];
-    V forward = vnorm(vsub(look, origin));
-    V right = vnorm(vcross(mkv(0, 1, 0), forward));
-    if (vdot(right, right) == 0) right = vnorm(vcross(mkv(0, 0, 1), forward));
-    V up = vcross(forward, right);
-    float theta = vfov * kPi / 180.0f;
-    float vh = 2.0f * tanf(theta * 0.5f);
-    float vw = aspect * vh;
-    float focus = 1.0f;
-    V horizontal = vmul(right, vw * focus);
-    V vertical = vmul(up, vh * focus);
-    V lower = vsub(vsub(vadd(origin, vmul(forward, focus)), vmul(horizontal, 0.5f)), vmul(vertical, 0.5f));
-    return DevCamera{origin.x, origin.y, origin.z, lower.x, lower.y, lower.z,
-                     horizontal.x, horizontal.y, horizontal.z, vertical.x, vertical.y, vertical.z};
-}
-
-// transform_instance_normal under the identity instance: adjugate product then
-// two normalisations (src/scene/scene_data.h:262-268, src/core/mat4.cuh:209-229).
-V instance_normal(V n) {
-    if (vdot(n, n) == 0) return mkv(0, 0, 0);
-    V t = mkv(1.0f * n.x + 0.0f * n.y + 0.0f * n.z, 0.0f * n.x + 1.0f * n.y + 0.0f * n.z,
-              0.0f * n.x + 0.0f * n.y + 1.0f * n.z);
-    V once = vnorm(t);
-    if (vdot(once, once) == 0) return mkv(0, 0, 0);
-    return vnorm(once);
-}
-
-// Power light selection with its alias table: prepare_scene_light_sampling
-// (src/scene/scene_prepare.h:268-387) and build_alias_table_from_weights
-// (src/light/light_distribution.cpp:8-64; same stack order, so the same aliases).
-void build_lights(const std::vector<lux_triangle>& soup, const int32_t* mat_ids, const lux_material* mats,
-                  std::vector<DevLight>& lights, std::vector<int32_t>& light_of_prim) {
-    int n = int(soup.size());
-    light_of_prim.assign(n, -1);
-    std::vector<float> cdf;
-    float total = 0;
-    for (int t = 0; t < n; ++t) {
-        const lux_material& m = mats[mat_ids[t]];
-        float emax = fmaxf(m.emission.x, fmaxf(m.emission.y, m.emission.z));
-        if (!(emax > 0)) continue;
-        V v0 = mkv(soup[t].v0.x, soup[t].v0.y, soup[t].v0.z);
-        V v1 = mkv(soup[t].v1.x, soup[t].v1.y, soup[t].v1.z);
-        V v2 = mkv(soup[t].v2.x, soup[t].v2.y, soup[t].v2.z);
-        V c = vcross(vsub(v1, v0), vsub(v2, v0));
-        float area = 0.5f * sqrtf(vdot(c, c));
-        if (area <= 0) continue;
-        float lum = float(0.2126) * m.emission.x + float(0.7152) * m.emission.y + float(0.0722) * m.emission.z;
-        float power = area * fmaxf(1e-6f, lum);
-        if (power <= 0) continue;
-        total += power;
-        DevLight L{};
-        L.prim = t;
-        L.er = m.emission.x;
-        L.eg = m.emission.y;
-        L.eb = m.emission.z;
-        L.two_sided = m.emission_two_sided;
-        L.area = area;
-        lights.push_back(L);
-        cdf.push_back(total);
-        int32_t& stored = light_of_prim[t];
-        stored = stored == -1 ? int32_t(lights.size()) - 1 : -2;
-    }
-    int count = int(lights.size());
-    if (count == 0 || !(total > 0)) {
-        lights.clear();
-        return;
-    }
-    std::vector<float> w(count);
-    float prev = 0;
-    for (int i = 0; i < count; ++i) {
-        w[i] = cdf[i] - prev;
-        prev = cdf[i];
-    }
-    float sum = 0;
-    for (float x : w) sum += fmaxf(0.0f, x);
-    std::vector<float> scaled(count);
-    std::vector<int> small, large;
-    for (int i = 0; i < count; ++i) {
-        float pmf = fmaxf(0.0f, w[i]) / sum;
-        lights[i].pmf = pmf;
-        lights[i].alias = i;
-        lights[i].prob = 1;
-        scaled[i] = pmf * float(count);
-        (scaled[i] < 1.0f ? small : large).push_back(i);
-    }
-    while (!small.empty() && !large.empty()) {
-        int l = small.back();
-        small.pop_back();
-        int g = large.back();
-        large.pop_back();
-        lights[l].prob = scaled[l];
-        lights[l].alias = g;
-        scaled[g] = (scaled[g] + scaled[l]) - 1.0f;
-        (scaled[g] < 1.0f ? small : large).push_back(g);
-    }
-    for (int i : large) {
-        lights[i].prob = 1;
-        lights[i].alias = i;
-    }
-    for (int i : small) {
-        lights[i].prob = 1;
-        lights[i].alias = i;
-    }
-}
-
-}  // namespace host
-
-void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
-    int nt = desc.triangle_count, nv = desc.vertex_count, nm = desc.material_count;
-    if (nt <= 0 || nv <= 0 || nm <= 0 || !desc.vertices || !desc.indices || !desc.material_ids || !desc.materials)
-        throw Error(LUX_ERR_INVALID_ARGUMENT, "scene needs vertices, indices, material ids and materials");
-    for (int t = 0; t < nt; ++t) {
-        for (int k = 0; k < 3; ++k)
-            if (desc.indices[3 * size_t(t) + k] >= uint32_t(nv))
-                throw Error(LUX_ERR_INVALID_ARGUMENT, "triangle index out of range");
-        if (desc.material_ids[t] < 0 || desc.material_ids[t] >= nm)
-            throw Error(LUX_ERR_INVALID_ARGUMENT, "material id out of range");
-    }
-    std::vector<lux_triangle> soup(nt);
-    std::vector<lux_aabb> boxes(nt);
-    std::vector<int32_t> ids(nt);
-    for (int t = 0; t < nt; ++t) {
-        const uint32_t* ix = desc.indices + 3 * size_t(t);
-        soup[t] = lux_triangle{desc.vertices[ix[0]], desc.vertices[ix[1]], desc.vertices[ix[2]]};
-        lux_aabb b;
-        b.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
-        b.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
-        for (const lux_vec3& p : {soup[t].v0, soup[t].v1, soup[t].v2}) {
-            b.p_min = lux_vec3{fminf(b.p_min.x, p.x), fminf(b.p_min.y, p.y), fminf(b.p_min.z, p.z)};
-            b.p_max = lux_vec3{fmaxf(b.p_max.x, p.x), fmaxf(b.p_max.y, p.y), fmaxf(b.p_max.z, p.z)};
-        }
-        boxes[t] = b;
-        ids[t] = t;
-    }
-    std::vector<float> normals;
-    if (desc.normals) {
-        normals.resize(3 * size_t(nv));
-        for (int v = 0; v < nv; ++v) {
-            host::V n = host::instance_normal(host::mkv(desc.normals[v].x, desc.normals[v].y, desc.normals[v].z));
-            normals[3 * v] = n.x;
-            normals[3 * v + 1] = n.y;
-            normals[3 * v + 2] = n.z;
-        }
-    }
-    std::vector<DevMaterial> mats(nm);
-    for (int m = 0; m < nm; ++m) {
-        const lux_material& in = desc.materials[m];
-        if (in.type < 0 || in.type >= kMaterialTypes) throw Error(LUX_ERR_INVALID_ARGUMENT, "unknown material type");
-        mats[m] = DevMaterial{in.type, in.albedo.x, in.albedo.y, in.albedo.z, in.emission.x, in.emission.y,
-                              in.emission.z, in.emission_two_sided, in.roughness, in.metallic, in.eta, 0};
-    }
-    std::vector<DevLight> lights;
-    std::vector<int32_t> light_of_prim;
-    host::build_lights(soup, desc.material_ids, desc.materials, lights, light_of_prim);
-
-    scene_release(s);
-    s.tri_count = nt;
-    s.vertex_count = nv;
-    s.soup = dalloc<lux_triangle>(nt);
-    s.indices = dalloc<uint32_t>(3 * size_t(nt));
-    s.mat_id = dalloc<int32_t>(nt);
-    s.mats = dalloc<DevMaterial>(nm);
-    s.mat_count = nm;
-    s.light_of_prim = dalloc<int32_t>(nt);
-    s.lights = dalloc<DevLight>(std::max<size_t>(lights.size(), 1));
-    s.light_count = int(lights.size());
-    s.host_lights = lights;
-    s.host_light_of_prim = light_of_prim;
-    if (desc.normals) s.normals = dalloc<float>(3 * size_t(nv));
-    s.camera = host::make_camera(desc.camera);
-    hip_check(hipMemcpyAsync(s.soup, soup.data(), sizeof(lux_triangle) * nt, hipMemcpyHostToDevice, stream), "H2D soup");
-    hip_check(hipMemcpyAsync(s.indices, desc.indices, 12 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D indices");
-    hip_check(hipMemcpyAsync(s.mat_id, desc.material_ids, 4 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D mat ids");
-    hip_check(hipMemcpyAsync(s.mats, mats.data(), sizeof(DevMaterial) * nm, hipMemcpyHostToDevice, stream), "H2D mats");
-    hip_check(hipMemcpyAsync(s.light_of_prim, light_of_prim.data(), 4 * size_t(nt), hipMemcpyHostToDevice, stream),
-              "H2D light map");
-    if (!lights.empty())
-        hip_check(hipMemcpyAsync(s.lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice,
-                                 stream),
-                  "H2D lights");
-    if (desc.normals)
-        hip_check(hipMemcpyAsync(s.normals, normals.data(), 4 * normals.size(), hipMemcpyHostToDevice, stream),
-                  "H2D normals");
-    // BLAS over the triangle boxes (ids = triangle index)
-    lux_aabb* d_boxes = dalloc<lux_aabb>(nt);
-    int32_t* d_ids = dalloc<int32_t>(nt);
-    hip_check(hipMemcpyAsync(d_boxes, boxes.data(), sizeof(lux_aabb) * nt, hipMemcpyHostToDevice, stream), "H2D boxes");
-    hip_check(hipMemcpyAsync(d_ids, ids.data(), 4 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D ids");
-    lbvh_build(s.bvh, d_boxes, d_ids, nt, stream);
-    hip_check(hipStreamSynchronize(stream), "scene upload");
-    (void)hipFree(d_boxes);
-    (void)hipFree(d_ids);
-}
-
-void scene_release(SceneDev& s) {
-    void* ptrs[] = {s.soup, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    bvh_release(s.bvh);
-    SceneDev empty;
-    s = empty;
-}
-
-SceneView scene_view(const SceneDev& s) {
-    SceneView v{};
-    v.nodes = s.bvh.nodes;
-    v.node_count = s.bvh.n > 0 ? 2 * s.bvh.n - 1 : 0;
-    v.soup = s.soup;
-    v.tri_count = s.tri_count;
-    v.indices = s.indices;
-    v.normals = s.normals;
-    v.mat_id = s.mat_id;
-    v.mats = s.mats;
-    v.mat_count = s.mat_count;
-    v.lights = s.lights;
-    v.light_count = s.light_count;
-    v.light_of_prim = s.light_of_prim;
-    v.cam = s.camera;
-    v.info = s.bvh.info;
-    return v;
-}
-
-// ---- host: render -------------------------------------------------------------------
-void pathbuf_allocate(PathBuf& b, int capacity) {
-    if (capacity <= b.capacity) return;
-    pathbuf_release(b);
-    b.capacity = capacity;
-    size_t P = size_t(capacity);
-    b.f4 = dalloc<float4>(11 * P);
-    b.rng = dalloc<ulonglong2>(P);
-    b.q = dalloc<uint32_t>(2 * P);
-    b.counts = dalloc<uint32_t>(4);
-    hip_check(hipMemset(b.counts, 0, 16), "hipMemset counts");
-}
-
-void pathbuf_release(PathBuf& b) {
-    void* ptrs[] = {b.f4, b.rng, b.q, b.counts};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    PathBuf empty;
-    b = empty;
-}
-
-namespace {
 Paths paths_of(const PathBuf& b) {
     size_t P = size_t(b.capacity);
     Paths p;
@@ -711,64 +507,302 @@ Paths paths_of(const PathBuf& b) {
     p.rng = b.rng;
     return p;
 }
+
 Queues queues_of(const PathBuf& b) {
     Queues q;
     q.q[0] = b.q;
     q.q[1] = b.q + b.capacity;
     q.counts = b.counts;
+    q.rays = b.rays;
     return q;
+}
+
+void pathbuf_release(PathBuf& b) {
+    void* ptrs[] = {b.f4, b.rng, b.q, b.counts, b.rays, b.desc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    b = PathBuf{};
+}
+
+Segment make_segment(const Request& rq, uint64_t req_begin, uint32_t slot_begin, uint32_t count) {
+    Segment s{};
+    s.sv = rq.sv;
+    s.film = rq.film;
+    s.seed = rq.rs.seed;
+    s.req_begin = req_begin;
+    s.slot_begin = slot_begin;
+    s.count = count;
+    s.width = rq.rs.width;
+    s.height = rq.rs.height;
+    s.max_depth = rq.rs.max_depth;
+    s.frame = rq.rs.frame_index;
+    s.sample_offset = rq.rs.sample_offset;
+    return s;
+}
+
+}  // namespace
+
+uint64_t request_paths(const lux_render_settings& rs) {
+    return uint64_t(rs.width) * uint64_t(rs.height) * uint64_t(rs.spp);
+}
+
+// ---- host: the batch engine ------------------------------------------------------------
+Renderer::Renderer() = default;
+
+Renderer::~Renderer() {
+    release_graph();
+    if (buf_.capacity) (void)hipDeviceSynchronize();
+    pathbuf_release(buf_);
+    for (int k = 0; k < kRing; ++k)
+        if (ring_ev_[k]) (void)hipEventDestroy(ring_ev_[k]);
+    if (ring_) (void)hipHostFree(ring_);
+}
+
+void Renderer::set_tuning(const Tuning& t) {
+    if (t.capacity <= 0 || t.grid <= 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "tuning needs capacity > 0 and grid > 0");
+    tune_ = t;
+}
+
+void Renderer::release_graph() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+    graph_depth_ = graph_grid_ = graph_capacity_ = -1;
+}
+
+void Renderer::ensure_capacity(int paths) {
+    if (paths <= buf_.capacity) return;
+    release_graph();  // the captured launches hold the old buffer addresses
+    if (buf_.capacity) hip_check(hipDeviceSynchronize(), "synchronize before path buffer growth");
+    pathbuf_release(buf_);
+    size_t P = size_t(paths);
+    buf_.capacity = paths;
+    buf_.f4 = dalloc<float4>(11 * P);
+    buf_.rng = dalloc<ulonglong2>(P);
+    buf_.q = dalloc<uint32_t>(2 * P);
+    buf_.counts = dalloc<uint32_t>(4);
+    buf_.rays = dalloc<unsigned long long>(2);
+    buf_.desc = static_cast<BatchDesc*>(static_cast<void*>(dalloc<uint8_t>(sizeof(BatchDesc))));
+    hip_check(hipMemset(buf_.counts, 0, 16), "hipMemset counts");
+    hip_check(hipMemset(buf_.rays, 0, 16), "hipMemset rays");
+}
+
+namespace {
+size_t pathbuf_bytes(int capacity) {
+    size_t P = size_t(capacity);
+    return P ? P * (11 * 16 + 16 + 8) + 16 + 16 + sizeof(BatchDesc) : 0;
 }
 }  // namespace
 
-void render(const SceneDev& s, PathBuf& buf, const lux_render_settings& rs, float* film, hipStream_t stream,
-            RenderStats* stats) {
-    const int w = rs.width, h = rs.height, spp = rs.spp, depth = rs.max_depth;
-    const int npix_total = w * h;
-    const bool plain = rs.mode == LUX_RENDER_PLAIN;
-    SceneView sv = scene_view(s);
-    LUX_LAUNCH("pt_zero_film", k_zero_film, div_up(3 * int64_t(npix_total), kBlock) > 4096 ? 4096
-               : div_up(3 * int64_t(npix_total), kBlock), kBlock, 0, stream, film, size_t(3) * npix_total);
-    // batch geometry: plain = one sample of up to batch_size pixels; wavefront =
-    // as many samples of the whole image as fit the path budget
-    int budget = rs.batch_size > 0 ? rs.batch_size : (plain ? 65536 : (1 << 21));
-    int npix = std::min(npix_total, budget);
-    int nsub = plain ? 1 : std::max(1, std::min(spp, budget / std::max(npix, 1)));
-    pathbuf_allocate(buf, npix * nsub);
-    Paths ps = paths_of(buf);
-    Queues qs = queues_of(buf);
-    int fixed_grid = std::min(div_up(int64_t(npix) * nsub, kBlock), 2048);
-    std::vector<uint32_t> host_counts(4);
-    for (int s0 = 0; s0 < spp; s0 += nsub) {
-        int sub = std::min(nsub, spp - s0);
-        for (int pix0 = 0; pix0 < npix_total; pix0 += npix) {
-            int np = std::min(npix, npix_total - pix0);
-            int batch = np * sub;
-            int grid = plain ? div_up(batch, kBlock) : std::min(div_up(batch, kBlock), fixed_grid);
-            LUX_LAUNCH("pt_generate", k_generate, grid, kBlock, 0, stream, sv, ps, qs, w, h, pix0, np, s0, sub,
-                       rs.seed, rs.frame_index, rs.sample_offset);
-            for (int b = 0; b <= depth; ++b) {
-                int in = b & 1;
-                LUX_LAUNCH("pt_intersect", k_intersect, grid, kBlock, 0, stream, sv, ps, qs, in);
-                if (plain) {
-                    for (int type = 0; type < kMaterialTypes; ++type)
-                        LUX_LAUNCH("pt_shade", k_shade, grid, kBlock, 0, stream, sv, ps, qs, in, b, depth, type);
-                } else {
-                    LUX_LAUNCH("pt_shade", k_shade, grid, kBlock, 0, stream, sv, ps, qs, in, b, depth, -1);
+size_t Renderer::device_bytes() const { return pathbuf_bytes(buf_.capacity); }
+
+size_t Renderer::planned_bytes() const { return pathbuf_bytes(std::max(buf_.capacity, tune_.capacity)); }
+
+BatchDesc* Renderer::next_slot() {
+    if (!ring_) {
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, sizeof(BatchDesc) * kRing, hipHostMallocDefault), "hipHostMalloc batch ring");
+        ring_ = static_cast<BatchDesc*>(p);
+        for (int k = 0; k < kRing; ++k)
+            hip_check(hipEventCreateWithFlags(&ring_ev_[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    int k = ring_next_;
+    if (ring_used_[k]) hip_check(hipEventSynchronize(ring_ev_[k]), "wait for batch ring slot");
+    return ring_ + k;
+}
+
+void Renderer::upload(BatchDesc* slot, hipStream_t stream) {
+    int k = int(slot - ring_);
+    hip_check(hipMemcpyAsync(buf_.desc, slot, desc_bytes(slot->nseg), hipMemcpyHostToDevice, stream), "H2D batch");
+    hip_check(hipEventRecord(ring_ev_[k], stream), "hipEventRecord");
+    ring_used_[k] = true;
+    ring_next_ = (k + 1) % kRing;
+}
+
+void Renderer::reset_ray_counts(hipStream_t stream) {
+    ensure_capacity(1);
+    LUX_LAUNCH("pt_zero_rays", k_zero_rays, 1, 64, 0, stream, buf_.rays);
+}
+
+void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* shadow) {
+    unsigned long long r[2] = {0, 0};
+    if (buf_.rays) {
+        hip_check(hipMemcpyAsync(r, buf_.rays, sizeof(r), hipMemcpyDeviceToHost, stream), "D2H ray counts");
+        hip_check(hipStreamSynchronize(stream), "synchronize");
+    }
+    *extension = r[0];
+    *shadow = r[1];
+}
+
+namespace {
+// The wavefront launch sequence of one batch (captured or launched directly).
+// Raw launches (no probe events) when captured.
+template <bool kProbe>
+void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+    const Paths ps = paths_of(buf);
+    const Queues qs = queues_of(buf);
+    const BatchDesc* d = buf.desc;
+#define PT_LAUNCH(NAME, K, ...)                                                      \
+    do {                                                                             \
+        if (kProbe) {                                                                \
+            LUX_LAUNCH(NAME, K, grid, kBlock, 0, stream, __VA_ARGS__);               \
+        } else {                                                                     \
+            hipLaunchKernelGGL(K, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__); \
+            hip_check(hipGetLastError(), NAME);                                      \
+        }                                                                            \
+    } while (0)
+    PT_LAUNCH("pt_generate", k_generate, d, ps, qs);
+    for (int b = 0; b <= depth; ++b) {
+        const int in = b & 1;
+        PT_LAUNCH("pt_trace", k_trace, d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1, 2 + (b & 1));
+        PT_LAUNCH("pt_shade", k_shade, d, ps, qs, in, b, -1);
+    }
+    PT_LAUNCH("pt_trace", k_trace, d, ps, qs, -1, 2 + (depth & 1), -1, -1);
+    PT_LAUNCH("pt_flush", k_flush, d, ps);
+#undef PT_LAUNCH
+}
+}  // namespace
+
+void Renderer::launch_wavefront(int depth, hipStream_t stream) {
+    const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, kBlock)));
+    if (!tune_.use_graph) {
+        wavefront_sequence<true>(buf_, depth, grid, stream);
+        return;
+    }
+    if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity) {
+        release_graph();
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+            wavefront_sequence<false>(buf_, depth, grid, stream);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+        graph_depth_ = depth;
+        graph_grid_ = grid;
+        graph_capacity_ = buf_.capacity;
+    }
+    hip_check(hipGraphLaunch(graph_, stream), "hipGraphLaunch");
+}
+
+void Renderer::launch_plain(int depth, uint32_t count, hipStream_t stream) {
+    const Paths ps = paths_of(buf_);
+    const Queues qs = queues_of(buf_);
+    const BatchDesc* d = buf_.desc;
+    const int grid = std::max(1, div_up(count, kBlock));
+    LUX_LAUNCH("pt_generate", k_generate, grid, kBlock, 0, stream, d, ps, qs);
+    for (int b = 0; b <= depth; ++b) {
+        const int in = b & 1;
+        LUX_LAUNCH("pt_trace", k_trace, grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1, 2 + (b & 1));
+        for (int type = 0; type < kMaterialTypes; ++type)
+            LUX_LAUNCH("pt_shade", k_shade, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
+        LUX_LAUNCH("pt_shadow", k_trace, grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1, -1);
+    }
+    LUX_LAUNCH("pt_flush", k_flush, grid, kBlock, 0, stream, d, ps);
+}
+
+uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
+                                 int64_t* batch_index) {
+    const uint64_t cap = uint64_t(tune_.capacity);
+    ensure_capacity(tune_.capacity);
+    BatchDesc* slot = next_slot();
+    uint64_t used = 0;
+    uint32_t nseg = 0;
+    for (Request* rq : active) {
+        if (used >= cap || nseg >= uint32_t(kMaxSegments)) break;
+        if (rq->issued >= rq->total) continue;
+        uint64_t take = std::min(rq->total - rq->issued, cap - used);
+        if (slice_spp > 0)
+            take = std::min(take, uint64_t(slice_spp) * uint64_t(rq->rs.width) * uint64_t(rq->rs.height));
+        slot->seg[nseg++] = make_segment(*rq, rq->issued, uint32_t(used), uint32_t(take));
+        rq->issued += take;
+        used += take;
+        if (rq->issued == rq->total) rq->last_batch = batch_counter_;
+    }
+    if (batch_index) *batch_index = batch_counter_;
+    if (used == 0) return 0;
+    slot->count = uint32_t(used);
+    slot->nseg = nseg;
+    upload(slot, stream);
+    launch_wavefront(max_depth, stream);
+    ++batch_counter_;
+    return used;
+}
+
+void Renderer::render_all(std::vector<Request*>& reqs, int mode, hipStream_t stream, RenderStats* stats) {
+    for (Request* rq : reqs) {
+        rq->total = request_paths(rq->rs);
+        rq->issued = 0;
+        rq->last_batch = -1;
+    }
+    if (mode == LUX_RENDER_PLAIN) {
+        // The reference's loop: for each sample, 65 536-pixel batches, each a
+        // complete bounce loop followed by a synchronisation.
+        for (Request* rq : reqs) {
+            const uint64_t npix = uint64_t(rq->rs.width) * uint64_t(rq->rs.height);
+            const uint64_t B = uint64_t(rq->rs.batch_size > 0 ? rq->rs.batch_size : kPlainBatch);
+            ensure_capacity(int(std::min(B, npix)));
+            uint32_t host_counts[4];
+            for (int s = 0; s < rq->rs.spp; ++s) {
+                for (uint64_t p0 = 0; p0 < npix; p0 += B) {
+                    const uint32_t n = uint32_t(std::min(B, npix - p0));
+                    BatchDesc* slot = next_slot();
+                    slot->count = n;
+                    slot->nseg = 1;
+                    slot->seg[0] = make_segment(*rq, uint64_t(s) * npix + p0, 0, n);
+                    upload(slot, stream);
+                    launch_plain(rq->rs.max_depth, n, stream);
+                    hip_check(hipMemcpyAsync(host_counts, buf_.counts, sizeof(host_counts), hipMemcpyDeviceToHost,
+                                             stream),
+                              "D2H counts");
+                    hip_check(hipStreamSynchronize(stream), "batch synchronize");
+                    ++batch_counter_;
+                    if (stats) {
+                        stats->batches += 1;
+                        stats->paths += n;
+                    }
                 }
-                LUX_LAUNCH("pt_shadow", k_shadow, grid, kBlock, 0, stream, sv, ps, qs);
             }
-            LUX_LAUNCH("pt_flush", k_flush, std::min(div_up(np, kBlock), 4096), kBlock, 0, stream, ps, film, pix0, np,
-                       sub);
+            rq->issued = rq->total;
+            rq->last_batch = batch_counter_ - 1;
+        }
+        return;
+    }
+    // wavefront: the caller's batch size (if any) caps this call's batches
+    Tuning saved = tune_;
+    uint64_t sum = 0;
+    int depth = 0;
+    int forced = 0;
+    for (Request* rq : reqs) {
+        sum += rq->total;
+        depth = std::max(depth, rq->rs.max_depth);
+        if (rq->rs.batch_size > 0) forced = forced ? std::min(forced, rq->rs.batch_size) : rq->rs.batch_size;
+    }
+    if (sum == 0) return;
+    Tuning t = tune_;
+    if (forced) t.capacity = forced;
+    t.capacity = int(std::min<uint64_t>(uint64_t(t.capacity), sum));
+    t.use_graph = mode == LUX_RENDER_WAVEFRONT ? tune_.use_graph : 0;
+    tune_ = t;
+    try {
+        for (;;) {
+            uint64_t n = enqueue_batch(reqs, 0, depth, stream, nullptr);
+            if (!n) break;
             if (stats) {
                 stats->batches += 1;
-                stats->paths += uint64_t(batch);
-            }
-            if (plain) {  // the reference's per-batch synchronisation and counter read-back
-                hip_check(hipMemcpyAsync(host_counts.data(), buf.counts, 16, hipMemcpyDeviceToHost, stream), "D2H counts");
-                hip_check(hipStreamSynchronize(stream), "batch synchronize");
+                stats->paths += n;
             }
         }
+    } catch (...) {
+        tune_ = saved;
+        throw;
     }
+    tune_ = saved;
 }
 
 }  // namespace pt

@@ -1,8 +1,11 @@
 // pt_runtime.hpp -- host-side state of the wavefront path tracer: the device
 // scene (triangles, shading normals, materials, power-alias light table, camera,
-// BLAS) and the per-batch path buffers.
+// BLAS), the per-renderer path buffers and the batch machinery shared by single
+// renders, multi-scene batches and the request server.
 #pragma once
 
+#include <cstddef>
+#include <cstdint>
 #include <vector>
 
 #include "lux_runtime.hpp"
@@ -14,14 +17,17 @@ namespace pt {
 // Scene in HBM. Layout (T triangles, V vertices, M materials, K lights):
 //   soup[T]          36-byte triangle records (vertex positions by triangle
 //                    index; the traversal reads only these and the nodes)
-//   indices[3T], normals[3V]  shading normals (after the identity-instance
-//                    transform), read once per hit
+//   vertices[V], indices[3T]  the indexed mesh (vertex updates gather the soup
+//                    from these on the device)
+//   normals[3V]      shading normals (after the identity-instance transform),
+//                    read once per hit
 //   mat_id[T], mats[M]        48-byte material records
 //   lights[K], light_of_prim[T]  power-alias light table, emitter lookup
 //   bvh              the LBVH over the triangle boxes (ids = triangle index)
 struct SceneDev {
     BvhDev bvh;
     lux_triangle* soup = nullptr;
+    lux_vec3* vertices = nullptr;
     uint32_t* indices = nullptr;
     float* normals = nullptr;
     int32_t* mat_id = nullptr;
@@ -32,34 +38,78 @@ struct SceneDev {
     DevCamera camera{};
     std::vector<DevLight> host_lights;
     std::vector<int32_t> host_light_of_prim;
+    // emissive triangles in triangle order, with their vertex indices and
+    // materials (the light table is rebuilt from these after a vertex update)
+    std::vector<int32_t> emitters;
+    std::vector<uint32_t> emitter_indices;
+    std::vector<lux_material> host_mats;
+    std::vector<int32_t> emitter_mat;
+    size_t device_bytes = 0;  // HBM held by the scene (mesh + BVH)
 };
 
-// POD view handed to kernels by value.
+// POD view of one scene handed to kernels.
 struct SceneView {
     const lux_bvh_node* nodes;
-    int node_count;
     const lux_triangle* soup;
-    int tri_count;
     const uint32_t* indices;
     const float* normals;
     const int32_t* mat_id;
     const DevMaterial* mats;
-    int mat_count;
     const DevLight* lights;
-    int light_count;
     const int32_t* light_of_prim;
-    DevCamera cam;
     DevInfo* info;
+    int32_t node_count, tri_count, mat_count, light_count;
+    DevCamera cam;
 };
 
-// Per-batch path state: 11 float4 planes (state, hit, shadow records) + PCG
-// state + two index queues + 4 counters; about 200 B per path slot.
+// One segment of a batch: a contiguous range of one request's paths. A
+// request's path index is r = sample * (width * height) + pixel (sample-major),
+// so a segment holds whole or partial samples of the request's image.
+struct Segment {
+    SceneView sv;
+    float* film;             // W*H*3 sums over the request's samples
+    uint64_t seed;
+    uint64_t req_begin;      // request path index of the segment's first path
+    uint32_t slot_begin;     // its first slot in the batch
+    uint32_t count;          // paths in the segment
+    int32_t width, height, max_depth, frame, sample_offset, pad;
+};
+
+constexpr int kMaxSegments = 256;
+
+struct BatchDesc {
+    uint32_t count;  // paths in the batch (sum of segment counts)
+    uint32_t nseg;
+    uint32_t pad[2];
+    Segment seg[kMaxSegments];
+};
+
+inline size_t desc_bytes(uint32_t nseg) { return offsetof(BatchDesc, seg) + sizeof(Segment) * nseg; }
+
+// Per-renderer path state: 11 float4 planes (state, hit, shadow records) + PCG
+// state + two index queues + counters; about 200 B per path slot.
 struct PathBuf {
     int capacity = 0;
     float4* f4 = nullptr;
     ulonglong2* rng = nullptr;
     uint32_t* q = nullptr;
+    // [0], [1] extension ray queues, [2], [3] shadow queues (by bounce parity)
     uint32_t* counts = nullptr;
+    // rays traced since the last reset: [0] extension (closest-hit) rays, [1]
+    // shadow rays, summed from the queue sizes of every bounce
+    unsigned long long* rays = nullptr;
+    BatchDesc* desc = nullptr;  // device copy of the current batch
+};
+
+// One render request in a renderer's work list.
+struct Request {
+    SceneView sv{};
+    lux_render_settings rs{};
+    float* film = nullptr;
+    uint64_t total = 0;     // width * height * spp paths
+    uint64_t issued = 0;    // paths already placed in batches
+    int64_t last_batch = -1;
+    uint64_t tag = 0;       // caller's identifier
 };
 
 struct RenderStats {
@@ -67,14 +117,71 @@ struct RenderStats {
     uint64_t paths = 0;
 };
 
+// Launch geometry of the wavefront schedule (set by the caller or the autotuner).
+struct Tuning {
+    int capacity = 1 << 21;  // path slots per batch
+    int grid = 2048;         // workgroups of the grid-stride kernels
+    int use_graph = 1;       // replay each batch's bounce loop as one hipGraph
+};
+
+// The batch engine: a path buffer, a pinned ring of batch descriptors and the
+// captured bounce loop. Not thread-safe; one stream at a time.
+class Renderer {
+public:
+    Renderer();
+    ~Renderer();
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    void set_tuning(const Tuning& t);
+    const Tuning& tuning() const { return tune_; }
+
+    // Enqueues every path of `reqs` (wavefront: multi-request batches of up to
+    // `capacity` paths; plain: the reference's 65 536-path single-sample
+    // batches, one shade launch per material type, a host sync per batch).
+    void render_all(std::vector<Request*>& reqs, int mode, hipStream_t stream, RenderStats* stats);
+
+    // Composes and enqueues ONE wavefront batch from `active` (in order; each
+    // request gives at most `slice_spp` samples of its image, 0 = no limit).
+    // Returns the number of paths placed (0 = nothing left). Sets last_batch of
+    // requests whose final path is in this batch to the batch's index.
+    uint64_t enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
+                           int64_t* batch_index);
+
+    void reset_ray_counts(hipStream_t stream);
+    void ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* shadow);
+    size_t device_bytes() const;
+    // HBM the engine holds once it runs batches of the tuned capacity.
+    size_t planned_bytes() const;
+    int64_t batches_enqueued() const { return batch_counter_; }
+
+private:
+    void ensure_capacity(int paths);
+    BatchDesc* next_slot();
+    void upload(BatchDesc* slot, hipStream_t stream);
+    void launch_wavefront(int depth, hipStream_t stream);
+    void launch_plain(int depth, uint32_t count, hipStream_t stream);
+    void release_graph();
+
+    Tuning tune_;
+    PathBuf buf_;
+    static constexpr int kRing = 8;
+    BatchDesc* ring_ = nullptr;  // pinned host slots
+    hipEvent_t ring_ev_[kRing] = {};
+    bool ring_used_[kRing] = {};
+    int ring_next_ = 0;
+    hipGraphExec_t graph_ = nullptr;
+    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1;
+    int64_t batch_counter_ = 0;
+};
+
 void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream);
+// Replaces vertex positions (same count), regathers the soup on the device,
+// refits the BVH and rebuilds the light table; the topology is kept.
+void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream);
 void scene_release(SceneDev& s);
 SceneView scene_view(const SceneDev& s);
-void pathbuf_allocate(PathBuf& b, int capacity);
-void pathbuf_release(PathBuf& b);
-// Enqueues a full render into `film` (device, W*H*3 sums over spp).
-void render(const SceneDev& s, PathBuf& buf, const lux_render_settings& rs, float* film, hipStream_t stream,
-            RenderStats* stats);
+uint64_t request_paths(const lux_render_settings& rs);
 
 }  // namespace pt
 }  // namespace lux

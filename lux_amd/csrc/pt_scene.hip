@@ -1,0 +1,353 @@
+// pt_scene.hip -- scene preparation for the wavefront path tracer: camera,
+// identity-instance shading normals, the power-alias light table, the device
+// mesh and its LBVH; vertex updates regather the triangles on the device and
+// refit the LBVH (same topology) instead of rebuilding it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pt_device.hpp"
+#include "pt_runtime.hpp"
+
+namespace lux {
+namespace pt {
+namespace {
+
+template <typename T>
+T* dalloc(size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) throw Error(LUX_ERR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return static_cast<T*>(p);
+}
+
+// soup[t] = the three indexed vertices of triangle t.
+__global__ __launch_bounds__(256) void k_gather_soup(const lux_vec3* __restrict__ vertices,
+                                                     const uint32_t* __restrict__ indices, int tri_count,
+                                                     int vertex_count, lux_triangle* __restrict__ soup,
+                                                     DevInfo* info) {
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < tri_count; t += gridDim.x * 256) {
+        uint32_t a = indices[3 * size_t(t)], b = indices[3 * size_t(t) + 1], c = indices[3 * size_t(t) + 2];
+        if (a >= uint32_t(vertex_count) || b >= uint32_t(vertex_count) || c >= uint32_t(vertex_count)) {
+            set_error(info, kErrPrimRange);
+            continue;
+        }
+        soup[t] = lux_triangle{vertices[a], vertices[b], vertices[c]};
+    }
+}
+
+}  // namespace
+
+// ---- host: scene preparation ------------------------------------------------------
+namespace host {
+
+struct V {
+    float x, y, z;
+};
+inline V mkv(float x, float y, float z) { return V{x, y, z}; }
+inline V vsub(V a, V b) { return mkv(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V vadd(V a, V b) { return mkv(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V vmul(V a, float s) { return mkv(a.x * s, a.y * s, a.z * s); }
+inline float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V vcross(V a, V b) { return mkv(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline V vnorm(V v) {
+    float len = sqrtf(vdot(v, v));
+    if (!(len > 0)) return v;
+    float inv = 1 / len;
+    return vmul(v, inv);
+}
+
+// make_perspective_camera with lens radius 0 and focal distance 1
+// (src/camera/camera.h:58-109, src/camera/pinhole_camera.h:13-17).
+DevCamera make_camera(const float* c) {
+    V origin = mkv(c[0], c[1], c[2]), look = mkv(c[3], c[4], c[5]);
+    float vfov = c[6], aspect = c[7];
+    V forward = vnorm(vsub(look, origin));
+    V right = vnorm(vcross(mkv(0, 1, 0), forward));
+    if (vdot(right, right) == 0) right = vnorm(vcross(mkv(0, 0, 1), forward));
+    V up = vcross(forward, right);
+    float theta = vfov * kPi / 180.0f;
+    float vh = 2.0f * tanf(theta * 0.5f);
+    float vw = aspect * vh;
+    float focus = 1.0f;
+    V horizontal = vmul(right, vw * focus);
+    V vertical = vmul(up, vh * focus);
+    V lower = vsub(vsub(vadd(origin, vmul(forward, focus)), vmul(horizontal, 0.5f)), vmul(vertical, 0.5f));
+    return DevCamera{origin.x, origin.y, origin.z, lower.x, lower.y, lower.z,
+                     horizontal.x, horizontal.y, horizontal.z, vertical.x, vertical.y, vertical.z};
+}
+
+// transform_instance_normal under the identity instance: adjugate product then
+// two normalisations (src/scene/scene_data.h:262-268, src/core/mat4.cuh:209-229).
+V instance_normal(V n) {
+    if (vdot(n, n) == 0) return mkv(0, 0, 0);
+    V t = mkv(1.0f * n.x + 0.0f * n.y + 0.0f * n.z, 0.0f * n.x + 1.0f * n.y + 0.0f * n.z,
+              0.0f * n.x + 0.0f * n.y + 1.0f * n.z);
+    V once = vnorm(t);
+    if (vdot(once, once) == 0) return mkv(0, 0, 0);
+    return vnorm(once);
+}
+
+// Power light selection with its alias table: prepare_scene_light_sampling
+// (src/scene/scene_prepare.h:268-387) and build_alias_table_from_weights
+// (src/light/light_distribution.cpp:8-64; same stack order, so the same aliases).
+// `emitters` lists the triangles whose material emits, in triangle order, with
+// their current vertices; the other triangles contribute nothing to the table.
+void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std::vector<lux_triangle>& tris,
+                  const std::vector<lux_material>& emitter_mats, std::vector<DevLight>& lights,
+                  std::vector<int32_t>& light_of_prim) {
+    light_of_prim.assign(tri_count, -1);
+    lights.clear();
+    std::vector<float> cdf;
+    float total = 0;
+    for (size_t e = 0; e < emitters.size(); ++e) {
+        const int t = emitters[e];
+        const lux_material& m = emitter_mats[e];
+        float emax = fmaxf(m.emission.x, fmaxf(m.emission.y, m.emission.z));
+        if (!(emax > 0)) continue;
+        V v0 = mkv(tris[e].v0.x, tris[e].v0.y, tris[e].v0.z);
+        V v1 = mkv(tris[e].v1.x, tris[e].v1.y, tris[e].v1.z);
+        V v2 = mkv(tris[e].v2.x, tris[e].v2.y, tris[e].v2.z);
+        V c = vcross(vsub(v1, v0), vsub(v2, v0));
+        float area = 0.5f * sqrtf(vdot(c, c));
+        if (area <= 0) continue;
+        float lum = float(0.2126) * m.emission.x + float(0.7152) * m.emission.y + float(0.0722) * m.emission.z;
+        float power = area * fmaxf(1e-6f, lum);
+        if (power <= 0) continue;
+        total += power;
+        DevLight L{};
+        L.prim = t;
+        L.er = m.emission.x;
+        L.eg = m.emission.y;
+        L.eb = m.emission.z;
+        L.two_sided = m.emission_two_sided;
+        L.area = area;
+        lights.push_back(L);
+        cdf.push_back(total);
+        int32_t& stored = light_of_prim[t];
+        stored = stored == -1 ? int32_t(lights.size()) - 1 : -2;
+    }
+    int count = int(lights.size());
+    if (count == 0 || !(total > 0)) {
+        lights.clear();
+        return;
+    }
+    std::vector<float> w(count);
+    float prev = 0;
+    for (int i = 0; i < count; ++i) {
+        w[i] = cdf[i] - prev;
+        prev = cdf[i];
+    }
+    float sum = 0;
+    for (float x : w) sum += fmaxf(0.0f, x);
+    std::vector<float> scaled(count);
+    std::vector<int> small, large;
+    for (int i = 0; i < count; ++i) {
+        float pmf = fmaxf(0.0f, w[i]) / sum;
+        lights[i].pmf = pmf;
+        lights[i].alias = i;
+        lights[i].prob = 1;
+        scaled[i] = pmf * float(count);
+        (scaled[i] < 1.0f ? small : large).push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        int l = small.back();
+        small.pop_back();
+        int g = large.back();
+        large.pop_back();
+        lights[l].prob = scaled[l];
+        lights[l].alias = g;
+        scaled[g] = (scaled[g] + scaled[l]) - 1.0f;
+        (scaled[g] < 1.0f ? small : large).push_back(g);
+    }
+    for (int i : large) {
+        lights[i].prob = 1;
+        lights[i].alias = i;
+    }
+    for (int i : small) {
+        lights[i].prob = 1;
+        lights[i].alias = i;
+    }
+}
+
+}  // namespace host
+
+namespace {
+void emitter_triangles(const SceneDev& s, const lux_vec3* vertices, std::vector<lux_triangle>& tris) {
+    tris.resize(s.emitters.size());
+    for (size_t e = 0; e < s.emitters.size(); ++e) {
+        const uint32_t* ix = s.emitter_indices.data() + 3 * e;
+        tris[e] = lux_triangle{vertices[ix[0]], vertices[ix[1]], vertices[ix[2]]};
+    }
+}
+
+void upload_lights(SceneDev& s, hipStream_t stream) {
+    if (!s.lights || s.light_count < int(s.host_lights.size())) {
+        if (s.lights) (void)hipFree(s.lights);
+        s.lights = dalloc<DevLight>(std::max<size_t>(s.host_lights.size(), 1));
+    }
+    s.light_count = int(s.host_lights.size());
+    if (!s.host_lights.empty())
+        hip_check(hipMemcpyAsync(s.lights, s.host_lights.data(), sizeof(DevLight) * s.host_lights.size(),
+                                 hipMemcpyHostToDevice, stream),
+                  "H2D lights");
+    hip_check(hipMemcpyAsync(s.light_of_prim, s.host_light_of_prim.data(), 4 * size_t(s.tri_count),
+                             hipMemcpyHostToDevice, stream),
+              "H2D light map");
+}
+
+size_t bvh_bytes(int n) {
+    // boxes, ids, 2x keys/vals, 5 tree links, leaf parents, arrivals, 2 slots,
+    // 2 layout maps, 2 node records, 2 frontier entries, 2 climb words
+    return size_t(std::max(n, 1)) * (24 + 4 + 16 + 20 + 4 + 4 + 48 + 8 + 64 + 8 + 8) + (size_t(n) / 2048 + 1) * 2048;
+}
+}  // namespace
+
+void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
+    int nt = desc.triangle_count, nv = desc.vertex_count, nm = desc.material_count;
+    if (nt <= 0 || nv <= 0 || nm <= 0 || !desc.vertices || !desc.indices || !desc.material_ids || !desc.materials)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "scene needs vertices, indices, material ids and materials");
+    for (int t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k)
+            if (desc.indices[3 * size_t(t) + k] >= uint32_t(nv))
+                throw Error(LUX_ERR_INVALID_ARGUMENT, "triangle index out of range");
+        if (desc.material_ids[t] < 0 || desc.material_ids[t] >= nm)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "material id out of range");
+    }
+    std::vector<lux_aabb> boxes(nt);
+    std::vector<int32_t> ids(nt);
+    for (int t = 0; t < nt; ++t) {
+        const uint32_t* ix = desc.indices + 3 * size_t(t);
+        lux_aabb b;
+        b.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
+        b.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 3; ++k) {
+            const lux_vec3& p = desc.vertices[ix[k]];
+            b.p_min = lux_vec3{fminf(b.p_min.x, p.x), fminf(b.p_min.y, p.y), fminf(b.p_min.z, p.z)};
+            b.p_max = lux_vec3{fmaxf(b.p_max.x, p.x), fmaxf(b.p_max.y, p.y), fmaxf(b.p_max.z, p.z)};
+        }
+        boxes[t] = b;
+        ids[t] = t;
+    }
+    std::vector<float> normals;
+    if (desc.normals) {
+        normals.resize(3 * size_t(nv));
+        for (int v = 0; v < nv; ++v) {
+            host::V n = host::instance_normal(host::mkv(desc.normals[v].x, desc.normals[v].y, desc.normals[v].z));
+            normals[3 * v] = n.x;
+            normals[3 * v + 1] = n.y;
+            normals[3 * v + 2] = n.z;
+        }
+    }
+    std::vector<DevMaterial> mats(nm);
+    for (int m = 0; m < nm; ++m) {
+        const lux_material& in = desc.materials[m];
+        if (in.type < 0 || in.type >= kMaterialTypes) throw Error(LUX_ERR_INVALID_ARGUMENT, "unknown material type");
+        mats[m] = DevMaterial{in.type, in.albedo.x, in.albedo.y, in.albedo.z, in.emission.x, in.emission.y,
+                              in.emission.z, in.emission_two_sided, in.roughness, in.metallic, in.eta, 0};
+    }
+
+    scene_release(s);
+    s.tri_count = nt;
+    s.vertex_count = nv;
+    s.mat_count = nm;
+    s.host_mats.assign(desc.materials, desc.materials + nm);
+    for (int t = 0; t < nt; ++t) {
+        const lux_material& m = desc.materials[desc.material_ids[t]];
+        if (fmaxf(m.emission.x, fmaxf(m.emission.y, m.emission.z)) > 0) {
+            s.emitters.push_back(t);
+            s.emitter_mat.push_back(desc.material_ids[t]);
+            for (int k = 0; k < 3; ++k) s.emitter_indices.push_back(desc.indices[3 * size_t(t) + k]);
+        }
+    }
+    std::vector<lux_triangle> etris;
+    emitter_triangles(s, desc.vertices, etris);
+    std::vector<lux_material> emats(s.emitters.size());
+    for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
+    host::build_lights(nt, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+
+    s.soup = dalloc<lux_triangle>(nt);
+    s.vertices = dalloc<lux_vec3>(nv);
+    s.indices = dalloc<uint32_t>(3 * size_t(nt));
+    s.mat_id = dalloc<int32_t>(nt);
+    s.mats = dalloc<DevMaterial>(nm);
+    s.light_of_prim = dalloc<int32_t>(nt);
+    if (desc.normals) s.normals = dalloc<float>(3 * size_t(nv));
+    s.camera = host::make_camera(desc.camera);
+    hip_check(hipMemcpyAsync(s.vertices, desc.vertices, sizeof(lux_vec3) * nv, hipMemcpyHostToDevice, stream),
+              "H2D vertices");
+    hip_check(hipMemcpyAsync(s.indices, desc.indices, 12 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D indices");
+    hip_check(hipMemcpyAsync(s.mat_id, desc.material_ids, 4 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D mat ids");
+    hip_check(hipMemcpyAsync(s.mats, mats.data(), sizeof(DevMaterial) * nm, hipMemcpyHostToDevice, stream), "H2D mats");
+    upload_lights(s, stream);
+    if (desc.normals)
+        hip_check(hipMemcpyAsync(s.normals, normals.data(), 4 * normals.size(), hipMemcpyHostToDevice, stream),
+                  "H2D normals");
+    // BLAS over the triangle boxes (ids = triangle index); the soup is gathered
+    // on the device from the indexed mesh
+    lux_aabb* d_boxes = dalloc<lux_aabb>(nt);
+    int32_t* d_ids = dalloc<int32_t>(nt);
+    hip_check(hipMemcpyAsync(d_boxes, boxes.data(), sizeof(lux_aabb) * nt, hipMemcpyHostToDevice, stream), "H2D boxes");
+    hip_check(hipMemcpyAsync(d_ids, ids.data(), 4 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D ids");
+    lbvh_build(s.bvh, d_boxes, d_ids, nt, stream);
+    LUX_LAUNCH("pt_gather_soup", k_gather_soup, std::min(div_up(nt, 256), 4096), 256, 0, stream, s.vertices,
+               s.indices, nt, nv, s.soup, s.bvh.info);
+    hip_check(hipStreamSynchronize(stream), "scene upload");
+    (void)hipFree(d_boxes);
+    (void)hipFree(d_ids);
+    s.device_bytes = size_t(nt) * (36 + 12 + 4 + 4) + size_t(nv) * (12 + (desc.normals ? 12 : 0)) +
+                     sizeof(DevMaterial) * nm + sizeof(DevLight) * std::max<size_t>(s.host_lights.size(), 1) +
+                     bvh_bytes(nt);
+}
+
+void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream) {
+    if (!s.soup) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene is empty");
+    if (!vertices || vertex_count != s.vertex_count)
+        throw Error(LUX_ERR_MISMATCH, "vertex update must keep the vertex count");
+    std::vector<lux_triangle> etris;
+    emitter_triangles(s, vertices, etris);
+    std::vector<lux_material> emats(s.emitters.size());
+    for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
+    host::build_lights(s.tri_count, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+    // stream order: renders enqueued before this update read the old geometry
+    hip_check(hipMemcpyAsync(s.vertices, vertices, sizeof(lux_vec3) * size_t(vertex_count), hipMemcpyHostToDevice,
+                             stream),
+              "H2D vertices");
+    LUX_LAUNCH("pt_gather_soup", k_gather_soup, std::min(div_up(s.tri_count, 256), 4096), 256, 0, stream, s.vertices,
+               s.indices, s.tri_count, s.vertex_count, s.soup, s.bvh.info);
+    lbvh_refit_triangles(s.bvh, s.soup, stream);
+    upload_lights(s, stream);
+    // the host arrays above are pageable: the copies complete before return
+    hip_check(hipStreamSynchronize(stream), "scene update");
+}
+
+void scene_release(SceneDev& s) {
+    void* ptrs[] = {s.soup, s.vertices, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    bvh_release(s.bvh);
+    s = SceneDev{};
+}
+
+SceneView scene_view(const SceneDev& s) {
+    SceneView v{};
+    v.nodes = s.bvh.nodes;
+    v.node_count = s.bvh.n > 0 ? 2 * s.bvh.n - 1 : 0;
+    v.soup = s.soup;
+    v.tri_count = s.tri_count;
+    v.indices = s.indices;
+    v.normals = s.normals;
+    v.mat_id = s.mat_id;
+    v.mats = s.mats;
+    v.mat_count = s.mat_count;
+    v.lights = s.lights;
+    v.light_count = s.light_count;
+    v.light_of_prim = s.light_of_prim;
+    v.cam = s.camera;
+    v.info = s.bvh.info;
+    return v;
+}
+
+}  // namespace pt
+}  // namespace lux

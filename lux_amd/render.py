@@ -35,6 +35,10 @@ class RenderSettings(ctypes.Structure):
                 ("mode", c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("capacity", c_int32), ("grid", c_int32), ("use_graph", c_int32)]
+
+
 assert ctypes.sizeof(Material) == 44
 
 
@@ -48,6 +52,7 @@ def _register():
         "lux_hip_scene_bvh": [c_void_p, POINTER(c_void_p)],
         "lux_hip_render": [c_void_p, POINTER(RenderSettings), c_void_p],
         "lux_hip_render_device": [c_void_p, POINTER(RenderSettings), c_void_p, c_void_p],
+        "lux_hip_render_rays": [c_void_p, c_void_p, POINTER(c_uint64), POINTER(c_uint64)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -129,9 +134,158 @@ class Scene:
         s = settings(width, height, spp, depth, **kw)
         call("lux_hip_render_device", self.handle, ctypes.byref(s), d_film.ptr, stream)
 
+    def rays(self, stream=None):
+        """(extension rays, shadow rays) traced by the last render."""
+        e, sh = c_uint64(), c_uint64()
+        call("lux_hip_render_rays", self.handle, stream, ctypes.byref(e), ctypes.byref(sh))
+        return e.value, sh.value
+
+    def info(self):
+        """(triangles, vertices, device bytes)."""
+        t, v, b = c_int32(), c_int32(), c_uint64()
+        call("lux_hip_scene_info", self.handle, ctypes.byref(t), ctypes.byref(v), ctypes.byref(b))
+        return t.value, v.value, b.value
+
+    def update_vertices(self, vertices):
+        """Move the vertices (same count): device regather + LBVH refit + light table."""
+        v = np.ascontiguousarray(vertices, np.float32)
+        assert v.shape == self._v.shape
+        call("lux_hip_scene_update_vertices", self.handle, ptr(v), len(v))
+        self._v = v
+
+
+def _batch_args(scenes, jobs):
+    n = len(scenes)
+    hs = (c_void_p * n)(*[sc.handle.value for sc in scenes])
+    ss = (RenderSettings * n)(*[settings(**j) for j in jobs])
+    return n, hs, ss
+
+
+def batch_render(scenes, jobs, mode=WAVEFRONT):
+    """Render jobs[i] (a dict of settings() arguments) of scenes[i], all in one
+    multi-scene wavefront loop (mode WAVEFRONT) or one after another on the
+    plain schedule. Returns the host film sums."""
+    n, hs, ss = _batch_args(scenes, jobs)
+    films = [np.zeros((j["height"], j["width"], 3), np.float32) for j in jobs]
+    fp = (c_void_p * n)(*[f.ctypes.data for f in films])
+    call("lux_hip_batch_render", hs, ss, fp, n, mode)
+    return films
+
+
+def batch_render_device(scenes, jobs, d_films, mode=WAVEFRONT, stream=None):
+    n, hs, ss = _batch_args(scenes, jobs)
+    fp = (c_void_p * n)(*[f.ptr.value for f in d_films])
+    call("lux_hip_batch_render_device", hs, ss, fp, n, mode, stream)
+
+
+def batch_rays(stream=None):
+    e, sh = c_uint64(), c_uint64()
+    call("lux_hip_batch_rays", stream, ctypes.byref(e), ctypes.byref(sh))
+    return e.value, sh.value
+
+
+def get_tuning():
+    lbvh.init()
+    t = Tuning()
+    call("lux_hip_get_render_tuning", ctypes.byref(t))
+    return {"capacity": t.capacity, "grid": t.grid, "use_graph": t.use_graph}
+
+
+def set_tuning(capacity=None, grid=None, use_graph=None):
+    cur = get_tuning()
+    t = Tuning(capacity or cur["capacity"], grid or cur["grid"], cur["use_graph"] if use_graph is None else int(use_graph))
+    call("lux_hip_set_render_tuning", ctypes.byref(t))
+
 
 def rel_rmse(test, ref, eps=1e-2):
     """sqrt(mean((g - c)^2 / (c^2 + eps))) over pixels and channels."""
     g = np.asarray(test, np.float64)
     c = np.asarray(ref, np.float64)
     return float(np.sqrt(np.mean((g - c) ** 2 / (c * c + eps))))
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", c_uint64), ("slice_spp", c_int32), ("max_inflight", c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, c_uint64) for n in ("submitted", "admitted", "completed", "refits", "batches", "paths",
+                                        "resident_bytes", "peak_bytes", "extension_rays", "shadow_rays")]
+
+
+class Server:
+    """Continuous-batching render server (lux_hip_server_*)."""
+
+    def __init__(self, hbm_budget=200 << 30, slice_spp=16, max_inflight=2):
+        lbvh.init()
+        cfg = ServeConfig(int(hbm_budget), int(slice_spp), int(max_inflight))
+        h = c_void_p()
+        call("lux_hip_server_create", ctypes.byref(cfg), ctypes.byref(h))
+        self.handle = h
+        self._shapes = {}
+        self._keep = {}
+
+    def close(self):
+        if self.handle:
+            call("lux_hip_server_destroy", self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                abi.load().lux_hip_server_destroy(self.handle)
+        except Exception:
+            pass
+
+    def submit(self, scene, job, vertices=None):
+        s = settings(**job)
+        v = None if vertices is None else np.ascontiguousarray(vertices, np.float32)
+        rid = c_uint64()
+        call("lux_hip_server_submit", self.handle, scene.handle, ctypes.byref(s), ptr(v),
+             0 if v is None else len(v), ctypes.byref(rid))
+        self._shapes[rid.value] = (job["height"], job["width"], 3)
+        return rid.value
+
+    def poll(self, block=False, max_ids=256):
+        ids = (c_uint64 * max_ids)()
+        n = c_int32()
+        call("lux_hip_server_poll", self.handle, int(bool(block)), ids, max_ids, ctypes.byref(n))
+        return [ids[i] for i in range(n.value)]
+
+    def result(self, rid):
+        film = np.zeros(self._shapes.pop(rid), np.float32)
+        lat = ctypes.c_double()
+        call("lux_hip_server_result", self.handle, c_uint64(rid), ptr(film), ctypes.byref(lat))
+        return film, lat.value
+
+    def stats(self):
+        st = ServeStats()
+        call("lux_hip_server_stats", self.handle, ctypes.byref(st))
+        return {n: getattr(st, n) for n, _ in ServeStats._fields_}
+
+    def run(self, scenes, jobs, arrivals, vertices=None, want_films=True):
+        """Replay an arrival trace (seconds, sorted). vertices[i] (or None) moves
+        scenes[i]'s vertices before request i. Returns (films or None, latencies
+        in seconds, wall seconds)."""
+        n = len(scenes)
+        hs = (c_void_p * n)(*[sc.handle.value for sc in scenes])
+        ss = (RenderSettings * n)(*[settings(**j) for j in jobs])
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        lat = np.zeros(n, np.float64)
+        wall = ctypes.c_double()
+        films = [np.zeros((j["height"], j["width"], 3), np.float32) for j in jobs] if want_films else None
+        fp = (c_void_p * n)(*[f.ctypes.data for f in films]) if want_films else None
+        vp = vc = None
+        keep = []
+        if vertices is not None:
+            vp = (c_void_p * n)()
+            vc = np.zeros(n, np.int32)
+            for i, v in enumerate(vertices):
+                if v is not None:
+                    a = np.ascontiguousarray(v, np.float32)
+                    keep.append(a)
+                    vp[i] = a.ctypes.data
+                    vc[i] = len(a)
+        call("lux_hip_server_run", self.handle, n, hs, ss, ptr(arr), vp, ptr(vc) if vc is not None else None, fp,
+             ptr(lat), ctypes.byref(wall))
+        return films, lat, wall.value

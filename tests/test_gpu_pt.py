@@ -132,3 +132,110 @@ def test_invalid_settings_rejected(rd):
         sc.render(0, 32, 4, 4)
     with pytest.raises(abi.LuxError):
         sc.render(32, 32, 4, -1)
+
+
+# ---- multi-scene batches, graph replay, vertex updates ---------------------------------
+
+def small_scenes(rd):
+    from lux_amd import scenes
+
+    specs = [
+        (scene_of(load_golden("pt_cornell_32")), dict(width=32, height=32, spp=5, depth=4, seed=3)),
+        (scenes.displaced_icosphere(freq=9, seed=0x4C555A, width=48, height=40), dict(width=48, height=40, spp=7,
+                                                                                      depth=6, seed=11)),
+        (scenes.displaced_icosphere(freq=14, seed=0x4C555B, width=24, height=24), dict(width=24, height=24, spp=3,
+                                                                                       depth=8, seed=5, frame=2)),
+        (scenes.cornell(20, 30), dict(width=20, height=30, spp=9, depth=3, seed=0, sample_offset=4)),
+    ]
+    return [rd.Scene(s) for s, _ in specs], [j for _, j in specs]
+
+
+def standalone(sc, j):
+    kw = {k: v for k, v in j.items() if k not in ("width", "height", "spp", "depth")}
+    return sc.render(j["width"], j["height"], j["spp"], j["depth"], **kw)
+
+
+@pytest.mark.parametrize("capacity", [0, 1000, 37])
+def test_batch_equals_standalone(rd, capacity):
+    """Each film of a multi-scene batch is bit-identical to the same job rendered
+    alone, also when batches cut requests mid-sample and hold several segments."""
+    print("case: batch vs standalone, capacity", capacity, flush=True)
+    saved = rd.get_tuning()
+    try:
+        scs, jobs = small_scenes(rd)
+        alone = [standalone(sc, j) for sc, j in zip(scs, jobs)]
+        if capacity:
+            rd.set_tuning(capacity=capacity)
+        films = rd.batch_render(scs, jobs)
+        for a, f in zip(alone, films):
+            assert np.array_equal(bits(a), bits(f))
+        e, sh = rd.batch_rays()
+        assert e >= sum(j["width"] * j["height"] * j["spp"] for j in jobs) and sh > 0
+    finally:
+        rd.set_tuning(**saved)
+
+
+def test_batch_plain_and_graphless_are_identical(rd):
+    print("case: batch plain / no graph", flush=True)
+    saved = rd.get_tuning()
+    try:
+        scs, jobs = small_scenes(rd)
+        ref = rd.batch_render(scs, jobs)
+        plain = rd.batch_render(scs, jobs, mode=rd.PLAIN)
+        rd.set_tuning(use_graph=0, capacity=4096, grid=64)
+        nograph = rd.batch_render(scs, jobs)
+        for a, b, c in zip(ref, plain, nograph):
+            assert np.array_equal(bits(a), bits(b))
+            assert np.array_equal(bits(a), bits(c))
+    finally:
+        rd.set_tuning(**saved)
+
+
+def test_graph_replays_are_stable(rd):
+    """The captured bounce loop replayed many times (small capacity -> many
+    batches, eager work and synchronisations between renders) gives the same
+    film every time."""
+    print("case: graph replay stability", flush=True)
+    saved = rd.get_tuning()
+    try:
+        g = load_golden("pt_icosphere")
+        sc = rd.Scene(scene_of(g))
+        rd.set_tuning(capacity=512, grid=8, use_graph=1)
+        first = sc.render(40, 40, 4, 8, seed=1)
+        for _ in range(3):
+            assert np.array_equal(bits(first), bits(sc.render(40, 40, 4, 8, seed=1)))
+        rd.set_tuning(use_graph=0)
+        assert np.array_equal(bits(first), bits(sc.render(40, 40, 4, 8, seed=1)))
+    finally:
+        rd.set_tuning(**saved)
+
+
+def test_vertex_update_refits(rd):
+    """Jittered vertices: the refit scene renders like a scene built from the
+    jittered mesh (same light table bit for bit; films equal up to hit-order
+    ties between the refit and rebuilt trees)."""
+    from lux_amd import scenes
+
+    print("case: vertex update + refit", flush=True)
+    base = scenes.displaced_icosphere(freq=12, seed=0x4C555C, width=40, height=40)
+    rng = np.random.default_rng(99)
+    moved = dict(base)
+    moved["vertices"] = (base["vertices"] + rng.normal(0, 1e-3, base["vertices"].shape)).astype(np.float32)
+    sc = rd.Scene(base)
+    before = sc.render(40, 40, 4, 6, seed=2)
+    sc.update_vertices(moved["vertices"])
+    fresh = rd.Scene(moved)
+    la, lb = sc.lights(), fresh.lights()
+    for k in la:
+        assert np.array_equal(np.asarray(la[k]).view(np.uint32), np.asarray(lb[k]).view(np.uint32)), k
+    a = sc.render(40, 40, 4, 6, seed=2)
+    b = fresh.render(40, 40, 4, 6, seed=2)
+    err = rd.rel_rmse(a / 4, b / 4)
+    print(f"refit vs rebuild relRMSE={err:.3e} identical={np.array_equal(bits(a), bits(b))}", flush=True)
+    assert err < 1e-3
+    assert not np.array_equal(bits(a), bits(before))
+    from lux_amd import abi
+
+    short = np.ascontiguousarray(moved["vertices"][:-1])
+    with pytest.raises(abi.LuxError):
+        abi.call("lux_hip_scene_update_vertices", sc.handle, abi.ptr(short), len(short))
