@@ -51,6 +51,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
+    ap.add_argument("--render", default="c3,c4,c5",
+                    help="render workloads to add to the line: any of c3,c4,c5 (comma-separated) or none")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule baselines of c3/c4")
+    ap.add_argument("--c4-scenes", type=int, default=64)
+    ap.add_argument("--c5-requests", type=int, default=96)
+    ap.add_argument("--c5-load", type=float, default=0.75, help="arrival rate as a fraction of the c4 capacity")
+    ap.add_argument("--cpu-render-rows", type=int, default=48,
+                    help="rows of the C3 image the CPU render baseline traces (centre band)")
     return ap.parse_args()
 
 
@@ -87,6 +95,204 @@ def _cpu_name():
     except OSError:
         pass
     return None
+
+
+# ---- render workloads ---------------------------------------------------------------
+# Algorithmic HBM bytes of the wavefront path tracer: per extension segment 336 B
+# (ray 32 r + hit 16 w + 16 r + path state 152 r/w + surface fetch 88 + next ray
+# 32 w), per shadow ray 112 B (record 2 x 44 + radiance read-modify-write 24),
+# 12 B per film pixel.
+EXT_BYTES, SHADOW_BYTES, FILM_BYTES = 336, 112, 12
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _render_rate(ext, sh, pixels, seconds):
+    algo = EXT_BYTES * ext + SHADOW_BYTES * sh + FILM_BYTES * pixels
+    gbps = algo / seconds / 1e9
+    return {"Mrays_per_s": round((ext + sh) / seconds / 1e6, 2), "extension_rays": int(ext), "shadow_rays": int(sh),
+            "algorithmic_GB": round(algo / 1e9, 3), "achieved_GBps": round(gbps, 2),
+            "hbm_frac": round(gbps / HBM_PEAK_GBPS, 5)}
+
+
+def bench_c3(args):
+    """Single-scene render: displaced icosphere (freq 208, 865 280 triangles),
+    1024x1024, 64 spp, depth 8, seed 0."""
+    from lux_amd import lbvh, render, scenes
+
+    W = H = 1024
+    spp, depth = 64, 8
+    t0 = time.perf_counter()
+    d = scenes.displaced_icosphere(freq=208, width=W, height=H)
+    t1 = time.perf_counter()
+    sc = render.Scene(d)
+    t2 = time.perf_counter()
+    film = lbvh.DeviceArray(W * H * 12)
+    out = {"workload": f"displaced icosphere, {sc.triangle_count} triangles, {W}x{H}, {spp} spp, depth {depth}",
+           "scene_generate_s": round(t1 - t0, 3), "scene_upload_build_s": round(t2 - t1, 3)}
+    modes = [("wavefront", render.WAVEFRONT)] + ([] if args.no_plain else [("plain", render.PLAIN)])
+    films = {}
+    for name, mode in modes:
+        sc.render_device(film, W, H, 1, depth, mode=mode)
+        lbvh.synchronize()
+        s0 = time.perf_counter()
+        sc.render_device(film, W, H, spp, depth, mode=mode)
+        lbvh.synchronize()
+        sec = time.perf_counter() - s0
+        ext, sh = sc.rays()
+        films[name] = film.to_numpy(np.float32, (H, W, 3))
+        out[name] = {"render_s": round(sec, 4), "renders_per_s": round(1 / sec, 4),
+                     **_render_rate(ext, sh, W * H, sec)}
+        _log(f"c3 {name}: {sec:.3f} s")
+    if "plain" in out:
+        out["speedup_vs_plain"] = round(out["plain"]["render_s"] / out["wavefront"]["render_s"], 3)
+        out["films_bit_identical"] = bool(np.array_equal(films["wavefront"].view(np.uint32),
+                                                         films["plain"].view(np.uint32)))
+    if args.cpu_render_rows > 0:
+        out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], args.cpu_render_rows)
+    sc.close()
+    return out
+
+
+def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows):
+    """The CPU restatement of render_cpu_path_tracer (pinned bit for bit to the
+    reference's renders) on a band of `nrows` rows through the image centre,
+    one thread and all host threads (rows split across threads, per-pixel
+    sample order unchanged)."""
+    import oracle
+
+    r0 = H // 2 - nrows // 2
+    rows = (r0, r0 + nrows)
+    t0 = time.perf_counter()
+    pt = oracle.OraclePathTracer(scene)
+    t1 = time.perf_counter()
+    s0 = time.perf_counter()
+    one = pt.render(W, H, spp, depth, rows=rows, threads=1)
+    t_one = time.perf_counter() - s0
+    threads = os.cpu_count() or 1
+    s0 = time.perf_counter()
+    pt.render(W, H, spp, depth, rows=rows, threads=threads)
+    t_all = time.perf_counter() - s0
+    from lux_amd.render import rel_rmse
+
+    band = slice(rows[0], rows[1])
+    err = rel_rmse(gpu_film[band] / spp, one[band] / spp)
+    paths = W * nrows * spp
+    return {"value": round(paths / t_one / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "sample": f"rows {rows[0]}..{rows[1] - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+            "all_threads_Mpaths_per_s": round(paths / t_all / 1e6, 4), "all_threads": threads,
+            "host_scene_prepare_s": round(t1 - t0, 3), "gpu_vs_cpu_band_relrmse": round(err, 8),
+            "host_cpu": _cpu_name()}
+
+
+def c4_scenes(count):
+    from lux_amd import scenes
+
+    specs = scenes.batched_mix(count)
+    dicts = [scenes.displaced_icosphere(freq=s["freq"], seed=s["seed"], width=s["width"], height=s["height"])
+             for s in specs]
+    jobs = [dict(width=s["width"], height=s["height"], spp=s["spp"], depth=s["depth"], seed=0) for s in specs]
+    return specs, dicts, jobs
+
+
+def bench_c4(args, prepared=None):
+    """Batched multi-scene launch vs the plain single-scene path, scene by scene."""
+    from lux_amd import lbvh, render
+
+    t0 = time.perf_counter()
+    specs, dicts, jobs = prepared or c4_scenes(args.c4_scenes)
+    t1 = time.perf_counter()
+    scs = [render.Scene(d) for d in dicts]
+    t2 = time.perf_counter()
+    d_films = [lbvh.DeviceArray(j["width"] * j["height"] * 12) for j in jobs]
+    warm = [dict(j, spp=1) for j in jobs]
+    render.batch_render_device(scs, warm, d_films)
+    lbvh.synchronize()
+    s0 = time.perf_counter()
+    render.batch_render_device(scs, jobs, d_films)
+    lbvh.synchronize()
+    sec = time.perf_counter() - s0
+    ext, sh = render.batch_rays()
+    pixels = sum(j["width"] * j["height"] for j in jobs)
+    tris = sum(sc.triangle_count for sc in scs)
+    paths = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    out = {"workload": (f"{len(scs)} displaced-icosphere scenes, {tris} triangles, {pixels} pixels, {paths} paths "
+                        f"(triangles log-U[1e3,5e5], sides {{128..1024}}, spp log-U[16,256], depth 6)"),
+           "scene_generate_s": round(t1 - t0, 3), "scene_upload_build_s": round(t2 - t1, 3),
+           "tuning": render.get_tuning(),
+           "batched": {"render_s": round(sec, 4), "renders_per_s": round(len(scs) / sec, 4),
+                       **_render_rate(ext, sh, pixels, sec)}}
+    _log(f"c4 batched: {sec:.3f} s")
+    if not args.no_plain:
+        check = [0, len(scs) // 2, len(scs) - 1]
+        batched = {i: d_films[i].to_numpy(np.float32, (jobs[i]["height"], jobs[i]["width"], 3)) for i in check}
+        s0 = time.perf_counter()
+        render.batch_render_device(scs, jobs, d_films, mode=render.PLAIN)
+        lbvh.synchronize()
+        psec = time.perf_counter() - s0
+        pe, ps = render.batch_rays()
+        same = all(np.array_equal(batched[i].view(np.uint32),
+                                  d_films[i].to_numpy(np.float32, batched[i].shape).view(np.uint32)) for i in check)
+        out["plain"] = {"render_s": round(psec, 4), "renders_per_s": round(len(scs) / psec, 4),
+                        **_render_rate(pe, ps, pixels, psec)}
+        out["speedup_vs_plain"] = round(psec / sec, 3)
+        out["equal_rmse"] = ("films bit-identical to the plain path (checked scenes "
+                             f"{check}: {same}), so the per-image relRMSE vs the CPU restatement is equal")
+        _log(f"c4 plain: {psec:.3f} s")
+    for f in d_films:
+        f.free()
+    return out, scs, jobs, dicts
+
+
+def bench_c5(args, c4_out, scs, jobs, dicts):
+    """Served stream: Poisson arrivals at a fraction of the measured batched
+    capacity, scenes drawn from the C4 mix, 30% of requests re-render a scene
+    after per-vertex N(0, 1e-3^2) jitter (refit, no rebuild), 16-spp slices,
+    200 GB HBM budget."""
+    from lux_amd import render
+
+    capacity = c4_out["batched"]["renders_per_s"]
+    rate = args.c5_load * capacity
+    rng = np.random.default_rng(0x4C555B)
+    n = args.c5_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals -= arrivals[0]
+    pick = rng.integers(0, len(scs), n)
+    # 30% of requests: a re-render of an already requested scene after jitter
+    verts, seen, rerender = [], set(), 0
+    for i in range(n):
+        k = int(pick[i])
+        if seen and rng.random() < 0.3:
+            k = int(rng.choice(sorted(seen)))
+            pick[i] = k
+            v = dicts[k]["vertices"]
+            verts.append((v + rng.normal(0, 1e-3, v.shape)).astype(np.float32))
+            rerender += 1
+        else:
+            verts.append(None)
+        seen.add(k)
+    sv = render.Server(hbm_budget=200 << 30, slice_spp=16)
+    req_scenes = [scs[int(k)] for k in pick]
+    req_jobs = [jobs[int(k)] for k in pick]
+    _, lat, wall = sv.run(req_scenes, req_jobs, arrivals, vertices=verts, want_films=False)
+    st = sv.stats()
+    sv.close()
+    lat_ms = lat * 1e3
+    paths = sum(j["width"] * j["height"] * j["spp"] for j in req_jobs)
+    out = {"workload": (f"{n} requests, Poisson arrivals at {rate:.3f}/s ({args.c5_load:.0%} of the measured "
+                        f"batched capacity {capacity:.3f} renders/s), scenes from the C4 mix, {rerender} re-renders "
+                        f"after sigma=1e-3 vertex jitter (refit), 16-spp slices, 200 GB HBM budget"),
+           "renders_per_s": round(n / wall, 4), "wall_s": round(wall, 3),
+           "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 2),
+           "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 2),
+           "mean_latency_ms": round(float(lat_ms.mean()), 2),
+           "refits": int(st["refits"]), "batches": int(st["batches"]), "paths": int(paths),
+           "peak_hbm_GB": round(st["peak_bytes"] / 1e9, 3),
+           "Mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2)}
+    _log(f"c5: {wall:.2f} s, p50 {out['p50_latency_ms']} ms p99 {out['p99_latency_ms']} ms")
+    return out
 
 
 def main():
@@ -239,6 +445,22 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(inp, min(args.cpu_sample_rays, R))
+    _log(f"lbvh: {out['value']} Mrays/s")
+    del d_boxes, d_ids, d_tris, d_jit, d_rays, d_qboxes, d_qids, d_hits, d_occ, d_pairs, d_found, bvh, qbvh
+
+    wanted = [w for w in args.render.split(",") if w and w != "none"]
+    if wanted:
+        out["render"] = {}
+    if "c3" in wanted:
+        out["render"]["c3_single_scene"] = bench_c3(args)
+    if "c4" in wanted or "c5" in wanted:
+        c4, scs, jobs, dicts = bench_c4(args)
+        if "c4" in wanted:
+            out["render"]["c4_batched"] = c4
+        if "c5" in wanted:
+            out["render"]["c5_served"] = bench_c5(args, c4, scs, jobs, dicts)
+        for sc in scs:
+            sc.close()
     print(json.dumps(out), flush=True)
 
 

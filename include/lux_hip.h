@@ -85,6 +85,11 @@ const char* lux_hip_last_error(void);
 /* Library version, e.g. 0x000100 for 0.1.0. */
 int lux_hip_version(void);
 
+/* Name, architecture (e.g. "gfx950:sramecc+:xnack-"), compute units and HBM
+ * size of the library's device; any pointer may be NULL. */
+int lux_hip_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* compute_units,
+                        uint64_t* global_mem_bytes);
+
 /* ---- device memory helpers (for callers without a GPU framework) ------- */
 int lux_hip_malloc(void** device_ptr, size_t bytes);
 int lux_hip_free(void* device_ptr);
@@ -250,6 +255,10 @@ int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out);
 int lux_hip_scene_destroy(lux_hip_scene* scene);
 /* Camera vectors {origin, lower_left, horizontal, vertical} (12 floats). */
 int lux_hip_scene_camera(lux_hip_scene* scene, float* out12);
+/* Replaces the camera with already built PerspectiveCamera vectors
+ * {origin, lower_left, horizontal, vertical} (src/camera/camera.h:29-38,
+ * pinhole: lens_radius 0), e.g. the Camera of a PreparedScene's Scene. */
+int lux_hip_scene_set_camera(lux_hip_scene* scene, const float* v12);
 /* Light table: count, and (if the arrays are non-NULL, sized by count) each
  * light's triangle, area, alias probability, alias index and pmf;
  * light_of_prim gets one entry per triangle (-1 = not an emitter). */
@@ -297,12 +306,14 @@ int lux_hip_batch_render_device(lux_hip_scene* const* scenes, const lux_render_s
 int lux_hip_batch_rays(lux_stream stream, uint64_t* extension_rays, uint64_t* shadow_rays);
 
 /* Launch geometry of the wavefront schedule: path slots per batch, workgroups
- * of the grid-stride kernels, graph replay on/off. Applies to renders started
- * after the call. No reference counterpart (the reference's capacity is
- * RenderSettings::batch_size, src/integrator/render_context.h:16-24). */
+ * of the grid-stride kernels, threads per workgroup (128, 256 or 512), graph
+ * replay on/off. Applies to renders started after the call. No reference
+ * counterpart (the reference's capacity is RenderSettings::batch_size,
+ * src/integrator/render_context.h:16-24). */
 typedef struct lux_render_tuning {
     int32_t capacity;
     int32_t grid;
+    int32_t block;
     int32_t use_graph;
 } lux_render_tuning;
 int lux_hip_set_render_tuning(const lux_render_tuning* tuning);

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "lux_hip_shutdown": [],
     "lux_hip_last_error": [],
     "lux_hip_version": [],
+    "lux_hip_device_info": [_P, c_int32, _P, c_int32, POINTER(c_int32), POINTER(c_uint64)],
     "lux_hip_malloc": [POINTER(c_void_p), c_size_t],
     "lux_hip_free": [_P],
     "lux_hip_memcpy_htod": [_P, _P, c_size_t],
@@ -95,6 +96,7 @@ _SIGNATURES = {
     "lux_hip_scene_create": [_P, POINTER(c_void_p)],
     "lux_hip_scene_destroy": [_P],
     "lux_hip_scene_camera": [_P, _P],
+    "lux_hip_scene_set_camera": [_P, _P],
     "lux_hip_scene_lights": [_P, POINTER(c_int32), _P, _P, _P, _P, _P, _P],
     "lux_hip_scene_bvh": [_P, POINTER(c_void_p)],
     "lux_hip_render": [_P, _P, _P],

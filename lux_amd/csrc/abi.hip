@@ -3,6 +3,7 @@
 // status codes + thread-local error text, and the kernel-time probe.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -185,6 +186,26 @@ int lux_hip_shutdown(void) {
         (void)hipStreamDestroy(g_stream);
         g_stream = nullptr;
         g_initialized = false;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* compute_units,
+                        uint64_t* global_mem_bytes) {
+    return guarded([&] {
+        require_init();
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties");
+        if (name && name_len > 0) {
+            std::strncpy(name, prop.name, size_t(name_len) - 1);
+            name[name_len - 1] = 0;
+        }
+        if (arch && arch_len > 0) {
+            std::strncpy(arch, prop.gcnArchName, size_t(arch_len) - 1);
+            arch[arch_len - 1] = 0;
+        }
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+        if (global_mem_bytes) *global_mem_bytes = uint64_t(prop.totalGlobalMem);
         return int(LUX_OK);
     });
 }
@@ -834,6 +855,16 @@ int lux_hip_scene_camera(lux_hip_scene* sc, float* out) {
     });
 }
 
+int lux_hip_scene_set_camera(lux_hip_scene* sc, const float* v12) {
+    return guarded([&] {
+        if (!sc || !v12) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(v12[k])) throw Error(LUX_ERR_INVALID_ARGUMENT, "camera vectors must be finite");
+        std::memcpy(&sc->dev.camera, v12, sizeof(float) * 12);
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float* area, float* prob, int32_t* alias,
                          float* pmf, int32_t* light_of_prim) {
     return guarded([&] {
@@ -1059,10 +1090,12 @@ int lux_hip_server_run(lux_hip_server* sv, int32_t n, lux_hip_scene* const* scen
 int lux_hip_set_render_tuning(const lux_render_tuning* t) {
     return guarded([&] {
         if (!t) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
-        if (t->capacity <= 0 || t->capacity > (1 << 28) || t->grid <= 0 || t->grid > (1 << 20))
+        if (t->capacity <= 0 || t->capacity > (1 << 28) || t->grid <= 0 || t->grid > (1 << 20) ||
+            (t->block != 128 && t->block != 256 && t->block != 512))
             throw Error(LUX_ERR_INVALID_ARGUMENT, "tuning out of range");
         g_tuning.capacity = t->capacity;
         g_tuning.grid = t->grid;
+        g_tuning.block = t->block;
         g_tuning.use_graph = t->use_graph ? 1 : 0;
         return int(LUX_OK);
     });
@@ -1073,6 +1106,7 @@ int lux_hip_get_render_tuning(lux_render_tuning* t) {
         if (!t) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
         t->capacity = g_tuning.capacity;
         t->grid = g_tuning.grid;
+        t->block = g_tuning.block;
         t->use_graph = g_tuning.use_graph;
         return int(LUX_OK);
     });

@@ -175,6 +175,20 @@ __device__ __forceinline__ bool box_overlap(float alox, float aloy, float aloz, 
 
 // ---- wave64 helpers --------------------------------------------------------
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// gfx950 hands workgroups to its 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), each XCD with its own L2. xcd_block() renumbers the workgroups so
+// that XCD x owns one contiguous run of per + (x < rem) virtual workgroups:
+// neighbouring work items (coherent rays, queue entries in path order) then
+// share one L2 instead of being spread over all eight. A bijection on
+// [0, gridDim.x) for every grid size.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t per = g / kXcds, rem = g % kXcds;
+    const uint32_t x = b % kXcds, k = b / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t l = lane_id();

@@ -148,7 +148,8 @@ __device__ __forceinline__ uint32_t segment_of_path(const Paths& ps, uint32_t pa
 // Batch slot j of segment s is request path r = req_begin + (j - slot_begin):
 // sample r / npix, pixel r % npix. Camera sample dimensions as
 // render_cpu_path_tracer (src/integrator/cpu_path_tracer.cpp:532-542).
-__global__ __launch_bounds__(kBlock) void k_generate(const BatchDesc* __restrict__ desc, Paths ps, Queues qs) {
+template <int B>
+__global__ __launch_bounds__(B) void k_generate(const BatchDesc* __restrict__ desc, Paths ps, Queues qs) {
     const uint32_t total = desc->count;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         qs.counts[0] = total;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(const BatchDesc* __restrict
         qs.counts[2] = 0;
         qs.counts[3] = 0;
     }
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
+    for (uint32_t j = xcd_block() * B + threadIdx.x; j < total; j += gridDim.x * B) {
         const uint32_t si = find_segment(desc, j);
         const Segment& sg = desc->seg[si];
         const int width = sg.width, height = sg.height;
@@ -230,7 +231,8 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
 // counted by counts[sh_in] (-1: none), in one grid-stride pass. Block 0 clears
 // the counters the following shade launch appends to (zero_a, zero_b; never
 // the ones read here) and adds the queue sizes to the ray totals.
-__global__ __launch_bounds__(kBlock) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
+template <int B>
+__global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
                                                   int sh_in, int zero_a, int zero_b) {
     const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
     const uint32_t n_sh = sh_in >= 0 ? qs.counts[sh_in] : 0u;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const BatchDesc* __restrict__ 
         qs.rays[1] += n_sh;
     }
     const uint32_t total = n_ext + n_sh;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < total; i += gridDim.x * kBlock) {
+    for (uint32_t i = xcd_block() * B + threadIdx.x; i < total; i += gridDim.x * B) {
         if (i < n_ext) {
             const uint32_t path = qs.q[ext_in][i];
             const float4 o = ps.ro[path], d = ps.rd[path];
@@ -259,14 +261,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(const BatchDesc* __restrict__ 
 // One bounce of trace_path for every live path of queue `in`. `filter` >= 0
 // restricts the launch to hits of that material type (misses go with filter 0).
 // Next rays go to queue in ^ 1, shadow records to counts[2 + (bounce & 1)].
-__global__ __launch_bounds__(kBlock) void k_shade(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int in,
+template <int B>
+__global__ __launch_bounds__(B) void k_shade(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int in,
                                                   int bounce, int filter) {
     const uint32_t count = qs.counts[in];
     uint32_t* const sh_count = qs.counts + 2 + (bounce & 1);
-    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t stride = gridDim.x * B;
     const uint32_t rounds = (count + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {  // uniform trip count: ballots see the whole wave
-        uint32_t i = r * stride + blockIdx.x * kBlock + threadIdx.x;
+        uint32_t i = r * stride + xcd_block() * B + threadIdx.x;
         bool live = i < count;
         uint32_t path = live ? qs.q[in][i] : 0u;
         bool push_ray = false, push_shadow = false;
@@ -451,9 +454,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(const BatchDesc* __restrict__ 
 // The first slot of each (segment, pixel) sums that pixel's samples of the
 // segment in sample order onto the film; a request's sample 0 overwrites the
 // film (0 + x == x for the non-negative radiance sums), later samples add.
-__global__ __launch_bounds__(kBlock) void k_flush(const BatchDesc* __restrict__ desc, Paths ps) {
+template <int B>
+__global__ __launch_bounds__(B) void k_flush(const BatchDesc* __restrict__ desc, Paths ps) {
     const uint32_t total = desc->count;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
+    for (uint32_t j = blockIdx.x * B + threadIdx.x; j < total; j += gridDim.x * B) {
         const Segment& sg = desc->seg[find_segment(desc, j)];
         const uint32_t local = j - sg.slot_begin;
         const uint64_t npix = uint64_t(sg.width) * uint64_t(sg.height);
@@ -560,13 +564,15 @@ Renderer::~Renderer() {
 
 void Renderer::set_tuning(const Tuning& t) {
     if (t.capacity <= 0 || t.grid <= 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "tuning needs capacity > 0 and grid > 0");
+    if (t.block != 128 && t.block != 256 && t.block != 512)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "tuning block must be 128, 256 or 512");
     tune_ = t;
 }
 
 void Renderer::release_graph() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     graph_ = nullptr;
-    graph_depth_ = graph_grid_ = graph_capacity_ = -1;
+    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = -1;
 }
 
 void Renderer::ensure_capacity(int paths) {
@@ -636,7 +642,7 @@ void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* sha
 namespace {
 // The wavefront launch sequence of one batch (captured or launched directly).
 // Raw launches (no probe events) when captured.
-template <bool kProbe>
+template <int B, bool kProbe>
 void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
@@ -644,9 +650,9 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
 #define PT_LAUNCH(NAME, K, ...)                                                      \
     do {                                                                             \
         if (kProbe) {                                                                \
-            LUX_LAUNCH(NAME, K, grid, kBlock, 0, stream, __VA_ARGS__);               \
+            LUX_LAUNCH(NAME, K<B>, grid, B, 0, stream, __VA_ARGS__);                 \
         } else {                                                                     \
-            hipLaunchKernelGGL(K, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__); \
+            hipLaunchKernelGGL(K<B>, dim3(grid), dim3(B), 0, stream, __VA_ARGS__);   \
             hip_check(hipGetLastError(), NAME);                                      \
         }                                                                            \
     } while (0)
@@ -662,18 +668,31 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
 }
 }  // namespace
 
+namespace {
+template <bool kProbe>
+void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+    switch (block) {
+        case 128: wavefront_sequence<128, kProbe>(buf, depth, grid, stream); break;
+        case 512: wavefront_sequence<512, kProbe>(buf, depth, grid, stream); break;
+        default: wavefront_sequence<256, kProbe>(buf, depth, grid, stream); break;
+    }
+}
+}  // namespace
+
 void Renderer::launch_wavefront(int depth, hipStream_t stream) {
-    const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, kBlock)));
+    const int block = tune_.block;
+    const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
     if (!tune_.use_graph) {
-        wavefront_sequence<true>(buf_, depth, grid, stream);
+        wavefront_sequence_b<true>(block, buf_, depth, grid, stream);
         return;
     }
-    if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity) {
+    if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity ||
+        graph_block_ != block) {
         release_graph();
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {
-            wavefront_sequence<false>(buf_, depth, grid, stream);
+            wavefront_sequence_b<false>(block, buf_, depth, grid, stream);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -686,6 +705,7 @@ void Renderer::launch_wavefront(int depth, hipStream_t stream) {
         graph_depth_ = depth;
         graph_grid_ = grid;
         graph_capacity_ = buf_.capacity;
+        graph_block_ = block;
     }
     hip_check(hipGraphLaunch(graph_, stream), "hipGraphLaunch");
 }
@@ -695,15 +715,15 @@ void Renderer::launch_plain(int depth, uint32_t count, hipStream_t stream) {
     const Queues qs = queues_of(buf_);
     const BatchDesc* d = buf_.desc;
     const int grid = std::max(1, div_up(count, kBlock));
-    LUX_LAUNCH("pt_generate", k_generate, grid, kBlock, 0, stream, d, ps, qs);
+    LUX_LAUNCH("pt_generate", k_generate<kBlock>, grid, kBlock, 0, stream, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        LUX_LAUNCH("pt_trace", k_trace, grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1, 2 + (b & 1));
+        LUX_LAUNCH("pt_trace", k_trace<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1, 2 + (b & 1));
         for (int type = 0; type < kMaterialTypes; ++type)
-            LUX_LAUNCH("pt_shade", k_shade, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
-        LUX_LAUNCH("pt_shadow", k_trace, grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1, -1);
+            LUX_LAUNCH("pt_shade", k_shade<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
+        LUX_LAUNCH("pt_shadow", k_trace<kBlock>, grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1, -1);
     }
-    LUX_LAUNCH("pt_flush", k_flush, grid, kBlock, 0, stream, d, ps);
+    LUX_LAUNCH("pt_flush", k_flush<kBlock>, grid, kBlock, 0, stream, d, ps);
 }
 
 uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,

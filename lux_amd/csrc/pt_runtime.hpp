@@ -121,6 +121,7 @@ struct RenderStats {
 struct Tuning {
     int capacity = 1 << 21;  // path slots per batch
     int grid = 2048;         // workgroups of the grid-stride kernels
+    int block = 256;         // threads per workgroup (128, 256 or 512)
     int use_graph = 1;       // replay each batch's bounce loop as one hipGraph
 };
 
@@ -171,7 +172,7 @@ private:
     bool ring_used_[kRing] = {};
     int ring_next_ = 0;
     hipGraphExec_t graph_ = nullptr;
-    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1;
+    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1;
     int64_t batch_counter_ = 0;
 };
 

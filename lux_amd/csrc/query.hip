@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __
                                                           int tri_count,
                                                           const lux_ray* __restrict__ rays, int n,
                                                           lux_hit* __restrict__ hits, DevInfo* info) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
+    int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
     RayIn r = load_ray(rays, i);
     float t;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __rest
                                                       int tri_count, const lux_ray* __restrict__ rays,
                                                       int n, uint8_t* __restrict__ occluded,
                                                       DevInfo* info) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
+    int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
     RayIn r = load_ray(rays, i);
     float t;

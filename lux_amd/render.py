@@ -36,7 +36,7 @@ class RenderSettings(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("capacity", c_int32), ("grid", c_int32), ("use_graph", c_int32)]
+    _fields_ = [("capacity", c_int32), ("grid", c_int32), ("block", c_int32), ("use_graph", c_int32)]
 
 
 assert ctypes.sizeof(Material) == 44
@@ -69,6 +69,9 @@ class Scene:
     def __init__(self, scene):
         lbvh.init()
         _register()
+        from . import autotune
+
+        autotune.apply()
         self._v = np.ascontiguousarray(scene["vertices"], np.float32)
         self._i = np.ascontiguousarray(scene["indices"], np.uint32)
         n = scene.get("normals")
@@ -188,12 +191,14 @@ def get_tuning():
     lbvh.init()
     t = Tuning()
     call("lux_hip_get_render_tuning", ctypes.byref(t))
-    return {"capacity": t.capacity, "grid": t.grid, "use_graph": t.use_graph}
+    return {"capacity": t.capacity, "grid": t.grid, "block": t.block, "use_graph": t.use_graph}
 
 
-def set_tuning(capacity=None, grid=None, use_graph=None):
+def set_tuning(capacity=None, grid=None, block=None, use_graph=None):
+    """Launch geometry of the wavefront schedule for renders started afterwards."""
     cur = get_tuning()
-    t = Tuning(capacity or cur["capacity"], grid or cur["grid"], cur["use_graph"] if use_graph is None else int(use_graph))
+    t = Tuning(capacity or cur["capacity"], grid or cur["grid"], block or cur["block"],
+               cur["use_graph"] if use_graph is None else int(use_graph))
     call("lux_hip_set_render_tuning", ctypes.byref(t))
 
 
