@@ -76,7 +76,7 @@ def calibration_mix():
     return jobs
 
 
-def tune(repeats=2, write=True, log=print):
+def tune(repeats=2, write=True, log=print, path=None):
     from . import lbvh, render
 
     mix = calibration_mix()
@@ -101,7 +101,7 @@ def tune(repeats=2, write=True, log=print):
     if write:
         table = load()
         table[device_key()] = entry
-        TUNING_FILE.write_text(json.dumps(table, indent=1) + "\n")
+        (path or TUNING_FILE).write_text(json.dumps(table, indent=1) + "\n")
     render.set_tuning(capacity=win["capacity"], grid=win["grid"], block=win["block"], use_graph=1)
     for f in films:
         f.free()
@@ -111,5 +111,8 @@ def tune(repeats=2, write=True, log=print):
 
 
 if __name__ == "__main__":
-    e = tune(write="--dry-run" not in sys.argv)
+    from pathlib import Path
+
+    out = Path(sys.argv[sys.argv.index("--out") + 1]) if "--out" in sys.argv else None
+    e = tune(write="--dry-run" not in sys.argv, path=out)
     print(json.dumps({k: v for k, v in e.items() if k != "candidates"}))

@@ -17,6 +17,8 @@ struct lux_hip_bvh {
     bool host_ids_known = false;     // host copy of the build ids is current
     std::vector<int32_t> host_ids;   // for refit's id check (src/bvh/gpu_bvh.cu:601-605)
     hipStream_t last_stream = nullptr;
+    lux::RaySortDev order;           // ray-order scratch of the traversal calls
+    bool order_rays = true;          // trace large ray sets in origin order
 };
 
 namespace lux {
@@ -284,7 +286,16 @@ int lux_hip_bvh_destroy(lux_hip_bvh* bvh) {
         if (!bvh) return int(LUX_OK);
         if (g_initialized) (void)hipStreamSynchronize(bvh->last_stream ? bvh->last_stream : g_stream);
         bvh_release(bvh->dev);
+        raysort_release(bvh->order);
         delete bvh;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_set_ray_order(lux_hip_bvh* bvh, int32_t enable) {
+    return guarded([&] {
+        require_handle(bvh);
+        bvh->order_rays = enable != 0;
         return int(LUX_OK);
     });
 }
@@ -514,7 +525,7 @@ int lux_hip_trace_closest(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tr
         hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
         hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
         bvh->last_stream = g_stream;
-        trace_closest(bvh->dev, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        trace_closest(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, dt.p, tri_count, dr.p, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(hits, dh.p, sizeof(lux_hit) * n, hipMemcpyDeviceToHost, g_stream), "D2H hits");
         sync_handle(bvh);
         return int(LUX_OK);
@@ -532,7 +543,7 @@ int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, i
         hipStream_t s = pick(stream);
         if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
         bvh->last_stream = s;
-        trace_closest(bvh->dev, d_tris, tri_count, d_rays, n, d_hits, s);
+        trace_closest(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, d_tris, tri_count, d_rays, n, d_hits, s);
         return int(LUX_OK);
     });
 }
@@ -554,7 +565,7 @@ int lux_hip_trace_any(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tri_co
         hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
         hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
         bvh->last_stream = g_stream;
-        trace_any(bvh->dev, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, dt.p, tri_count, dr.p, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(occluded, dh.p, size_t(n), hipMemcpyDeviceToHost, g_stream), "D2H occluded");
         sync_handle(bvh);
         return int(LUX_OK);
@@ -572,7 +583,7 @@ int lux_hip_trace_any_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32
         hipStream_t s = pick(stream);
         if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
         bvh->last_stream = s;
-        trace_any(bvh->dev, d_tris, tri_count, d_rays, n, d_occluded, s);
+        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, d_tris, tri_count, d_rays, n, d_occluded, s);
         return int(LUX_OK);
     });
 }
@@ -822,6 +833,7 @@ int lux_hip_scene_destroy(lux_hip_scene* sc) {
         sc->renderer.reset();
         pt::scene_release(sc->dev);
         sc->bvh_alias.dev = BvhDev{};
+        raysort_release(sc->bvh_alias.order);
         delete sc;
         return int(LUX_OK);
     });

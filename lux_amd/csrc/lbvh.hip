@@ -219,6 +219,43 @@ __global__ __launch_bounds__(kBlock) void k_morton_hist(const lux_aabb* __restri
     }
 }
 
+// Ray-order keys: the 30-bit Morton code of each ray origin in the tree's root
+// box (origins outside clamp to the boundary cells), payload = ray index, and
+// the histograms of the digits at bits 14..21 and 22..29 (hist zeroed before).
+__global__ __launch_bounds__(kBlock) void k_ray_keys(const lux_ray* __restrict__ rays, int n,
+                                                     const DevInfo* __restrict__ info,
+                                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                     uint32_t* hist) {
+    __shared__ uint32_t s_hist[2 * kRadix];
+    for (int w = threadIdx.x; w < 2 * kRadix; w += kBlock) s_hist[w] = 0;
+    __syncthreads();
+    const lux_aabb root = info->root;
+    const float dx = root.p_max.x - root.p_min.x, dy = root.p_max.y - root.p_min.y,
+                dz = root.p_max.z - root.p_min.z;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const lux_vec3 o = rays[i].origin;
+        float ox = dx > 0 ? (o.x - root.p_min.x) / dx : 0.5f;
+        float oy = dy > 0 ? (o.y - root.p_min.y) / dy : 0.5f;
+        float oz = dz > 0 ? (o.z - root.p_min.z) / dz : 0.5f;
+        // NaN origins quantize to cell 0 (fmaxf drops the NaN)
+        uint32_t c = (spread_bits3(quantize10(ox)) << 2) | (spread_bits3(quantize10(oy)) << 1) |
+                     spread_bits3(quantize10(oz));
+        keys[i] = c;
+        vals[i] = uint32_t(i);
+        atomicAdd(&s_hist[(c >> 14) & 0xffu], 1u);
+        atomicAdd(&s_hist[kRadix + ((c >> 22) & 0xffu)], 1u);
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < 2 * kRadix; w += kBlock) {
+        uint32_t c = s_hist[w];
+        if (c) __hip_atomic_fetch_add(&hist[w], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // 3. one stable LSD radix pass (8-bit digit) with decoupled look-back
 // ---------------------------------------------------------------------------
@@ -748,6 +785,42 @@ void bvh_release(BvhDev& d) {
         if (p) (void)hipFree(p);
     BvhDev empty;
     d = empty;
+}
+
+void raysort_release(RaySortDev& r) {
+    void* ptrs[] = {r.keys[0], r.keys[1], r.vals[0], r.vals[1], r.hist, r.status, r.tickets};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    r = RaySortDev{};
+}
+
+const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays, int n, hipStream_t s) {
+    if (n > r.capacity) {
+        if (r.capacity) hip_check(hipDeviceSynchronize(), "synchronize before ray-sort growth");
+        raysort_release(r);
+        r.capacity = n;
+        for (int b = 0; b < 2; ++b) {
+            r.keys[b] = dalloc<uint32_t>(n);
+            r.vals[b] = dalloc<uint32_t>(n);
+        }
+        r.hist = dalloc<uint32_t>(2 * kRadix);
+        r.status = dalloc<uint64_t>(size_t(sort_tiles(n)) * kRadix);
+        r.tickets = dalloc<uint32_t>(4);
+        hip_check(hipMemset(r.status, 0, sizeof(uint64_t) * size_t(sort_tiles(n)) * kRadix), "hipMemset status");
+        hip_check(hipMemset(r.tickets, 0, sizeof(uint32_t) * 4), "hipMemset tickets");
+    }
+    LUX_LAUNCH("ray_order_zero", k_zero_u32, 1, kBlock, 0, s, r.hist, 2 * kRadix);
+    LUX_LAUNCH("ray_order_keys", k_ray_keys, max(1, min(1024, div_up(n, kBlock * 16))), kBlock, 0, s, rays, n,
+               (const DevInfo*)tree.info, r.keys[0], r.vals[0], r.hist);
+    const int tiles = sort_tiles(n);
+    for (int p = 0; p < 2; ++p) {  // digits 14..21, then 22..29: the top 16 bits
+        const uint32_t epoch = ++r.epoch;
+        const int src = p & 1, dst = src ^ 1;
+        LUX_LAUNCH("ray_order_sort", k_sort_pass, tiles, kBlock, 0, s, (const uint32_t*)r.keys[src],
+                   (const uint32_t*)r.vals[src], r.keys[dst], r.vals[dst], n, 14 + 8 * p,
+                   (const uint32_t*)(r.hist + p * kRadix), r.status, r.tickets, epoch, tiles, tree.info);
+    }
+    return r.vals[0];
 }
 
 void bvh_allocate(BvhDev& d, int n) {

@@ -97,6 +97,24 @@ struct BvhDev {
 void bvh_allocate(BvhDev& d, int n);
 void bvh_release(BvhDev& d);
 
+// Scratch of the ray-order sort of the traversal entry points (grown on
+// demand; one per tree handle, used on one stream at a time).
+struct RaySortDev {
+    int capacity = 0;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;    // 2 x 256 digit counts
+    uint64_t* status = nullptr;  // epoch-tagged look-back words
+    uint32_t* tickets = nullptr;
+    uint32_t epoch = 0;
+};
+void raysort_release(RaySortDev& r);
+// Enqueues the origin order of `rays` and returns the device array of the n
+// ray indices sorted by the top 16 bits of the 30-bit Morton code of each
+// origin in the tree's root box (2 radix passes). Tracing in this order keeps
+// the rays in flight on one XCD spatially close, so their nodes stay in L2.
+const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays, int n, hipStream_t s);
+
 // Enqueue a full build of n primitives; in_boxes/in_ids are device arrays
 // (may alias d.boxes/d.ids).
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s);
@@ -105,10 +123,12 @@ void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);
 void lbvh_refit_triangles(BvhDev& d, const lux_triangle* tris, hipStream_t s);
 
 // Queries (all enqueue only).
-void trace_closest(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays,
-                   int n, lux_hit* hits, hipStream_t s);
-void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
-               uint8_t* occluded, hipStream_t s);
+// With `order` non-null and n large enough, the rays are traced in origin
+// order (results still land at each ray's own index).
+void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count,
+                   const lux_ray* rays, int n, lux_hit* hits, hipStream_t s);
+void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+               int n, uint8_t* occluded, hipStream_t s);
 void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s);
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s);

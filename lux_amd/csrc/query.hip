@@ -93,32 +93,43 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __
                                                           const lux_triangle* __restrict__ tris,
                                                           int tri_count,
                                                           const lux_ray* __restrict__ rays, int n,
+                                                          const uint32_t* __restrict__ order,
                                                           lux_hit* __restrict__ hits, DevInfo* info) {
-    int i = int(xcd_block()) * kBlock + threadIdx.x;
+    const int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
-    RayIn r = load_ray(rays, i);
+    const int ray = order ? int(order[i]) : i;  // order is a permutation of [0, n)
+    if (uint32_t(ray) >= uint32_t(n)) {          // only if the sort itself reported an error
+        set_error(info, kErrSpinLimit);
+        return;
+    }
+    RayIn r = load_ray(rays, ray);
     float t;
     uint32_t visits = 0, tests = 0;
     int id = walk<false>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
     lux_hit h;
     h.primitive_id = id;
     h.t = id >= 0 ? t : INFINITY;
-    hits[i] = h;
+    hits[ray] = h;
 }
 
 __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __restrict__ nodes,
                                                       int node_count,
                                                       const lux_triangle* __restrict__ tris,
                                                       int tri_count, const lux_ray* __restrict__ rays,
-                                                      int n, uint8_t* __restrict__ occluded,
-                                                      DevInfo* info) {
-    int i = int(xcd_block()) * kBlock + threadIdx.x;
+                                                      int n, const uint32_t* __restrict__ order,
+                                                      uint8_t* __restrict__ occluded, DevInfo* info) {
+    const int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
-    RayIn r = load_ray(rays, i);
+    const int ray = order ? int(order[i]) : i;  // order is a permutation of [0, n)
+    if (uint32_t(ray) >= uint32_t(n)) {          // only if the sort itself reported an error
+        set_error(info, kErrSpinLimit);
+        return;
+    }
+    RayIn r = load_ray(rays, ray);
     float t;
     uint32_t visits = 0, tests = 0;
     int id = walk<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
-    occluded[i] = id >= 0 ? 1 : 0;
+    occluded[ray] = id >= 0 ? 1 : 0;
 }
 
 // Same walks, counting node visits and triangle tests per ray (summed per wave,
@@ -230,20 +241,32 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const lux_bvh_node* __restri
 
 }  // namespace
 
-void trace_closest(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays,
-                   int n, lux_hit* hits, hipStream_t s) {
+namespace {
+// Below this many rays the origin sort costs more than the locality it buys.
+constexpr int kOrderMinRays = 1 << 16;
+
+const uint32_t* maybe_order(const BvhDev& d, RaySortDev* order, const lux_ray* rays, int n, hipStream_t s) {
+    if (!order || n < kOrderMinRays || d.n < 2) return nullptr;
+    return ray_order(*order, d, rays, n, s);
+}
+}  // namespace
+
+void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count,
+                   const lux_ray* rays, int n, lux_hit* hits, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    const uint32_t* ord = maybe_order(d, order, rays, n, s);
     LUX_LAUNCH("trace_closest", k_trace_closest, div_up(n, kBlock), kBlock, 0, s,
-               (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, hits, d.info);
+               (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, hits, d.info);
 }
 
-void trace_any(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
-               uint8_t* occluded, hipStream_t s) {
+void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+               int n, uint8_t* occluded, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    const uint32_t* ord = maybe_order(d, order, rays, n, s);
     LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
-               node_count, tris, tri_count, rays, n, occluded, d.info);
+               node_count, tris, tri_count, rays, n, ord, occluded, d.info);
 }
 
 void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
