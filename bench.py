@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--render", default="c3,c4,c5",
                     help="render workloads to add to the line: any of c3,c4,c5 (comma-separated) or none")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule baselines of c3/c4")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the wavefront bounce loop directly instead of replaying a captured hipGraph")
     ap.add_argument("--c4-scenes", type=int, default=64)
     ap.add_argument("--c5-requests", type=int, default=96)
     ap.add_argument("--c5-load", type=float, default=0.75, help="arrival rate as a fraction of the c4 capacity")
@@ -449,6 +451,11 @@ def main():
     del d_boxes, d_ids, d_tris, d_jit, d_rays, d_qboxes, d_qids, d_hits, d_occ, d_pairs, d_found, bvh, qbvh
 
     wanted = [w for w in args.render.split(",") if w and w != "none"]
+    if wanted and args.no_graph:
+        from lux_amd import autotune, render
+
+        autotune.apply()
+        render.set_tuning(use_graph=0)
     if wanted:
         out["render"] = {}
     if "c3" in wanted:

@@ -133,10 +133,11 @@ int lux_hip_bvh_refit_triangles_device(lux_hip_bvh* bvh,
                                        const lux_triangle* d_triangles, int32_t n,
                                        lux_stream stream);
 
-/* Traversal calls with at least 65 536 rays trace them in the order of their
- * origins' Morton codes (2-pass radix sort of the top 16 bits); hits and
- * occlusion flags still land at each ray's own index and are identical either
- * way. enable = 0 traces in input order. No reference counterpart. */
+/* Any-hit calls with at least 65 536 rays trace them in the order of their
+ * origins' Morton codes (2-pass radix sort of the top 16 bits); occlusion
+ * flags still land at each ray's own index and are identical either way.
+ * enable = 0 traces in input order. Closest-hit calls always trace in input
+ * order. No reference counterpart. */
 int lux_hip_bvh_set_ray_order(lux_hip_bvh* bvh, int32_t enable);
 
 /* GpuBvh::is_valid / primitive_count / root_bounds (src/bvh/gpu_bvh.cuh:35-47).

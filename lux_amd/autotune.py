@@ -21,7 +21,7 @@ from . import PACKAGE_DIR
 
 TUNING_FILE = PACKAGE_DIR / "tuning.json"
 CANDIDATES = {
-    "capacity": [1 << 19, 1 << 20, 1 << 21, 1 << 22],
+    "capacity": [1 << 21, 1 << 22, 1 << 23, 1 << 24],
     "grid": [1024, 2048, 4096, 8192],
     "block": [128, 256, 512],
 }

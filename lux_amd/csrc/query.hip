@@ -255,7 +255,12 @@ void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
-    const uint32_t* ord = maybe_order(d, order, rays, n, s);
+    // Closest-hit walks the whole segment [t_min, t_hit] in the fixed right-first
+    // order, so a ray's nodes lie along its whole line and origin order does not
+    // help (measured 12.95 ms ordered vs 12.46 ms in input order on 2^22 random
+    // rays over 2^20 triangles); any-hit stops early and gains from it.
+    (void)order;
+    const uint32_t* ord = nullptr;
     LUX_LAUNCH("trace_closest", k_trace_closest, div_up(n, kBlock), kBlock, 0, s,
                (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, hits, d.info);
 }
