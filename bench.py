@@ -384,12 +384,13 @@ def main():
         lbvh.synchronize()
         stage_ms[name] = round((time.perf_counter() - s0) * 1e3 / args.steps, 4)
     build()
-    stats = (ctypes.c_uint64 * 4)()
+    stats = (ctypes.c_uint64 * 6)()
     call("lux_hip_trace_stats_device", bvh.handle, d_tris.ptr, N, d_rays.ptr, R, stats)
 
     algo_bytes = 40 * R  # per ray: o, d, t_min, t_max in (32 B) + {id, t} out (8 B)
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
-    touched = 32 * stats[0] + 36 * stats[1]
+    # closest-hit walk (binary right-first array): 32-B nodes + 36-B triangles
+    touched = 32 * stats[0] + 32 * stats[1] + 36 * stats[2]
     traffic = None
     if TRAFFIC_FILE.exists():
         t = json.loads(TRAFFIC_FILE.read_text()).get("trace_closest")
@@ -432,7 +433,7 @@ def main():
             "touched_bytes_per_launch": int(touched),
             "touched_GBps": round(touched / (kernel_ms * 1e-3) / 1e9, 2),
             "node_visits_per_ray": round(stats[0] / R, 2),
-            "triangle_tests_per_ray": round(stats[1] / R, 3),
+            "triangle_tests_per_ray": round(stats[2] / R, 3),
         },
         "stages_ms": stage_ms,
         "stage_rates": {

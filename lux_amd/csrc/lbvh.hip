@@ -29,6 +29,7 @@
 // stable key order src/bvh/cpu_lbvh_builder.h:212-216, Karras 2012 range/split
 // with merged keys :63-131, internal box = union of children :139-159.
 #include "lux_runtime.hpp"
+#include "wide.hpp"
 
 namespace lux {
 namespace {
@@ -780,7 +781,8 @@ void bvh_release(BvhDev& d) {
                     d.left,      d.right,       d.parent,       d.first,      d.last,       d.leaf_parent,
                     d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
                     d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
-                    d.rtop_int,  d.rtop_leaf};
+                    d.rtop_int,  d.rtop_leaf,   d.wide,         d.wsrc,       d.wq[0],      d.wq[1],
+                    d.wcounts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     BvhDev empty;
@@ -859,6 +861,7 @@ void bvh_allocate(BvhDev& d, int n) {
     d.partials = dalloc<uint8_t>(sizeof(BoxIds) * kReduceBlocksMax);
     d.info = dalloc<DevInfo>(1);
     d.counter = dalloc<int32_t>(4);
+    d.wide_capacity = 0;  // the wide tree is allocated by its first build
     // One-time clears at allocation: look-back words (epoch 0 is never used),
     // tickets, arrival counters (even = idle), info, frontier count.
     hip_check(hipMemset(d.status, 0, sizeof(uint64_t) * d.status_words), "hipMemset status");
@@ -871,7 +874,10 @@ void bvh_allocate(BvhDev& d, int n) {
 
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s) {
     bvh_allocate(d, n);
-    if (n <= 0) return;
+    if (n <= 0) {
+        if (d.want_wide) wide_build(d, s);  // clears the record count
+        return;
+    }
     int copy = (in_boxes != d.boxes || in_ids != d.ids) ? 1 : 0;
     int rb = reduce_blocks(n);
     uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
@@ -883,6 +889,7 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
                    (const lux_triangle*)nullptr, (const int32_t*)d.ids, d.boxes, 0, d.nodes, d.pre_leaf,
                    d.keys[0], d.vals[0], d.info, 1);
         // root box from the one leaf (the reduction result is identical)
+        if (d.want_wide) wide_build(d, s);
         return;
     }
     int mh_blocks = max(1, min(1024, div_up(n, kBlock * 16)));
@@ -910,6 +917,7 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
     LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+    if (d.want_wide) wide_build(d, s);
 }
 
 namespace {
@@ -921,6 +929,7 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
     if (n == 1) {
         LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, in_boxes, tris, (const int32_t*)d.ids, d.boxes,
                    wb, d.nodes, d.pre_leaf, d.keys[0], d.vals[0], d.info, 0);
+        if (d.want_wide && d.wide_capacity) wide_refit(d, s);
         return;
     }
     LUX_LAUNCH(kMode == 2 ? "lbvh_local_tris" : "lbvh_local_boxes", k_local<kMode>, div_up(n, kLocalLeaves),
@@ -931,6 +940,7 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
     LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+    if (d.want_wide && d.wide_capacity) wide_refit(d, s);
 }
 }  // namespace
 

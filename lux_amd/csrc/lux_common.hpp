@@ -25,6 +25,7 @@ enum DeviceError : uint32_t {
     kErrClimbLimit = 1u << 2,     // bottom-up pass climbed more than n levels
     kErrPrimRange = 1u << 3,      // a leaf id is outside the triangle array
     kErrNegativeId = 1u << 4,     // an input primitive id is negative
+    kErrWideBuild = 1u << 5,      // the wide traversal tree could not be built
 };
 
 // Per-handle device scratch that survives across kernels of one build.
@@ -34,7 +35,8 @@ struct DevInfo {
     uint32_t error;        // DeviceError bits
     uint32_t max_depth;    // deepest leaf (root = depth 0)
     uint32_t done_blocks;  // last-block-done counter of the root reduction
-    uint32_t pad[3];
+    uint32_t wide_count;   // records of the wide traversal tree (0 = none)
+    uint32_t pad[2];
 };
 
 struct f3 {

@@ -62,6 +62,8 @@ inline int div_up(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 //   nodes[2n-1]                   32-byte stackless nodes, right-first preorder
 //   sort scratch: hist[4*256], status[tiles*256] (epoch-tagged look-back words),
 //                 tickets[1], partials for the root reduction
+struct WideNode;
+
 struct BvhDev {
     int n = 0;
     int capacity = 0;
@@ -92,6 +94,14 @@ struct BvhDev {
     int32_t* counter = nullptr;
     uint32_t epoch = 0;
     int status_words = 0;
+    // 8-wide traversal tree (wide.hpp): records, the binary positions of each
+    // record's children (for refits), two level queues and 4 counters
+    WideNode* wide = nullptr;
+    uint32_t* wsrc = nullptr;
+    uint4* wq[2] = {nullptr, nullptr};
+    uint32_t* wcounts = nullptr;
+    int wide_capacity = 0;
+    bool want_wide = false;  // build/refit the wide tree with the binary one
 };
 
 void bvh_allocate(BvhDev& d, int n);
@@ -118,18 +128,24 @@ const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays
 // Enqueue a full build of n primitives; in_boxes/in_ids are device arrays
 // (may alias d.boxes/d.ids).
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s);
+// Enqueue the collapse of the built binary tree into the wide traversal tree
+// (allocating it on first use), and its refresh after a refit; build and refit
+// call them when `want_wide` is set.
+void wide_build(BvhDev& d, hipStream_t s);
+void wide_refit(BvhDev& d, hipStream_t s);
 // Enqueue a refit with new boxes (input order) or triangles (input order).
 void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);
 void lbvh_refit_triangles(BvhDev& d, const lux_triangle* tris, hipStream_t s);
 
 // Queries (all enqueue only).
-// With `order` non-null and n large enough, the rays are traced in origin
-// order (results still land at each ray's own index).
-void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count,
+// With `order` non-null and n large enough, any-hit rays are traced in origin
+// order (results still land at each ray's own index). `wide` selects the wide
+// tree walk (default) or the binary right-first walk; results are identical.
+void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s);
-void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
-               int n, uint8_t* occluded, hipStream_t s);
-void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+void trace_any(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
+               const lux_ray* rays, int n, uint8_t* occluded, hipStream_t s);
+void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s);
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "lux_runtime.hpp"
+#include "wide.hpp"
 
 namespace lux {
 namespace {
@@ -88,7 +89,31 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
     return best;
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __restrict__ nodes,
+// One ray through the wide tree (kWideWalk) or the binary right-first array;
+// both give bit-identical results (wide.hpp). Counters: records or nodes
+// loaded, exact leaf boxes loaded (wide walk only), triangle tests.
+template <bool kAny, bool kWideWalk, bool kCount>
+__device__ __forceinline__ int trace_one(const WideNode* __restrict__ wide, const lux_bvh_node* __restrict__ nodes,
+                                         int node_count, const lux_triangle* __restrict__ tris, int tri_count,
+                                         const RayIn& r, float& t, DevInfo* info, uint32_t& visits,
+                                         uint32_t& leaf_boxes, uint32_t& tests) {
+    if (kWideWalk) {
+        WideStats st;
+        float t_max = r.t_max, u = 0.0f, v = 0.0f;
+        int id = wide_walk<kAny, kCount>(wide, info->wide_count, nodes, node_count, tris, tri_count, r.o, r.d,
+                                         r.t_min, t_max, -1, u, v, info, st);
+        t = t_max;
+        visits += st.records;
+        leaf_boxes += st.leaf_boxes;
+        tests += st.tests;
+        return id;
+    }
+    return walk<kAny>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
+}
+
+template <bool kWideWalk>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(const WideNode* __restrict__ wide,
+                                                          const lux_bvh_node* __restrict__ nodes,
                                                           int node_count,
                                                           const lux_triangle* __restrict__ tris,
                                                           int tri_count,
@@ -104,15 +129,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __
     }
     RayIn r = load_ray(rays, ray);
     float t;
-    uint32_t visits = 0, tests = 0;
-    int id = walk<false>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
+    uint32_t visits = 0, leaf_boxes = 0, tests = 0;
+    int id = trace_one<false, kWideWalk, false>(wide, nodes, node_count, tris, tri_count, r, t, info, visits,
+                                                leaf_boxes, tests);
     lux_hit h;
     h.primitive_id = id;
     h.t = id >= 0 ? t : INFINITY;
     hits[ray] = h;
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __restrict__ nodes,
+template <bool kWideWalk>
+__global__ __launch_bounds__(kBlock) void k_trace_any(const WideNode* __restrict__ wide,
+                                                      const lux_bvh_node* __restrict__ nodes,
                                                       int node_count,
                                                       const lux_triangle* __restrict__ tris,
                                                       int tri_count, const lux_ray* __restrict__ rays,
@@ -127,30 +155,35 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __rest
     }
     RayIn r = load_ray(rays, ray);
     float t;
-    uint32_t visits = 0, tests = 0;
-    int id = walk<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
+    uint32_t visits = 0, leaf_boxes = 0, tests = 0;
+    int id = trace_one<true, kWideWalk, false>(wide, nodes, node_count, tris, tri_count, r, t, info, visits,
+                                               leaf_boxes, tests);
     occluded[ray] = id >= 0 ? 1 : 0;
 }
 
-// Same walks, counting node visits and triangle tests per ray (summed per wave,
-// one atomic per wave): stats[0..1] closest-hit, stats[2..3] any-hit. Used to
-// state the bytes a traversal actually touches (32 B per node, 36 B per triangle).
-__global__ __launch_bounds__(kBlock) void k_trace_stats(const lux_bvh_node* __restrict__ nodes,
+// The walks of the handle, counting per ray the nodes loaded (32-byte binary
+// nodes, or 128-byte wide records), the exact leaf boxes loaded separately
+// (none in either layout) and the triangle tests (summed per wave, one atomic
+// per wave): stats[0..2] closest-hit, stats[3..5] any-hit. Used to state the
+// bytes a traversal actually touches.
+template <bool kWideWalk>
+__global__ __launch_bounds__(kBlock) void k_trace_stats(const WideNode* __restrict__ wide,
+                                                        const lux_bvh_node* __restrict__ nodes,
                                                         int node_count,
                                                         const lux_triangle* __restrict__ tris,
                                                         int tri_count, const lux_ray* __restrict__ rays,
                                                         int n, unsigned long long* stats, DevInfo* info) {
     int i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t v0 = 0, t0 = 0, v1 = 0, t1 = 0;
+    uint32_t v0 = 0, l0 = 0, t0 = 0, v1 = 0, l1 = 0, t1 = 0;
     if (i < n) {
         RayIn r = load_ray(rays, i);
         float t;
-        walk<false>(nodes, node_count, tris, tri_count, r, t, info, v0, t0);
-        walk<true>(nodes, node_count, tris, tri_count, r, t, info, v1, t1);
+        trace_one<false, kWideWalk, true>(wide, nodes, node_count, tris, tri_count, r, t, info, v0, l0, t0);
+        trace_one<true, kWideWalk, true>(wide, nodes, node_count, tris, tri_count, r, t, info, v1, l1, t1);
     }
-    unsigned long long c[4] = {v0, t0, v1, t1};
+    unsigned long long c[6] = {v0, l0, t0, v1, l1, t1};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 6; ++k) {
         unsigned long long x = c[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
         if (lane_id() == 0) atomicAdd(&stats[k], x);
@@ -251,7 +284,7 @@ const uint32_t* maybe_order(const BvhDev& d, RaySortDev* order, const lux_ray* r
 }
 }  // namespace
 
-void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count,
+void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
@@ -261,25 +294,38 @@ void trace_closest(const BvhDev& d, RaySortDev* order, const lux_triangle* tris,
     // rays over 2^20 triangles); any-hit stops early and gains from it.
     (void)order;
     const uint32_t* ord = nullptr;
-    LUX_LAUNCH("trace_closest", k_trace_closest, div_up(n, kBlock), kBlock, 0, s,
-               (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, hits, d.info);
+    if (wide)
+        LUX_LAUNCH("trace_closest", k_trace_closest<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
+                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, hits, d.info);
+    else
+        LUX_LAUNCH("trace_closest", k_trace_closest<false>, div_up(n, kBlock), kBlock, 0, s,
+                   (const WideNode*)d.wide, (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord,
+                   hits, d.info);
 }
 
-void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
-               int n, uint8_t* occluded, hipStream_t s) {
+void trace_any(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
+               const lux_ray* rays, int n, uint8_t* occluded, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
     const uint32_t* ord = maybe_order(d, order, rays, n, s);
-    LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
-               node_count, tris, tri_count, rays, n, ord, occluded, d.info);
+    if (wide)
+        LUX_LAUNCH("trace_any", k_trace_any<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
+                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
+    else
+        LUX_LAUNCH("trace_any", k_trace_any<false>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
+                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
 }
 
-void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s) {
-    LUX_LAUNCH("trace_stats_zero", k_zero_u64, 1, 64, 0, s, d_stats, 4);
+    LUX_LAUNCH("trace_stats_zero", k_zero_u64, 1, 64, 0, s, d_stats, 6);
     if (n <= 0 || d.n <= 0) return;
-    LUX_LAUNCH("trace_stats", k_trace_stats, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.nodes,
-               2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
+    if (wide)
+        LUX_LAUNCH("trace_stats", k_trace_stats<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
+                   (const lux_bvh_node*)d.nodes, 2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
+    else
+        LUX_LAUNCH("trace_stats", k_trace_stats<false>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
+                   (const lux_bvh_node*)d.nodes, 2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
 }
 
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
