@@ -782,11 +782,101 @@ void bvh_release(BvhDev& d) {
                     d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
                     d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
                     d.rtop_int,  d.rtop_leaf,   d.wide,         d.wsrc,       d.wq[0],      d.wq[1],
-                    d.wcounts};
+                    d.wcounts,   d.pairs,       d.pair_pos};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     BvhDev empty;
     d = empty;
+}
+
+// ---------------------------------------------------------------------------
+// sibling-pair traversal copy of the node array
+// ---------------------------------------------------------------------------
+// Internal node i (Karras order) sits at preorder position k = pre_internal[i];
+// in right-first preorder the nodes before it are the n-1-last(i) leaves right
+// of its range and k-(n-1-last(i)) internal nodes, which is its internal rank
+// r. Its children go to the pair r+1: right child (position k+1) at slot
+// 2r+2, left child (escape of k+1) at slot 2r+3.
+__global__ __launch_bounds__(kBlock) void k_pair_layout(int n, const uint32_t* __restrict__ last,
+                                                        const int32_t* __restrict__ pre_internal,
+                                                        const lux_bvh_node* __restrict__ nodes,
+                                                        uint32_t* __restrict__ pair_pos, DevInfo* info) {
+    const int node_count = 2 * n - 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) pair_pos[0] = 0;  // the root (leaf or internal) stays at slot 0
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n - 1; i += gridDim.x * kBlock) {
+        const int k = pre_internal[i];
+        const int rank = k - (n - 1 - int(last[i]));
+        if (k < 0 || k + 1 >= node_count || rank < 0 || rank >= n - 1) {
+            set_error(info, kErrWalkOrder);
+            continue;
+        }
+        const int left = nodes[k + 1].escape;
+        if (left <= k + 1 || left >= node_count) {
+            set_error(info, kErrWalkOrder);
+            continue;
+        }
+        pair_pos[k + 1] = uint32_t(2 * rank + 2);
+        pair_pos[left] = uint32_t(2 * rank + 3);
+    }
+}
+
+// Copies every node to its slot: boxes and leaf ids as they are, internal
+// nodes point at their right child's slot, escapes are translated to slots.
+// Slot 1 (the root's unused sibling) gets an empty box that no walk reaches.
+__global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node* __restrict__ nodes,
+                                                      const uint32_t* __restrict__ pair_pos,
+                                                      lux_bvh_node* __restrict__ pairs, DevInfo* info) {
+    const int node_count = 2 * n - 1;
+    const int slots = 2 * n;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < node_count; k += gridDim.x * kBlock) {
+        lux_bvh_node nd = nodes[k];
+        const uint32_t dst = pair_pos[k];
+        if (dst >= uint32_t(slots)) {
+            set_error(info, kErrWalkOrder);
+            continue;
+        }
+        if (nd.primitive_id < 0) {
+            const uint32_t rc = (k + 1 < node_count) ? pair_pos[k + 1] : uint32_t(slots);
+            if (rc >= uint32_t(slots)) {
+                set_error(info, kErrWalkOrder);
+                continue;
+            }
+            nd.primitive_id = ~int32_t(rc);
+        }
+        if (nd.escape >= 0) {
+            const uint32_t e = nd.escape < node_count ? pair_pos[nd.escape] : uint32_t(slots);
+            if (e >= uint32_t(slots)) {
+                set_error(info, kErrWalkOrder);
+                continue;
+            }
+            nd.escape = int32_t(e);
+        }
+        pairs[dst] = nd;
+    }
+    if (n >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+        lux_bvh_node pad;
+        pad.bounds = box_empty();
+        pad.primitive_id = -1;
+        pad.escape = -1;
+        pairs[1] = pad;
+    }
+}
+
+void pair_fill(BvhDev& d, hipStream_t s) {
+    if (d.n <= 0) return;
+    LUX_LAUNCH("lbvh_pair_fill", k_pair_fill, max(1, min(2048, div_up(2 * d.n - 1, kBlock))), kBlock, 0, s, d.n,
+               (const lux_bvh_node*)d.nodes, (const uint32_t*)d.pair_pos, d.pairs, d.info);
+}
+
+void pair_build(BvhDev& d, hipStream_t s) {
+    if (d.n <= 0) return;
+    if (d.n >= 2)
+        LUX_LAUNCH("lbvh_pair_layout", k_pair_layout, max(1, min(2048, div_up(d.n - 1, kBlock))), kBlock, 0, s, d.n,
+                   (const uint32_t*)d.last, (const int32_t*)d.pre_internal, (const lux_bvh_node*)d.nodes,
+                   d.pair_pos, d.info);
+    else
+        LUX_LAUNCH("lbvh_pair_layout", k_zero_u32, 1, 64, 0, s, d.pair_pos, 1);
+    pair_fill(d, s);
 }
 
 void raysort_release(RaySortDev& r) {
@@ -862,6 +952,8 @@ void bvh_allocate(BvhDev& d, int n) {
     d.info = dalloc<DevInfo>(1);
     d.counter = dalloc<int32_t>(4);
     d.wide_capacity = 0;  // the wide tree is allocated by its first build
+    d.pairs = dalloc<lux_bvh_node>(2 * size_t(cap));
+    d.pair_pos = dalloc<uint32_t>(2 * size_t(cap));
     // One-time clears at allocation: look-back words (epoch 0 is never used),
     // tickets, arrival counters (even = idle), info, frontier count.
     hip_check(hipMemset(d.status, 0, sizeof(uint64_t) * d.status_words), "hipMemset status");
@@ -889,6 +981,7 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
                    (const lux_triangle*)nullptr, (const int32_t*)d.ids, d.boxes, 0, d.nodes, d.pre_leaf,
                    d.keys[0], d.vals[0], d.info, 1);
         // root box from the one leaf (the reduction result is identical)
+        pair_build(d, s);
         if (d.want_wide) wide_build(d, s);
         return;
     }
@@ -917,6 +1010,7 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
     LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+    pair_build(d, s);
     if (d.want_wide) wide_build(d, s);
 }
 
@@ -929,6 +1023,7 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
     if (n == 1) {
         LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, in_boxes, tris, (const int32_t*)d.ids, d.boxes,
                    wb, d.nodes, d.pre_leaf, d.keys[0], d.vals[0], d.info, 0);
+        pair_fill(d, s);
         if (d.want_wide && d.wide_capacity) wide_refit(d, s);
         return;
     }
@@ -940,6 +1035,7 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
     LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+    pair_fill(d, s);
     if (d.want_wide && d.wide_capacity) wide_refit(d, s);
 }
 }  // namespace

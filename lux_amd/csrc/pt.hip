@@ -77,8 +77,9 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
     return base + uint32_t(__popcll(m & lanemask_lt()));
 }
 
-// Closest hit / occlusion walk over the right-first stackless LBVH; `ignore`
-// skips one leaf's triangle test (the source primitive of a shadow ray).
+// Closest hit / occlusion walk over the sibling-pair copy of the right-first
+// stackless LBVH (same visits and order as the preorder walk); `ignore` skips
+// one leaf's triangle test (the source primitive of a shadow ray).
 __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
                                        int ignore, float& t_hit, float& u_hit, float& v_hit) {
     const lux_bvh_node* nodes = sv.nodes;
@@ -88,6 +89,7 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
     SlabRay sr = make_slab_ray(O, D, t_min);
     int best = -1;
     int k = 0;
+    int guard = node_count + 1;  // a walk visits each node at most once
     while (uint32_t(k) < uint32_t(node_count)) {
         float4 a, b;
         load_node(nodes, k, a, b);
@@ -114,9 +116,9 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
                 }
             }
         } else {
-            next = k + 1;
+            next = ~prim;  // right child's slot
         }
-        if (next >= 0 && next <= k) {
+        if (--guard == 0) {
             set_error(sv.info, kErrWalkOrder);
             break;
         }

@@ -332,8 +332,8 @@ void scene_release(SceneDev& s) {
 
 SceneView scene_view(const SceneDev& s) {
     SceneView v{};
-    v.nodes = s.bvh.nodes;
-    v.node_count = s.bvh.n > 0 ? 2 * s.bvh.n - 1 : 0;
+    v.nodes = s.bvh.pairs;  // sibling-pair traversal copy
+    v.node_count = s.bvh.n > 1 ? 2 * s.bvh.n : (s.bvh.n > 0 ? 1 : 0);
     v.soup = s.soup;
     v.tri_count = s.tri_count;
     v.indices = s.indices;

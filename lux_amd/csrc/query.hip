@@ -41,6 +41,9 @@ __device__ __forceinline__ void load_tri(const lux_triangle* tris, int i, f3& v0
     v2 = make3(p[6], p[7], p[8]);
 }
 
+// Right-first walk over the sibling-pair copy of the node array (`nodes` =
+// BvhDev::pairs, node_count = its slot count): the same node visits in the
+// same order as the preorder walk, with a node's siblings on one cache line.
 // kAny = false: closest hit; kAny = true: stop at the first accepted triangle.
 // visits/tests count node loads and triangle tests (only read by the stats kernel).
 template <bool kAny>
@@ -52,6 +55,7 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
     float t_max = ray.t_max;
     int best = -1;
     int k = 0;
+    int guard = node_count + 1;  // a walk visits each node at most once
     while (uint32_t(k) < uint32_t(node_count)) {
         float4 a, b;
         load_node(nodes, k, a, b);
@@ -77,9 +81,9 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
                 }
             }
         } else {
-            next = k + 1;
+            next = ~prim;  // right child's slot
         }
-        if (next >= 0 && next <= k) {
+        if (--guard == 0) {
             set_error(info, kErrWalkOrder);
             break;
         }
@@ -287,7 +291,7 @@ const uint32_t* maybe_order(const BvhDev& d, RaySortDev* order, const lux_ray* r
 void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s) {
     if (n <= 0) return;
-    int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
     // Closest-hit walks the whole segment [t_min, t_hit] in the fixed right-first
     // order, so a ray's nodes lie along its whole line and origin order does not
     // help (measured 12.95 ms ordered vs 12.46 ms in input order on 2^22 random
@@ -296,24 +300,24 @@ void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_tria
     const uint32_t* ord = nullptr;
     if (wide)
         LUX_LAUNCH("trace_closest", k_trace_closest<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, hits, d.info);
+                   (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord, hits, d.info);
     else
         LUX_LAUNCH("trace_closest", k_trace_closest<false>, div_up(n, kBlock), kBlock, 0, s,
-                   (const WideNode*)d.wide, (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord,
+                   (const WideNode*)d.wide, (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord,
                    hits, d.info);
 }
 
 void trace_any(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
                const lux_ray* rays, int n, uint8_t* occluded, hipStream_t s) {
     if (n <= 0) return;
-    int node_count = d.n > 0 ? 2 * d.n - 1 : 0;
+    int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
     const uint32_t* ord = maybe_order(d, order, rays, n, s);
     if (wide)
         LUX_LAUNCH("trace_any", k_trace_any<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
+                   (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
     else
         LUX_LAUNCH("trace_any", k_trace_any<false>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.nodes, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
+                   (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
 }
 
 void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
@@ -322,10 +326,10 @@ void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_c
     if (n <= 0 || d.n <= 0) return;
     if (wide)
         LUX_LAUNCH("trace_stats", k_trace_stats<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.nodes, 2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
+                   (const lux_bvh_node*)d.pairs, d.n > 1 ? 2 * d.n : 1, tris, tri_count, rays, n, d_stats, d.info);
     else
         LUX_LAUNCH("trace_stats", k_trace_stats<false>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.nodes, 2 * d.n - 1, tris, tri_count, rays, n, d_stats, d.info);
+                   (const lux_bvh_node*)d.pairs, d.n > 1 ? 2 * d.n : 1, tris, tri_count, rays, n, d_stats, d.info);
 }
 
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
