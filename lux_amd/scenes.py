@@ -83,6 +83,62 @@ def cornell(width=256, height=256):
     return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
 
 
+def mixed_materials(width=40, height=40):
+    """Every material type and both emitter kinds in the Cornell room: a thin
+    Lambert left wall, a transparent sheet and an interface panel (pass-through
+    surfaces, also for shadow rays), a metallic GGX box, a rough dielectric-free
+    GGX box, a glass box, the single-sided ceiling light and a two-sided
+    emitting panel hanging in the room."""
+    f = np.float32
+    materials = [
+        material(LAMBERT, (0.78, 0.78, 0.74)),                                          # 0 white
+        material(THIN_LAMBERT, (0.65, 0.10, 0.08)),                                     # 1 thin red wall
+        material(LAMBERT, (0.12, 0.45, 0.15)),                                          # 2 green
+        material(LAMBERT, (0, 0, 0), emission=(15.0, 14.0, 12.0)),                      # 3 ceiling light
+        material(LAMBERT, (0, 0, 0), emission=(2.0, 3.0, 6.0), two_sided=1),            # 4 two-sided panel
+        material(TRANSPARENT, (0.6, 0.8, 0.9)),                                         # 5 tinted sheet
+        material(INTERFACE, (1.0, 1.0, 1.0)),                                           # 6 interface panel
+        material(GGX, (0.9, 0.7, 0.4), roughness=f(0.2), metallic=f(1.0)),              # 7 gold-ish metal
+        material(GGX, (0.3, 0.5, 0.8), roughness=f(0.45), metallic=f(0.0)),             # 8 rough plastic
+        material(DIELECTRIC, (1.0, 1.0, 1.0), roughness=0.0, eta=f(1.5)),               # 9 glass
+    ]
+    tris, mats = [], []
+
+    def quad(p0, p1, p2, p3, m):
+        tris.append([*p0, *p1, *p2])
+        tris.append([*p0, *p2, *p3])
+        mats.extend([m, m])
+
+    def box(lo, hi, m):
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        p000, p001, p010, p011 = (x0, y0, z0), (x0, y0, z1), (x0, y1, z0), (x0, y1, z1)
+        p100, p101, p110, p111 = (x1, y0, z0), (x1, y0, z1), (x1, y1, z0), (x1, y1, z1)
+        quad(p000, p100, p110, p010, m)
+        quad(p101, p001, p011, p111, m)
+        quad(p001, p000, p010, p011, m)
+        quad(p100, p101, p111, p110, m)
+        quad(p010, p110, p111, p011, m)
+        quad(p001, p101, p100, p000, m)
+
+    quad((-1, 0, 2), (1, 0, 2), (1, 0, 0), (-1, 0, 0), 0)
+    quad((-1, 2, 0), (1, 2, 0), (1, 2, 2), (-1, 2, 2), 0)
+    quad((-1, 0, 0), (1, 0, 0), (1, 2, 0), (-1, 2, 0), 0)
+    quad((-1, 0, 2), (-1, 0, 0), (-1, 2, 0), (-1, 2, 2), 1)
+    quad((1, 0, 0), (1, 0, 2), (1, 2, 2), (1, 2, 0), 2)
+    box((f(-0.7), 0, f(0.3)), (f(-0.3), f(0.7), f(0.7)), 7)
+    box((f(0.3), 0, f(0.9)), (f(0.7), f(0.5), f(1.3)), 9)
+    box((f(-0.2), 0, f(0.15)), (f(0.2), f(0.3), f(0.45)), 8)
+    quad((f(-0.9), f(0.05), f(1.6)), (f(-0.1), f(0.05), f(1.6)), (f(-0.1), f(1.1), f(1.6)),
+         (f(-0.9), f(1.1), f(1.6)), 5)
+    quad((f(0.1), f(0.6), f(1.5)), (f(0.9), f(0.6), f(1.5)), (f(0.9), f(1.4), f(1.5)), (f(0.1), f(1.4), f(1.5)), 6)
+    quad((f(-0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(1.35)),
+         (f(-0.35), f(1.99), f(1.35)), 3)
+    quad((f(0.2), f(1.2), f(0.4)), (f(0.6), f(1.2), f(0.4)), (f(0.6), f(1.6), f(0.4)), (f(0.2), f(1.6), f(0.4)), 4)
+    aspect = np.float32(width) / np.float32(height)
+    return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
+
+
 # ---- seeded displaced icosphere -----------------------------------------------
 
 def _icosahedron():

@@ -9,6 +9,13 @@ Cases:
   pt_cornell_c1   Cornell box 256x256, 64 spp, depth 4, seed 0: the C1 oracle image
   pt_icosphere    displaced icosphere (freq 12, 2882 tris, Lambert/GGX/dielectric,
                   smooth normals) 40x40, 8 spp, depth 8, seed 7: film + 2048 samples
+  pt_mixed        every material type (thin Lambert, transparent, interface,
+                  metallic GGX, GGX, dielectric) and a two-sided emitter,
+                  40x40, 8 spp, depth 6, seed 3, frame 2: film + 2048 samples
+  pt_mixed_d0     the same scene at depth 0: black, since the only emitter in
+                  view sits behind the pass-through panel (the bounce limit
+                  counts pass-through hits)
+  pt_mixed_d1     the same scene at depth 1
 Each stores the scene arrays, the camera and light tables the reference derived.
 """
 
@@ -26,24 +33,24 @@ from lux_amd import scenes  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
-def case(name, scene, w, h, spp, depth, seed, sample_pixels=None):
+def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
     r = oracle.RefPathTracer(scene)
-    film = r.render(w, h, spp, depth, seed=seed)
+    film = r.render(w, h, spp, depth, seed=seed, frame=frame)
     data = {k: scene[k] for k in ("vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera")}
     data.update(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
-                seed=np.uint64(seed), film_sum=film, camera_vectors=r.camera())
+                seed=np.uint64(seed), frame=np.int32(frame), film_sum=film, camera_vectors=r.camera())
     for k, v in r.lights().items():
         data["light_" + k] = v
     if sample_pixels is not None:
         px, sm = sample_pixels
         data["sample_pixels"] = px.astype(np.int32)
         data["sample_index"] = sm.astype(np.int32)
-        data["sample_radiance"] = r.samples(w, h, depth, px, sm, seed=seed)
+        data["sample_radiance"] = r.samples(w, h, depth, px, sm, seed=seed, frame=frame)
     np.savez_compressed(OUT / f"{name}.npz", **data)
     print(f"{name}: {w}x{h} spp={spp} depth={depth} tris={len(scene['indices'])} mean={film.mean() / spp:.5f}")
 
 
-def main():
+def main(only=None):
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libluxref.so missing: run `make -f oracle/ref.mk` where the reference exists")
     w = h = 32
@@ -56,6 +63,13 @@ def main():
     px = rng.integers(0, 40 * 40, 2048)
     sm = rng.integers(0, 8, 2048)
     case("pt_icosphere", ico, 40, 40, 8, 8, 7, (px, sm))
+    mixed = scenes.mixed_materials(40, 40)
+    rng = np.random.default_rng(6)
+    px = rng.integers(0, 40 * 40, 2048)
+    sm = rng.integers(0, 8, 2048)
+    case("pt_mixed", mixed, 40, 40, 8, 6, 3, (px, sm), frame=2)
+    case("pt_mixed_d0", mixed, 40, 40, 2, 0, 3)
+    case("pt_mixed_d1", mixed, 40, 40, 2, 1, 3)
 
 
 if __name__ == "__main__":

@@ -34,7 +34,7 @@ def rd(gpu):
     return render
 
 
-@pytest.mark.parametrize("case", ["pt_cornell_32", "pt_icosphere", "pt_cornell_c1"])
+@pytest.mark.parametrize("case", ["pt_cornell_32", "pt_icosphere", "pt_cornell_c1", "pt_mixed"])
 def test_scene_tables_bit_exact(rd, case):
     print("case: tables", case, flush=True)
     g = load_golden(case)
@@ -61,15 +61,38 @@ def rd_rel(a, c):
     return rel_rmse(a, c)
 
 
-@pytest.mark.parametrize("case", ["pt_cornell_32", "pt_icosphere"])
+@pytest.mark.parametrize("case", ["pt_cornell_32", "pt_icosphere", "pt_mixed", "pt_mixed_d0", "pt_mixed_d1"])
 def test_render_matches_reference(rd, case):
+    """pt_mixed covers thin Lambert, transparent and interface (pass-through,
+    also for shadow rays), metallic GGX, glass and a two-sided emitter; the d0 /
+    d1 variants the bounce limit with pass-through hits."""
     print("case: render", case, flush=True)
     g = load_golden(case)
     sc = rd.Scene(scene_of(g))
     w, h, spp, depth, seed = int(g["width"]), int(g["height"]), int(g["spp"]), int(g["depth"]), int(g["seed"])
+    frame = int(g["frame"]) if "frame" in g else 0
     for mode in (rd.WAVEFRONT, rd.PLAIN):
-        film = sc.render(w, h, spp, depth, seed=seed, mode=mode)
+        film = sc.render(w, h, spp, depth, seed=seed, frame=frame, mode=mode)
         check_film(film, g["film_sum"], spp, RELRMSE_SMALL)
+
+
+def test_mixed_materials_per_sample(rd):
+    """Per (pixel, sample) radiance of the mixed-material scene against
+    trace_path's: the large majority agree to 1e-4 relative."""
+    print("case: per-sample mixed", flush=True)
+    g = load_golden("pt_mixed")
+    sc = rd.Scene(scene_of(g))
+    px, sm, ref = g["sample_pixels"], g["sample_index"], g["sample_radiance"]
+    close = 0
+    for k in range(int(g["spp"])):
+        film = sc.render(40, 40, 1, int(g["depth"]), seed=int(g["seed"]), frame=int(g["frame"]),
+                         sample_offset=k).reshape(-1, 3)
+        sel = sm == k
+        got, exp = film[px[sel]], ref[sel]
+        close += int(np.all(np.abs(got - exp) <= 1e-4 * np.maximum(np.abs(exp), 1e-2), axis=1).sum())
+    frac = close / len(px)
+    print(f"samples within 1e-4: {frac:.4f}", flush=True)
+    assert frac > 0.95
 
 
 def test_per_sample_radiance(rd):

@@ -8,7 +8,11 @@ import pytest
 import oracle
 from conftest import load_golden
 
-PT_CASES = ["pt_cornell_32", "pt_icosphere"]
+PT_CASES = ["pt_cornell_32", "pt_icosphere", "pt_mixed", "pt_mixed_d0", "pt_mixed_d1"]
+
+
+def frame_of(g):
+    return int(g["frame"]) if "frame" in g else 0
 
 
 def scene_of(g):
@@ -34,9 +38,11 @@ def test_film_and_samples_match_reference(case):
     g = load_golden(case)
     o = oracle.OraclePathTracer(scene_of(g))
     w, h, spp, depth, seed = int(g["width"]), int(g["height"]), int(g["spp"]), int(g["depth"]), int(g["seed"])
-    assert np.array_equal(bits(o.render(w, h, spp, depth, seed=seed)), bits(g["film_sum"]))
-    s = o.samples(w, h, depth, g["sample_pixels"], g["sample_index"], seed=seed)
-    assert np.array_equal(bits(s), bits(g["sample_radiance"]))
+    frame = frame_of(g)
+    assert np.array_equal(bits(o.render(w, h, spp, depth, seed=seed, frame=frame)), bits(g["film_sum"]))
+    if "sample_pixels" in g:
+        s = o.samples(w, h, depth, g["sample_pixels"], g["sample_index"], seed=seed, frame=frame)
+        assert np.array_equal(bits(s), bits(g["sample_radiance"]))
 
 
 def test_c1_oracle_image_rows():
