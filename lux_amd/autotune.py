@@ -21,8 +21,8 @@ from . import PACKAGE_DIR
 
 TUNING_FILE = PACKAGE_DIR / "tuning.json"
 CANDIDATES = {
-    "capacity": [1 << 21, 1 << 22, 1 << 23, 1 << 24],
-    "grid": [1024, 2048, 4096, 8192],
+    "capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
+    "grid": [1024, 2048, 4096],
     "block": [128, 256, 512],
 }
 _applied = False
@@ -63,12 +63,13 @@ def apply(force=False):
 
 
 def calibration_mix():
-    """One 512x512 16-spp image of a 216k-triangle displaced icosphere and six
-    batched-mix scenes at 8 spp (about 6 M paths in all)."""
+    """One 1024x1024 32-spp image of a 216k-triangle displaced icosphere and six
+    batched-mix scenes at 8 spp (about 38 M paths, more than the largest
+    capacity tried, so every candidate runs several batches)."""
     from . import scenes
 
-    jobs = [(scenes.displaced_icosphere(freq=104, width=512, height=512),
-             dict(width=512, height=512, spp=16, depth=8, seed=0))]
+    jobs = [(scenes.displaced_icosphere(freq=104, width=1024, height=1024),
+             dict(width=1024, height=1024, spp=32, depth=8, seed=0))]
     for spec in scenes.batched_mix(6, seed=0x4C5570):
         d = scenes.displaced_icosphere(freq=spec["freq"], seed=spec["seed"], width=spec["width"],
                                        height=spec["height"])
