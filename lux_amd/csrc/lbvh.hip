@@ -792,6 +792,7 @@ void bvh_release(BvhDev& d) {
 // ---------------------------------------------------------------------------
 // sibling-pair traversal copy of the node array
 // ---------------------------------------------------------------------------
+namespace {
 // Internal node i (Karras order) sits at preorder position k = pre_internal[i];
 // in right-first preorder the nodes before it are the n-1-last(i) leaves right
 // of its range and k-(n-1-last(i)) internal nodes, which is its internal rank
@@ -861,6 +862,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node*
         pairs[1] = pad;
     }
 }
+
+}  // namespace
 
 void pair_fill(BvhDev& d, hipStream_t s) {
     if (d.n <= 0) return;
