@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--c4-scenes", type=int, default=64)
     ap.add_argument("--c5-requests", type=int, default=96)
     ap.add_argument("--c5-load", type=float, default=0.75, help="arrival rate as a fraction of the c4 capacity")
-    ap.add_argument("--cpu-render-rows", type=int, default=48,
+    ap.add_argument("--cpu-render-rows", type=int, default=32,
                     help="rows of the C3 image the CPU render baseline traces (centre band)")
     return ap.parse_args()
 
@@ -159,34 +159,55 @@ def bench_c3(args):
 
 
 def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows):
-    """The CPU restatement of render_cpu_path_tracer (pinned bit for bit to the
-    reference's renders) on a band of `nrows` rows through the image centre,
-    one thread and all host threads (rows split across threads, per-pixel
-    sample order unchanged)."""
+    """A band of `nrows` rows through the image centre on the CPU:
+    - the reference's own trace_path (oracle/_ref, compiled from the LUX
+      sources), one thread, camera samples drawn as render_cpu_path_tracer
+      draws them and summed per pixel in sample order -> `value`, kind
+      "reference" (the CPU restatement stands in where oracle/_ref is absent);
+    - the CPU restatement (bit-identical to it) on one thread and on all host
+      threads (rows split across threads, per-pixel sample order unchanged)."""
     import oracle
 
     r0 = H // 2 - nrows // 2
     rows = (r0, r0 + nrows)
+    band = slice(rows[0], rows[1])
+    paths = W * nrows * spp
+    out = {"unit": "Mpaths/s", "cores": 1,
+           "sample": f"rows {rows[0]}..{rows[1] - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+           "host_cpu": _cpu_name()}
     t0 = time.perf_counter()
     pt = oracle.OraclePathTracer(scene)
-    t1 = time.perf_counter()
+    out["port_scene_prepare_s"] = round(time.perf_counter() - t0, 3)
     s0 = time.perf_counter()
-    one = pt.render(W, H, spp, depth, rows=rows, threads=1)
-    t_one = time.perf_counter() - s0
+    port = pt.render(W, H, spp, depth, rows=rows, threads=1)
+    t_port = time.perf_counter() - s0
     threads = os.cpu_count() or 1
     s0 = time.perf_counter()
     pt.render(W, H, spp, depth, rows=rows, threads=threads)
     t_all = time.perf_counter() - s0
+    out.update(port_Mpaths_per_s=round(paths / t_port / 1e6, 4),
+               port_all_threads_Mpaths_per_s=round(paths / t_all / 1e6, 4), all_threads=threads)
     from lux_amd.render import rel_rmse
 
-    band = slice(rows[0], rows[1])
-    err = rel_rmse(gpu_film[band] / spp, one[band] / spp)
-    paths = W * nrows * spp
-    return {"value": round(paths / t_one / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
-            "sample": f"rows {rows[0]}..{rows[1] - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
-            "all_threads_Mpaths_per_s": round(paths / t_all / 1e6, 4), "all_threads": threads,
-            "host_scene_prepare_s": round(t1 - t0, 3), "gpu_vs_cpu_band_relrmse": round(err, 8),
-            "host_cpu": _cpu_name()}
+    out["gpu_vs_cpu_band_relrmse"] = round(rel_rmse(gpu_film[band] / spp, port[band] / spp), 8)
+    if oracle.ref_available():
+        t0 = time.perf_counter()
+        ref = oracle.RefPathTracer(scene)
+        out["reference_scene_prepare_s"] = round(time.perf_counter() - t0, 3)
+        pix = np.repeat(np.arange(rows[0] * W, rows[1] * W, dtype=np.int32), spp)
+        smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
+        s0 = time.perf_counter()
+        rad = ref.samples(W, H, depth, pix, smp)
+        t_ref = time.perf_counter() - s0
+        film = np.zeros((nrows * W, 3), np.float32)
+        for k in range(spp):  # per pixel, in sample order, as render_cpu_path_tracer adds them
+            film += rad[k::spp]
+        out.update(value=round(paths / t_ref / 1e6, 4), kind="reference",
+                   reference_equals_port=bool(np.array_equal(film.view(np.uint32),
+                                                             port[band].reshape(-1, 3).view(np.uint32))))
+    else:
+        out.update(value=out["port_Mpaths_per_s"], kind="port")
+    return out
 
 
 def c4_scenes(count):
@@ -438,7 +459,11 @@ def main():
         "stages_ms": stage_ms,
         "stage_rates": {
             "build_Mprims_per_s": round(N / stage_ms["build"] / 1e3, 2),
+            # algorithmic bytes: build 360 B/prim, refit 144 B/prim (input, sort,
+            # links, boxes, layout, nodes; vertices, links, boxes)
+            "build_hbm_frac": round(360 * N / (stage_ms["build"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 5),
             "refit_Mprims_per_s": round(N / stage_ms["refit"] / 1e3, 2),
+            "refit_hbm_frac": round(144 * N / (stage_ms["refit"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 5),
             "closest_Mrays_per_s": round(R / stage_ms["closest"] / 1e3, 2),
             "any_Mrays_per_s": round(R / stage_ms["any"] / 1e3, 2),
             "overlap_Mqueries_per_s": round(Q / stage_ms["overlap"] / 1e3, 2),

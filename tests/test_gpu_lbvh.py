@@ -207,6 +207,16 @@ def test_full_size_microbench_inputs(gpu):
     assert np.array_equal(bits(hits["t"][sample]), bits(exp["t"]))
     occ = gpu.trace_any(bvh, tris, rays)
     assert np.array_equal(occ.astype(bool), hits["primitive_id"] >= 0)
+    # FP32 hit distance of every hit against FP64 Moller-Trumbore on the same
+    # triangle. On this input (3.31M hits) the oracle's FP32 t, which the GPU
+    # equals bit for bit, has median relative error 6.0e-8, 99.9th percentile
+    # 1.4e-6 and maximum 2.3e-3 (a grazing hit, where the determinant is tiny).
+    found = hits["primitive_id"] >= 0
+    t64 = oracle.tri_t_fp64(tris, hits["primitive_id"], rays)[found]
+    rel = np.abs(hits["t"][found].astype(np.float64) - t64) / np.maximum(np.abs(t64), 1e-30)
+    print(f"fp64 t error over {found.sum()} hits: max {rel.max():.3e}, median {np.median(rel):.3e}, "
+          f"p99.9 {np.quantile(rel, 0.999):.3e}", flush=True)
+    assert rel.max() < 5e-3 and np.quantile(rel, 0.999) < 1e-5 and np.median(rel) < 1e-6
     # refit from vertices (device entry point) == refit from host boxes
     d_tris = gpu.DeviceArray.from_numpy(inp["tris_jittered"])
     bvh.refit_triangles_device(d_tris, len(tris))

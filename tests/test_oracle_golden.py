@@ -166,14 +166,20 @@ def test_overlap_vectors():
     assert set(map(tuple, cp)) == set(map(tuple, g["cross_pairs"]))
 
 
-def test_fp64_error_statement_for_hit_distance():
-    """FP32 hit t against an FP64 Moller-Trumbore of the same triangle: the
-    relative error stays small (stated bound 1e-4 relative)."""
-    g = load_golden("lbvh_soup2000")
+@pytest.mark.parametrize("name", ["lbvh_soup2000", "lbvh_clustered1500", "lbvh_flat300", "lbvh_identical64",
+                                  "lbvh_sparse_ids777", "lbvh_tiny1", "lbvh_tiny2", "lbvh_tiny3", "lbvh_tiny7"])
+def test_fp64_error_statement_for_hit_distance(name):
+    """The reference's FP32 hit t (the golden vectors, which the GPU matches bit
+    for bit) against an FP64 Moller-Trumbore of the same triangle: measured
+    at most 1.9e-5 relative (168 ulp, on the clustered case), median ~7e-8;
+    the stated bound is 5e-5 relative."""
+    g = load_golden(name)
     found = g["hit_ids"] >= 0
+    assert found.any()
     t64 = oracle.tri_t_fp64(g["tri_by_id"], g["hit_ids"], g["rays"])
     rel = np.abs(g["hit_t"][found].astype(np.float64) - t64[found]) / np.maximum(np.abs(t64[found]), 1e-30)
-    assert np.nanmax(rel) < 1e-4
+    assert np.nanmax(rel) < 5e-5
+    assert np.median(rel) < 1e-6
 
 
 def test_oracle_refit_equals_rebuild_boxes_on_same_topology():
