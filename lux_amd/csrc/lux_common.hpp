@@ -148,6 +148,27 @@ __device__ __forceinline__ bool slab_hit(const SlabRay& r, float t_max, float lo
     return !miss && !(t_near > t_far);
 }
 
+// slab_hit for a ray with no axis parallel to the box faces (|d| >= 1e-12 on
+// every axis): the same operations and result without the per-axis branches.
+__device__ __forceinline__ bool slab_hit_np(const SlabRay& r, float t_max, float lox, float loy, float loz, float hix,
+                                            float hiy, float hiz) {
+    const float ax = ((r.neg_x ? hix : lox) - r.o.x) * r.inv.x;
+    const float cx = ((r.neg_x ? lox : hix) - r.o.x) * r.inv.x;
+    const float ay = ((r.neg_y ? hiy : loy) - r.o.y) * r.inv.y;
+    const float cy = ((r.neg_y ? loy : hiy) - r.o.y) * r.inv.y;
+    const float az = ((r.neg_z ? hiz : loz) - r.o.z) * r.inv.z;
+    const float cz = ((r.neg_z ? loz : hiz) - r.o.z) * r.inv.z;
+    const float t_near = fmaxf(fmaxf(fmaxf(r.t_min, ax), ay), az);
+    const float t_far = fminf(fminf(fminf(t_max, cx), cy), cz);
+    return !(t_near > t_far);
+}
+
+// True when any active lane of the wave has an axis-parallel direction; walks
+// take the branch-free slab test when none has (a wave-uniform choice).
+__device__ __forceinline__ bool wave_any_parallel(const SlabRay& r) {
+    return __ballot(r.par_x || r.par_y || r.par_z) != 0;
+}
+
 // Moller-Trumbore, operation order of src/geometry/triangle.h:33-52.
 __device__ __forceinline__ bool tri_hit(const f3& o, const f3& d, float t_min, float t_max,
                                         const f3& v0, const f3& v1, const f3& v2, float& t,

@@ -79,14 +79,16 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
 
 // Closest hit / occlusion walk over the sibling-pair copy of the right-first
 // stackless LBVH (same visits and order as the preorder walk); `ignore` skips
-// one leaf's triangle test (the source primitive of a shadow ray).
-__device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
-                                       int ignore, float& t_hit, float& u_hit, float& v_hit) {
+// one leaf's triangle test (the source primitive of a shadow ray). kParallel =
+// false drops the axis-parallel branches of the slab test (no active lane of
+// the wave has such an axis).
+template <bool kParallel>
+__device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
+                                            float t_min, float t_max, int ignore, float& t_hit, float& u_hit,
+                                            float& v_hit) {
     const lux_bvh_node* nodes = sv.nodes;
     const lux_triangle* soup = sv.soup;
     const int node_count = sv.node_count, tri_count = sv.tri_count;
-    f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
-    SlabRay sr = make_slab_ray(O, D, t_min);
     int best = -1;
     int k = 0;
     int guard = node_count + 1;  // a walk visits each node at most once
@@ -96,7 +98,9 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
         int prim = __float_as_int(b.z);
         int esc = __float_as_int(b.w);
         int next;
-        if (!slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)) {
+        const bool pass = kParallel ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
+                                    : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
+        if (!pass) {
             next = esc;
         } else if (prim >= 0) {
             next = esc;
@@ -126,6 +130,14 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
     }
     t_hit = t_max;
     return best;
+}
+
+__device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
+                                       int ignore, float& t_hit, float& u_hit, float& v_hit) {
+    const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
+    const SlabRay sr = make_slab_ray(O, D, t_min);
+    if (wave_any_parallel(sr)) return closest_walk<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+    return closest_walk<false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
 }
 
 __device__ __forceinline__ vec xyz(const float4& f) { return mk(f.x, f.y, f.z); }

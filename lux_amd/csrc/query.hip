@@ -46,12 +46,11 @@ __device__ __forceinline__ void load_tri(const lux_triangle* tris, int i, f3& v0
 // same order as the preorder walk, with a node's siblings on one cache line.
 // kAny = false: closest hit; kAny = true: stop at the first accepted triangle.
 // visits/tests count node loads and triangle tests (only read by the stats kernel).
-template <bool kAny>
-__device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int node_count,
-                                    const lux_triangle* __restrict__ tris, int tri_count,
-                                    const RayIn& ray, float& t_best, DevInfo* info,
-                                    uint32_t& visits, uint32_t& tests) {
-    SlabRay sr = make_slab_ray(ray.o, ray.d, ray.t_min);
+template <bool kAny, bool kParallel>
+__device__ __forceinline__ int walk_impl(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                         const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
+                                         const SlabRay& sr, float& t_best, DevInfo* info, uint32_t& visits,
+                                         uint32_t& tests) {
     float t_max = ray.t_max;
     int best = -1;
     int k = 0;
@@ -63,7 +62,9 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
         int prim = __float_as_int(b.z);
         int esc = __float_as_int(b.w);
         int next;
-        if (!slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)) {
+        const bool pass = kParallel ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
+                                    : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
+        if (!pass) {
             next = esc;
         } else if (prim >= 0) {
             next = esc;
@@ -91,6 +92,16 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
     }
     t_best = t_max;
     return best;
+}
+
+template <bool kAny>
+__device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                    const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
+                                    float& t_best, DevInfo* info, uint32_t& visits, uint32_t& tests) {
+    const SlabRay sr = make_slab_ray(ray.o, ray.d, ray.t_min);
+    if (wave_any_parallel(sr))
+        return walk_impl<kAny, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
+    return walk_impl<kAny, false>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
 }
 
 // One ray through the wide tree (kWideWalk) or the binary right-first array;
