@@ -82,6 +82,16 @@ __device__ __forceinline__ void load_node(const lux_bvh_node* nodes, int i, floa
     b = p[1];
 }
 
+// The same for node arrays below 4 GiB (node index < 2^27): base plus a
+// zero-extended 32-bit byte offset, one shift per node instead of 64-bit index
+// arithmetic.
+constexpr uint32_t kOffset32Nodes = 1u << 27;
+__device__ __forceinline__ void load_node32(const lux_bvh_node* nodes, uint32_t i, float4& a, float4& b) {
+    const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + (i << 5));
+    a = p[0];
+    b = p[1];
+}
+
 // Per-ray constants of the slab test. 1/d is a per-ray constant, so computing it
 // once gives the same value the reference recomputes at every box
 // (src/core/aabb.cuh:96). Axes with |d| < 1e-12 take the "origin inside slab"

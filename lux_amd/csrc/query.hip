@@ -46,50 +46,49 @@ __device__ __forceinline__ void load_tri(const lux_triangle* tris, int i, f3& v0
 // same order as the preorder walk, with a node's siblings on one cache line.
 // kAny = false: closest hit; kAny = true: stop at the first accepted triangle.
 // visits/tests count node loads and triangle tests (only read by the stats kernel).
-template <bool kAny, bool kParallel>
+template <bool kAny, bool kParallel, bool kOffset32>
 __device__ __forceinline__ int walk_impl(const lux_bvh_node* __restrict__ nodes, int node_count,
                                          const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
                                          const SlabRay& sr, float& t_best, DevInfo* info, uint32_t& visits,
                                          uint32_t& tests) {
-    float t_max = ray.t_max;
+    // canonical from the start, so the loop's min/max need no per-step canonicalise
+    float t_max = __builtin_canonicalizef(ray.t_max);
     int best = -1;
-    int k = 0;
+    uint32_t k = 0;
+    const uint32_t nc = uint32_t(node_count);
     int guard = node_count + 1;  // a walk visits each node at most once
-    while (uint32_t(k) < uint32_t(node_count)) {
+    bool bad_prim = false;
+    // The rare checks (leaf id range, step guard) are folded into lane flags and
+    // reported after the loop, keeping the loop body to one branch (the leaf's
+    // triangle test).
+    while (k < nc && --guard > 0) {
         float4 a, b;
-        load_node(nodes, k, a, b);
+        if (kOffset32)
+            load_node32(nodes, k, a, b);
+        else
+            load_node(nodes, int(k), a, b);
         ++visits;
-        int prim = __float_as_int(b.z);
-        int esc = __float_as_int(b.w);
-        int next;
+        const int prim = __float_as_int(b.z);
         const bool pass = kParallel ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
                                     : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
-        if (!pass) {
-            next = esc;
-        } else if (prim >= 0) {
-            next = esc;
-            if (uint32_t(prim) >= uint32_t(tri_count)) {
-                set_error(info, kErrPrimRange);
-            } else {
-                f3 v0, v1, v2;
-                load_tri(tris, prim, v0, v1, v2);
-                ++tests;
-                float t, u, v;
-                if (tri_hit(ray.o, ray.d, ray.t_min, t_max, v0, v1, v2, t, u, v)) {
-                    best = prim;
-                    t_max = t;
-                    if (kAny) break;
-                }
+        // internal hit: the right child's slot; otherwise the escape
+        uint32_t next = (pass && prim < 0) ? uint32_t(~prim) : __float_as_uint(b.w);
+        bad_prim |= pass && prim >= tri_count;
+        if (pass && prim >= 0 && prim < tri_count) {
+            f3 v0, v1, v2;
+            load_tri(tris, prim, v0, v1, v2);
+            ++tests;
+            float t, u, v;
+            if (tri_hit(ray.o, ray.d, ray.t_min, t_max, v0, v1, v2, t, u, v)) {
+                best = prim;
+                t_max = t;
+                if (kAny) next = 0xffffffffu;  // done
             }
-        } else {
-            next = ~prim;  // right child's slot
-        }
-        if (--guard == 0) {
-            set_error(info, kErrWalkOrder);
-            break;
         }
         k = next;
     }
+    if (bad_prim) set_error(info, kErrPrimRange);
+    if (guard <= 0 && k < nc) set_error(info, kErrWalkOrder);
     t_best = t_max;
     return best;
 }
@@ -99,9 +98,11 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
                                     const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
                                     float& t_best, DevInfo* info, uint32_t& visits, uint32_t& tests) {
     const SlabRay sr = make_slab_ray(ray.o, ray.d, ray.t_min);
+    if (uint32_t(node_count) > kOffset32Nodes)
+        return walk_impl<kAny, true, false>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
     if (wave_any_parallel(sr))
-        return walk_impl<kAny, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
-    return walk_impl<kAny, false>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
+        return walk_impl<kAny, true, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
+    return walk_impl<kAny, false, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
 }
 
 // One ray through the wide tree (kWideWalk) or the binary right-first array;
