@@ -82,7 +82,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
 // one leaf's triangle test (the source primitive of a shadow ray). kParallel =
 // false drops the axis-parallel branches of the slab test (no active lane of
 // the wave has such an axis).
-template <bool kParallel>
+template <bool kParallel, bool kAny>
 __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
                                             float t_min, float t_max, int ignore, float& t_hit, float& u_hit,
                                             float& v_hit) {
@@ -116,6 +116,7 @@ __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, co
                         t_max = t;
                         u_hit = u;
                         v_hit = v;
+                        if (kAny) break;
                     }
                 }
             }
@@ -136,8 +137,22 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
                                        int ignore, float& t_hit, float& u_hit, float& v_hit) {
     const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
     const SlabRay sr = make_slab_ray(O, D, t_min);
-    if (wave_any_parallel(sr)) return closest_walk<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
-    return closest_walk<false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+    if (wave_any_parallel(sr))
+        return closest_walk<true, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+    return closest_walk<false, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+}
+
+// True when the segment (t_min, t_max) hits any triangle other than `ignore`:
+// the same walk stopped at its first accepted triangle. It accepts a triangle
+// exactly when the closest-hit walk finds a hit, since both test the same
+// leaves in the same order with the same t_max up to that first acceptance.
+__device__ __forceinline__ bool occluded(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
+                                         int ignore) {
+    const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
+    const SlabRay sr = make_slab_ray(O, D, t_min);
+    float t = 0, u = 0, v = 0;
+    if (wave_any_parallel(sr)) return closest_walk<true, true>(sv, O, D, sr, t_min, t_max, ignore, t, u, v) >= 0;
+    return closest_walk<false, true>(sv, O, D, sr, t_min, t_max, ignore, t, u, v) >= 0;
 }
 
 __device__ __forceinline__ vec xyz(const float4& f) { return mk(f.x, f.y, f.z); }
@@ -207,6 +222,11 @@ __global__ __launch_bounds__(B) void k_generate(const BatchDesc* __restrict__ de
 // ---- shadow rays -------------------------------------------------------------------
 // trace_shadow (src/integrator/cpu_path_tracer.cpp:27-72): repeated closest hits
 // through pass-through surfaces; radiance += T * (contribution * Tr * mis).
+// kAnyShadow (launched only for batches whose scenes have no pass-through
+// material): the first hit found blocks the ray, so an any-hit walk gives the
+// same transmittance (0 or 1) as the closest-hit loop. The choice is made per
+// batch on the host; reading the scene's flag per ray instead measured slower.
+template <bool kAnyShadow>
 __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, const Paths& ps, uint32_t i) {
     float4 so = ps.so[i], sd = ps.sd[i], sc = ps.sc[i], st = ps.st[i];
     const uint32_t path = __float_as_uint(sc.w);
@@ -215,7 +235,8 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
     float t_max = so.w, t_min = kEps;
     int ignore = __float_as_int(st.w);
     vec tr = splat(1);
-    for (int guard = 0; guard < 1 << 16; ++guard) {
+    if (kAnyShadow && occluded(sv, o, d, t_min, t_max, ignore)) tr = splat(0);
+    for (int guard = 0; !kAnyShadow && guard < 1 << 16; ++guard) {
         float t = 0, u = 0, v = 0;
         int prim = closest(sv, o, d, t_min, t_max, ignore, t, u, v);
         if (prim < 0) break;
@@ -245,7 +266,7 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
 // counted by counts[sh_in] (-1: none), in one grid-stride pass. Block 0 clears
 // the counters the following shade launch appends to (zero_a, zero_b; never
 // the ones read here) and adds the queue sizes to the ray totals.
-template <int B>
+template <int B, bool kAnyShadow>
 __global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
                                                   int sh_in, int zero_a, int zero_b) {
     const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
@@ -266,7 +287,7 @@ __global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc,
             int prim = closest(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
             ps.hit[path] = make_float4(__int_as_float(prim), t, u, v);
         } else {
-            shadow_ray(desc, ps, i - n_ext);
+            shadow_ray<kAnyShadow>(desc, ps, i - n_ext);
         }
     }
 }
@@ -641,7 +662,7 @@ void Renderer::set_tuning(const Tuning& t) {
 void Renderer::release_graph() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     graph_ = nullptr;
-    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = -1;
+    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = graph_opaque_ = -1;
 }
 
 void Renderer::ensure_capacity(int paths) {
@@ -711,7 +732,7 @@ void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* sha
 namespace {
 // The wavefront launch sequence of one batch (captured or launched directly).
 // Raw launches (no probe events) when captured.
-template <int B, bool kProbe>
+template <int B, bool kProbe, bool kOpaque>
 void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
@@ -719,49 +740,58 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
 #define PT_LAUNCH(NAME, K, ...)                                                      \
     do {                                                                             \
         if (kProbe) {                                                                \
-            LUX_LAUNCH(NAME, K<B>, grid, B, 0, stream, __VA_ARGS__);                 \
+            LUX_LAUNCH(NAME, K, grid, B, 0, stream, __VA_ARGS__);                    \
         } else {                                                                     \
-            hipLaunchKernelGGL(K<B>, dim3(grid), dim3(B), 0, stream, __VA_ARGS__);   \
+            hipLaunchKernelGGL(K, dim3(grid), dim3(B), 0, stream, __VA_ARGS__);      \
             hip_check(hipGetLastError(), NAME);                                      \
         }                                                                            \
     } while (0)
-    PT_LAUNCH("pt_generate", k_generate, d, ps, qs);
+    PT_LAUNCH("pt_generate", k_generate<B>, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        PT_LAUNCH("pt_trace", k_trace, d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1, 2 + (b & 1));
-        PT_LAUNCH("pt_shade", k_shade, d, ps, qs, in, b, -1);
+        PT_LAUNCH("pt_trace", (k_trace<B, kOpaque>), d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1,
+                  2 + (b & 1));
+        PT_LAUNCH("pt_shade", k_shade<B>, d, ps, qs, in, b, -1);
     }
-    PT_LAUNCH("pt_trace", k_trace, d, ps, qs, -1, 2 + (depth & 1), -1, -1);
-    PT_LAUNCH("pt_flush", k_flush, d, ps);
+    PT_LAUNCH("pt_trace", (k_trace<B, kOpaque>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
+    PT_LAUNCH("pt_flush", k_flush<B>, d, ps);
 #undef PT_LAUNCH
 }
 }  // namespace
 
 namespace {
-template <bool kProbe>
+template <bool kProbe, bool kOpaque>
 void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     switch (block) {
-        case 128: wavefront_sequence<128, kProbe>(buf, depth, grid, stream); break;
-        case 512: wavefront_sequence<512, kProbe>(buf, depth, grid, stream); break;
-        default: wavefront_sequence<256, kProbe>(buf, depth, grid, stream); break;
+        case 128: wavefront_sequence<128, kProbe, kOpaque>(buf, depth, grid, stream); break;
+        case 512: wavefront_sequence<512, kProbe, kOpaque>(buf, depth, grid, stream); break;
+        default: wavefront_sequence<256, kProbe, kOpaque>(buf, depth, grid, stream); break;
     }
+}
+
+template <bool kProbe>
+void wavefront_sequence_o(bool opaque, int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+    if (opaque)
+        wavefront_sequence_b<kProbe, true>(block, buf, depth, grid, stream);
+    else
+        wavefront_sequence_b<kProbe, false>(block, buf, depth, grid, stream);
 }
 }  // namespace
 
-void Renderer::launch_wavefront(int depth, hipStream_t stream) {
+void Renderer::launch_wavefront(int depth, bool opaque, hipStream_t stream) {
     const int block = tune_.block;
     const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
     if (!tune_.use_graph) {
-        wavefront_sequence_b<true>(block, buf_, depth, grid, stream);
+        wavefront_sequence_o<true>(opaque, block, buf_, depth, grid, stream);
         return;
     }
     if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity ||
-        graph_block_ != block) {
+        graph_block_ != block || graph_opaque_ != int(opaque)) {
         release_graph();
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {
-            wavefront_sequence_b<false>(block, buf_, depth, grid, stream);
+            wavefront_sequence_o<false>(opaque, block, buf_, depth, grid, stream);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -775,6 +805,7 @@ void Renderer::launch_wavefront(int depth, hipStream_t stream) {
         graph_grid_ = grid;
         graph_capacity_ = buf_.capacity;
         graph_block_ = block;
+        graph_opaque_ = int(opaque);
     }
     hip_check(hipGraphLaunch(graph_, stream), "hipGraphLaunch");
 }
@@ -787,10 +818,12 @@ void Renderer::launch_plain(int depth, uint32_t count, hipStream_t stream) {
     LUX_LAUNCH("pt_generate", k_generate<kBlock>, grid, kBlock, 0, stream, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        LUX_LAUNCH("pt_trace", k_trace<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1, 2 + (b & 1));
+        LUX_LAUNCH("pt_trace", (k_trace<kBlock, false>), grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1,
+                   2 + (b & 1));
         for (int type = 0; type < kMaterialTypes; ++type)
             LUX_LAUNCH("pt_shade", k_shade<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
-        LUX_LAUNCH("pt_shadow", k_trace<kBlock>, grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1, -1);
+        LUX_LAUNCH("pt_shadow", (k_trace<kBlock, false>), grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1,
+                   -1);
     }
     LUX_LAUNCH("pt_flush", k_flush<kBlock>, grid, kBlock, 0, stream, d, ps);
 }
@@ -802,12 +835,14 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     BatchDesc* slot = next_slot();
     uint64_t used = 0;
     uint32_t nseg = 0;
+    bool opaque = true;  // every scene of the batch: shadow rays may use the any-hit walk
     for (Request* rq : active) {
         if (used >= cap || nseg >= uint32_t(kMaxSegments)) break;
         if (rq->issued >= rq->total) continue;
         uint64_t take = std::min(rq->total - rq->issued, cap - used);
         if (slice_spp > 0)
             take = std::min(take, uint64_t(slice_spp) * uint64_t(rq->rs.width) * uint64_t(rq->rs.height));
+        opaque = opaque && rq->sv.opaque != 0;
         slot->seg[nseg++] = make_segment(*rq, rq->issued, uint32_t(used), uint32_t(take));
         rq->issued += take;
         used += take;
@@ -818,7 +853,7 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     slot->count = uint32_t(used);
     slot->nseg = nseg;
     upload(slot, stream);
-    launch_wavefront(max_depth, stream);
+    launch_wavefront(max_depth, opaque, stream);
     ++batch_counter_;
     return used;
 }

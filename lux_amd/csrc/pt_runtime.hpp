@@ -45,6 +45,7 @@ struct SceneDev {
     std::vector<lux_material> host_mats;
     std::vector<int32_t> emitter_mat;
     size_t device_bytes = 0;  // HBM held by the scene (mesh + BVH)
+    bool opaque = true;       // no pass-through (interface, transparent) material
 };
 
 // POD view of one scene handed to kernels.
@@ -59,6 +60,7 @@ struct SceneView {
     const int32_t* light_of_prim;
     DevInfo* info;
     int32_t node_count, tri_count, mat_count, light_count;
+    int32_t opaque;  // 1: every hit blocks a shadow ray (no pass-through material)
     DevCamera cam;
 };
 
@@ -160,7 +162,7 @@ private:
     void ensure_capacity(int paths);
     BatchDesc* next_slot();
     void upload(BatchDesc* slot, hipStream_t stream);
-    void launch_wavefront(int depth, hipStream_t stream);
+    void launch_wavefront(int depth, bool opaque, hipStream_t stream);
     void launch_plain(int depth, uint32_t count, hipStream_t stream);
     void release_graph();
 
@@ -172,7 +174,7 @@ private:
     bool ring_used_[kRing] = {};
     int ring_next_ = 0;
     hipGraphExec_t graph_ = nullptr;
-    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1;
+    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1, graph_opaque_ = -1;
     int64_t batch_counter_ = 0;
 };
 

@@ -253,6 +253,8 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
     s.vertex_count = nv;
     s.mat_count = nm;
     s.host_mats.assign(desc.materials, desc.materials + nm);
+    s.opaque = true;
+    for (const DevMaterial& m : mats) s.opaque = s.opaque && m.type != kInterface && m.type != kTransparent;
     for (int t = 0; t < nt; ++t) {
         const lux_material& m = desc.materials[desc.material_ids[t]];
         if (fmaxf(m.emission.x, fmaxf(m.emission.y, m.emission.z)) > 0) {
@@ -344,6 +346,7 @@ SceneView scene_view(const SceneDev& s) {
     v.lights = s.lights;
     v.light_count = s.light_count;
     v.light_of_prim = s.light_of_prim;
+    v.opaque = s.opaque ? 1 : 0;
     v.cam = s.camera;
     v.info = s.bvh.info;
     return v;
