@@ -20,7 +20,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-ORACLE_SRCS = [REPO_ROOT / "oracle" / "lbvh_oracle.cpp", REPO_ROOT / "oracle" / "pt_oracle.cpp"]
+ORACLE_SRCS = [REPO_ROOT / "oracle" / "lbvh_oracle.cpp", REPO_ROOT / "oracle" / "pt_oracle.cpp",
+               REPO_ROOT / "oracle" / "two_level_oracle.cpp"]
 ORACLE_LIB = REPO_ROOT / "oracle" / "_build" / "liblux_oracle.so"
 
 

@@ -86,3 +86,57 @@ def c2_inputs(n_tris=1 << 20, n_rays=1 << 22, n_queries=100_000, seed=SEED_SOUP)
         "rays": rays(n_rays, seed=seed + 2),
         "query_boxes": triangle_boxes(soup(n_queries, seed + 3)),
     }
+
+
+# ---- two-level scenes (meshes + affine instances) ------------------------------
+def affine(rotation=None, scale=(1.0, 1.0, 1.0), translation=(0.0, 0.0, 0.0)):
+    """Row-major 4x4 object_to_world = T * R * S (float32; row 3 = 0 0 0 1).
+    `rotation` is a quaternion (w, x, y, z), normalised here."""
+    m = np.eye(4, dtype=np.float64)
+    if rotation is not None:
+        w, x, y, z = np.asarray(rotation, np.float64) / np.linalg.norm(rotation)
+        m[:3, :3] = [[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]]
+    m[:3, :3] = m[:3, :3] * np.asarray(scale, np.float64)[None, :]
+    m[:3, 3] = translation
+    return m.astype(np.float32)
+
+
+def soup_mesh(n, seed, edge=(0.02, 0.1)):
+    """A soup of n triangles in [0,1]^3 as an indexed mesh (vertices, indices)."""
+    t = soup(n, seed=seed, edge=edge)
+    return t.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+
+
+def instances_random(n, n_meshes, seed, extent=3.0, scale=(0.4, 1.6)):
+    """n instances over n_meshes meshes: random rotation, scale and translation."""
+    rng = np.random.default_rng(seed)
+    mesh = rng.integers(0, n_meshes, n).astype(np.int32)
+    m = np.stack([affine(rng.standard_normal(4), rng.uniform(scale[0], scale[1], 3),
+                         rng.uniform(-extent, extent, 3)) for _ in range(n)])
+    return mesh, m
+
+
+def scene_rays(n, seed, extent=4.0, aim=3.0):
+    """Rays from random points of [-extent, extent]^3 towards random points of
+    [-aim, aim]^3 (unit directions, t_min 0, t_max +inf)."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-extent, extent, (n, 3))
+    d = rng.uniform(-aim, aim, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 9), np.float32)
+    r[:, 0:3] = o
+    r[:, 3:6] = d
+    r[:, 7] = np.inf
+    return r
+
+
+def two_level_inputs(n_meshes=4, tris_per_mesh=1 << 16, n_instances=64, n_rays=1 << 22, seed=SEED_SOUP + 5):
+    """Instanced microbench: n_meshes triangle soups of tris_per_mesh triangles
+    (edges U(0.002, 0.02) in the unit cube), n_instances random affine
+    instances in [-3, 3]^3 (scales U(0.4, 1.6)), and n_rays rays aimed into the
+    instanced region."""
+    meshes = [soup_mesh(tris_per_mesh, seed + 10 * k, edge=(0.002, 0.02)) for k in range(n_meshes)]
+    inst_mesh, inst_m = instances_random(n_instances, n_meshes, seed + 1)
+    return {"meshes": meshes, "inst_mesh": inst_mesh, "inst_m": inst_m, "rays": scene_rays(n_rays, seed + 2)}

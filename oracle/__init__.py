@@ -342,6 +342,158 @@ class RefPathTracer:
         return out
 
 
+# ---- two-level restatement (two_level_oracle.cpp) ------------------------------
+def _tl_lib():
+    L = lib()
+    if not hasattr(L, "_tl_ready"):
+        L.orc_inverse_affine.argtypes = [c_void_p, c_void_p]
+        L.orc_transform_bounds.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.orc_instance_transform_valid.restype = c_int32
+        L.orc_instance_transform_valid.argtypes = [c_void_p, c_void_p]
+        L.orc_tl_build.restype = c_void_p
+        L.orc_tl_build.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32]
+        L.orc_tl_free.argtypes = [c_void_p]
+        L.orc_tl_first_primitive.restype = c_int32
+        L.orc_tl_first_primitive.argtypes = [c_void_p, c_int32]
+        L.orc_tl_trace.restype = c_int32
+        L.orc_tl_trace.argtypes = [c_void_p, c_void_p, c_void_p, c_int32] + [c_void_p] * 6
+        L._tl_ready = True
+    return L
+
+
+def inverse_affine(m):
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    _tl_lib().orc_inverse_affine(_p(a), _p(out))
+    return out
+
+
+def transform_bounds(m, box):
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    b = np.ascontiguousarray(box, np.float32).reshape(6)
+    out = np.zeros(6, np.float32)
+    _tl_lib().orc_transform_bounds(_p(a), _p(b), _p(out))
+    return out
+
+
+def instance_transform_valid(m, inv):
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    b = np.ascontiguousarray(inv, np.float32).reshape(16)
+    return bool(_tl_lib().orc_instance_transform_valid(_p(a), _p(b)))
+
+
+def mesh_soups(vertices, vert_off, indices, tri_off):
+    """Object-space triangle soups (9 floats each) of indexed meshes, concatenated."""
+    V = np.asarray(vertices, np.float32).reshape(-1, 3)
+    I = np.asarray(indices, np.int64).reshape(-1, 3)
+    out = []
+    for m in range(len(vert_off) - 1):
+        out.append(V[vert_off[m] + I[tri_off[m]:tri_off[m + 1]]].reshape(-1, 9))
+    return np.ascontiguousarray(np.concatenate(out) if out else np.zeros((0, 9), np.float32), np.float32)
+
+
+class OracleTwoLevel:
+    """BLAS per mesh + TLAS over instance world bounds and the host two-level walk."""
+
+    def __init__(self, tris, tri_off, inst_mesh, inst_inv, inst_world_bounds):
+        self._keep = (np.ascontiguousarray(tris, np.float32).reshape(-1, 9), np.ascontiguousarray(tri_off, np.int32),
+                      np.ascontiguousarray(inst_mesh, np.int32),
+                      np.ascontiguousarray(inst_inv, np.float32).reshape(-1, 16),
+                      np.ascontiguousarray(inst_world_bounds, np.float32).reshape(-1, 6))
+        t, to, im, inv, wb = self._keep
+        self.h = _tl_lib().orc_tl_build(_p(t), _p(to), len(to) - 1, _p(im), _p(inv), _p(wb), len(im))
+
+    def __del__(self):
+        try:
+            _tl_lib().orc_tl_free(self.h)
+        except Exception:
+            pass
+
+    def first_primitive(self):
+        return np.array([_tl_lib().orc_tl_first_primitive(self.h, i) for i in range(len(self._keep[2]))], np.int32)
+
+    def trace(self, rays, ignore=None):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        k = len(r)
+        ig = None if ignore is None else np.ascontiguousarray(ignore, np.int32)
+        out = {"prim": np.zeros(k, np.int32), "t": np.zeros(k, np.float32), "u": np.zeros(k, np.float32),
+               "v": np.zeros(k, np.float32), "instance": np.zeros(k, np.int32), "local": np.zeros(k, np.int32)}
+        ov = _tl_lib().orc_tl_trace(self.h, _p(r), _p(ig), k, _p(out["prim"]), _p(out["t"]), _p(out["u"]),
+                                    _p(out["v"]), _p(out["instance"]), _p(out["local"]))
+        out["overflows"] = int(ov)
+        return out
+
+
+# ---- the reference's own two-level scene walk (oracle/_ref) ----------------------
+def _ref_tl_lib():
+    L = ref()
+    if not hasattr(L, "_tl_ready"):
+        L.ref_tl_create.restype = c_void_p
+        L.ref_tl_create.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                    c_int32]
+        L.ref_tl_free.argtypes = [c_void_p]
+        L.ref_tl_instances.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+        L.ref_tl_trace.restype = c_int32
+        L.ref_tl_trace.argtypes = [c_void_p, c_void_p, c_void_p, c_int32] + [c_void_p] * 6
+        L._tl_ready = True
+    return L
+
+
+def two_level_arrays(meshes):
+    """meshes: list of (vertices (v, 3), indices (t, 3)) -> flat arrays + offsets."""
+    verts = [np.asarray(v, np.float32).reshape(-1, 3) for v, _ in meshes]
+    idx = [np.asarray(i, np.uint32).reshape(-1, 3) for _, i in meshes]
+    vo = np.zeros(len(meshes) + 1, np.int32)
+    to = np.zeros(len(meshes) + 1, np.int32)
+    vo[1:] = np.cumsum([len(v) for v in verts])
+    to[1:] = np.cumsum([len(i) for i in idx])
+    V = np.ascontiguousarray(np.concatenate(verts) if verts else np.zeros((0, 3), np.float32), np.float32)
+    I = np.ascontiguousarray(np.concatenate(idx) if idx else np.zeros((0, 3), np.uint32), np.uint32)
+    return V, vo, I, to
+
+
+class RefTwoLevel:
+    """The reference's two-level scene: add_scene_mesh / add_scene_instance
+    (make_transform) + PreparedScene::host_bvh + intersect_scene_bvh."""
+
+    def __init__(self, meshes, inst_mesh, inst_m, inst_override=None):
+        L = _ref_tl_lib()
+        V, vo, I, to = two_level_arrays(meshes)
+        self.inst_mesh = np.ascontiguousarray(inst_mesh, np.int32)
+        self.inst_m = np.ascontiguousarray(inst_m, np.float32).reshape(-1, 16)
+        n = len(self.inst_mesh)
+        self.inst_override = (np.full(n, -1, np.int32) if inst_override is None
+                              else np.ascontiguousarray(inst_override, np.int32))
+        self._keep = (V, vo, I, to)
+        self.h = L.ref_tl_create(_p(V), _p(vo), _p(I), _p(to), len(meshes), _p(self.inst_mesh), _p(self.inst_m),
+                                 _p(self.inst_override), n)
+
+    def __del__(self):
+        try:
+            _ref_tl_lib().ref_tl_free(self.h)
+        except Exception:
+            pass
+
+    def instances(self):
+        n = len(self.inst_mesh)
+        inv = np.zeros((n, 16), np.float32)
+        wb = np.zeros((n, 6), np.float32)
+        first = np.zeros(n, np.int32)
+        _ref_tl_lib().ref_tl_instances(self.h, _p(inv), _p(wb), _p(first))
+        return {"inv": inv, "world_bounds": wb, "first_primitive": first}
+
+    def trace(self, rays, ignore=None):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        k = len(r)
+        ig = None if ignore is None else np.ascontiguousarray(ignore, np.int32)
+        out = {"prim": np.zeros(k, np.int32), "t": np.zeros(k, np.float32), "u": np.zeros(k, np.float32),
+               "v": np.zeros(k, np.float32), "instance": np.zeros(k, np.int32), "local": np.zeros(k, np.int32)}
+        ov = _ref_tl_lib().ref_tl_trace(self.h, _p(r), _p(ig), k, _p(out["prim"]), _p(out["t"]), _p(out["u"]),
+                                        _p(out["v"]), _p(out["instance"]), _p(out["local"]))
+        out["overflows"] = int(ov)
+        return out
+
+
 # ---- path-tracer restatement (pt_oracle.cpp) ----------------------------------
 def _pt_lib():
     L = lib()

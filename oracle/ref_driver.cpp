@@ -335,3 +335,98 @@ void ref_pt_samples(const ref_pt* r, int32_t w, int32_t h, int32_t depth, uint64
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Two-level scenes: meshes and affine instances built through the reference's
+// scene API (add_scene_mesh, add_scene_instance with make_transform, so the
+// inverse comes from inverse_affine and the world bounds from transform_bounds)
+// and traced with its host two-level walk (intersect_scene_bvh ->
+// intersect_scene_two_level_bvh over PreparedScene::host_bvh).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+struct ref_tl {
+    std::unique_ptr<PreparedScene> prepared;
+};
+
+// Mesh m has vertices [vert_off[m], vert_off[m+1]) (xyz triples) and triangles
+// [tri_off[m], tri_off[m+1]) (mesh-local vertex indices, 3 each). Instance i
+// uses mesh inst_mesh[i], the row-major 4x4 matrix inst_m[16 i..] as
+// object_to_world, and material override inst_override[i].
+ref_tl* ref_tl_create(const float* vertices, const int32_t* vert_off, const uint32_t* indices,
+                      const int32_t* tri_off, int32_t mesh_count, const int32_t* inst_mesh, const float* inst_m,
+                      const int32_t* inst_override, int32_t instance_count) {
+    Scene scene;
+    scene.materials.push_back(Material(static_cast<MaterialType>(0), vec3(0.5f), vec3(0), 0.5f, 0, 1.5f,
+                                       EmissionSidedness::SingleSided));
+    for (int m = 0; m < mesh_count; ++m) {
+        SceneMesh mesh;
+        const int v0 = vert_off[m], nv = vert_off[m + 1] - vert_off[m];
+        const int t0 = tri_off[m], nt = tri_off[m + 1] - tri_off[m];
+        mesh.vertices.resize(nv);
+        mesh.normals.assign(nv, vec3(0));
+        mesh.uvs.assign(nv, vec2(0));
+        for (int i = 0; i < nv; ++i)
+            mesh.vertices[i] = vec3(vertices[3 * (v0 + i)], vertices[3 * (v0 + i) + 1], vertices[3 * (v0 + i) + 2]);
+        mesh.indices.resize(nt);
+        mesh.material_ids.assign(nt, 0);
+        mesh.boundary_edges.assign(nt, 0);
+        for (int t = 0; t < nt; ++t)
+            mesh.indices[t] =
+                TriangleIndices{indices[3 * (t0 + t)], indices[3 * (t0 + t) + 1], indices[3 * (t0 + t) + 2]};
+        add_scene_mesh(scene, std::move(mesh));
+    }
+    for (int i = 0; i < instance_count; ++i) {
+        mat4 M;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) M.m[r][c] = inst_m[16 * i + 4 * r + c];
+        add_scene_instance(scene, inst_mesh[i], make_transform(M), inst_override[i]);
+    }
+    scene.camera = make_pinhole_camera(vec3(0, 0, 3), vec3(0), 40.0f, 1.0f);
+    ref_tl* r = new ref_tl();
+    r->prepared = std::make_unique<PreparedScene>(std::move(scene));
+    r->prepared->prepare_host_bvh();
+    return r;
+}
+
+void ref_tl_free(ref_tl* r) { delete r; }
+
+// Per instance: Transform::inv (16 floats, row-major), world_bounds (6 floats)
+// and the scene ordinal of its local triangle 0 (-1 for an empty mesh).
+void ref_tl_instances(const ref_tl* r, float* inv, float* world_bounds, int32_t* first_primitive) {
+    const Scene& scene = r->prepared->host_scene();
+    for (int i = 0; i < int(scene.instances.size()); ++i) {
+        const SceneInstance& in = scene.instances[i];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) inv[16 * i + 4 * a + b] = in.object_to_world.inv.m[a][b];
+        std::memcpy(world_bounds + 6 * i, &in.world_bounds, sizeof(AABB));
+        first_primitive[i] = scene_primitive_index(scene, PrimitiveRef{i, in.mesh_id, 0, PrimitiveKind::Triangle});
+    }
+}
+
+// intersect_scene_bvh per ray, or intersect_scene_bvh_skip_ref when ignore[i]
+// >= 0 (a scene ordinal). Returns the number of stack overflows.
+int32_t ref_tl_trace(const ref_tl* r, const float* rays, const int32_t* ignore, int32_t n, int32_t* prim, float* t,
+                     float* u, float* v, int32_t* instance, int32_t* local) {
+    const Scene& scene = r->prepared->host_scene();
+    const CpuBvh& bvh = r->prepared->host_bvh();
+    const Ray* rs = reinterpret_cast<const Ray*>(rays);
+    int32_t overflows = 0;
+    for (int i = 0; i < n; ++i) {
+        SurfaceHit hit;
+        bool ov = false;
+        bool ok = (ignore && ignore[i] >= 0)
+                      ? intersect_scene_bvh_skip_ref(scene, bvh, rs[i], scene_primitive_ref(scene, ignore[i]), hit, &ov)
+                      : intersect_scene_bvh(scene, bvh, rs[i], hit, &ov);
+        if (ov) ++overflows;
+        prim[i] = ok ? hit.triangle_id : -1;
+        t[i] = ok ? hit.t : INFINITY;
+        u[i] = ok ? hit.u : 0;
+        v[i] = ok ? hit.v : 0;
+        instance[i] = ok ? hit.primitive_ref.instance_id : -1;
+        local[i] = ok ? hit.primitive_ref.local_primitive_id : -1;
+    }
+    return overflows;
+}
+
+}  // extern "C"
