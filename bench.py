@@ -53,6 +53,8 @@ def parse():
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--render", default="c3,c4,c5",
                     help="render workloads to add to the line: any of c3,c4,c5 (comma-separated) or none")
+    ap.add_argument("--two-level", default="on", choices=["on", "off"],
+                    help="add the instanced (two-level TLAS/BLAS) workload to the line")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule baselines of c3/c4")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the wavefront bounce loop directly instead of replaying a captured hipGraph")
@@ -269,6 +271,70 @@ def bench_c4(args, prepared=None):
     return out, scs, jobs, dicts
 
 
+def bench_two_level(args):
+    """Instanced scene: 4 meshes of 65 536 triangles, 64 random affine instances
+    (4 194 304 instanced triangles), 4 194 304 rays aimed into the instanced
+    region. Build (every BLAS + TLAS, synchronous, host triangles in), TLAS-only
+    rebuild (new instance transforms), closest hit and occlusion from
+    HBM-resident rays. CPU baseline: the two-level restatement on one thread over
+    a ray sample."""
+    from lux_amd import lbvh, two_level as TL, workloads as W
+
+    R = args.rays
+    inp = W.two_level_inputs(n_rays=R)
+    tris = np.concatenate([v[i].reshape(-1, 9) for v, i in inp["meshes"]]).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum([len(i) for _, i in inp["meshes"]])]).astype(np.int32)
+    rec = TL.make_instances(inp["inst_mesh"], inp["inst_m"], tris=tris, mesh_offsets=off)
+    mesh2, m2 = W.instances_random(len(rec), len(off) - 1, 4242)
+    rec2 = TL.make_instances(mesh2, m2, tris=tris, mesh_offsets=off)
+    s = TL.TwoLevelScene(tris, off, rec)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        s.build(tris, off, rec)
+    build_s = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for k in range(6):
+        s.set_instances(rec2 if k % 2 == 0 else rec)
+    tlas_s = (time.perf_counter() - t0) / 6
+    d_rays = lbvh.DeviceArray.from_numpy(inp["rays"])
+    d_hits = lbvh.DeviceArray(24 * R)
+    d_occ = lbvh.DeviceArray(R)
+    stage = {}
+    for name, fn in (("closest", lambda: s.intersect_device(d_rays, R, d_hits)),
+                     ("occluded", lambda: s.occluded_device(d_rays, R, d_occ))):
+        fn()
+        lbvh.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        lbvh.synchronize()
+        stage[name] = (time.perf_counter() - t0) / args.steps
+    hits = np.frombuffer(d_hits.to_numpy(np.uint8, (24 * R,)).tobytes(), TL.SCENE_HIT)
+    info = s.info()
+    out = {"workload": (f"{len(off) - 1} meshes x {int(np.diff(off)[0])} triangles, {len(rec)} affine instances "
+                        f"({info['scene_primitives']} instanced triangles), {R} rays"),
+           "build_ms": round(build_s * 1e3, 3), "tlas_rebuild_ms": round(tlas_s * 1e3, 3),
+           "closest_ms": round(stage["closest"] * 1e3, 3), "occluded_ms": round(stage["occluded"] * 1e3, 3),
+           "closest_Mrays_per_s": round(R / stage["closest"] / 1e6, 2),
+           "occluded_Mrays_per_s": round(R / stage["occluded"] / 1e6, 2),
+           "hit_fraction": round(float((hits["primitive_id"] >= 0).mean()), 4),
+           "device_bytes": info["device_bytes"]}
+    if not args.no_cpu_baseline:
+        import oracle
+
+        k = min(R, 1 << 16)
+        o = oracle.OracleTwoLevel(tris, off, inp["inst_mesh"], rec["world_to_object"], rec["world_bounds"])
+        t0 = time.perf_counter()
+        r = o.trace(inp["rays"][:k])
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(k / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+                               "sample": f"closest hit of the first {k} rays, two-level restatement, 1 thread",
+                               "equal_to_gpu": bool(np.array_equal(r["prim"], hits["primitive_id"][:k]))}
+    s.close()
+    _log(f"two-level: closest {out['closest_Mrays_per_s']} Mrays/s, occluded {out['occluded_Mrays_per_s']} Mrays/s")
+    return out
+
+
 def bench_c5(args, c4_out, scs, jobs, dicts):
     """Served stream: Poisson arrivals at a fraction of the measured batched
     capacity, scenes drawn from the C4 mix, 30% of requests re-render a scene
@@ -475,6 +541,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(inp, min(args.cpu_sample_rays, R))
     _log(f"lbvh: {out['value']} Mrays/s")
     del d_boxes, d_ids, d_tris, d_jit, d_rays, d_qboxes, d_qids, d_hits, d_occ, d_pairs, d_found, bvh, qbvh
+
+    if args.two_level == "on":
+        out["two_level"] = bench_two_level(args)
 
     wanted = [w for w in args.render.split(",") if w and w != "none"]
     if wanted and args.no_graph:

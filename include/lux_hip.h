@@ -386,6 +386,70 @@ int lux_hip_server_run(lux_hip_server* server, int32_t n, lux_hip_scene* const* 
                        const lux_vec3* const* new_vertices, const int32_t* vertex_counts, float* const* films,
                        double* latency_s, double* wall_s);
 
+/* ---- two-level (instanced) scenes ---------------------------------------
+ * A mesh is a run of object-space triangles; an instance places a mesh with an
+ * affine transform. One LBVH per non-empty mesh over its triangle boxes (BLAS)
+ * and one over the non-empty instance world bounds with the instance ids
+ * (TLAS) replace PreparedScene::prepare_gpu's BVH part
+ * (src/scene/prepared_scene.cu:59-78, src/scene/scene_prepare.h:420-462); the
+ * walk replaces intersect_scene_bvh_device (src/scene/gpu_scene_bvh.cuh:44-135):
+ * TLAS right-first walk; at an instance, the ray through Transform::inv
+ * (inverse_transform_ray, src/core/mat4.cuh:173-252) with t_max = the current
+ * hit, the BLAS walk, and a nearer hit becomes scene ordinal
+ * first_primitive + local id (src/scene/gpu_scene.h:149-172). Ids, t, u, v and
+ * tie winners equal the reference's host two-level walk
+ * (intersect_scene_two_level_bvh, src/scene/scene_intersection.h:147-240). */
+typedef struct lux_instance {
+    int32_t mesh_id;            /* SceneInstance::mesh_id */
+    int32_t material_override;  /* SceneInstance::material_override (-1: none); not used by the walk */
+    float object_to_world[16];  /* Transform::m, row-major */
+    float world_to_object[16];  /* Transform::inv, row-major (inverse_affine(m) in the reference) */
+    lux_aabb world_bounds;      /* SceneInstance::world_bounds; an empty box keeps it out of the TLAS */
+} lux_instance;
+typedef struct lux_scene_hit {
+    int32_t primitive_id;       /* scene ordinal (-1 = no hit) */
+    float t, u, v;
+    int32_t instance_id;        /* PrimitiveRef::instance_id */
+    int32_t local_primitive_id; /* PrimitiveRef::local_primitive_id */
+} lux_scene_hit;
+typedef struct lux_hip_two_level lux_hip_two_level;
+
+int lux_hip_two_level_create(lux_hip_two_level** out);
+int lux_hip_two_level_destroy(lux_hip_two_level* scene);
+/* Builds the BLAS of every mesh (mesh m owns triangles [mesh_offsets[m],
+ * mesh_offsets[m+1]) of `triangles`) and the TLAS. Instances are checked as
+ * scene validation checks them (src/scene/scene_validate.h:140-147: mesh id in
+ * range, both matrices affine-invertible, m * inv within 1e-4 of the identity);
+ * a failure returns LUX_ERR_INVALID_ARGUMENT. Synchronous. */
+int lux_hip_two_level_build(lux_hip_two_level* scene, const lux_triangle* triangles, const int32_t* mesh_offsets,
+                            int32_t mesh_count, const lux_instance* instances, int32_t instance_count);
+/* New transforms / world bounds / mesh ids for the same instance count: only
+ * the TLAS and the instance records are rebuilt. Synchronous. */
+int lux_hip_two_level_set_instances(lux_hip_two_level* scene, const lux_instance* instances,
+                                    int32_t instance_count);
+/* Closest hit per ray (intersect_scene_bvh / intersect_scene_bvh_skip_ref):
+ * ignore (nullable) holds one scene ordinal per ray to skip, -1 for none.
+ * Host arrays, synchronous; the _device forms take device pointers and are
+ * stream-ordered. */
+int lux_hip_two_level_intersect(lux_hip_two_level* scene, const lux_ray* rays, const int32_t* ignore,
+                                int32_t ray_count, lux_scene_hit* hits);
+int lux_hip_two_level_intersect_device(lux_hip_two_level* scene, const lux_ray* d_rays, const int32_t* d_ignore,
+                                       int32_t ray_count, lux_scene_hit* d_hits, lux_stream stream);
+/* 1 where the ray hits any triangle in [t_min, t_max) other than `ignore`
+ * (exactly where the closest-hit walk finds a hit), else 0. */
+int lux_hip_two_level_occluded(lux_hip_two_level* scene, const lux_ray* rays, const int32_t* ignore,
+                               int32_t ray_count, uint8_t* occluded);
+int lux_hip_two_level_occluded_device(lux_hip_two_level* scene, const lux_ray* d_rays, const int32_t* d_ignore,
+                                      int32_t ray_count, uint8_t* d_occluded, lux_stream stream);
+/* Instances in the TLAS, scene ordinals (triangles over all instances), HBM held. */
+int lux_hip_two_level_info(lux_hip_two_level* scene, int32_t* tlas_instances, int64_t* scene_primitives,
+                           uint64_t* device_bytes);
+/* Host helpers with the reference's arithmetic: make_transform's inverse
+ * (inverse_affine, src/core/mat4.cuh:126-159) and transform_bounds
+ * (src/core/mat4.cuh:254-270), for callers that build lux_instance records. */
+int lux_hip_inverse_affine(const float* m16, float* inv16);
+int lux_hip_transform_bounds(const float* m16, const lux_aabb* bounds, lux_aabb* out);
+
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart (the reference only prints wall-clock stage times,
  * src/app/render_runner.cpp:274-286). While a probe is open, every launch of

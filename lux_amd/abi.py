@@ -118,6 +118,17 @@ _SIGNATURES = {
     "lux_hip_server_result": [_P, c_uint64, _P, POINTER(c_double)],
     "lux_hip_server_stats": [_P, _P],
     "lux_hip_server_run": [_P, c_int32, _P, _P, _P, _P, _P, _P, _P, POINTER(c_double)],
+    "lux_hip_two_level_create": [POINTER(c_void_p)],
+    "lux_hip_two_level_destroy": [_P],
+    "lux_hip_two_level_build": [_P, _P, _P, c_int32, _P, c_int32],
+    "lux_hip_two_level_set_instances": [_P, _P, c_int32],
+    "lux_hip_two_level_intersect": [_P, _P, _P, c_int32, _P],
+    "lux_hip_two_level_intersect_device": [_P, _P, _P, c_int32, _P, _P],
+    "lux_hip_two_level_occluded": [_P, _P, _P, c_int32, _P],
+    "lux_hip_two_level_occluded_device": [_P, _P, _P, c_int32, _P, _P],
+    "lux_hip_two_level_info": [_P, POINTER(c_int32), POINTER(c_int64), POINTER(c_uint64)],
+    "lux_hip_inverse_affine": [_P, _P],
+    "lux_hip_transform_bounds": [_P, _P, _P],
 }
 _RESTYPES = {"lux_hip_last_error": c_char_p}
 

@@ -10,6 +10,13 @@
 #include <vector>
 
 #include "lux_runtime.hpp"
+#include "two_level.hpp"
+
+struct lux_hip_two_level {
+    lux::TwoLevelDev dev;
+    bool built = false;
+    hipStream_t last_stream = nullptr;
+};
 
 struct lux_hip_bvh {
     lux::BvhDev dev;
@@ -672,16 +679,200 @@ int lux_hip_trace_stats_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int
         require_handle(bvh);
         if (n < 0 || tri_count < 0 || !stats) throw Error(LUX_ERR_INVALID_ARGUMENT, "bad stats arguments");
         if (n > 0 && !d_rays) throw Error(LUX_ERR_INVALID_ARGUMENT, "null rays");
-        for (int k = 0; k < 4; ++k) stats[k] = 0;
+        constexpr int kStats = 6;  // {closest, any-hit} x {node units, visits, triangle tests}
+        for (int k = 0; k < kStats; ++k) stats[k] = 0;
         if (!bvh->valid) return int(LUX_OK);
         sync_handle(bvh);
-        DeviceBuffer<unsigned long long> ds(4);
+        DeviceBuffer<unsigned long long> ds(kStats);
         bvh->last_stream = g_stream;
         trace_stats(bvh->dev, bvh->wide_walk, d_tris, tri_count, d_rays, n, ds.p, g_stream);
-        unsigned long long h[4];
+        unsigned long long h[kStats];
         hip_check(hipMemcpyAsync(h, ds.p, sizeof(h), hipMemcpyDeviceToHost, g_stream), "D2H stats");
         sync_handle(bvh);
-        for (int k = 0; k < 4; ++k) stats[k] = h[k];
+        for (int k = 0; k < kStats; ++k) stats[k] = h[k];
+        return int(LUX_OK);
+    });
+}
+
+// ---- two-level scenes -----------------------------------------------------------
+namespace lux {
+namespace {
+void require_two_level(lux_hip_two_level* s) {
+    if (!s) throw Error(LUX_ERR_INVALID_ARGUMENT, "null lux_hip_two_level handle");
+}
+
+// Waits for the scene's last stream and turns device check bits into an error.
+void sync_two_level(lux_hip_two_level* s) {
+    hip_check(hipStreamSynchronize(s->last_stream ? s->last_stream : g_stream), "synchronize");
+    DevInfo* info = s->dev.tlas.info;
+    if (!info) return;
+    DevInfo h;
+    hip_check(hipMemcpy(&h, info, sizeof(DevInfo), hipMemcpyDeviceToHost), "read device info");
+    if (h.error == 0) return;
+    uint32_t zero = 0;
+    hip_check(hipMemcpy(reinterpret_cast<char*>(info) + offsetof(DevInfo, error), &zero, sizeof(zero),
+                        hipMemcpyHostToDevice),
+              "clear device info");
+    std::string what;
+    if (h.error & kErrPrimRange) what += " leaf primitive id outside its mesh;";
+    if (h.error & kErrWalkOrder) what += " two-level walk exceeded its step limit;";
+    throw Error(LUX_ERR_DEVICE_CHECK, "device check failed:" + what);
+}
+
+hipStream_t use_stream(lux_hip_two_level* s, lux_stream stream) {
+    hipStream_t st = pick(stream);
+    if (s->last_stream && s->last_stream != st) hip_check(hipStreamSynchronize(s->last_stream), "synchronize");
+    s->last_stream = st;
+    return st;
+}
+}  // namespace
+}  // namespace lux
+
+int lux_hip_two_level_create(lux_hip_two_level** out) {
+    return guarded([&] {
+        require_init();
+        if (!out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output pointer");
+        *out = new lux_hip_two_level();
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_destroy(lux_hip_two_level* s) {
+    return guarded([&] {
+        if (!s) return int(LUX_OK);
+        if (g_initialized) (void)hipStreamSynchronize(s->last_stream ? s->last_stream : g_stream);
+        two_level_release(s->dev);
+        delete s;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_build(lux_hip_two_level* s, const lux_triangle* tris, const int32_t* mesh_offsets,
+                            int32_t mesh_count, const lux_instance* instances, int32_t instance_count) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (s->last_stream) hip_check(hipStreamSynchronize(s->last_stream), "synchronize");
+        s->built = false;
+        s->last_stream = g_stream;
+        two_level_build(s->dev, tris, mesh_offsets, mesh_count, instances, instance_count, g_stream);
+        s->built = true;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_set_instances(lux_hip_two_level* s, const lux_instance* instances, int32_t instance_count) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
+        if (s->last_stream) hip_check(hipStreamSynchronize(s->last_stream), "synchronize");
+        s->last_stream = g_stream;
+        two_level_set_instances(s->dev, instances, instance_count, g_stream);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_intersect(lux_hip_two_level* s, const lux_ray* rays, const int32_t* ignore, int32_t n,
+                                lux_scene_hit* hits) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
+        if (n > 0 && (!rays || !hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
+        if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
+        if (n == 0) return int(LUX_OK);
+        sync_two_level(s);
+        DeviceBuffer<lux_ray> dr(n);
+        DeviceBuffer<int32_t> di(ignore ? size_t(n) : 1);
+        DeviceBuffer<lux_scene_hit> dh(n);
+        hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
+        if (ignore)
+            hip_check(hipMemcpyAsync(di.p, ignore, sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream),
+                      "H2D ignore");
+        s->last_stream = g_stream;
+        two_level_closest(two_level_view(s->dev), dr.p, ignore ? di.p : nullptr, n, dh.p, g_stream);
+        hip_check(hipMemcpyAsync(hits, dh.p, sizeof(lux_scene_hit) * n, hipMemcpyDeviceToHost, g_stream),
+                  "D2H hits");
+        sync_two_level(s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_intersect_device(lux_hip_two_level* s, const lux_ray* d_rays, const int32_t* d_ignore,
+                                       int32_t n, lux_scene_hit* d_hits, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
+        if (n > 0 && (!d_rays || !d_hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
+        if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
+        two_level_closest(two_level_view(s->dev), d_rays, d_ignore, n, d_hits, use_stream(s, stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_occluded(lux_hip_two_level* s, const lux_ray* rays, const int32_t* ignore, int32_t n,
+                               uint8_t* occluded) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
+        if (n > 0 && (!rays || !occluded)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/output arrays");
+        if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
+        if (n == 0) return int(LUX_OK);
+        sync_two_level(s);
+        DeviceBuffer<lux_ray> dr(n);
+        DeviceBuffer<int32_t> di(ignore ? size_t(n) : 1);
+        DeviceBuffer<uint8_t> dh(n);
+        hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
+        if (ignore)
+            hip_check(hipMemcpyAsync(di.p, ignore, sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream),
+                      "H2D ignore");
+        s->last_stream = g_stream;
+        two_level_occluded(two_level_view(s->dev), dr.p, ignore ? di.p : nullptr, n, dh.p, g_stream);
+        hip_check(hipMemcpyAsync(occluded, dh.p, size_t(n), hipMemcpyDeviceToHost, g_stream), "D2H occluded");
+        sync_two_level(s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_occluded_device(lux_hip_two_level* s, const lux_ray* d_rays, const int32_t* d_ignore,
+                                      int32_t n, uint8_t* d_occluded, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_two_level(s);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
+        if (n > 0 && (!d_rays || !d_occluded)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/output arrays");
+        if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
+        two_level_occluded(two_level_view(s->dev), d_rays, d_ignore, n, d_occluded, use_stream(s, stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_two_level_info(lux_hip_two_level* s, int32_t* tlas_instances, int64_t* scene_primitives,
+                           uint64_t* device_bytes) {
+    return guarded([&] {
+        require_two_level(s);
+        if (tlas_instances) *tlas_instances = s->dev.tlas_instances;
+        if (scene_primitives) *scene_primitives = s->dev.scene_primitives;
+        if (device_bytes) *device_bytes = uint64_t(s->dev.device_bytes);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_inverse_affine(const float* m16, float* inv16) {
+    return guarded([&] {
+        if (!m16 || !inv16) throw Error(LUX_ERR_INVALID_ARGUMENT, "null matrix");
+        inverse_affine(m16, inv16);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_transform_bounds(const float* m16, const lux_aabb* bounds, lux_aabb* out) {
+    return guarded([&] {
+        if (!m16 || !bounds || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = transform_bounds(m16, *bounds);
         return int(LUX_OK);
     });
 }

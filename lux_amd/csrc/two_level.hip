@@ -1,0 +1,445 @@
+// two_level.hip -- builds and walks two-level (instanced) scenes.
+//
+// The walk is one flat loop over two levels. A lane starts in the TLAS (world
+// ray); at an instance leaf it switches to that mesh's BLAS with the ray moved
+// to object space and keeps the TLAS escape of the leaf; when the BLAS walk
+// ends it returns to that escape with the world ray. One loop keeps the lanes
+// of a wave in step whichever level each is on. t_max is shared by both
+// levels: the reference gives the BLAS walk t_max = the current hit and keeps a
+// BLAS hit only when it is nearer, which every hit the BLAS walk accepts is
+// (it only accepts t < t_max), so updating t_max at once is the same.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "two_level.hpp"
+
+namespace lux {
+namespace {
+
+constexpr int kBlock = 256;
+
+// transform_point / transform_vector (src/core/mat4.cuh:173-203): row-major
+// rows summed left to right; a point divides by w unless w == 1.
+__device__ __forceinline__ f3 xform_point(const float* m, const f3& p) {
+    const float x = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    const float y = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    const float z = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    const float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    return (w == 1.0f) ? make3(x, y, z) : make3(x / w, y / w, z / w);
+}
+
+__device__ __forceinline__ f3 xform_vector(const float* m, const f3& v) {
+    return make3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+                 m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+
+struct TwoLevelHit {
+    int prim = -1, instance = -1, local = -1;
+    float t = 0, u = 0, v = 0;
+};
+
+template <bool kAny>
+__device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, const lux_ray& ray, int ignore) {
+    const f3 WO = ld3(ray.origin), WD = ld3(ray.direction);
+    const float t_min = ray.t_min;
+    float t_max = ray.t_max;
+    const SlabRay wsr = make_slab_ray(WO, WD, t_min);
+    SlabRay sr = wsr;
+    f3 O = WO, D = WD;
+    const lux_bvh_node* nodes = s.tlas_nodes;
+    uint32_t nc = uint32_t(s.tlas_node_count);
+    const lux_triangle* tris = nullptr;
+    int tcount = 0, first = -1, inst_cur = -1, ignore_local = -1;
+    bool in_blas = false;
+    uint32_t resume = 0;
+    uint32_t k = 0;
+    uint32_t steps = s.step_limit;
+    TwoLevelHit best;
+    for (;;) {
+        if (k >= nc) {
+            if (!in_blas) break;
+            in_blas = false;  // back to the TLAS at the instance leaf's escape
+            nodes = s.tlas_nodes;
+            nc = uint32_t(s.tlas_node_count);
+            k = resume;
+            sr = wsr;
+            O = WO;
+            D = WD;
+            continue;
+        }
+        if (steps-- == 0u) {
+            set_error(s.info, kErrWalkOrder);
+            break;
+        }
+        float4 a, b;
+        load_node(nodes, int(k), a, b);
+        const int prim = __float_as_int(b.z);
+        const uint32_t esc = __float_as_uint(b.w);
+        if (!slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)) {
+            k = esc;
+            continue;
+        }
+        if (prim < 0) {
+            k = uint32_t(~prim);  // right child's slot
+            continue;
+        }
+        k = esc;
+        if (!in_blas) {  // instance leaf: enter its BLAS if it has one
+            if (prim >= s.instance_count) continue;
+            const DevInstance& in = s.instances[prim];
+            if (in.mesh < 0 || in.mesh >= s.mesh_count) continue;
+            const DevMesh mesh = s.meshes[in.mesh];
+            if (mesh.node_count <= 0 || in.first < 0) continue;
+            float inv[16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float4 row = reinterpret_cast<const float4*>(in.inv)[i];
+                inv[4 * i] = row.x;
+                inv[4 * i + 1] = row.y;
+                inv[4 * i + 2] = row.z;
+                inv[4 * i + 3] = row.w;
+            }
+            resume = esc;
+            in_blas = true;
+            nodes = mesh.nodes;
+            nc = uint32_t(mesh.node_count);
+            tris = mesh.tris;
+            tcount = mesh.tri_count;
+            first = in.first;
+            inst_cur = prim;
+            ignore_local = (ignore >= first && ignore - first < tcount) ? ignore - first : -1;
+            O = xform_point(inv, WO);
+            D = xform_vector(inv, WD);
+            sr = make_slab_ray(O, D, t_min);
+            k = 0;
+            continue;
+        }
+        if (prim == ignore_local) continue;  // should_skip_primitive
+        if (prim >= tcount) {
+            set_error(s.info, kErrPrimRange);
+            continue;
+        }
+        const float* p = reinterpret_cast<const float*>(tris + prim);
+        const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
+        float t, u, v;
+        if (tri_hit(O, D, t_min, t_max, v0, v1, v2, t, u, v)) {
+            t_max = t;
+            best.prim = first + prim;
+            best.instance = inst_cur;
+            best.local = prim;
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            if (kAny) break;
+        }
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, const lux_ray* __restrict__ rays,
+                                                              const int32_t* __restrict__ ignore, int n,
+                                                              lux_scene_hit* __restrict__ hits) {
+    const int i = int(xcd_block()) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const TwoLevelHit h = two_level_walk<false>(s, rays[i], ignore ? ignore[i] : -1);
+    lux_scene_hit out;
+    out.primitive_id = h.prim;
+    out.t = h.prim >= 0 ? h.t : INFINITY;
+    out.u = h.prim >= 0 ? h.u : 0.0f;
+    out.v = h.prim >= 0 ? h.v : 0.0f;
+    out.instance_id = h.instance;
+    out.local_primitive_id = h.local;
+    hits[i] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_two_level_occluded(TwoLevelView s, const lux_ray* __restrict__ rays,
+                                                               const int32_t* __restrict__ ignore, int n,
+                                                               uint8_t* __restrict__ occluded) {
+    const int i = int(xcd_block()) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    occluded[i] = two_level_walk<true>(s, rays[i], ignore ? ignore[i] : -1).prim >= 0 ? 1 : 0;
+}
+
+template <typename T>
+T* dalloc_n(size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) throw Error(LUX_ERR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+struct TempBuffer {
+    T* p = nullptr;
+    explicit TempBuffer(size_t count) : p(dalloc_n<T>(count)) {}
+    ~TempBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    TempBuffer(const TempBuffer&) = delete;
+    TempBuffer& operator=(const TempBuffer&) = delete;
+};
+
+bool box_is_empty(const lux_aabb& b) {
+    return b.p_min.x > b.p_max.x || b.p_min.y > b.p_max.y || b.p_min.z > b.p_max.z;
+}
+
+// det3 and the affine test of src/core/mat4.cuh:86-104, host side.
+float affine_det3(const float* m) {
+    return m[0] * (m[5] * m[10] - m[6] * m[9]) - m[1] * (m[4] * m[10] - m[6] * m[8]) +
+           m[2] * (m[4] * m[9] - m[5] * m[8]);
+}
+bool affine_invertible(const float* m) {
+    return std::fabs(m[12]) <= 1e-6f && std::fabs(m[13]) <= 1e-6f && std::fabs(m[14]) <= 1e-6f &&
+           std::fabs(m[15] - 1.0f) <= 1e-6f && std::fabs(affine_det3(m)) > 1e-12f;
+}
+bool product_near_identity(const float* a, const float* b) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float s = 0.0f;
+            for (int k = 0; k < 4; ++k) s += a[4 * i + k] * b[4 * k + j];
+            if (!(std::fabs(s - (i == j ? 1.0f : 0.0f)) <= 1e-4f)) return false;
+        }
+    return true;
+}
+
+void validate_instances(const lux_instance* instances, int count, int mesh_count) {
+    if (count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative instance count");
+    if (count > 0 && !instances) throw Error(LUX_ERR_INVALID_ARGUMENT, "null instance array");
+    for (int i = 0; i < count; ++i) {
+        const lux_instance& in = instances[i];
+        if (in.mesh_id < 0 || in.mesh_id >= mesh_count)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": mesh id out of range");
+        if (!instance_transform_valid(in.object_to_world, in.world_to_object))
+            throw Error(LUX_ERR_INVALID_ARGUMENT,
+                        "instance " + std::to_string(i) + ": transform must be affine-invertible with a consistent inverse");
+    }
+}
+
+void sync_and_check(const BvhDev& d, hipStream_t stream, const char* what) {
+    hip_check(hipStreamSynchronize(stream), what);
+    if (!d.info) return;
+    DevInfo info;
+    hip_check(hipMemcpy(&info, d.info, sizeof(DevInfo), hipMemcpyDeviceToHost), "read device info");
+    if (info.error != 0) throw Error(LUX_ERR_DEVICE_CHECK, std::string("device check failed in ") + what);
+}
+
+// TLAS over the non-empty world bounds (ids = instance ids) and the instance
+// records; first ordinals follow the instances in order (each adds its mesh's
+// triangle count).
+void upload_instances(TwoLevelDev& s, const lux_instance* instances, int count, hipStream_t stream) {
+    std::vector<DevInstance> rec(static_cast<size_t>(std::max(count, 0)));
+    std::vector<lux_aabb> boxes;
+    std::vector<int32_t> ids;
+    int64_t cursor = 0;
+    for (int i = 0; i < count; ++i) {
+        const lux_instance& in = instances[i];
+        DevInstance& r = rec[i];
+        std::memcpy(r.inv, in.world_to_object, sizeof(r.inv));
+        r.mesh = in.mesh_id;
+        const int64_t tri = s.mesh_tri_off[in.mesh_id + 1] - s.mesh_tri_off[in.mesh_id];
+        if (cursor + tri > INT32_MAX) throw Error(LUX_ERR_INVALID_ARGUMENT, "more than 2^31 scene primitives");
+        r.first = int32_t(cursor);
+        r.pad[0] = r.pad[1] = 0;
+        cursor += tri;
+        if (box_is_empty(in.world_bounds)) continue;
+        boxes.push_back(in.world_bounds);
+        ids.push_back(i);
+    }
+    s.host_instances.assign(instances, instances + count);
+    s.instance_count = count;
+    s.scene_primitives = cursor;
+    s.tlas_instances = int(boxes.size());
+    if (s.instances) hip_check(hipFree(s.instances), "hipFree instances");
+    s.instances = dalloc_n<DevInstance>(rec.size());
+    if (!rec.empty())
+        hip_check(hipMemcpyAsync(s.instances, rec.data(), sizeof(DevInstance) * rec.size(), hipMemcpyHostToDevice,
+                                 stream),
+                  "H2D instances");
+    const int n = int(boxes.size());
+    TempBuffer<lux_aabb> d_boxes(size_t(std::max(n, 1)));
+    TempBuffer<int32_t> d_ids(size_t(std::max(n, 1)));
+    if (n > 0) {
+        hip_check(hipMemcpyAsync(d_boxes.p, boxes.data(), sizeof(lux_aabb) * n, hipMemcpyHostToDevice, stream),
+                  "H2D world bounds");
+        hip_check(hipMemcpyAsync(d_ids.p, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream),
+                  "H2D instance ids");
+    }
+    lbvh_build(s.tlas, d_boxes.p, d_ids.p, n, stream);
+    sync_and_check(s.tlas, stream, "TLAS build");
+    uint64_t limit = n > 1 ? 2u * uint64_t(n) : uint64_t(n);
+    for (int i = 0; i < count; ++i) {
+        const int tri = s.mesh_tri_off[instances[i].mesh_id + 1] - s.mesh_tri_off[instances[i].mesh_id];
+        limit += tri > 1 ? 2u * uint64_t(tri) : uint64_t(tri);
+    }
+    s.step_limit = uint32_t(std::min<uint64_t>(limit + 1, 0xffffffffu));
+}
+
+size_t bvh_device_bytes(const BvhDev& d) {
+    const size_t cap = size_t(d.capacity);
+    // boxes, ids, keys/vals, tree links, slots, preorder maps, nodes, pairs
+    return cap * (24 + 4 + 16 + 28 + 48 + 8 + 64 + 64 + 8);
+}
+
+}  // namespace
+
+bool instance_transform_valid(const float* m16, const float* inv16) {
+    return affine_invertible(m16) && affine_invertible(inv16) && product_near_identity(m16, inv16) &&
+           product_near_identity(inv16, m16);
+}
+
+void inverse_affine(const float* m, float* r) {
+    for (int i = 0; i < 16; ++i) r[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    const float a00 = m[0], a01 = m[1], a02 = m[2];
+    const float a10 = m[4], a11 = m[5], a12 = m[6];
+    const float a20 = m[8], a21 = m[9], a22 = m[10];
+    const float det = affine_det3(m);
+    if (std::fabs(det) <= 1e-12f) return;
+    const float inv_det = 1.0f / det;
+    r[0] = (a11 * a22 - a12 * a21) * inv_det;
+    r[1] = -(a01 * a22 - a02 * a21) * inv_det;
+    r[2] = (a01 * a12 - a02 * a11) * inv_det;
+    r[4] = -(a10 * a22 - a12 * a20) * inv_det;
+    r[5] = (a00 * a22 - a02 * a20) * inv_det;
+    r[6] = -(a00 * a12 - a02 * a10) * inv_det;
+    r[8] = (a10 * a21 - a11 * a20) * inv_det;
+    r[9] = -(a00 * a21 - a01 * a20) * inv_det;
+    r[10] = (a00 * a11 - a01 * a10) * inv_det;
+    const float tx = m[3], ty = m[7], tz = m[11];
+    r[3] = -(r[0] * tx + r[1] * ty + r[2] * tz);
+    r[7] = -(r[4] * tx + r[5] * ty + r[6] * tz);
+    r[11] = -(r[8] * tx + r[9] * ty + r[10] * tz);
+    r[12] = r[13] = r[14] = 0.0f;
+    r[15] = 1.0f;
+}
+
+lux_aabb transform_bounds(const float* m, const lux_aabb& b) {
+    lux_aabb r;
+    r.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
+    r.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
+    if (box_is_empty(b)) return r;
+    for (int z = 0; z <= 1; ++z)
+        for (int y = 0; y <= 1; ++y)
+            for (int x = 0; x <= 1; ++x) {
+                const float cx = x ? b.p_max.x : b.p_min.x, cy = y ? b.p_max.y : b.p_min.y,
+                            cz = z ? b.p_max.z : b.p_min.z;
+                float px = m[0] * cx + m[1] * cy + m[2] * cz + m[3];
+                float py = m[4] * cx + m[5] * cy + m[6] * cz + m[7];
+                float pz = m[8] * cx + m[9] * cy + m[10] * cz + m[11];
+                const float w = m[12] * cx + m[13] * cy + m[14] * cz + m[15];
+                if (w != 1.0f) {
+                    px = px / w;
+                    py = py / w;
+                    pz = pz / w;
+                }
+                r.p_min = lux_vec3{std::fmin(r.p_min.x, px), std::fmin(r.p_min.y, py), std::fmin(r.p_min.z, pz)};
+                r.p_max = lux_vec3{std::fmax(r.p_max.x, px), std::fmax(r.p_max.y, py), std::fmax(r.p_max.z, pz)};
+            }
+    return r;
+}
+
+void two_level_release(TwoLevelDev& s) {
+    for (BvhDev& b : s.blas) bvh_release(b);
+    bvh_release(s.tlas);
+    void* ptrs[] = {s.soup, s.meshes, s.instances};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    s = TwoLevelDev{};
+}
+
+void two_level_build(TwoLevelDev& s, const lux_triangle* tris, const int32_t* offsets, int mesh_count,
+                     const lux_instance* instances, int instance_count, hipStream_t stream) {
+    if (mesh_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative mesh count");
+    if (mesh_count > 0 && !offsets) throw Error(LUX_ERR_INVALID_ARGUMENT, "null mesh offsets");
+    if (mesh_count > 0 && offsets[0] != 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh_offsets[0] must be 0");
+    for (int m = 0; m < mesh_count; ++m)
+        if (offsets[m + 1] < offsets[m]) throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh offsets must not decrease");
+    const int total = mesh_count > 0 ? offsets[mesh_count] : 0;
+    if (total > 0 && !tris) throw Error(LUX_ERR_INVALID_ARGUMENT, "null triangle array");
+    validate_instances(instances, instance_count, mesh_count);
+    two_level_release(s);
+    s.mesh_count = mesh_count;
+    s.mesh_tri_off.assign(size_t(mesh_count) + 1, 0);
+    for (int m = 0; m <= mesh_count; ++m) s.mesh_tri_off[m] = mesh_count > 0 ? offsets[m] : 0;
+    s.soup = dalloc_n<lux_triangle>(size_t(std::max(total, 1)));
+    if (total > 0)
+        hip_check(hipMemcpyAsync(s.soup, tris, sizeof(lux_triangle) * total, hipMemcpyHostToDevice, stream),
+                  "H2D mesh triangles");
+    s.blas.resize(size_t(mesh_count));
+    std::vector<DevMesh> meshes(static_cast<size_t>(mesh_count));
+    for (int m = 0; m < mesh_count; ++m) {
+        const int n = offsets[m + 1] - offsets[m];
+        std::vector<lux_aabb> boxes(static_cast<size_t>(std::max(n, 0)));
+        std::vector<int32_t> ids(static_cast<size_t>(std::max(n, 0)));
+        for (int i = 0; i < n; ++i) {  // triangle_bounds of each triangle (exact min/max)
+            const lux_triangle& t = tris[offsets[m] + i];
+            boxes[i].p_min = lux_vec3{std::fmin(std::fmin(t.v0.x, t.v1.x), t.v2.x),
+                                      std::fmin(std::fmin(t.v0.y, t.v1.y), t.v2.y),
+                                      std::fmin(std::fmin(t.v0.z, t.v1.z), t.v2.z)};
+            boxes[i].p_max = lux_vec3{std::fmax(std::fmax(t.v0.x, t.v1.x), t.v2.x),
+                                      std::fmax(std::fmax(t.v0.y, t.v1.y), t.v2.y),
+                                      std::fmax(std::fmax(t.v0.z, t.v1.z), t.v2.z)};
+            ids[i] = i;
+        }
+        TempBuffer<lux_aabb> d_boxes(size_t(std::max(n, 1)));
+        TempBuffer<int32_t> d_ids(size_t(std::max(n, 1)));
+        if (n > 0) {
+            hip_check(hipMemcpyAsync(d_boxes.p, boxes.data(), sizeof(lux_aabb) * n, hipMemcpyHostToDevice, stream),
+                      "H2D mesh boxes");
+            hip_check(hipMemcpyAsync(d_ids.p, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream),
+                      "H2D mesh ids");
+        }
+        lbvh_build(s.blas[m], d_boxes.p, d_ids.p, n, stream);
+        sync_and_check(s.blas[m], stream, "BLAS build");
+        meshes[m].nodes = s.blas[m].pairs;
+        meshes[m].tris = s.soup + offsets[m];
+        meshes[m].node_count = n > 1 ? 2 * n : n;
+        meshes[m].tri_count = n;
+        s.device_bytes += bvh_device_bytes(s.blas[m]);
+    }
+    s.meshes = dalloc_n<DevMesh>(meshes.size());
+    if (!meshes.empty())
+        hip_check(hipMemcpyAsync(s.meshes, meshes.data(), sizeof(DevMesh) * meshes.size(), hipMemcpyHostToDevice,
+                                 stream),
+                  "H2D meshes");
+    upload_instances(s, instances, instance_count, stream);
+    s.device_bytes += sizeof(lux_triangle) * size_t(total) + sizeof(DevMesh) * meshes.size() +
+                      sizeof(DevInstance) * size_t(instance_count) + bvh_device_bytes(s.tlas);
+}
+
+void two_level_set_instances(TwoLevelDev& s, const lux_instance* instances, int instance_count,
+                             hipStream_t stream) {
+    if (instance_count != s.instance_count)
+        throw Error(LUX_ERR_MISMATCH, "instance count differs from the built scene");
+    validate_instances(instances, instance_count, s.mesh_count);
+    upload_instances(s, instances, instance_count, stream);
+}
+
+TwoLevelView two_level_view(const TwoLevelDev& s) {
+    TwoLevelView v;
+    v.tlas_nodes = s.tlas.pairs;
+    v.instances = s.instances;
+    v.meshes = s.meshes;
+    v.info = s.tlas.info;
+    v.tlas_node_count = s.tlas.n > 1 ? 2 * s.tlas.n : s.tlas.n;
+    v.instance_count = s.instance_count;
+    v.mesh_count = s.mesh_count;
+    v.step_limit = s.step_limit;
+    return v;
+}
+
+void two_level_closest(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
+                       lux_scene_hit* hits, hipStream_t stream) {
+    if (n <= 0) return;
+    LUX_LAUNCH("two_level_closest", k_two_level_closest, div_up(n, kBlock), kBlock, 0, stream, v, rays, ignore, n,
+               hits);
+}
+
+void two_level_occluded(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
+                        uint8_t* occluded, hipStream_t stream) {
+    if (n <= 0) return;
+    LUX_LAUNCH("two_level_occluded", k_two_level_occluded, div_up(n, kBlock), kBlock, 0, stream, v, rays, ignore, n,
+               occluded);
+}
+
+}  // namespace lux

@@ -1,0 +1,152 @@
+"""GPU parity of the two-level (instanced) scene walk, through the C ABI
+(lux_hip_two_level_*): against the reference's own host two-level walk (golden
+vectors of tests/golden/make_golden_tl.py) and the CPU restatement. Ids and
+every float bit-exact."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_golden
+from lux_amd import abi, two_level as TL, workloads as W
+
+pytestmark = pytest.mark.gpu
+TL_CASES = ["tl_mixed", "tl_single", "tl_many"]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def print_case(*what):
+    print("case:", *what, flush=True)
+
+
+def golden_scene(g):
+    tris = oracle.mesh_soups(g["vertices"], g["vert_off"], g["indices"], g["tri_off"])
+    rec = TL.make_instances(g["inst_mesh"], g["inst_m"], inverses=g["inv"], world_bounds=g["world_bounds"])
+    return tris, g["tri_off"], rec
+
+
+def check_hits(h, g, prefix):
+    assert np.array_equal(h["primitive_id"], g[f"{prefix}_prim"])
+    assert np.array_equal(h["instance_id"], g[f"{prefix}_instance"])
+    assert np.array_equal(h["local_primitive_id"], g[f"{prefix}_local"])
+    for k in ("t", "u", "v"):
+        assert np.array_equal(bits(h[k]), bits(g[f"{prefix}_{k}"])), k
+
+
+@pytest.mark.parametrize("case", TL_CASES)
+def test_two_level_closest_matches_reference(gpu, case):
+    print_case("two-level closest", case)
+    g = load_golden(case)
+    tris, off, rec = golden_scene(g)
+    s = TL.TwoLevelScene(tris, off, rec)
+    check_hits(s.intersect(g["rays"]), g, "hit")
+    check_hits(s.intersect(g["rays"], g["ignore"]), g, "skip")
+    info = s.info()
+    assert info["tlas_instances"] == int((g["world_bounds"][:, 0] <= g["world_bounds"][:, 3]).sum())
+    assert info["scene_primitives"] == int(np.diff(g["tri_off"])[g["inst_mesh"]].sum())
+    s.close()
+
+
+@pytest.mark.parametrize("case", TL_CASES)
+def test_two_level_occluded_is_closest_found(gpu, case):
+    print_case("two-level occluded", case)
+    g = load_golden(case)
+    tris, off, rec = golden_scene(g)
+    s = TL.TwoLevelScene(tris, off, rec)
+    assert np.array_equal(s.occluded(g["rays"]).astype(bool), g["hit_prim"] >= 0)
+    assert np.array_equal(s.occluded(g["rays"], g["ignore"]).astype(bool), g["skip_prim"] >= 0)
+    s.close()
+
+
+def test_library_computed_instances_give_the_same_hits(gpu):
+    """Instances whose inverse and world bounds come from the library's own
+    inverse_affine / transform_bounds."""
+    print_case("two-level library instances")
+    g = load_golden("tl_mixed")
+    tris = oracle.mesh_soups(g["vertices"], g["vert_off"], g["indices"], g["tri_off"])
+    rec = TL.make_instances(g["inst_mesh"], g["inst_m"], tris=tris, mesh_offsets=g["tri_off"])
+    s = TL.TwoLevelScene(tris, g["tri_off"], rec)
+    check_hits(s.intersect(g["rays"]), g, "hit")
+    s.close()
+
+
+def test_set_instances_equals_fresh_build_and_oracle(gpu):
+    """Moving the instances rebuilds only the TLAS: the hits equal a scene built
+    from scratch with the new instances and the oracle walk."""
+    print_case("two-level set_instances")
+    g = load_golden("tl_many")
+    tris = oracle.mesh_soups(g["vertices"], g["vert_off"], g["indices"], g["tri_off"])
+    off = g["tri_off"]
+    s = TL.TwoLevelScene(tris, off, TL.make_instances(g["inst_mesh"], g["inst_m"], tris=tris, mesh_offsets=off))
+    mesh2, m2 = W.instances_random(len(g["inst_mesh"]), len(off) - 1, 999, extent=2.0)
+    rec2 = TL.make_instances(mesh2, m2, tris=tris, mesh_offsets=off)
+    s.set_instances(rec2)
+    moved = s.intersect(g["rays"])
+    fresh = TL.TwoLevelScene(tris, off, rec2)
+    again = fresh.intersect(g["rays"])
+    assert np.array_equal(moved.view(np.uint8), again.view(np.uint8))
+    o = oracle.OracleTwoLevel(tris, off, mesh2, rec2["world_to_object"], rec2["world_bounds"]).trace(g["rays"])
+    assert np.array_equal(moved["primitive_id"], o["prim"])
+    assert np.array_equal(bits(moved["t"]), bits(o["t"]))
+    assert int((moved["primitive_id"] >= 0).sum()) > 50
+    s.close()
+    fresh.close()
+
+
+def test_two_level_rejects_what_scene_validation_rejects(gpu):
+    print_case("two-level validation")
+    g = load_golden("tl_single")
+    tris, off, rec = golden_scene(g)
+    bad = rec.copy()
+    bad["object_to_world"][0, 5] = 0.0  # singular
+    with pytest.raises(abi.LuxError) as e:
+        TL.TwoLevelScene(tris, off, bad)
+    assert e.value.status == abi.LUX_ERR_INVALID_ARGUMENT
+    bad = rec.copy()
+    bad["mesh_id"][0] = 5
+    with pytest.raises(abi.LuxError) as e:
+        TL.TwoLevelScene(tris, off, bad)
+    assert e.value.status == abi.LUX_ERR_INVALID_ARGUMENT
+    s = TL.TwoLevelScene(tris, off, rec)
+    with pytest.raises(abi.LuxError) as e:
+        s.set_instances(np.concatenate([rec, rec]))
+    assert e.value.status == abi.LUX_ERR_MISMATCH
+    check_hits(s.intersect(g["rays"]), g, "hit")  # still usable
+    s.close()
+
+
+def test_two_level_without_instances_misses_everything(gpu):
+    print_case("two-level empty")
+    g = load_golden("tl_single")
+    tris, off, _ = golden_scene(g)
+    s = TL.TwoLevelScene(tris, off, np.zeros(0, TL.INSTANCE))
+    h = s.intersect(g["rays"])
+    assert np.all(h["primitive_id"] == -1) and np.all(np.isinf(h["t"]))
+    assert not s.occluded(g["rays"]).any()
+    s.close()
+
+
+def test_instanced_microbench_inputs_match_oracle(gpu):
+    """The instanced microbench scene (4 meshes of 65 536 triangles, 64 instances):
+    a 32 768-ray sample against the oracle walk; occlusion of every ray equals
+    closest-found."""
+    print_case("two-level microbench inputs")
+    inp = W.two_level_inputs(n_rays=1 << 18)
+    tris = np.concatenate([v[i].reshape(-1, 9) for v, i in inp["meshes"]]).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum([len(i) for _, i in inp["meshes"]])]).astype(np.int32)
+    rec = TL.make_instances(inp["inst_mesh"], inp["inst_m"], tris=tris, mesh_offsets=off)
+    s = TL.TwoLevelScene(tris, off, rec)
+    h = s.intersect(inp["rays"])
+    sample = np.random.default_rng(3).choice(len(inp["rays"]), 1 << 15, replace=False)
+    o = oracle.OracleTwoLevel(tris, off, inp["inst_mesh"], rec["world_to_object"], rec["world_bounds"])
+    r = o.trace(inp["rays"][sample])
+    assert r["overflows"] == 0
+    assert np.array_equal(h["primitive_id"][sample], r["prim"])
+    assert np.array_equal(bits(h["t"][sample]), bits(r["t"]))
+    assert np.array_equal(h["instance_id"][sample], r["instance"])
+    assert np.array_equal(s.occluded(inp["rays"]).astype(bool), h["primitive_id"] >= 0)
+    assert (h["primitive_id"] >= 0).mean() > 0.2
+    s.close()

@@ -85,3 +85,16 @@ def test_single_identity_instance_equals_single_level_walk():
     single = oracle.OracleBvh(boxes).trace_closest(tris, g["rays"])
     assert np.array_equal(single["ids"], g["hit_prim"])
     assert np.array_equal(bits(single["t"]), bits(g["hit_t"]))
+
+
+@pytest.mark.parametrize("case", TL_CASES)
+def test_library_host_helpers_match_reference(case):
+    """lux_hip_inverse_affine / lux_hip_transform_bounds (host code of the C ABI,
+    no GPU needed) give the reference's inverse and world bounds bit for bit."""
+    from lux_amd import two_level as TL
+
+    g = load_golden(case)
+    tris, to = scene_of(g)
+    rec = TL.make_instances(g["inst_mesh"], g["inst_m"], tris=tris, mesh_offsets=to)
+    assert np.array_equal(bits(rec["world_to_object"]), bits(g["inv"]))
+    assert np.array_equal(bits(rec["world_bounds"]), bits(g["world_bounds"]))
