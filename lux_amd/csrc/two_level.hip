@@ -76,7 +76,12 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
         load_node(nodes, int(k), a, b);
         const int prim = __float_as_int(b.z);
         const uint32_t esc = __float_as_uint(b.w);
-        if (!slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)) {
+        // the branch-free test unless some active lane's current ray (world or
+        // object space) has an axis-parallel component; a wave-uniform choice
+        const bool any_parallel = wave_any_parallel(sr);
+        const bool pass = any_parallel ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
+                                       : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
+        if (!pass) {
             k = esc;
             continue;
         }
