@@ -335,6 +335,34 @@ def bench_two_level(args):
     return out
 
 
+def bench_film(args):
+    """Film output of a 3840x2160 film of per-pixel sums resident in HBM:
+    write_ppm's pixel bytes (15 B per pixel: 12 read, 3 written) and write_pfm's
+    float body (24 B per pixel), as HBM-bound kernels."""
+    from lux_amd import film as F, lbvh
+
+    w, h, spp = 3840, 2160, 64
+    rng = np.random.default_rng(7)
+    d_film = lbvh.DeviceArray.from_numpy((rng.random((h, w, 3), dtype=np.float32) * 4 * spp).astype(np.float32))
+    d8 = lbvh.DeviceArray(3 * w * h)
+    d32 = lbvh.DeviceArray(12 * w * h)
+    out = {"workload": f"{w}x{h} film, {spp} spp"}
+    for name, fn, per_px in (("ppm", lambda: F.ppm_pixels_device(d_film, w, h, spp, d8), 15),
+                             ("pfm", lambda: F.pfm_pixels_device(d_film, w, h, spp, d32), 24)):
+        fn()
+        lbvh.synchronize()
+        probe = lbvh.Probe(f"film_{name}")
+        with probe:
+            for _ in range(args.steps):
+                fn()
+            lbvh.synchronize()
+        ms = probe.total_ms / max(probe.launches, 1)
+        gbps = per_px * w * h / (ms * 1e-3) / 1e9
+        out[name] = {"kernel_ms": round(ms, 4), "GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    _log(f"film: ppm {out['ppm']['GBps']} GB/s, pfm {out['pfm']['GBps']} GB/s")
+    return out
+
+
 def bench_c5(args, c4_out, scs, jobs, dicts):
     """Served stream: Poisson arrivals at a fraction of the measured batched
     capacity, scenes drawn from the C4 mix, 30% of requests re-render a scene
@@ -544,6 +572,7 @@ def main():
 
     if args.two_level == "on":
         out["two_level"] = bench_two_level(args)
+    out["film_output"] = bench_film(args)
 
     wanted = [w for w in args.render.split(",") if w and w != "none"]
     if wanted and args.no_graph:

@@ -450,6 +450,27 @@ int lux_hip_two_level_info(lux_hip_two_level* scene, int32_t* tlas_instances, in
 int lux_hip_inverse_affine(const float* m16, float* inv16);
 int lux_hip_transform_bounds(const float* m16, const lux_aabb* bounds, lux_aabb* out);
 
+/* ---- film output ---------------------------------------------------------
+ * The files of write_ppm (src/film/ppm.h:37-56: 1/spp, Reinhard c/(1+c),
+ * clamp to [0,1], 255.999 * c^(1/2.2) truncated, pixels in film order) and
+ * write_pfm (src/film/pfm.h:27-44: 1/spp, rows bottom to top, "PF" header,
+ * scale -1.0) from a film of per-pixel sums in device memory (W*H*3 floats,
+ * the layout lux_hip_render_device writes). Byte-identical to the reference's
+ * writers. The _device forms write only the pixel body (3 bytes or 3 floats per
+ * pixel) and are stream-ordered; lux_hip_write_* write the whole file and
+ * create its parent directories, as the reference does. */
+int lux_hip_film_ppm_device(const float* d_film, int32_t width, int32_t height, int32_t samples_per_pixel,
+                            uint8_t* d_rgb8, lux_stream stream);
+int lux_hip_film_pfm_device(const float* d_film, int32_t width, int32_t height, int32_t samples_per_pixel,
+                            float* d_rgb, lux_stream stream);
+int lux_hip_write_ppm(const char* path, const float* d_film, int32_t width, int32_t height,
+                      int32_t samples_per_pixel);
+int lux_hip_write_pfm(const char* path, const float* d_film, int32_t width, int32_t height,
+                      int32_t samples_per_pixel);
+/* The 255 gamma thresholds of the PPM byte conversion (no reference
+ * counterpart; host only): byte = number of thresholds <= the clamped value. */
+int lux_hip_ppm_thresholds(float* out255);
+
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart (the reference only prints wall-clock stage times,
  * src/app/render_runner.cpp:274-286). While a probe is open, every launch of

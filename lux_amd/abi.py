@@ -129,6 +129,11 @@ _SIGNATURES = {
     "lux_hip_two_level_info": [_P, POINTER(c_int32), POINTER(c_int64), POINTER(c_uint64)],
     "lux_hip_inverse_affine": [_P, _P],
     "lux_hip_transform_bounds": [_P, _P, _P],
+    "lux_hip_film_ppm_device": [_P, c_int32, c_int32, c_int32, _P, _P],
+    "lux_hip_film_pfm_device": [_P, c_int32, c_int32, c_int32, _P, _P],
+    "lux_hip_write_ppm": [c_char_p, _P, c_int32, c_int32, c_int32],
+    "lux_hip_write_pfm": [c_char_p, _P, c_int32, c_int32, c_int32],
+    "lux_hip_ppm_thresholds": [_P],
 }
 _RESTYPES = {"lux_hip_last_error": c_char_p}
 

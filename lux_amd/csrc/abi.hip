@@ -877,6 +877,65 @@ int lux_hip_transform_bounds(const float* m16, const lux_aabb* bounds, lux_aabb*
     });
 }
 
+// ---- film output ----------------------------------------------------------------
+namespace lux {
+namespace {
+void check_film_args(const float* d_film, int32_t w, int32_t h) {
+    if (w < 0 || h < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative film size");
+    if (int64_t(w) * h > 0 && !d_film) throw Error(LUX_ERR_INVALID_ARGUMENT, "null film");
+}
+}  // namespace
+}  // namespace lux
+
+int lux_hip_film_ppm_device(const float* d_film, int32_t w, int32_t h, int32_t spp, uint8_t* d_rgb8,
+                            lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        check_film_args(d_film, w, h);
+        if (int64_t(w) * h > 0 && !d_rgb8) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output");
+        film_ppm(d_film, w, h, spp, d_rgb8, pick(stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_film_pfm_device(const float* d_film, int32_t w, int32_t h, int32_t spp, float* d_rgb, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        check_film_args(d_film, w, h);
+        if (int64_t(w) * h > 0 && !d_rgb) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output");
+        film_pfm(d_film, w, h, spp, d_rgb, pick(stream));
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_write_ppm(const char* path, const float* d_film, int32_t w, int32_t h, int32_t spp) {
+    return guarded([&] {
+        require_init();
+        check_film_args(d_film, w, h);
+        if (!path) throw Error(LUX_ERR_INVALID_ARGUMENT, "null path");
+        write_film_file(path, true, d_film, w, h, spp, g_stream);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_write_pfm(const char* path, const float* d_film, int32_t w, int32_t h, int32_t spp) {
+    return guarded([&] {
+        require_init();
+        check_film_args(d_film, w, h);
+        if (!path) throw Error(LUX_ERR_INVALID_ARGUMENT, "null path");
+        write_film_file(path, false, d_film, w, h, spp, g_stream);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_ppm_thresholds(float* out255) {
+    return guarded([&] {
+        if (!out255) throw Error(LUX_ERR_INVALID_ARGUMENT, "null output");
+        ppm_thresholds(out255);
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_probe_begin(const char* kernel_name) {
     return guarded([&] {
         require_init();

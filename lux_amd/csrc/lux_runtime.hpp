@@ -138,6 +138,15 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
 // (allocating it on first use), and its refresh after a refit; build and refit
 // call them when `want_wide` is set.
 void wide_build(BvhDev& d, hipStream_t s);
+
+// Film output (film.hip): write_ppm's pixel bytes (3 per pixel, film order),
+// write_pfm's float body (rows bottom to top), both from a device film of
+// per-pixel sums; whole-file writers; the 255 gamma thresholds of the byte
+// conversion (host powf).
+void film_ppm(const float* film, int width, int height, int spp, uint8_t* out, hipStream_t s);
+void film_pfm(const float* film, int width, int height, int spp, float* out, hipStream_t s);
+void write_film_file(const char* path, bool ppm, const float* d_film, int width, int height, int spp, hipStream_t s);
+void ppm_thresholds(float* out255);
 void wide_refit(BvhDev& d, hipStream_t s);
 // Enqueue a refit with new boxes (input order) or triangles (input order).
 void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);

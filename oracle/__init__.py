@@ -494,6 +494,43 @@ class RefTwoLevel:
         return out
 
 
+# ---- film writers restatement (film_oracle.cpp) ---------------------------------
+def film_file(kind, film, spp):
+    """The bytes of write_ppm (kind "ppm") or write_pfm ("pfm") of a film of
+    per-pixel sums (H, W, 3), restated."""
+    L = lib()
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    fn = L.orc_ppm_file if kind == "ppm" else L.orc_pfm_file
+    fn.restype = c_int64
+    fn.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_int64]
+    n = fn(_p(f), w, h, spp, None, 0)
+    out = np.zeros(n, np.uint8)
+    fn(_p(f), w, h, spp, _p(out), n)
+    return out
+
+
+def ppm_table_mismatches(t255, threads=8):
+    L = lib()
+    L.orc_ppm_table_mismatches.restype = c_int64
+    L.orc_ppm_table_mismatches.argtypes = [c_void_p, c_int32]
+    t = np.ascontiguousarray(t255, np.float32)
+    assert len(t) == 255
+    return int(L.orc_ppm_table_mismatches(_p(t), threads))
+
+
+# ---- the reference's own film writers (oracle/_ref) ------------------------------
+def ref_write_film(kind, path, film, spp):
+    """write_ppm (kind "ppm") or write_pfm ("pfm") of the reference on a film of
+    per-pixel sums (H, W, 3)."""
+    L = ref()
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    fn = L.ref_write_ppm if kind == "ppm" else L.ref_write_pfm
+    fn.argtypes = [ctypes.c_char_p, c_void_p, c_int32, c_int32, c_int32]
+    fn(str(path).encode(), _p(f), w, h, spp)
+
+
 # ---- path-tracer restatement (pt_oracle.cpp) ----------------------------------
 def _pt_lib():
     L = lib()

@@ -430,3 +430,28 @@ int32_t ref_tl_trace(const ref_tl* r, const float* rays, const int32_t* ignore, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Film output: the reference's own write_ppm / write_pfm (src/film/ppm.h,
+// src/film/pfm.h) on a film given as W*H*3 float sums.
+// ---------------------------------------------------------------------------
+#include "film/pfm.h"
+#include "film/ppm.h"
+
+extern "C" {
+
+static std::vector<vec3> film_pixels(const float* film, int32_t w, int32_t h) {
+    std::vector<vec3> px(size_t(w) * size_t(h));
+    for (size_t i = 0; i < px.size(); ++i) px[i] = vec3(film[3 * i], film[3 * i + 1], film[3 * i + 2]);
+    return px;
+}
+
+void ref_write_ppm(const char* path, const float* film, int32_t w, int32_t h, int32_t spp) {
+    write_ppm(path, film_pixels(film, w, h), w, h, spp);
+}
+
+void ref_write_pfm(const char* path, const float* film, int32_t w, int32_t h, int32_t spp) {
+    write_pfm(path, film_pixels(film, w, h), w, h, spp);
+}
+
+}  // extern "C"
