@@ -39,6 +39,140 @@ struct TwoLevelView {
     uint32_t step_limit;  // TLAS slots + the BLAS slots of every instance: no walk takes more steps
 };
 
+// ---- device walk ------------------------------------------------------------------
+// transform_point / transform_vector (src/core/mat4.cuh:173-203): row-major
+// rows summed left to right; a point divides by w unless w == 1.
+__device__ __forceinline__ f3 xform_point(const float* m, const f3& p) {
+    const float x = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    const float y = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    const float z = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    const float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    return (w == 1.0f) ? make3(x, y, z) : make3(x / w, y / w, z / w);
+}
+
+__device__ __forceinline__ f3 xform_vector(const float* m, const f3& v) {
+    return make3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+                 m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+
+struct TwoLevelHit {
+    int prim = -1, instance = -1, local = -1;
+    float t = 0, u = 0, v = 0;
+};
+
+// The walk is one flat loop over two levels. A lane starts in the TLAS (world
+// ray); at an instance leaf it switches to that mesh's BLAS with the ray moved
+// to object space and keeps the TLAS escape of the leaf; when the BLAS walk
+// ends it returns to that escape with the world ray. One loop keeps the lanes
+// of a wave in step whichever level each is on. t_max is shared by both
+// levels: the reference gives the BLAS walk t_max = the current hit and keeps a
+// BLAS hit only when it is nearer, which every hit the BLAS walk accepts is
+// (it only accepts t < t_max), so updating t_max at once is the same.
+// kAny stops at the first accepted triangle. `ignore` is a scene ordinal
+// (should_skip_primitive) or -1.
+template <bool kAny>
+__device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, const f3& WO, const f3& WD,
+                                                      float t_min, float t_max, int ignore) {
+    const SlabRay wsr = make_slab_ray(WO, WD, t_min);
+    SlabRay sr = wsr;
+    f3 O = WO, D = WD;
+    const lux_bvh_node* nodes = s.tlas_nodes;
+    uint32_t nc = uint32_t(s.tlas_node_count);
+    const lux_triangle* tris = nullptr;
+    int tcount = 0, first = -1, inst_cur = -1, ignore_local = -1;
+    bool in_blas = false;
+    uint32_t resume = 0;
+    uint32_t k = 0;
+    uint32_t steps = s.step_limit;
+    TwoLevelHit best;
+    for (;;) {
+        if (k >= nc) {
+            if (!in_blas) break;
+            in_blas = false;  // back to the TLAS at the instance leaf's escape
+            nodes = s.tlas_nodes;
+            nc = uint32_t(s.tlas_node_count);
+            k = resume;
+            sr = wsr;
+            O = WO;
+            D = WD;
+            continue;
+        }
+        if (steps-- == 0u) {
+            set_error(s.info, kErrWalkOrder);
+            break;
+        }
+        float4 a, b;
+        load_node(nodes, int(k), a, b);
+        const int prim = __float_as_int(b.z);
+        const uint32_t esc = __float_as_uint(b.w);
+        // the branch-free test unless some active lane's current ray (world or
+        // object space) has an axis-parallel component; a wave-uniform choice
+        const bool any_parallel = wave_any_parallel(sr);
+        const bool pass = any_parallel ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
+                                       : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
+        if (!pass) {
+            k = esc;
+            continue;
+        }
+        if (prim < 0) {
+            k = uint32_t(~prim);  // right child's slot
+            continue;
+        }
+        k = esc;
+        if (!in_blas) {  // instance leaf: enter its BLAS if it has one
+            if (prim >= s.instance_count) continue;
+            const DevInstance& in = s.instances[prim];
+            if (in.mesh < 0 || in.mesh >= s.mesh_count) continue;
+            const DevMesh mesh = s.meshes[in.mesh];
+            if (mesh.node_count <= 0 || in.first < 0) continue;
+            float inv[16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float4 row = reinterpret_cast<const float4*>(in.inv)[i];
+                inv[4 * i] = row.x;
+                inv[4 * i + 1] = row.y;
+                inv[4 * i + 2] = row.z;
+                inv[4 * i + 3] = row.w;
+            }
+            resume = esc;
+            in_blas = true;
+            nodes = mesh.nodes;
+            nc = uint32_t(mesh.node_count);
+            tris = mesh.tris;
+            tcount = mesh.tri_count;
+            first = in.first;
+            inst_cur = prim;
+            ignore_local = (ignore >= first && ignore - first < tcount) ? ignore - first : -1;
+            O = xform_point(inv, WO);
+            D = xform_vector(inv, WD);
+            sr = make_slab_ray(O, D, t_min);
+            k = 0;
+            continue;
+        }
+        if (prim == ignore_local) continue;  // should_skip_primitive
+        if (prim >= tcount) {
+            set_error(s.info, kErrPrimRange);
+            continue;
+        }
+        const float* p = reinterpret_cast<const float*>(tris + prim);
+        const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
+        float t, u, v;
+        if (tri_hit(O, D, t_min, t_max, v0, v1, v2, t, u, v)) {
+            t_max = t;
+            best.prim = first + prim;
+            best.instance = inst_cur;
+            best.local = prim;
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            if (kAny) break;
+        }
+    }
+    return best;
+}
+
+
+// ---- host side ----------------------------------------------------------------------
 struct TwoLevelDev {
     std::vector<BvhDev> blas;  // per mesh (n = 0 for an empty mesh)
     BvhDev tlas;               // over the non-empty instance world bounds, ids = instance ids
