@@ -450,6 +450,58 @@ int lux_hip_two_level_info(lux_hip_two_level* scene, int32_t* tlas_instances, in
 int lux_hip_inverse_affine(const float* m16, float* inv16);
 int lux_hip_transform_bounds(const float* m16, const lux_aabb* bounds, lux_aabb* out);
 
+/* ---- instanced scenes for the path tracer ---------------------------------
+ * A Scene of several meshes placed by affine instances (Scene::mesh_assets +
+ * Scene::instances, src/scene/scene_data.h:35-58, built by add_scene_mesh /
+ * add_scene_instance, src/scene/scene_edit.h), rendered through the same
+ * lux_hip_render / lux_hip_batch_render / server entry points as a one-mesh
+ * scene. Rays walk the two-level structure (the lux_hip_two_level walk);
+ * shading reads each scene ordinal's triangle in world space as
+ * scene_triangle_view gives it (src/scene/scene_data.h:335-372): corners by
+ * transform_point(object_to_world), normals by transform_instance_normal
+ * (adjugate transform, normalised), material = the instance's override if
+ * >= 0, else the mesh triangle's. Emissive ordinals become power-sampled area
+ * lights with their world-space area (prepare_scene_light_sampling,
+ * src/scene/scene_prepare.h:268-387). Mesh m owns vertices
+ * [vertex_offsets[m], vertex_offsets[m+1]) and triangles
+ * [triangle_offsets[m], triangle_offsets[m+1]); indices are mesh-local. */
+typedef struct lux_instanced_scene_desc {
+    const lux_vec3* vertices;
+    const lux_vec3* normals;         /* per vertex or NULL (flat shading) */
+    const int32_t* vertex_offsets;   /* mesh_count + 1 */
+    const uint32_t* indices;         /* 3 per triangle, into the mesh's own vertices */
+    const int32_t* material_ids;     /* per triangle */
+    const int32_t* triangle_offsets; /* mesh_count + 1 */
+    int32_t mesh_count;
+    const lux_instance* instances;   /* mesh_id, material_override, both matrices; world_bounds is recomputed */
+    int32_t instance_count;
+    const lux_material* materials;
+    int32_t material_count;
+    float camera[8];
+} lux_instanced_scene_desc;
+/* PreparedScene(scene) + prepare_gpu() for an instanced scene. Scene
+ * validation (src/scene/scene_validate.h:95-147) rejects with
+ * LUX_ERR_INVALID_ARGUMENT: indices or material ids out of range, a mesh id
+ * or material override out of range, a transform pair that is not
+ * affine-invertible and consistent; at least one instance must place a
+ * triangle. lux_hip_scene_bvh and lux_hip_scene_update_vertices do not apply
+ * to such a scene (LUX_ERR_INVALID_ARGUMENT). Synchronous. */
+int lux_hip_scene_create_instanced(const lux_instanced_scene_desc* desc, lux_hip_scene** out);
+/* Moves the instances of an instanced scene (same count; mesh ids and
+ * overrides may change): the TLAS, the world-space triangles and normals and
+ * the light table are rebuilt, the per-mesh BLASes are kept. It waits for the
+ * device first, so renders enqueued earlier finish with the old placement; a
+ * scene with admitted, unfinished server requests is refused
+ * (LUX_ERR_INVALID_ARGUMENT). Synchronous. */
+int lux_hip_scene_set_instances(lux_hip_scene* scene, const lux_instance* instances, int32_t instance_count);
+/* The shading view the renderer holds for every scene ordinal, as
+ * scene_triangle_view gives it (src/scene/scene_data.h:335-372): world corners
+ * (9 floats per ordinal), world corner normals (9 floats; zeros for a mesh
+ * without normals) and the material id after the instance override. Works for
+ * one-mesh scenes too (identity instance). Arrays sized by the ordinal count
+ * (lux_hip_scene_info's triangle_count); any may be NULL. Synchronous. */
+int lux_hip_scene_triangles(lux_hip_scene* scene, float* corners, float* normals, int32_t* material_ids);
+
 /* ---- film output ---------------------------------------------------------
  * The files of write_ppm (src/film/ppm.h:37-56: 1/spp, Reinhard c/(1+c),
  * clamp to [0,1], 255.999 * c^(1/2.2) truncated, pixels in film order) and

@@ -1007,14 +1007,18 @@ void check_settings(const lux_render_settings* s) {
     if (s->batch_size < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "negative batch size");
 }
 
-// Reads and clears the device check bits of a scene (caller synchronised).
+// Reads and clears the device check bits of a scene (caller synchronised):
+// the LBVH's, or the TLAS's for an instanced scene (the two-level walk
+// reports there).
 void scene_check(lux_hip_scene* sc) {
+    lux::DevInfo* d = sc->dev.instanced ? sc->dev.tl.tlas.info : sc->dev.bvh.info;
+    if (!d) return;
     lux::DevInfo info;
-    lux::hip_check(hipMemcpy(&info, sc->dev.bvh.info, sizeof(info), hipMemcpyDeviceToHost), "read device info");
+    lux::hip_check(hipMemcpy(&info, d, sizeof(info), hipMemcpyDeviceToHost), "read device info");
     if (info.error) {
         uint32_t zero = 0;
-        lux::hip_check(hipMemcpy(reinterpret_cast<char*>(sc->dev.bvh.info) + offsetof(lux::DevInfo, error), &zero,
-                                 4, hipMemcpyHostToDevice),
+        lux::hip_check(hipMemcpy(reinterpret_cast<char*>(d) + offsetof(lux::DevInfo, error), &zero, 4,
+                                 hipMemcpyHostToDevice),
                        "clear device info");
         throw lux::Error(LUX_ERR_DEVICE_CHECK, "device check failed during render");
     }
@@ -1092,6 +1096,63 @@ int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out) {
     });
 }
 
+int lux_hip_scene_create_instanced(const lux_instanced_scene_desc* desc, lux_hip_scene** out) {
+    return guarded([&] {
+        require_init();
+        if (!desc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto* sc = new lux_hip_scene();
+        try {
+            pt::scene_upload_instanced(sc->dev, *desc, g_stream);
+        } catch (...) {
+            pt::scene_release(sc->dev);
+            delete sc;
+            throw;
+        }
+        *out = sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_set_instances(lux_hip_scene* sc, const lux_instance* instances, int32_t instance_count) {
+    return guarded([&] {
+        require_init();
+        if (!sc) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        pt::scene_set_instances(sc->dev, instances, instance_count, g_stream);
+        scene_check(sc);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_triangles(lux_hip_scene* sc, float* corners, float* normals, int32_t* material_ids) {
+    return guarded([&] {
+        require_init();
+        if (!sc) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const pt::SceneDev& d = sc->dev;
+        const size_t T = size_t(std::max(d.tri_count, 0));
+        hip_check(hipStreamSynchronize(g_stream), "synchronize");
+        if (corners && T)
+            hip_check(hipMemcpy(corners, d.soup, sizeof(lux_triangle) * T, hipMemcpyDeviceToHost), "D2H triangles");
+        if (material_ids && T)
+            hip_check(hipMemcpy(material_ids, d.mat_id, 4 * T, hipMemcpyDeviceToHost), "D2H materials");
+        if (normals && T) {
+            if (!d.normals) {
+                std::memset(normals, 0, sizeof(float) * 9 * T);
+            } else if (d.instanced) {  // already per corner
+                hip_check(hipMemcpy(normals, d.normals, sizeof(float) * 9 * T, hipMemcpyDeviceToHost), "D2H normals");
+            } else {  // per vertex: gather the corners on the host
+                std::vector<float> vn(3 * size_t(d.vertex_count));
+                std::vector<uint32_t> ix(3 * T);
+                hip_check(hipMemcpy(vn.data(), d.normals, 4 * vn.size(), hipMemcpyDeviceToHost), "D2H normals");
+                hip_check(hipMemcpy(ix.data(), d.indices, 4 * ix.size(), hipMemcpyDeviceToHost), "D2H indices");
+                for (size_t c = 0; c < 3 * T; ++c)
+                    for (int k = 0; k < 3; ++k) normals[3 * c + k] = vn[3 * size_t(ix[c]) + k];
+            }
+        }
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_scene_destroy(lux_hip_scene* sc) {
     return guarded([&] {
         if (!sc) return int(LUX_OK);
@@ -1109,6 +1170,8 @@ int lux_hip_scene_update_vertices(lux_hip_scene* sc, const lux_vec3* vertices, i
     return guarded([&] {
         require_init();
         if (!sc || !vertices) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        if (sc->dev.instanced)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: move instances with lux_hip_scene_set_instances");
         pt::scene_update_vertices(sc->dev, vertices, vertex_count, g_stream);
         scene_check(sc);
         return int(LUX_OK);
@@ -1165,6 +1228,7 @@ int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float
 int lux_hip_scene_bvh(lux_hip_scene* sc, lux_hip_bvh** out) {
     return guarded([&] {
         if (!sc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        if (sc->dev.instanced) throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: no single-level LBVH");
         // The alias shares the scene's device buffers; it must not be destroyed.
         sc->bvh_alias.dev = sc->dev.bvh;
         sc->bvh_alias.valid = sc->dev.bvh.n > 0;

@@ -155,6 +155,37 @@ __device__ __forceinline__ bool occluded(const SceneView& sv, const vec& o, cons
     return closest_walk<false, true>(sv, O, D, sr, t_min, t_max, ignore, t, u, v) >= 0;
 }
 
+// The same two queries on an instanced scene: the two-level walk (TLAS, then
+// the BLAS of each instance the ray reaches, in object space), returning the
+// scene ordinal. kTwoLevel is chosen per batch on the host, like kAnyShadow.
+template <bool kTwoLevel>
+__device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, const vec& d, float t_min,
+                                             float t_max, int ignore, float& t_hit, float& u_hit, float& v_hit) {
+    if constexpr (kTwoLevel) {
+        const TwoLevelHit h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
+                                                    ignore);
+        if (h.prim >= 0) {
+            t_hit = h.t;
+            u_hit = h.u;
+            v_hit = h.v;
+        } else {
+            t_hit = t_max;
+        }
+        return h.prim;
+    } else {
+        return closest(sv, o, d, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+    }
+}
+
+template <bool kTwoLevel>
+__device__ __forceinline__ bool scene_occluded(const SceneView& sv, const vec& o, const vec& d, float t_min,
+                                               float t_max, int ignore) {
+    if constexpr (kTwoLevel)
+        return two_level_walk<true>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore).prim >= 0;
+    else
+        return occluded(sv, o, d, t_min, t_max, ignore);
+}
+
 __device__ __forceinline__ vec xyz(const float4& f) { return mk(f.x, f.y, f.z); }
 __device__ __forceinline__ float4 pack(const vec& v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
@@ -226,7 +257,7 @@ __global__ __launch_bounds__(B) void k_generate(const BatchDesc* __restrict__ de
 // material): the first hit found blocks the ray, so an any-hit walk gives the
 // same transmittance (0 or 1) as the closest-hit loop. The choice is made per
 // batch on the host; reading the scene's flag per ray instead measured slower.
-template <bool kAnyShadow>
+template <bool kAnyShadow, bool kTwoLevel>
 __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, const Paths& ps, uint32_t i) {
     float4 so = ps.so[i], sd = ps.sd[i], sc = ps.sc[i], st = ps.st[i];
     const uint32_t path = __float_as_uint(sc.w);
@@ -235,10 +266,10 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
     float t_max = so.w, t_min = kEps;
     int ignore = __float_as_int(st.w);
     vec tr = splat(1);
-    if (kAnyShadow && occluded(sv, o, d, t_min, t_max, ignore)) tr = splat(0);
+    if (kAnyShadow && scene_occluded<kTwoLevel>(sv, o, d, t_min, t_max, ignore)) tr = splat(0);
     for (int guard = 0; !kAnyShadow && guard < 1 << 16; ++guard) {
         float t = 0, u = 0, v = 0;
-        int prim = closest(sv, o, d, t_min, t_max, ignore, t, u, v);
+        int prim = scene_closest<kTwoLevel>(sv, o, d, t_min, t_max, ignore, t, u, v);
         if (prim < 0) break;
         const DevMaterial& m = sv.mats[sv.mat_id[prim]];
         if (!passthrough(m)) {
@@ -266,7 +297,7 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
 // counted by counts[sh_in] (-1: none), in one grid-stride pass. Block 0 clears
 // the counters the following shade launch appends to (zero_a, zero_b; never
 // the ones read here) and adds the queue sizes to the ray totals.
-template <int B, bool kAnyShadow>
+template <int B, bool kAnyShadow, bool kTwoLevel>
 __global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
                                                   int sh_in, int zero_a, int zero_b) {
     const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
@@ -284,10 +315,10 @@ __global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc,
             const float4 o = ps.ro[path], d = ps.rd[path];
             const SceneView& sv = desc->seg[__float_as_uint(d.w)].sv;
             float t = INFINITY, u = 0, v = 0;
-            int prim = closest(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
+            int prim = scene_closest<kTwoLevel>(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
             ps.hit[path] = make_float4(__int_as_float(prim), t, u, v);
         } else {
-            shadow_ray<kAnyShadow>(desc, ps, i - n_ext);
+            shadow_ray<kAnyShadow, kTwoLevel>(desc, ps, i - n_ext);
         }
     }
 }
@@ -662,7 +693,7 @@ void Renderer::set_tuning(const Tuning& t) {
 void Renderer::release_graph() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     graph_ = nullptr;
-    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = graph_opaque_ = -1;
+    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = graph_opaque_ = graph_two_level_ = -1;
 }
 
 void Renderer::ensure_capacity(int paths) {
@@ -732,7 +763,7 @@ void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* sha
 namespace {
 // The wavefront launch sequence of one batch (captured or launched directly).
 // Raw launches (no probe events) when captured.
-template <int B, bool kProbe, bool kOpaque>
+template <int B, bool kProbe, bool kOpaque, bool kTwoLevel>
 void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
@@ -749,49 +780,57 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
     PT_LAUNCH("pt_generate", k_generate<B>, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        PT_LAUNCH("pt_trace", (k_trace<B, kOpaque>), d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1,
+        PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1,
                   2 + (b & 1));
         PT_LAUNCH("pt_shade", k_shade<B>, d, ps, qs, in, b, -1);
     }
-    PT_LAUNCH("pt_trace", (k_trace<B, kOpaque>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
+    PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
     PT_LAUNCH("pt_flush", k_flush<B>, d, ps);
 #undef PT_LAUNCH
 }
 }  // namespace
 
 namespace {
-template <bool kProbe, bool kOpaque>
+template <bool kProbe, bool kOpaque, bool kTwoLevel>
 void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     switch (block) {
-        case 128: wavefront_sequence<128, kProbe, kOpaque>(buf, depth, grid, stream); break;
-        case 512: wavefront_sequence<512, kProbe, kOpaque>(buf, depth, grid, stream); break;
-        default: wavefront_sequence<256, kProbe, kOpaque>(buf, depth, grid, stream); break;
+        case 128: wavefront_sequence<128, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
+        case 512: wavefront_sequence<512, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
+        default: wavefront_sequence<256, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
     }
 }
 
 template <bool kProbe>
-void wavefront_sequence_o(bool opaque, int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
-    if (opaque)
-        wavefront_sequence_b<kProbe, true>(block, buf, depth, grid, stream);
-    else
-        wavefront_sequence_b<kProbe, false>(block, buf, depth, grid, stream);
+void wavefront_sequence_o(bool opaque, bool two_level, int block, const PathBuf& buf, int depth, int grid,
+                          hipStream_t stream) {
+    if (two_level) {
+        if (opaque)
+            wavefront_sequence_b<kProbe, true, true>(block, buf, depth, grid, stream);
+        else
+            wavefront_sequence_b<kProbe, false, true>(block, buf, depth, grid, stream);
+    } else {
+        if (opaque)
+            wavefront_sequence_b<kProbe, true, false>(block, buf, depth, grid, stream);
+        else
+            wavefront_sequence_b<kProbe, false, false>(block, buf, depth, grid, stream);
+    }
 }
 }  // namespace
 
-void Renderer::launch_wavefront(int depth, bool opaque, hipStream_t stream) {
+void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, hipStream_t stream) {
     const int block = tune_.block;
     const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
     if (!tune_.use_graph) {
-        wavefront_sequence_o<true>(opaque, block, buf_, depth, grid, stream);
+        wavefront_sequence_o<true>(opaque, two_level, block, buf_, depth, grid, stream);
         return;
     }
     if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity ||
-        graph_block_ != block || graph_opaque_ != int(opaque)) {
+        graph_block_ != block || graph_opaque_ != int(opaque) || graph_two_level_ != int(two_level)) {
         release_graph();
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {
-            wavefront_sequence_o<false>(opaque, block, buf_, depth, grid, stream);
+            wavefront_sequence_o<false>(opaque, two_level, block, buf_, depth, grid, stream);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -806,26 +845,37 @@ void Renderer::launch_wavefront(int depth, bool opaque, hipStream_t stream) {
         graph_capacity_ = buf_.capacity;
         graph_block_ = block;
         graph_opaque_ = int(opaque);
+        graph_two_level_ = int(two_level);
     }
     hip_check(hipGraphLaunch(graph_, stream), "hipGraphLaunch");
 }
 
-void Renderer::launch_plain(int depth, uint32_t count, hipStream_t stream) {
-    const Paths ps = paths_of(buf_);
-    const Queues qs = queues_of(buf_);
-    const BatchDesc* d = buf_.desc;
+namespace {
+template <bool kTwoLevel>
+void plain_sequence(const PathBuf& buf, int depth, uint32_t count, hipStream_t stream) {
+    const Paths ps = paths_of(buf);
+    const Queues qs = queues_of(buf);
+    const BatchDesc* d = buf.desc;
     const int grid = std::max(1, div_up(count, kBlock));
     LUX_LAUNCH("pt_generate", k_generate<kBlock>, grid, kBlock, 0, stream, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        LUX_LAUNCH("pt_trace", (k_trace<kBlock, false>), grid, kBlock, 0, stream, d, ps, qs, in, -1, in ^ 1,
-                   2 + (b & 1));
+        LUX_LAUNCH("pt_trace", (k_trace<kBlock, false, kTwoLevel>), grid, kBlock, 0, stream, d, ps, qs, in, -1,
+                   in ^ 1, 2 + (b & 1));
         for (int type = 0; type < kMaterialTypes; ++type)
             LUX_LAUNCH("pt_shade", k_shade<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
-        LUX_LAUNCH("pt_shadow", (k_trace<kBlock, false>), grid, kBlock, 0, stream, d, ps, qs, -1, 2 + (b & 1), -1,
-                   -1);
+        LUX_LAUNCH("pt_shadow", (k_trace<kBlock, false, kTwoLevel>), grid, kBlock, 0, stream, d, ps, qs, -1,
+                   2 + (b & 1), -1, -1);
     }
     LUX_LAUNCH("pt_flush", k_flush<kBlock>, grid, kBlock, 0, stream, d, ps);
+}
+}  // namespace
+
+void Renderer::launch_plain(int depth, uint32_t count, bool two_level, hipStream_t stream) {
+    if (two_level)
+        plain_sequence<true>(buf_, depth, count, stream);
+    else
+        plain_sequence<false>(buf_, depth, count, stream);
 }
 
 uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
@@ -836,9 +886,15 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     uint64_t used = 0;
     uint32_t nseg = 0;
     bool opaque = true;  // every scene of the batch: shadow rays may use the any-hit walk
+    // One traversal kind per batch (the kernels are built for one): the kind of
+    // the first request with paths left; requests of the other kind wait for a
+    // later batch and keep their place in `active`.
+    int kind = -1;
     for (Request* rq : active) {
         if (used >= cap || nseg >= uint32_t(kMaxSegments)) break;
         if (rq->issued >= rq->total) continue;
+        if (kind < 0) kind = rq->sv.instanced;
+        if (rq->sv.instanced != kind) continue;
         uint64_t take = std::min(rq->total - rq->issued, cap - used);
         if (slice_spp > 0)
             take = std::min(take, uint64_t(slice_spp) * uint64_t(rq->rs.width) * uint64_t(rq->rs.height));
@@ -853,7 +909,7 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     slot->count = uint32_t(used);
     slot->nseg = nseg;
     upload(slot, stream);
-    launch_wavefront(max_depth, opaque, stream);
+    launch_wavefront(max_depth, opaque, kind == 1, stream);
     ++batch_counter_;
     return used;
 }
@@ -880,7 +936,7 @@ void Renderer::render_all(std::vector<Request*>& reqs, int mode, hipStream_t str
                     slot->nseg = 1;
                     slot->seg[0] = make_segment(*rq, uint64_t(s) * npix + p0, 0, n);
                     upload(slot, stream);
-                    launch_plain(rq->rs.max_depth, n, stream);
+                    launch_plain(rq->rs.max_depth, n, rq->sv.instanced != 0, stream);
                     hip_check(hipMemcpyAsync(host_counts, buf_.counts, sizeof(host_counts), hipMemcpyDeviceToHost,
                                              stream),
                               "D2H counts");

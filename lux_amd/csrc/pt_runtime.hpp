@@ -10,6 +10,7 @@
 
 #include "lux_runtime.hpp"
 #include "pt_device.hpp"
+#include "two_level.hpp"
 
 namespace lux {
 namespace pt {
@@ -24,6 +25,13 @@ namespace pt {
 //   mat_id[T], mats[M]        48-byte material records
 //   lights[K], light_of_prim[T]  power-alias light table, emitter lookup
 //   bvh              the LBVH over the triangle boxes (ids = triangle index)
+// An instanced scene (meshes placed by affine instances) keeps the same shading
+// arrays, indexed by scene ordinal (instances in order, each its mesh's
+// triangles): soup[T] in world space, normals[9T] per corner in world space,
+// indices[3T] = 0, 1, 2, ... (corner c of ordinal t is 3t + c), mat_id[T] after
+// the instance's material override. Traversal uses the two-level structure
+// (tl: a BLAS per mesh over its object-space triangles, a TLAS over the
+// instances' world bounds) instead of bvh.
 struct SceneDev {
     BvhDev bvh;
     lux_triangle* soup = nullptr;
@@ -46,6 +54,17 @@ struct SceneDev {
     std::vector<int32_t> emitter_mat;
     size_t device_bytes = 0;  // HBM held by the scene (mesh + BVH)
     bool opaque = true;       // no pass-through (interface, transparent) material
+    // instanced scenes
+    bool instanced = false;
+    TwoLevelDev tl;
+    // the meshes as given (object space), kept to re-flatten after an instance update
+    std::vector<lux_vec3> mesh_vertices, mesh_normals;
+    std::vector<uint32_t> mesh_indices;
+    std::vector<int32_t> mesh_vertex_off, mesh_tri_off, mesh_mat_id;
+    int mesh_count = 0;
+    // admitted, unfinished server requests: they hold this scene's device
+    // pointers, so the instances may not move under them
+    int server_jobs = 0;
 };
 
 // POD view of one scene handed to kernels.
@@ -60,8 +79,10 @@ struct SceneView {
     const int32_t* light_of_prim;
     DevInfo* info;
     int32_t node_count, tri_count, mat_count, light_count;
-    int32_t opaque;  // 1: every hit blocks a shadow ray (no pass-through material)
+    int32_t opaque;     // 1: every hit blocks a shadow ray (no pass-through material)
+    int32_t instanced;  // 1: traverse `tl` (two-level) instead of `nodes`
     DevCamera cam;
+    TwoLevelView tl;
 };
 
 // One segment of a batch: a contiguous range of one request's paths. A
@@ -147,7 +168,9 @@ public:
     // Composes and enqueues ONE wavefront batch from `active` (in order; each
     // request gives at most `slice_spp` samples of its image, 0 = no limit).
     // Returns the number of paths placed (0 = nothing left). Sets last_batch of
-    // requests whose final path is in this batch to the batch's index.
+    // requests whose final path is in this batch to the batch's index. A batch
+    // holds one traversal kind (one-mesh or instanced scenes): the first
+    // unfinished request's; the others wait for a later batch.
     uint64_t enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
                            int64_t* batch_index);
 
@@ -162,8 +185,8 @@ private:
     void ensure_capacity(int paths);
     BatchDesc* next_slot();
     void upload(BatchDesc* slot, hipStream_t stream);
-    void launch_wavefront(int depth, bool opaque, hipStream_t stream);
-    void launch_plain(int depth, uint32_t count, hipStream_t stream);
+    void launch_wavefront(int depth, bool opaque, bool two_level, hipStream_t stream);
+    void launch_plain(int depth, uint32_t count, bool two_level, hipStream_t stream);
     void release_graph();
 
     Tuning tune_;
@@ -174,7 +197,8 @@ private:
     bool ring_used_[kRing] = {};
     int ring_next_ = 0;
     hipGraphExec_t graph_ = nullptr;
-    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1, graph_opaque_ = -1;
+    int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1, graph_opaque_ = -1,
+        graph_two_level_ = -1;
     int64_t batch_counter_ = 0;
 };
 
@@ -182,6 +206,12 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream);
 // Replaces vertex positions (same count), regathers the soup on the device,
 // refits the BVH and rebuilds the light table; the topology is kept.
 void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream);
+// An instanced scene: BLAS per mesh, TLAS over the instances, world-space
+// shading arrays by scene ordinal and the light table over them.
+void scene_upload_instanced(SceneDev& s, const lux_instanced_scene_desc& desc, hipStream_t stream);
+// New instance transforms (same count): rebuilds the TLAS, the world-space
+// shading arrays and the light table; the BLASes are kept.
+void scene_set_instances(SceneDev& s, const lux_instance* instances, int instance_count, hipStream_t stream);
 void scene_release(SceneDev& s);
 SceneView scene_view(const SceneDev& s);
 uint64_t request_paths(const lux_render_settings& rs);

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "pt_device.hpp"
@@ -87,6 +88,39 @@ V instance_normal(V n) {
     V once = vnorm(t);
     if (vdot(once, once) == 0) return mkv(0, 0, 0);
     return vnorm(once);
+}
+
+// transform_instance_normal (src/scene/scene_data.h:262-268) under an affine
+// object_to_world m (row-major): transform_normal's cofactor matrix of the
+// upper 3x3 times n, normalised (src/core/mat4.cuh:209-229), then a zero stays
+// zero and the result is normalised again.
+V instance_normal(const float* m, V n) {
+    if (vdot(n, n) == 0) return mkv(0, 0, 0);
+    const float c00 = m[5] * m[10] - m[6] * m[9];
+    const float c01 = m[6] * m[8] - m[4] * m[10];
+    const float c02 = m[4] * m[9] - m[5] * m[8];
+    const float c10 = m[2] * m[9] - m[1] * m[10];
+    const float c11 = m[0] * m[10] - m[2] * m[8];
+    const float c12 = m[1] * m[8] - m[0] * m[9];
+    const float c20 = m[1] * m[6] - m[2] * m[5];
+    const float c21 = m[2] * m[4] - m[0] * m[6];
+    const float c22 = m[0] * m[5] - m[1] * m[4];
+    V t = mkv(c00 * n.x + c01 * n.y + c02 * n.z, c10 * n.x + c11 * n.y + c12 * n.z,
+              c20 * n.x + c21 * n.y + c22 * n.z);
+    V once = vnorm(t);
+    if (vdot(once, once) == 0) return mkv(0, 0, 0);
+    return vnorm(once);
+}
+
+// transform_point (src/core/mat4.cuh:173-190): rows summed left to right, a
+// division by w unless w == 1.
+lux_vec3 world_point(const float* m, const lux_vec3& p) {
+    const float x = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    const float y = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    const float z = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    const float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    if (w == 1.0f) return lux_vec3{x, y, z};
+    return lux_vec3{x / w, y / w, z / w};
 }
 
 // Power light selection with its alias table: prepare_scene_light_sampling
@@ -305,6 +339,7 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
 
 void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream) {
     if (!s.soup) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene is empty");
+    if (s.instanced) throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: vertex updates do not apply");
     if (!vertices || vertex_count != s.vertex_count)
         throw Error(LUX_ERR_MISMATCH, "vertex update must keep the vertex count");
     std::vector<lux_triangle> etris;
@@ -324,18 +359,253 @@ void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_cou
     hip_check(hipStreamSynchronize(stream), "scene update");
 }
 
+// ---- instanced scenes ------------------------------------------------------------------
+namespace {
+
+bool finite3(const lux_vec3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+// The checks of validate_scene_shape (src/scene/scene_validate.h:125-147) that
+// the two-level build does not make itself: material overrides in range.
+void check_overrides(const lux_instance* instances, int count, int material_count) {
+    for (int i = 0; i < count; ++i)
+        if (instances[i].material_override < -1 || instances[i].material_override >= material_count)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": material override out of range");
+}
+
+// Mesh bounds (finalize_scene_mesh: union of the triangle bounds) through
+// transform_bounds: SceneInstance::world_bounds as add_scene_instance sets it.
+std::vector<lux_instance> with_world_bounds(const SceneDev& s, const lux_instance* instances, int count) {
+    std::vector<lux_aabb> mesh_box(static_cast<size_t>(s.mesh_count));
+    for (int m = 0; m < s.mesh_count; ++m) {
+        lux_aabb b;
+        b.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
+        b.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
+        const int v0 = s.mesh_vertex_off[m];
+        for (int t = s.mesh_tri_off[m]; t < s.mesh_tri_off[m + 1]; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const lux_vec3& p = s.mesh_vertices[size_t(v0) + s.mesh_indices[3 * size_t(t) + k]];
+                b.p_min = lux_vec3{std::fmin(b.p_min.x, p.x), std::fmin(b.p_min.y, p.y), std::fmin(b.p_min.z, p.z)};
+                b.p_max = lux_vec3{std::fmax(b.p_max.x, p.x), std::fmax(b.p_max.y, p.y), std::fmax(b.p_max.z, p.z)};
+            }
+        mesh_box[m] = b;
+    }
+    std::vector<lux_instance> out(instances, instances + count);
+    for (lux_instance& in : out)
+        if (in.mesh_id >= 0 && in.mesh_id < s.mesh_count)
+            in.world_bounds = transform_bounds(in.object_to_world, mesh_box[in.mesh_id]);
+    return out;
+}
+
+int64_t ordinal_count(const SceneDev& s, const lux_instance* instances, int count) {
+    int64_t n = 0;
+    for (int i = 0; i < count; ++i) n += s.mesh_tri_off[instances[i].mesh_id + 1] - s.mesh_tri_off[instances[i].mesh_id];
+    return n;
+}
+
+// scene_triangle_view of every scene ordinal (src/scene/scene_data.h:335-372):
+// world corners, world corner normals, material after the override; then the
+// light table over the emissive ordinals and the upload of all of it.
+void flatten_instances(SceneDev& s, const lux_instance* instances, int count, hipStream_t stream) {
+    const int64_t total = ordinal_count(s, instances, count);
+    if (total <= 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "the instances place no triangle");
+    if (total > INT32_MAX / 3) throw Error(LUX_ERR_INVALID_ARGUMENT, "too many scene primitives");
+    const int T = int(total);
+    const bool has_normals = !s.mesh_normals.empty();
+    std::vector<lux_triangle> soup(static_cast<size_t>(T));
+    std::vector<float> normals(has_normals ? 9 * size_t(T) : 0);
+    std::vector<int32_t> mat_id(static_cast<size_t>(T));
+    std::vector<int32_t> emitters, emitter_mat;
+    std::vector<lux_triangle> etris;
+    size_t o = 0;
+    for (int i = 0; i < count; ++i) {
+        const lux_instance& in = instances[i];
+        const float* m = in.object_to_world;
+        const int v0 = s.mesh_vertex_off[in.mesh_id];
+        for (int t = s.mesh_tri_off[in.mesh_id]; t < s.mesh_tri_off[in.mesh_id + 1]; ++t, ++o) {
+            const uint32_t* ix = s.mesh_indices.data() + 3 * size_t(t);
+            lux_vec3 c[3];
+            for (int k = 0; k < 3; ++k) {
+                c[k] = host::world_point(m, s.mesh_vertices[size_t(v0) + ix[k]]);
+                if (has_normals) {
+                    const lux_vec3& n = s.mesh_normals[size_t(v0) + ix[k]];
+                    const host::V w = host::instance_normal(m, host::mkv(n.x, n.y, n.z));
+                    normals[9 * o + 3 * k] = w.x;
+                    normals[9 * o + 3 * k + 1] = w.y;
+                    normals[9 * o + 3 * k + 2] = w.z;
+                }
+            }
+            soup[o] = lux_triangle{c[0], c[1], c[2]};
+            mat_id[o] = in.material_override >= 0 ? in.material_override : s.mesh_mat_id[t];
+            const lux_material& mm = s.host_mats[mat_id[o]];
+            if (fmaxf(mm.emission.x, fmaxf(mm.emission.y, mm.emission.z)) > 0) {
+                emitters.push_back(int32_t(o));
+                emitter_mat.push_back(mat_id[o]);
+                etris.push_back(soup[o]);
+            }
+        }
+    }
+    std::vector<lux_material> emats(emitters.size());
+    for (size_t e = 0; e < emitters.size(); ++e) emats[e] = s.host_mats[emitter_mat[e]];
+    host::build_lights(T, emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+    s.emitters = emitters;
+    s.emitter_mat = emitter_mat;
+
+    if (!s.soup || T != s.tri_count) {
+        void* old[] = {s.soup, s.indices, s.normals, s.mat_id, s.light_of_prim};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        s.soup = dalloc<lux_triangle>(T);
+        s.indices = dalloc<uint32_t>(3 * size_t(T));
+        s.normals = has_normals ? dalloc<float>(9 * size_t(T)) : nullptr;
+        s.mat_id = dalloc<int32_t>(T);
+        s.light_of_prim = dalloc<int32_t>(T);
+        std::vector<uint32_t> iota(3 * size_t(T));
+        for (size_t k = 0; k < iota.size(); ++k) iota[k] = uint32_t(k);
+        hip_check(hipMemcpyAsync(s.indices, iota.data(), 4 * iota.size(), hipMemcpyHostToDevice, stream),
+                  "H2D corner indices");
+        s.tri_count = T;
+        // the pageable source above must outlive the copy
+        hip_check(hipStreamSynchronize(stream), "corner indices");
+    }
+    hip_check(hipMemcpyAsync(s.soup, soup.data(), sizeof(lux_triangle) * soup.size(), hipMemcpyHostToDevice, stream),
+              "H2D world triangles");
+    if (has_normals)
+        hip_check(hipMemcpyAsync(s.normals, normals.data(), 4 * normals.size(), hipMemcpyHostToDevice, stream),
+                  "H2D world normals");
+    hip_check(hipMemcpyAsync(s.mat_id, mat_id.data(), 4 * mat_id.size(), hipMemcpyHostToDevice, stream),
+              "H2D ordinal materials");
+    upload_lights(s, stream);
+    hip_check(hipStreamSynchronize(stream), "instanced scene upload");
+}
+
+}  // namespace
+
+void scene_upload_instanced(SceneDev& s, const lux_instanced_scene_desc& d, hipStream_t stream) {
+    const int nm = d.mesh_count, nmat = d.material_count;
+    if (nm <= 0 || nmat <= 0 || !d.vertex_offsets || !d.triangle_offsets || !d.materials)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene needs meshes, offsets and materials");
+    if (d.instance_count < 0 || (d.instance_count > 0 && !d.instances))
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "bad instance array");
+    if (d.vertex_offsets[0] != 0 || d.triangle_offsets[0] != 0)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh offsets must start at 0");
+    for (int m = 0; m < nm; ++m)
+        if (d.vertex_offsets[m + 1] < d.vertex_offsets[m] || d.triangle_offsets[m + 1] < d.triangle_offsets[m])
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh offsets must not decrease");
+    const int nv = d.vertex_offsets[nm], nt = d.triangle_offsets[nm];
+    if ((nv > 0 && !d.vertices) || (nt > 0 && (!d.indices || !d.material_ids)))
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "null mesh arrays");
+    for (int v = 0; v < nv; ++v)
+        if (!finite3(d.vertices[v]) || (d.normals && !finite3(d.normals[v])))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh vertex or normal is not finite");
+    for (int m = 0; m < nm; ++m) {
+        const uint32_t count = uint32_t(d.vertex_offsets[m + 1] - d.vertex_offsets[m]);
+        for (int t = d.triangle_offsets[m]; t < d.triangle_offsets[m + 1]; ++t) {
+            for (int k = 0; k < 3; ++k)
+                if (d.indices[3 * size_t(t) + k] >= count)
+                    throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh triangle index out of range");
+            if (d.material_ids[t] < 0 || d.material_ids[t] >= nmat)
+                throw Error(LUX_ERR_INVALID_ARGUMENT, "mesh material id out of range");
+        }
+    }
+    std::vector<DevMaterial> mats(nmat);
+    for (int m = 0; m < nmat; ++m) {
+        const lux_material& in = d.materials[m];
+        if (in.type < 0 || in.type >= kMaterialTypes) throw Error(LUX_ERR_INVALID_ARGUMENT, "unknown material type");
+        mats[m] = DevMaterial{in.type, in.albedo.x, in.albedo.y, in.albedo.z, in.emission.x, in.emission.y,
+                              in.emission.z, in.emission_two_sided, in.roughness, in.metallic, in.eta, 0};
+    }
+    check_overrides(d.instances, d.instance_count, nmat);
+
+    SceneDev n;
+    n.instanced = true;
+    n.mesh_count = nm;
+    n.mesh_vertices.assign(d.vertices, d.vertices + nv);
+    if (d.normals) n.mesh_normals.assign(d.normals, d.normals + nv);
+    n.mesh_indices.assign(d.indices, d.indices + 3 * size_t(nt));
+    n.mesh_mat_id.assign(d.material_ids, d.material_ids + nt);
+    n.mesh_vertex_off.assign(d.vertex_offsets, d.vertex_offsets + nm + 1);
+    n.mesh_tri_off.assign(d.triangle_offsets, d.triangle_offsets + nm + 1);
+    n.host_mats.assign(d.materials, d.materials + nmat);
+    n.mat_count = nmat;
+    n.vertex_count = nv;
+    n.opaque = true;
+    for (const DevMaterial& m : mats) n.opaque = n.opaque && m.type != kInterface && m.type != kTransparent;
+    n.camera = host::make_camera(d.camera);
+    for (int i = 0; i < d.instance_count; ++i)  // mesh ids first: the ordinal count reads them
+        if (d.instances[i].mesh_id < 0 || d.instances[i].mesh_id >= nm)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": mesh id out of range");
+    if (ordinal_count(n, d.instances, d.instance_count) <= 0)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "the instances place no triangle");
+    const std::vector<lux_instance> inst = with_world_bounds(n, d.instances, d.instance_count);
+    // object-space soup per mesh for the BLASes
+    std::vector<lux_triangle> obj(static_cast<size_t>(nt));
+    for (int m = 0; m < nm; ++m)
+        for (int t = d.triangle_offsets[m]; t < d.triangle_offsets[m + 1]; ++t) {
+            const lux_vec3* V = d.vertices + d.vertex_offsets[m];
+            const uint32_t* ix = d.indices + 3 * size_t(t);
+            obj[t] = lux_triangle{V[ix[0]], V[ix[1]], V[ix[2]]};
+        }
+    try {
+        two_level_build(n.tl, obj.data(), d.triangle_offsets, nm, inst.data(), d.instance_count, stream);
+        n.mats = dalloc<DevMaterial>(nmat);
+        hip_check(hipMemcpyAsync(n.mats, mats.data(), sizeof(DevMaterial) * nmat, hipMemcpyHostToDevice, stream),
+                  "H2D mats");
+        flatten_instances(n, inst.data(), d.instance_count, stream);
+    } catch (...) {
+        scene_release(n);
+        throw;
+    }
+    n.device_bytes = n.tl.device_bytes + size_t(n.tri_count) * (36 + 12 + (d.normals ? 36 : 0) + 4 + 4) +
+                     sizeof(DevMaterial) * nmat + sizeof(DevLight) * std::max<size_t>(n.host_lights.size(), 1);
+    scene_release(s);
+    s = std::move(n);
+}
+
+void scene_set_instances(SceneDev& s, const lux_instance* instances, int count, hipStream_t stream) {
+    if (!s.instanced) throw Error(LUX_ERR_INVALID_ARGUMENT, "not an instanced scene");
+    if (count != s.tl.instance_count) throw Error(LUX_ERR_MISMATCH, "instance count differs from the scene's");
+    if (s.server_jobs > 0)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "the scene has unfinished server requests (poll them to completion first)");
+    if (count > 0 && !instances) throw Error(LUX_ERR_INVALID_ARGUMENT, "null instance array");
+    check_overrides(instances, count, s.mat_count);
+    for (int i = 0; i < count; ++i)
+        if (instances[i].mesh_id < 0 || instances[i].mesh_id >= s.mesh_count)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": mesh id out of range");
+    for (int i = 0; i < count; ++i)
+        if (!instance_transform_valid(instances[i].object_to_world, instances[i].world_to_object))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) +
+                                                      ": transform must be affine-invertible with a consistent inverse");
+    if (ordinal_count(s, instances, count) <= 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "the instances place no triangle");
+    const std::vector<lux_instance> inst = with_world_bounds(s, instances, count);
+    // renders enqueued earlier (any stream) finish with the old placement
+    hip_check(hipDeviceSynchronize(), "synchronize before instance update");
+    two_level_set_instances(s.tl, inst.data(), count, stream);
+    flatten_instances(s, inst.data(), count, stream);
+    s.device_bytes = s.tl.device_bytes + size_t(s.tri_count) * (36 + 12 + (s.normals ? 36 : 0) + 4 + 4) +
+                     sizeof(DevMaterial) * s.mat_count + sizeof(DevLight) * std::max<size_t>(s.host_lights.size(), 1);
+}
+
 void scene_release(SceneDev& s) {
     void* ptrs[] = {s.soup, s.vertices, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     bvh_release(s.bvh);
+    two_level_release(s.tl);
     s = SceneDev{};
 }
 
 SceneView scene_view(const SceneDev& s) {
     SceneView v{};
-    v.nodes = s.bvh.pairs;  // sibling-pair traversal copy
-    v.node_count = s.bvh.n > 1 ? 2 * s.bvh.n : (s.bvh.n > 0 ? 1 : 0);
+    if (s.instanced) {
+        v.nodes = nullptr;
+        v.node_count = 0;
+        v.tl = two_level_view(s.tl);
+        v.instanced = 1;
+    } else {
+        v.nodes = s.bvh.pairs;  // sibling-pair traversal copy
+        v.node_count = s.bvh.n > 1 ? 2 * s.bvh.n : (s.bvh.n > 0 ? 1 : 0);
+        v.instanced = 0;
+    }
     v.soup = s.soup;
     v.tri_count = s.tri_count;
     v.indices = s.indices;
@@ -348,7 +618,7 @@ SceneView scene_view(const SceneDev& s) {
     v.light_of_prim = s.light_of_prim;
     v.opaque = s.opaque ? 1 : 0;
     v.cam = s.camera;
-    v.info = s.bvh.info;
+    v.info = s.instanced ? s.tl.tlas.info : s.bvh.info;
     return v;
 }
 

@@ -19,6 +19,7 @@ Server::Server(const ServeConfig& cfg, const Tuning& tuning, hipStream_t stream)
 
 Server::~Server() {
     (void)hipStreamSynchronize(stream_);
+    for (Job* j : active_) --j->scene->server_jobs;  // admitted, never retired
     for (auto& f : inflight_) (void)hipEventDestroy(f.ev);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     for (auto& kv : jobs_)
@@ -35,6 +36,8 @@ uint64_t Server::submit(SceneDev* scene, const lux_render_settings& rs, const lu
                         double arrival) {
     if (!scene || !scene->soup) throw Error(LUX_ERR_INVALID_ARGUMENT, "request needs a scene");
     if (rs.mode != LUX_RENDER_WAVEFRONT) throw Error(LUX_ERR_INVALID_ARGUMENT, "the server renders wavefront only");
+    if (vertices && scene->instanced)
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: vertex updates do not apply");
     if (vertices && vertex_count != scene->vertex_count)
         throw Error(LUX_ERR_MISMATCH, "vertex update must keep the vertex count");
     auto job = std::make_unique<Job>();
@@ -125,6 +128,7 @@ void Server::admit() {
         j->rq.tag = j->id;
         j->admitted = now();
         ++scene_users_[j->scene];
+        ++j->scene->server_jobs;
         active_.push_back(j);
         issue_.push_back(j);
         jobs_[j->id] = std::move(*it);
@@ -192,6 +196,7 @@ void Server::retire(bool block, std::vector<uint64_t>* done) {
                 j->done = true;
                 j->finish = t;
                 --scene_users_[j->scene];
+                --j->scene->server_jobs;
                 ++stats_.completed;
                 if (done) done->push_back(j->id);
                 it = active_.erase(it);

@@ -1,7 +1,12 @@
 """numpy-facing wavefront path tracer over the C ABI.
 
 `Scene(scene_dict)` uploads a scene (see lux_amd.scenes for the dict layout):
-PreparedScene + prepare_gpu (src/scene/prepared_scene.cu:19-23, :61-79).
+PreparedScene + prepare_gpu (src/scene/prepared_scene.cu:19-23, :61-79). A
+dict with "inst_mesh" is an instanced scene (several meshes placed by affine
+instances): "vertices" / "indices" / "material_ids" / "normals" of all meshes
+concatenated, "vertex_offsets" and "triangle_offsets" (mesh_count + 1 each,
+indices mesh-local), "inst_mesh", "inst_m" (row-major object_to_world 4x4) and
+optionally "inst_override" (material override, -1 for none).
 `Scene.render(...)` is WavefrontPathTracer::render
 (src/integrator/wavefront_pt.cu:51-84) and returns the film SUM over spp,
 shape (height, width, 3), row 0 at the top (Film::pixels).
@@ -29,6 +34,13 @@ class SceneDesc(ctypes.Structure):
                 ("material_count", c_int32), ("camera", c_float * 8)]
 
 
+class InstancedSceneDesc(ctypes.Structure):
+    _fields_ = [("vertices", c_void_p), ("normals", c_void_p), ("vertex_offsets", c_void_p), ("indices", c_void_p),
+                ("material_ids", c_void_p), ("triangle_offsets", c_void_p), ("mesh_count", c_int32),
+                ("instances", c_void_p), ("instance_count", c_int32), ("materials", c_void_p),
+                ("material_count", c_int32), ("camera", c_float * 8)]
+
+
 class RenderSettings(ctypes.Structure):
     _fields_ = [("width", c_int32), ("height", c_int32), ("spp", c_int32), ("max_depth", c_int32),
                 ("seed", c_uint64), ("frame_index", c_int32), ("sample_offset", c_int32), ("batch_size", c_int32),
@@ -46,6 +58,9 @@ def _register():
     lib = abi.load()
     sigs = {
         "lux_hip_scene_create": [POINTER(SceneDesc), POINTER(c_void_p)],
+        "lux_hip_scene_create_instanced": [POINTER(InstancedSceneDesc), POINTER(c_void_p)],
+        "lux_hip_scene_set_instances": [c_void_p, c_void_p, c_int32],
+        "lux_hip_scene_triangles": [c_void_p, c_void_p, c_void_p, c_void_p],
         "lux_hip_scene_destroy": [c_void_p],
         "lux_hip_scene_camera": [c_void_p, c_void_p],
         "lux_hip_scene_lights": [c_void_p, POINTER(c_int32)] + [c_void_p] * 6,
@@ -66,6 +81,8 @@ def settings(width, height, spp, depth, seed=0, frame=0, sample_offset=0, batch_
 
 
 class Scene:
+    instanced = False
+
     def __init__(self, scene):
         lbvh.init()
         _register()
@@ -82,6 +99,9 @@ class Scene:
         for k, row in enumerate(mats):
             self._mats[k] = Material(int(row[0]), Vec3(*row[1:4]), Vec3(*row[4:7]), int(row[7]), float(row[8]),
                                      float(row[9]), float(row[10]))
+        if "inst_mesh" in scene:
+            self._create_instanced(scene, len(mats))
+            return
         desc = SceneDesc()
         desc.vertices = self._v.ctypes.data
         desc.vertex_count = len(self._v)
@@ -98,6 +118,56 @@ class Scene:
         call("lux_hip_scene_create", ctypes.byref(desc), ctypes.byref(h))
         self.handle = h
         self.triangle_count = len(self._m)
+
+    def _create_instanced(self, scene, material_count):
+        from . import two_level as TL
+
+        self.instanced = True
+        self._voff = np.ascontiguousarray(scene["vertex_offsets"], np.int32)
+        self._toff = np.ascontiguousarray(scene["triangle_offsets"], np.int32)
+        self._inst = np.ascontiguousarray(TL.make_instances(
+            scene["inst_mesh"], scene["inst_m"], world_bounds=np.zeros((len(scene["inst_mesh"]), 6), np.float32),
+            material_override=scene.get("inst_override")), TL.INSTANCE)
+        desc = InstancedSceneDesc()
+        desc.vertices = self._v.ctypes.data
+        desc.normals = None if self._n is None else self._n.ctypes.data
+        desc.vertex_offsets = self._voff.ctypes.data
+        desc.indices = self._i.ctypes.data
+        desc.material_ids = self._m.ctypes.data
+        desc.triangle_offsets = self._toff.ctypes.data
+        desc.mesh_count = len(self._voff) - 1
+        desc.instances = self._inst.ctypes.data
+        desc.instance_count = len(self._inst)
+        desc.materials = ctypes.addressof(self._mats)
+        desc.material_count = material_count
+        cam = np.asarray(scene["camera"], np.float32)
+        for k in range(8):
+            desc.camera[k] = float(cam[k])
+        h = c_void_p()
+        call("lux_hip_scene_create_instanced", ctypes.byref(desc), ctypes.byref(h))
+        self.handle = h
+        self.triangle_count = self.info()[0]
+
+    def set_instances(self, inst_mesh, inst_m, inst_override=None):
+        """Move the instances of an instanced scene (same count): TLAS rebuild,
+        world-space triangles, normals and light table; the BLASes are kept."""
+        from . import two_level as TL
+
+        inst = np.ascontiguousarray(TL.make_instances(
+            inst_mesh, inst_m, world_bounds=np.zeros((len(inst_mesh), 6), np.float32),
+            material_override=inst_override), TL.INSTANCE)
+        call("lux_hip_scene_set_instances", self.handle, inst.ctypes.data, len(inst))
+        self._inst = inst
+        self.triangle_count = self.info()[0]
+
+    def triangles(self):
+        """The renderer's shading view per scene ordinal: world corners (T, 9),
+        world corner normals (T, 9), material after the override (T,)."""
+        t = self.info()[0]
+        out = {"corners": np.zeros((t, 9), np.float32), "normals": np.zeros((t, 9), np.float32),
+               "material": np.zeros(t, np.int32)}
+        call("lux_hip_scene_triangles", self.handle, ptr(out["corners"]), ptr(out["normals"]), ptr(out["material"]))
+        return out
 
     def close(self):
         if self.handle:

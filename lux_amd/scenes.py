@@ -139,6 +139,97 @@ def mixed_materials(width=40, height=40):
     return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
 
 
+# ---- instanced scenes ------------------------------------------------------------
+
+def instanced_room(width=40, height=40, pass_through=True):
+    """Meshes placed by affine instances (an instanced scene dict: the mesh
+    arrays concatenated with "vertex_offsets" / "triangle_offsets", mesh-local
+    indices, and "inst_mesh", "inst_m", "inst_override"):
+      mesh 0  the Cornell room (walls + ceiling light), identity instance
+      mesh 1  a unit geodesic sphere (freq 4, 320 triangles) with smooth
+              normals: uniformly scaled (plastic), non-uniformly scaled and
+              rotated (metal override) and mirrored (glass override)
+      mesh 2  a unit cube (flat): a rotated block, a thin two-sided emitting
+              slab (emitter override: an instanced light, non-uniform scale)
+              and, with pass_through, a tinted transparent sheet in front
+    """
+    from .workloads import affine
+
+    f = np.float32
+    materials = [
+        material(LAMBERT, (0.78, 0.78, 0.74)),                                  # 0 white
+        material(LAMBERT, (0.65, 0.05, 0.05)),                                  # 1 red
+        material(LAMBERT, (0.12, 0.45, 0.15)),                                  # 2 green
+        material(LAMBERT, (0, 0, 0), emission=(15.0, 14.0, 12.0)),              # 3 ceiling light
+        material(GGX, (0.3, 0.5, 0.8), roughness=f(0.45), metallic=f(0.0)),     # 4 plastic
+        material(GGX, (0.9, 0.7, 0.4), roughness=f(0.2), metallic=f(1.0)),      # 5 metal
+        material(DIELECTRIC, (1.0, 1.0, 1.0), roughness=0.0, eta=f(1.5)),       # 6 glass
+        material(LAMBERT, (0, 0, 0), emission=(2.0, 3.0, 6.0), two_sided=1),    # 7 two-sided emitter
+        material(TRANSPARENT, (0.6, 0.8, 0.9)),                                 # 8 tinted sheet
+    ]
+    room, room_m = [], []
+
+    def quad(p0, p1, p2, p3, m):
+        room.append([*p0, *p1, *p2])
+        room.append([*p0, *p2, *p3])
+        room_m.extend([m, m])
+
+    quad((-1, 0, 2), (1, 0, 2), (1, 0, 0), (-1, 0, 0), 0)
+    quad((-1, 2, 0), (1, 2, 0), (1, 2, 2), (-1, 2, 2), 0)
+    quad((-1, 0, 0), (1, 0, 0), (1, 2, 0), (-1, 2, 0), 0)
+    quad((-1, 0, 2), (-1, 0, 0), (-1, 2, 0), (-1, 2, 2), 1)
+    quad((1, 0, 0), (1, 0, 2), (1, 2, 2), (1, 2, 0), 2)
+    quad((f(-0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(0.75)), (f(0.35), f(1.99), f(1.35)),
+         (f(-0.35), f(1.99), f(1.35)), 3)
+    room_v = np.asarray(room, np.float32).reshape(-1, 3)
+    room_i = np.arange(len(room_v), dtype=np.uint32).reshape(-1, 3)
+    sv, si = geodesic_sphere(4)
+    sv = sv.astype(np.float32)
+    c = np.array([[x, y, z] for z in (-0.5, 0.5) for y in (-0.5, 0.5) for x in (-0.5, 0.5)], np.float32)
+    cube_i = np.array([[0, 2, 3], [0, 3, 1], [4, 5, 7], [4, 7, 6], [0, 1, 5], [0, 5, 4],
+                       [2, 6, 7], [2, 7, 3], [0, 4, 6], [0, 6, 2], [1, 3, 7], [1, 7, 5]], np.uint32)
+    verts = np.concatenate([room_v, sv, c]) + np.float32(0.0)
+    normals = np.concatenate([np.zeros_like(room_v), sv, np.zeros_like(c)]) + np.float32(0.0)
+    indices = np.concatenate([room_i, si.astype(np.uint32), cube_i])
+    mat_ids = np.concatenate([room_m, np.full(len(si), 4), np.zeros(len(cube_i))]).astype(np.int32)
+    vo = np.cumsum([0, len(room_v), len(sv), len(c)]).astype(np.int32)
+    to = np.cumsum([0, len(room_i), len(si), len(cube_i)]).astype(np.int32)
+
+    def ry(deg):
+        a = np.radians(deg) / 2
+        return (np.cos(a), 0.0, np.sin(a), 0.0)
+
+    def rx(deg):
+        a = np.radians(deg) / 2
+        return (np.cos(a), np.sin(a), 0.0, 0.0)
+
+    inst = [
+        (0, affine(), -1),
+        (1, affine(ry(30), (0.3, 0.3, 0.3), (-0.45, 0.3, 0.6)), -1),
+        (1, affine(ry(-20), (0.35, 0.2, 0.25), (0.45, 0.22, 1.1)), 5),
+        (1, affine(None, (-0.18, 0.18, 0.18), (0.0, 0.9, 1.2)), 6),
+        (2, affine(ry(45), (0.3, 0.7, 0.3), (-0.5, 0.35, 1.35)), -1),
+        (2, affine(rx(30), (0.25, 0.02, 0.25), (0.5, 1.3, 0.5)), 7),
+    ]
+    if pass_through:
+        inst.append((2, affine(None, (0.8, 0.8, 0.01), (0.0, 0.5, 1.7)), 8))
+    aspect = np.float32(width) / np.float32(height)
+    return {
+        "vertices": np.ascontiguousarray(verts, np.float32),
+        "indices": np.ascontiguousarray(indices, np.uint32),
+        "normals": np.ascontiguousarray(normals, np.float32),
+        "uvs": np.zeros((len(verts), 2), np.float32),
+        "material_ids": mat_ids,
+        "materials": np.asarray(materials, np.float32),
+        "camera": np.asarray([0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect], np.float32),
+        "vertex_offsets": vo,
+        "triangle_offsets": to,
+        "inst_mesh": np.array([k for k, _, _ in inst], np.int32),
+        "inst_m": np.stack([m for _, m, _ in inst]).astype(np.float32),
+        "inst_override": np.array([o for _, _, o in inst], np.int32),
+    }
+
+
 # ---- seeded displaced icosphere -----------------------------------------------
 
 def _icosahedron():

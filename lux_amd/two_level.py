@@ -65,8 +65,9 @@ def make_instances(mesh_ids, matrices, tris=None, mesh_offsets=None, inverses=No
     rec["mesh_id"] = mesh_ids
     rec["material_override"] = -1 if material_override is None else np.asarray(material_override, np.int32)
     rec["object_to_world"] = m
-    rec["world_to_object"] = (np.stack([inverse_affine(x).reshape(16) for x in m]) if inverses is None
-                              else np.asarray(inverses, np.float32).reshape(-1, 16))
+    if inverses is None:
+        inverses = [inverse_affine(x).reshape(16) for x in m] if len(m) else np.zeros((0, 16), np.float32)
+    rec["world_to_object"] = np.asarray(inverses, np.float32).reshape(-1, 16)
     if world_bounds is None:
         mb = mesh_bounds(tris, mesh_offsets)
         world_bounds = np.stack([transform_bounds(x, mb[k]) for x, k in zip(m, mesh_ids)]) if len(m) else \

@@ -282,6 +282,11 @@ def _ref_pt_lib():
                                     c_void_p]
         L.ref_pt_samples.argtypes = [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
                                      c_void_p, c_void_p, c_int32, c_void_p]
+        L.ref_pt_create_instanced.restype = c_void_p
+        L.ref_pt_create_instanced.argtypes = [c_void_p] * 7 + [c_int32, c_void_p, c_void_p, c_void_p, c_int32,
+                                                               c_void_p, c_int32, c_void_p]
+        L.ref_pt_triangles.restype = c_int32
+        L.ref_pt_triangles.argtypes = [c_void_p] * 4
         L._pt_ready = True
     return L
 
@@ -298,13 +303,38 @@ def _scene_args(scene):
 
 
 class RefPathTracer:
-    """render_cpu_path_tracer / trace_path of the reference on a scene dict."""
+    """render_cpu_path_tracer / trace_path of the reference on a scene dict (an
+    instanced one when the dict has "inst_mesh": meshes + affine instances,
+    built with add_scene_mesh / add_scene_instance(make_transform))."""
 
     def __init__(self, scene):
         L = _ref_pt_lib()
         self._keep = _scene_args(scene)
         v, i, n, uv, m, mats, cam = self._keep
-        self.h = L.ref_pt_create(_p(v), len(v), _p(i), _p(n), _p(uv), _p(m), len(m), _p(mats), len(mats), _p(cam))
+        if "inst_mesh" in scene:
+            vo = np.ascontiguousarray(scene["vertex_offsets"], np.int32)
+            to = np.ascontiguousarray(scene["triangle_offsets"], np.int32)
+            im = np.ascontiguousarray(scene["inst_mesh"], np.int32)
+            mm = np.ascontiguousarray(scene["inst_m"], np.float32).reshape(-1, 16)
+            ov = scene.get("inst_override")
+            ov = np.full(len(im), -1, np.int32) if ov is None else np.ascontiguousarray(ov, np.int32)
+            self._inst = (vo, to, im, mm, ov)
+            self.h = L.ref_pt_create_instanced(_p(v), _p(vo), _p(i), _p(to), _p(n), _p(uv), _p(m), len(vo) - 1,
+                                               _p(im), _p(mm), _p(ov), len(im), _p(mats), len(mats), _p(cam))
+            self.ordinals = L.ref_pt_triangles(self.h, None, None, None)
+        else:
+            self.h = L.ref_pt_create(_p(v), len(v), _p(i), _p(n), _p(uv), _p(m), len(m), _p(mats), len(mats),
+                                     _p(cam))
+            self.ordinals = len(m)
+
+    def triangles(self):
+        """scene_triangle_view per scene ordinal: world corners (T, 9), world
+        corner normals (T, 9), material after the instance override (T,)."""
+        t = self.ordinals
+        out = {"corners": np.zeros((t, 9), np.float32), "normals": np.zeros((t, 9), np.float32),
+               "material": np.zeros(t, np.int32)}
+        _ref_pt_lib().ref_pt_triangles(self.h, _p(out["corners"]), _p(out["normals"]), _p(out["material"]))
+        return out
 
     def __del__(self):
         try:
@@ -320,7 +350,7 @@ class RefPathTracer:
     def lights(self):
         L = _ref_pt_lib()
         n = L.ref_pt_lights(self.h, None, None, None, None, None, None, None, None)
-        t = len(self._keep[4])
+        t = self.ordinals
         out = {"prim": np.zeros(n, np.int32), "area": np.zeros(n, np.float32), "cdf": np.zeros(n, np.float32),
                "prob": np.zeros(n, np.float32), "alias": np.zeros(n, np.int32), "pmf": np.zeros(n, np.float32),
                "primitive_light_index": np.zeros(t, np.int32), "total_weight": np.zeros(1, np.float32)}

@@ -232,6 +232,84 @@ ref_pt* ref_pt_create(const float* vertices, int32_t nv, const uint32_t* indices
     return r;
 }
 
+// Several meshes placed by affine instances (add_scene_mesh per mesh,
+// add_scene_instance(make_transform(M), override) per instance): mesh m owns
+// vertices [vert_off[m], vert_off[m+1]) and triangles [tri_off[m], tri_off[m+1])
+// with mesh-local indices; normals / uvs per vertex (nullable).
+ref_pt* ref_pt_create_instanced(const float* vertices, const int32_t* vert_off, const uint32_t* indices,
+                                const int32_t* tri_off, const float* normals, const float* uvs,
+                                const int32_t* material_ids, int32_t mesh_count, const int32_t* inst_mesh,
+                                const float* inst_m, const int32_t* inst_override, int32_t instance_count,
+                                const float* materials, int32_t nm, const float* camera) {
+    Scene scene;
+    for (int m = 0; m < nm; ++m) {
+        const float* p = materials + 11 * m;
+        Material mat(static_cast<MaterialType>(int(p[0])), vec3(p[1], p[2], p[3]), vec3(p[4], p[5], p[6]), p[8],
+                     p[9], p[10],
+                     p[7] != 0 ? EmissionSidedness::DoubleSided : EmissionSidedness::SingleSided);
+        scene.materials.push_back(mat);
+    }
+    for (int m = 0; m < mesh_count; ++m) {
+        SceneMesh mesh;
+        const int v0 = vert_off[m], nv = vert_off[m + 1] - vert_off[m];
+        const int t0 = tri_off[m], nt = tri_off[m + 1] - tri_off[m];
+        mesh.vertices.resize(nv);
+        mesh.normals.assign(nv, vec3(0));
+        mesh.uvs.assign(nv, vec2(0));
+        for (int i = 0; i < nv; ++i) {
+            const int g = v0 + i;
+            mesh.vertices[i] = vec3(vertices[3 * g], vertices[3 * g + 1], vertices[3 * g + 2]);
+            if (normals) mesh.normals[i] = vec3(normals[3 * g], normals[3 * g + 1], normals[3 * g + 2]);
+            if (uvs) mesh.uvs[i] = vec2(uvs[2 * g], uvs[2 * g + 1]);
+        }
+        mesh.indices.resize(nt);
+        mesh.material_ids.resize(nt);
+        mesh.boundary_edges.assign(nt, 0);
+        for (int t = 0; t < nt; ++t) {
+            mesh.indices[t] =
+                TriangleIndices{indices[3 * (t0 + t)], indices[3 * (t0 + t) + 1], indices[3 * (t0 + t) + 2]};
+            mesh.material_ids[t] = material_ids[t0 + t];
+        }
+        add_scene_mesh(scene, std::move(mesh));
+    }
+    for (int i = 0; i < instance_count; ++i) {
+        mat4 M;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) M.m[r][c] = inst_m[16 * i + 4 * r + c];
+        add_scene_instance(scene, inst_mesh[i], make_transform(M), inst_override[i]);
+    }
+    scene.camera = make_pinhole_camera(vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]),
+                                       camera[6], camera[7]);
+    ref_pt* r = new ref_pt();
+    r->prepared = std::make_unique<PreparedScene>(std::move(scene));
+    r->prepared->prepare_host_bvh();
+    return r;
+}
+
+// scene_triangle_view of every scene ordinal: world corners (9 floats), world
+// corner normals (9 floats) and the material after the override. Returns the
+// ordinal count (call with null arrays first).
+int32_t ref_pt_triangles(const ref_pt* r, float* corners, float* normals, int32_t* material) {
+    const Scene& scene = r->prepared->host_scene();
+    const int n = scene_triangle_count(scene);
+    if (!corners) return n;
+    for (int t = 0; t < n; ++t) {
+        const SceneTriangle st = scene_triangle_view(scene, t);
+        const vec3 c[3] = {st.triangle.v0, st.triangle.v1, st.triangle.v2};
+        const vec3 nn[3] = {st.n0, st.n1, st.n2};
+        for (int k = 0; k < 3; ++k) {
+            corners[9 * t + 3 * k] = c[k].x;
+            corners[9 * t + 3 * k + 1] = c[k].y;
+            corners[9 * t + 3 * k + 2] = c[k].z;
+            normals[9 * t + 3 * k] = nn[k].x;
+            normals[9 * t + 3 * k + 1] = nn[k].y;
+            normals[9 * t + 3 * k + 2] = nn[k].z;
+        }
+        material[t] = st.material_id;
+    }
+    return n;
+}
+
 void ref_pt_free(ref_pt* r) { delete r; }
 
 // {origin, lower_left, horizontal, vertical, lens_u, lens_v} xyz, lens_radius, focal_distance

@@ -16,7 +16,16 @@ Cases:
                   view sits behind the pass-through panel (the bounce limit
                   counts pass-through hits)
   pt_mixed_d1     the same scene at depth 1
+  pt_instanced    an instanced scene (scenes.instanced_room: a mesh placed by a
+                  uniform, a non-uniform and a mirroring instance, material
+                  overrides, an instanced emitter, a transparent sheet)
+                  40x40, 8 spp, depth 6, seed 11: film + 2048 samples + the
+                  world-space triangle views (scene_triangle_view per ordinal)
+  pt_instanced_opaque  the same without the transparent sheet (every hit
+                  blocks a shadow ray), 40x40, 8 spp, depth 5, seed 12
 Each stores the scene arrays, the camera and light tables the reference derived.
+
+    python tests/golden/make_golden_pt.py instanced   # only the instanced cases
 """
 
 import sys
@@ -36,7 +45,12 @@ OUT = Path(__file__).resolve().parent
 def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
     r = oracle.RefPathTracer(scene)
     film = r.render(w, h, spp, depth, seed=seed, frame=frame)
-    data = {k: scene[k] for k in ("vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera")}
+    keys = ["vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera"]
+    if "inst_mesh" in scene:
+        keys += ["vertex_offsets", "triangle_offsets", "inst_mesh", "inst_m", "inst_override"]
+    data = {k: scene[k] for k in keys}
+    if "inst_mesh" in scene:
+        data.update({"tri_" + k: v for k, v in r.triangles().items()})
     data.update(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
                 seed=np.uint64(seed), frame=np.int32(frame), film_sum=film, camera_vectors=r.camera())
     for k, v in r.lights().items():
@@ -50,9 +64,20 @@ def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
     print(f"{name}: {w}x{h} spp={spp} depth={depth} tris={len(scene['indices'])} mean={film.mean() / spp:.5f}")
 
 
+def instanced_cases():
+    rng = np.random.default_rng(8)
+    px = rng.integers(0, 40 * 40, 2048)
+    sm = rng.integers(0, 8, 2048)
+    case("pt_instanced", scenes.instanced_room(40, 40), 40, 40, 8, 6, 11, (px, sm))
+    case("pt_instanced_opaque", scenes.instanced_room(40, 40, pass_through=False), 40, 40, 8, 5, 12, (px, sm))
+
+
 def main(only=None):
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libluxref.so missing: run `make -f oracle/ref.mk` where the reference exists")
+    if only == "instanced":
+        instanced_cases()
+        return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
     sm = np.tile(np.arange(4), w * h)
@@ -70,7 +95,8 @@ def main(only=None):
     case("pt_mixed", mixed, 40, 40, 8, 6, 3, (px, sm), frame=2)
     case("pt_mixed_d0", mixed, 40, 40, 2, 0, 3)
     case("pt_mixed_d1", mixed, 40, 40, 2, 1, 3)
+    instanced_cases()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)
