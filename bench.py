@@ -51,8 +51,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
-    ap.add_argument("--render", default="c3,c4,c5",
-                    help="render workloads to add to the line: any of c3,c4,c5 (comma-separated) or none")
+    ap.add_argument("--render", default="c3,instanced,c4,c5",
+                    help="render workloads to add to the line: any of c3,instanced,c4,c5 (comma-separated) or none")
     ap.add_argument("--two-level", default="on", choices=["on", "off"],
                     help="add the instanced (two-level TLAS/BLAS) workload to the line")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule baselines of c3/c4")
@@ -157,6 +157,80 @@ def bench_c3(args):
     if args.cpu_render_rows > 0:
         out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], args.cpu_render_rows)
     sc.close()
+    return out
+
+
+def bench_instanced(args):
+    """Instanced-scene render: 64 instances of 4 displaced icospheres (81 920
+    triangles each, 5 242 880 instanced triangles) over a ground plane,
+    1024x1024, 16 spp, depth 8, seed 0; the same geometry flattened into one
+    mesh (the instanced scene's own world-space shading view) rendered through
+    the one-level path beside it. CPU baseline: the reference's trace_path on a
+    band of rows, one thread."""
+    from lux_amd import lbvh, render, scenes
+
+    W = H = 1024
+    spp, depth = 16, 8
+    t0 = time.perf_counter()
+    d = scenes.instanced_field(W, H)
+    t1 = time.perf_counter()
+    sc = render.Scene(d)
+    t2 = time.perf_counter()
+    flat = scenes.flattened(sc.triangles(), d["materials"], d["camera"])
+    t3 = time.perf_counter()
+    fsc = render.Scene(flat)
+    t4 = time.perf_counter()
+    film = lbvh.DeviceArray(W * H * 12)
+    out = {"workload": (f"{len(d['inst_mesh']) - 1} instances of {len(d['triangle_offsets']) - 2} displaced icospheres "
+                        f"+ ground and light, {sc.triangle_count} instanced triangles, {W}x{H}, {spp} spp, "
+                        f"depth {depth}"),
+           "scene_generate_s": round(t1 - t0, 3), "instanced_upload_build_s": round(t2 - t1, 3),
+           "flat_upload_build_s": round(t4 - t3, 3),
+           "instanced_device_bytes": sc.info()[2], "flat_device_bytes": fsc.info()[2]}
+    films = {}
+    for name, s_ in (("instanced", sc), ("flattened", fsc)):
+        s_.render_device(film, W, H, 1, depth)
+        lbvh.synchronize()
+        best = None
+        for _ in range(3):
+            s0 = time.perf_counter()
+            s_.render_device(film, W, H, spp, depth)
+            lbvh.synchronize()
+            sec = time.perf_counter() - s0
+            best = sec if best is None else min(best, sec)
+        ext, sh = s_.rays()
+        films[name] = film.to_numpy(np.float32, (H, W, 3))
+        out[name] = {"render_s": round(best, 4), "Mpaths_per_s": round(W * H * spp / best / 1e6, 2),
+                     **_render_rate(ext, sh, W * H, best)}
+        _log(f"instanced render, {name}: {best:.3f} s")
+    out["instanced_vs_flattened_time"] = round(out["instanced"]["render_s"] / out["flattened"]["render_s"], 3)
+    out["instanced_vs_flattened_relrmse"] = round(render.rel_rmse(films["instanced"] / spp,
+                                                                  films["flattened"] / spp), 6)
+    if args.cpu_render_rows > 0:
+        import oracle
+
+        if oracle.ref_available():
+            nrows = max(1, args.cpu_render_rows // 4)
+            r0 = H // 2 - nrows // 2
+            paths = W * nrows * spp
+            ref = oracle.RefPathTracer(d)
+            pix = np.repeat(np.arange(r0 * W, (r0 + nrows) * W, dtype=np.int32), spp)
+            smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
+            s0 = time.perf_counter()
+            rad = ref.samples(W, H, depth, pix, smp)
+            t_ref = time.perf_counter() - s0
+            band = np.zeros((nrows * W, 3), np.float32)
+            for k in range(spp):
+                band += rad[k::spp]
+            gpu_band = films["instanced"][r0:r0 + nrows].reshape(-1, 3)
+            out["cpu_baseline"] = {
+                "value": round(paths / t_ref / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "reference",
+                "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+                "host_cpu": _cpu_name(),
+                "gpu_vs_reference_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6)}
+    sc.close()
+    fsc.close()
+    film.free()
     return out
 
 
@@ -584,6 +658,8 @@ def main():
         out["render"] = {}
     if "c3" in wanted:
         out["render"]["c3_single_scene"] = bench_c3(args)
+    if "instanced" in wanted:
+        out["render"]["instanced_scene"] = bench_instanced(args)
     if "c4" in wanted or "c5" in wanted:
         c4, scs, jobs, dicts = bench_c4(args)
         if "c4" in wanted:

@@ -502,6 +502,63 @@ int lux_hip_scene_set_instances(lux_hip_scene* scene, const lux_instance* instan
  * (lux_hip_scene_info's triangle_count); any may be NULL. Synchronous. */
 int lux_hip_scene_triangles(lux_hip_scene* scene, float* corners, float* normals, int32_t* material_ids);
 
+/* ---- OBJ import ----------------------------------------------------------
+ * load_obj_into_scene (src/util/obj_loader.cpp:613-1311, contract
+ * README.md:180-226) for the path tracer's instanced scenes: the file parsed
+ * as the bundled tinyobjloader parses it with triangulation (quads along the
+ * shorter diagonal, larger polygons by ear clipping), faces with invalid or
+ * non-finite vertices and degenerate faces (area <= 1e-12) skipped, one vertex
+ * per distinct (position, used normal, used texcoord) in first-use order,
+ * normals per normal_mode, material ids from the usemtl bindings or the
+ * default material, the boundary-edge mask per triangle, and the instance
+ * transform object_to_world * placement (make_obj_placement_transform). The
+ * mesh is object space; it enters a scene through lux_instanced_scene_desc
+ * with that transform. Boundary-hole capping is not built. Host code: needs no
+ * GPU and no lux_hip_init. */
+enum { LUX_OBJ_NORMALS_SOURCE_OR_GEOMETRY = 0, LUX_OBJ_NORMALS_GENERATE_MISSING_SMOOTH = 1,
+       LUX_OBJ_NORMALS_FACE_ONLY = 2 };
+enum { LUX_OBJ_SCALE_NONE = 0, LUX_OBJ_SCALE_MAX_EXTENT = 1, LUX_OBJ_SCALE_X_EXTENT = 2 };
+enum { LUX_OBJ_TOPOLOGY_SOURCE_VERTICES = 0, LUX_OBJ_TOPOLOGY_WELDED_POSITIONS = 1 };
+/* ObjLoadOptions (src/util/obj_loader.h:52-67); lux_obj_default_options fills
+ * the reference's defaults. */
+typedef struct lux_obj_options {
+    int32_t normal_mode;           /* LUX_OBJ_NORMALS_* */
+    int32_t reverse_orientation;
+    int32_t flip_texcoord_v;
+    int32_t boundary_topology;     /* LUX_OBJ_TOPOLOGY_* */
+    float position_weld_epsilon;
+    int32_t default_material_id;   /* -1: a Lambert(0.8) appended after the scene's materials, if no bindings */
+    int32_t scene_material_count;  /* materials already in the scene (the ids bindings may name) */
+    int32_t placement_scale_mode;  /* LUX_OBJ_SCALE_* */
+    float placement_target_extent;
+    int32_t placement_center_xz;
+    int32_t placement_ground_to_y;
+    float placement_target_y;
+    float object_to_world[16];     /* Transform applied after the placement (m, row-major) */
+    float world_to_object[16];     /* its inverse */
+    int32_t binding_count;         /* usemtl name -> scene material id */
+    const char* const* binding_names;
+    const int32_t* binding_material_ids;
+} lux_obj_options;
+typedef struct lux_obj_mesh lux_obj_mesh;
+
+int lux_obj_default_options(lux_obj_options* options);
+/* LUX_ERR_INVALID_ARGUMENT for what the reference loader reports as a failure
+ * (unreadable file, a parse error, a usemtl without binding, bad options). A
+ * file whose faces are all skipped loads as a mesh of 0 triangles. */
+int lux_obj_load(const char* path, const lux_obj_options* options, lux_obj_mesh** out);
+int lux_obj_mesh_free(lux_obj_mesh* mesh);
+/* Counts: vertices, triangles, 1 if the default material was appended; skipped
+ * faces (invalid index, non-finite vertex, degenerate). Any pointer may be NULL. */
+int lux_obj_mesh_counts(const lux_obj_mesh* mesh, int32_t* vertex_count, int32_t* triangle_count,
+                        int32_t* appended_default_material, int32_t* skipped_invalid, int32_t* skipped_nonfinite,
+                        int32_t* skipped_degenerate);
+/* The mesh arrays (sized by the counts; any may be NULL) and the instance
+ * transform (16 + 16 floats, row-major). */
+int lux_obj_mesh_arrays(const lux_obj_mesh* mesh, lux_vec3* vertices, lux_vec3* normals, float* uvs,
+                        uint32_t* indices, int32_t* material_ids, uint8_t* boundary_edges, float* object_to_world,
+                        float* world_to_object);
+
 /* ---- film output ---------------------------------------------------------
  * The files of write_ppm (src/film/ppm.h:37-56: 1/spp, Reinhard c/(1+c),
  * clamp to [0,1], 255.999 * c^(1/2.2) truncated, pixels in film order) and

@@ -983,6 +983,10 @@ struct lux_hip_server {
     std::vector<uint64_t> ready;  // completed ids not yet returned by poll
 };
 
+struct lux_obj_mesh {
+    lux::obj::Mesh mesh;
+};
+
 struct lux_hip_scene {
     lux::pt::SceneDev dev;
     std::unique_ptr<lux::pt::Renderer> renderer;  // path buffers of this scene's single renders
@@ -1149,6 +1153,67 @@ int lux_hip_scene_triangles(lux_hip_scene* sc, float* corners, float* normals, i
                     for (int k = 0; k < 3; ++k) normals[3 * c + k] = vn[3 * size_t(ix[c]) + k];
             }
         }
+        return int(LUX_OK);
+    });
+}
+
+int lux_obj_default_options(lux_obj_options* o) {
+    return guarded([&] {
+        if (!o) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        lux::obj::default_options(*o);
+        return int(LUX_OK);
+    });
+}
+
+int lux_obj_load(const char* path, const lux_obj_options* o, lux_obj_mesh** out) {
+    return guarded([&] {
+        if (!path || !o || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto m = std::make_unique<lux_obj_mesh>();
+        lux::obj::load(path, *o, m->mesh);
+        *out = m.release();
+        return int(LUX_OK);
+    });
+}
+
+int lux_obj_mesh_free(lux_obj_mesh* m) {
+    delete m;
+    return int(LUX_OK);
+}
+
+int lux_obj_mesh_counts(const lux_obj_mesh* m, int32_t* vertex_count, int32_t* triangle_count,
+                        int32_t* appended_default_material, int32_t* skipped_invalid, int32_t* skipped_nonfinite,
+                        int32_t* skipped_degenerate) {
+    return guarded([&] {
+        if (!m) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const lux::obj::Mesh& x = m->mesh;
+        if (vertex_count) *vertex_count = int32_t(x.vertices.size() / 3);
+        if (triangle_count) *triangle_count = int32_t(x.indices.size() / 3);
+        if (appended_default_material) *appended_default_material = x.appended_default_material;
+        if (skipped_invalid) *skipped_invalid = x.skipped_invalid;
+        if (skipped_nonfinite) *skipped_nonfinite = x.skipped_nonfinite;
+        if (skipped_degenerate) *skipped_degenerate = x.skipped_degenerate;
+        return int(LUX_OK);
+    });
+}
+
+int lux_obj_mesh_arrays(const lux_obj_mesh* m, lux_vec3* vertices, lux_vec3* normals, float* uvs, uint32_t* indices,
+                        int32_t* material_ids, uint8_t* boundary_edges, float* object_to_world,
+                        float* world_to_object) {
+    return guarded([&] {
+        if (!m) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const lux::obj::Mesh& x = m->mesh;
+        auto put = [](void* dst, const auto& v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+        };
+        put(vertices, x.vertices);
+        put(normals, x.normals);
+        put(uvs, x.uvs);
+        put(indices, x.indices);
+        put(material_ids, x.material_ids);
+        put(boundary_edges, x.boundary_edges);
+        if (object_to_world) std::memcpy(object_to_world, x.object_to_world, sizeof(x.object_to_world));
+        if (world_to_object) std::memcpy(world_to_object, x.world_to_object, sizeof(x.world_to_object));
         return int(LUX_OK);
     });
 }

@@ -147,6 +147,23 @@ void film_ppm(const float* film, int width, int height, int spp, uint8_t* out, h
 void film_pfm(const float* film, int width, int height, int spp, float* out, hipStream_t s);
 void write_film_file(const char* path, bool ppm, const float* d_film, int width, int height, int spp, hipStream_t s);
 void ppm_thresholds(float* out255);
+
+// ---- OBJ import (obj_import.hip) ----
+namespace obj {
+// The mesh load_obj_into_scene appends (object space) and its instance transform.
+struct Mesh {
+    std::vector<float> vertices, normals, uvs;  // 3, 3, 2 per vertex
+    std::vector<uint32_t> indices;              // 3 per triangle
+    std::vector<int32_t> material_ids;
+    std::vector<uint8_t> boundary_edges;        // bit k: edge (k, k+1 mod 3) is on the boundary
+    float object_to_world[16], world_to_object[16];
+    int appended_default_material = 0;
+    int skipped_invalid = 0, skipped_nonfinite = 0, skipped_degenerate = 0, nonfinite_vertices = 0;
+    int invalid_normal_refs = 0, nonfinite_normal_refs = 0, zero_normal_refs = 0;
+};
+void load(const std::string& path, const lux_obj_options& options, Mesh& out);
+void default_options(lux_obj_options& options);
+}  // namespace obj
 void wide_refit(BvhDev& d, hipStream_t s);
 // Enqueue a refit with new boxes (input order) or triangles (input order).
 void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);

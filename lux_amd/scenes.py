@@ -230,6 +230,82 @@ def instanced_room(width=40, height=40, pass_through=True):
     }
 
 
+def instanced_field(width=1024, height=1024, n_meshes=4, freq=64, grid=8, seed=0x4C5560):
+    """A field of grid x grid instanced displaced icospheres over a ground plane
+    (an instanced scene dict, see instanced_room): n_meshes distinct spheres
+    (freq -> 20 freq^2 triangles each, smooth normals, the 48-material
+    Lambert / GGX / dielectric palette of displaced_icosphere), each instance a
+    random rotation, uniform scale U(0.35, 0.6) and position on the grid, every
+    fourth with a palette material override; mesh 0 holds the 12 x 12 ground
+    plane and a 4 x 4 ceiling light at y = 6 (emission (6, 6, 5.5)) under an
+    identity instance. Camera (0, 5, 9) -> (0, -0.5, 0), vertical fov 40."""
+    from .workloads import affine
+
+    spheres = [displaced_icosphere(freq=freq, seed=seed + k, width=width, height=height) for k in range(n_meshes)]
+    palette = spheres[0]["materials"][:-1]  # drop the sphere scene's own light material
+    ground, light = len(palette), len(palette) + 1
+    materials = np.concatenate([palette, np.asarray([material(LAMBERT, (0.7, 0.7, 0.68)),
+                                                     material(LAMBERT, (0, 0, 0), emission=(6.0, 6.0, 5.5))],
+                                                    np.float32)])
+    g0 = np.array([[-6, -1.05, -6], [6, -1.05, -6], [6, -1.05, 6], [-6, -1.05, 6]], np.float32)
+    l0 = np.array([[-2, 6, -2], [2, 6, -2], [2, 6, 2], [-2, 6, 2]], np.float32)
+    mesh0_v = np.concatenate([g0, l0])
+    mesh0_i = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7]], np.uint32)  # ground up, light down
+    verts, normals, indices, mat_ids = [mesh0_v], [np.zeros_like(mesh0_v)], [mesh0_i], [[ground, ground, light, light]]
+    for sp in spheres:  # each sphere mesh without its light quad
+        nv = len(sp["vertices"]) - 4
+        verts.append(sp["vertices"][:nv])
+        normals.append(sp["normals"][:nv])
+        indices.append(sp["indices"][:-2])
+        mat_ids.append(sp["material_ids"][:-2])
+    vo = np.cumsum([0] + [len(v) for v in verts]).astype(np.int32)
+    to = np.cumsum([0] + [len(i) for i in indices]).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    inst_mesh, inst_m, inst_ov = [0], [affine()], [-1]
+    step = 10.0 / grid
+    for gz in range(grid):
+        for gx in range(grid):
+            k = len(inst_mesh) - 1
+            sc = rng.uniform(0.35, 0.6)
+            pos = (-5 + step * (gx + 0.5) + rng.uniform(-0.1, 0.1), -1.05 + sc, -5 + step * (gz + 0.5))
+            inst_mesh.append(1 + int(rng.integers(0, n_meshes)))
+            inst_m.append(affine(rng.standard_normal(4), (sc, sc, sc), pos))
+            inst_ov.append(int(rng.integers(0, len(palette))) if k % 4 == 3 else -1)
+    aspect = np.float32(width) / np.float32(height)
+    v = np.concatenate(verts)
+    return {
+        "vertices": np.ascontiguousarray(v, np.float32) + np.float32(0.0),
+        "indices": np.ascontiguousarray(np.concatenate(indices), np.uint32),
+        "normals": np.ascontiguousarray(np.concatenate(normals), np.float32) + np.float32(0.0),
+        "uvs": np.zeros((len(v), 2), np.float32),
+        "material_ids": np.concatenate(mat_ids).astype(np.int32),
+        "materials": materials.astype(np.float32),
+        "camera": np.asarray([0, 5, 9, 0, -0.5, 0, 40.0, aspect], np.float32),
+        "vertex_offsets": vo,
+        "triangle_offsets": to,
+        "inst_mesh": np.asarray(inst_mesh, np.int32),
+        "inst_m": np.stack(inst_m).astype(np.float32),
+        "inst_override": np.asarray(inst_ov, np.int32),
+    }
+
+
+def flattened(view, materials, camera):
+    """A one-mesh scene dict holding an instanced scene's shading view (world
+    corners, world corner normals, materials after the override: the
+    renderer's Scene.triangles()), for comparing one-level and two-level
+    traversal on the same geometry."""
+    t = len(view["material"])
+    return {
+        "vertices": np.ascontiguousarray(view["corners"].reshape(-1, 3), np.float32),
+        "indices": np.arange(3 * t, dtype=np.uint32).reshape(t, 3),
+        "normals": np.ascontiguousarray(view["normals"].reshape(-1, 3), np.float32),
+        "uvs": np.zeros((3 * t, 2), np.float32),
+        "material_ids": np.ascontiguousarray(view["material"], np.int32),
+        "materials": np.asarray(materials, np.float32),
+        "camera": np.asarray(camera, np.float32),
+    }
+
+
 # ---- seeded displaced icosphere -----------------------------------------------
 
 def _icosahedron():

@@ -372,6 +372,48 @@ class RefPathTracer:
         return out
 
 
+# ---- the reference's own OBJ loader (oracle/_ref) --------------------------------
+def ref_load_obj(path, normal_mode=0, reverse_orientation=False, flip_texcoord_v=False, boundary_topology=0,
+                 position_weld_epsilon=1e-6, default_material_id=-1, scene_material_count=0, scale_mode=0,
+                 target_extent=1.0, center_xz=False, ground_to_y=False, target_y=0.0, object_to_world=None,
+                 material_bindings=None):
+    """load_obj_into_scene_with_result into a scene holding
+    `scene_material_count` materials: ok flag, the appended mesh arrays and
+    the created instance's transform (m, inv)."""
+    L = ref()
+    if not hasattr(L, "_obj_ready"):
+        L.ref_obj_load.restype = c_void_p
+        L.ref_obj_load.argtypes = [ctypes.c_char_p, c_int32, c_int32, c_int32, c_int32, ctypes.c_float, c_int32,
+                                   c_int32, c_int32, ctypes.c_float, c_int32, c_int32, ctypes.c_float, c_void_p,
+                                   c_int32, c_void_p, c_void_p]
+        L.ref_obj_free.argtypes = [c_void_p]
+        L.ref_obj_counts.argtypes = [c_void_p, c_void_p]
+        L.ref_obj_arrays.argtypes = [c_void_p] * 9
+        L._obj_ready = True
+    m = None if object_to_world is None else np.ascontiguousarray(object_to_world, np.float32).reshape(16)
+    b = material_bindings or {}
+    names = (ctypes.c_char_p * max(len(b), 1))(*[k.encode() for k in b])
+    ids = np.ascontiguousarray(list(b.values()) or [0], np.int32)
+    h = L.ref_obj_load(str(path).encode(), normal_mode, int(reverse_orientation), int(flip_texcoord_v),
+                       boundary_topology, position_weld_epsilon, default_material_id, scene_material_count,
+                       scale_mode, target_extent, int(center_xz), int(ground_to_y), target_y,
+                       None if m is None else _p(m), len(b), ctypes.cast(names, c_void_p), _p(ids))
+    try:
+        c = np.zeros(7, np.int32)
+        L.ref_obj_counts(h, _p(c))
+        nv, nt = int(c[1]), int(c[2])
+        out = {"vertices": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
+               "uvs": np.zeros((nv, 2), np.float32), "indices": np.zeros((nt, 3), np.uint32),
+               "material_ids": np.zeros(nt, np.int32), "boundary_edges": np.zeros(nt, np.uint8),
+               "object_to_world": np.zeros((4, 4), np.float32), "world_to_object": np.zeros((4, 4), np.float32)}
+        L.ref_obj_arrays(h, *[_p(v) for v in out.values()])
+        out.update(ok=bool(c[0]), appended_default_material=int(c[3]), skipped_invalid=int(c[4]),
+                   skipped_nonfinite=int(c[5]), skipped_degenerate=int(c[6]))
+        return out
+    finally:
+        L.ref_obj_free(h)
+
+
 # ---- two-level restatement (two_level_oracle.cpp) ------------------------------
 def _tl_lib():
     L = lib()

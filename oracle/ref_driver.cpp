@@ -533,3 +533,91 @@ void ref_write_pfm(const char* path, const float* film, int32_t w, int32_t h, in
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// OBJ import: the reference's own load_obj_into_scene_with_result
+// (src/util/obj_loader.cpp, bundled tinyobjloader) into a scene that already
+// holds `scene_material_count` materials; the appended mesh and its instance
+// transform are read back.
+// ---------------------------------------------------------------------------
+#include "util/obj_loader.h"
+
+extern "C" {
+
+struct ref_obj {
+    Scene scene;
+    ObjLoadResult result;
+    bool ok = false;
+};
+
+ref_obj* ref_obj_load(const char* path, int32_t normal_mode, int32_t reverse, int32_t flip_v,
+                      int32_t boundary_topology, float weld_epsilon, int32_t default_material_id,
+                      int32_t scene_material_count, int32_t scale_mode, float target_extent, int32_t center_xz,
+                      int32_t ground_to_y, float target_y, const float* object_to_world, int32_t binding_count,
+                      const char* const* binding_names, const int32_t* binding_ids) {
+    ref_obj* r = new ref_obj();
+    for (int m = 0; m < scene_material_count; ++m) r->scene.materials.push_back(Material::lambert(vec3(0.5f)));
+    ObjLoadOptions o;
+    o.normal_mode = static_cast<ObjNormalMode>(normal_mode);
+    o.reverse_orientation = reverse != 0;
+    o.flip_texcoord_v = flip_v != 0;
+    o.boundary_topology = static_cast<ObjBoundaryTopology>(boundary_topology);
+    o.position_weld_epsilon = weld_epsilon;
+    o.default_material_id = default_material_id;
+    o.placement.scale_mode = static_cast<ObjPlacementScaleMode>(scale_mode);
+    o.placement.target_extent = target_extent;
+    o.placement.center_xz = center_xz != 0;
+    o.placement.ground_to_y = ground_to_y != 0;
+    o.placement.target_y = target_y;
+    if (object_to_world) {
+        mat4 M;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) M.m[a][b] = object_to_world[4 * a + b];
+        o.object_to_world = make_transform(M);
+    }
+    for (int b = 0; b < binding_count; ++b) o.material_bindings[binding_names[b]] = binding_ids[b];
+    std::fflush(stderr);
+    r->ok = load_obj_into_scene_with_result(path, r->scene, o, r->result);
+    return r;
+}
+
+void ref_obj_free(ref_obj* r) { delete r; }
+
+// ok, vertices, triangles, appended materials, skipped (invalid, non-finite, degenerate)
+void ref_obj_counts(const ref_obj* r, int32_t* out7) {
+    const ObjLoadResult& x = r->result;
+    out7[0] = r->ok ? 1 : 0;
+    out7[1] = x.mesh_id >= 0 ? int32_t(r->scene.mesh_assets[x.mesh_id].vertices.size()) : 0;
+    out7[2] = x.mesh_id >= 0 ? int32_t(r->scene.mesh_assets[x.mesh_id].indices.size()) : 0;
+    out7[3] = x.material_count;
+    out7[4] = x.skipped_invalid_triangle_count;
+    out7[5] = x.skipped_nonfinite_triangle_count;
+    out7[6] = x.skipped_degenerate_triangle_count;
+}
+
+void ref_obj_arrays(const ref_obj* r, float* vertices, float* normals, float* uvs, uint32_t* indices,
+                    int32_t* material_ids, uint8_t* boundary_edges, float* m16, float* inv16) {
+    const ObjLoadResult& x = r->result;
+    if (x.mesh_id < 0) return;
+    const SceneMesh& mesh = r->scene.mesh_assets[x.mesh_id];
+    for (size_t i = 0; i < mesh.vertices.size(); ++i) {
+        std::memcpy(vertices + 3 * i, &mesh.vertices[i], 12);
+        std::memcpy(normals + 3 * i, &mesh.normals[i], 12);
+        std::memcpy(uvs + 2 * i, &mesh.uvs[i], 8);
+    }
+    for (size_t t = 0; t < mesh.indices.size(); ++t) {
+        indices[3 * t] = mesh.indices[t].v0;
+        indices[3 * t + 1] = mesh.indices[t].v1;
+        indices[3 * t + 2] = mesh.indices[t].v2;
+        material_ids[t] = mesh.material_ids[t];
+        boundary_edges[t] = mesh.boundary_edges[t];
+    }
+    const Transform& tr = x.instance_id >= 0 ? r->scene.instances[x.instance_id].object_to_world : Transform{};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            m16[4 * a + b] = tr.m.m[a][b];
+            inv16[4 * a + b] = tr.inv.m[a][b];
+        }
+}
+
+}  // extern "C"
