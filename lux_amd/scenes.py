@@ -289,6 +289,48 @@ def instanced_field(width=1024, height=1024, n_meshes=4, freq=64, grid=8, seed=0
     }
 
 
+def bunny(obj_path, width=512, height=512, loader=None):
+    """The reference's "bunny" scene (make_stanford_bunny_scene,
+    src/scene/stanford_bunny.h:15-56, camera of make_bunny_scene_from_config,
+    src/app/render_runner.cpp:148-154) as an instanced scene dict: mesh 0 holds
+    the grey floor quad at y = 0 and the ceiling light quad at y = 0.5 under an
+    identity instance; mesh 1 is the OBJ imported with default options and
+    ground_to_y (its instance grounds it at y = 0), shaded with the default
+    material turned ThinLambert(0.8). `loader(path, **options)` returns the
+    imported mesh dict (lux_amd.obj.load by default)."""
+    if loader is None:
+        from .obj import load as loader
+    m = loader(obj_path, scene_material_count=2, ground_to_y=True)
+    materials = np.asarray([material(LAMBERT, (0.5, 0.5, 0.5)),
+                            material(LAMBERT, (0, 0, 0), emission=(15.0, 14.0, 12.0)),
+                            material(THIN_LAMBERT, (0.8, 0.8, 0.8))], np.float32)
+    f = np.float32
+    quads = [[(-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1)],
+             [(f(-0.3), f(0.5), f(-0.3)), (f(0.3), f(0.5), f(-0.3)), (f(0.3), f(0.5), f(0.3)),
+              (f(-0.3), f(0.5), f(0.3))]]
+    tris = []
+    for q in quads:  # add_quad: (p0, p1, p2), (p0, p2, p3)
+        tris += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+    mesh0_v = np.asarray(tris, np.float32).reshape(-1, 3)
+    mesh0_i = np.arange(len(mesh0_v), dtype=np.uint32).reshape(-1, 3)
+    verts = np.concatenate([mesh0_v, m["vertices"]])
+    aspect = np.float32(width) / np.float32(height)
+    return {
+        "vertices": np.ascontiguousarray(verts, np.float32),
+        "indices": np.ascontiguousarray(np.concatenate([mesh0_i, m["indices"]]), np.uint32),
+        "normals": np.ascontiguousarray(np.concatenate([np.zeros_like(mesh0_v), m["normals"]]), np.float32),
+        "uvs": np.ascontiguousarray(np.concatenate([np.zeros((len(mesh0_v), 2), np.float32), m["uvs"]])),
+        "material_ids": np.concatenate([[0, 0, 1, 1], m["material_ids"]]).astype(np.int32),
+        "materials": materials,
+        "camera": np.asarray([0, f(0.3), f(0.5), f(-0.02), f(0.11), 0, 40.0, aspect], np.float32),
+        "vertex_offsets": np.array([0, len(mesh0_v), len(verts)], np.int32),
+        "triangle_offsets": np.array([0, len(mesh0_i), len(mesh0_i) + len(m["indices"])], np.int32),
+        "inst_mesh": np.array([0, 1], np.int32),
+        "inst_m": np.stack([np.eye(4, dtype=np.float32), m["object_to_world"]]).astype(np.float32),
+        "inst_override": np.array([-1, -1], np.int32),
+    }
+
+
 def flattened(view, materials, camera):
     """A one-mesh scene dict holding an instanced scene's shading view (world
     corners, world corner normals, materials after the override: the

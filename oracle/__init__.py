@@ -285,6 +285,8 @@ def _ref_pt_lib():
         L.ref_pt_create_instanced.restype = c_void_p
         L.ref_pt_create_instanced.argtypes = [c_void_p] * 7 + [c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                                                c_void_p, c_int32, c_void_p]
+        L.ref_pt_create_bunny.restype = c_void_p
+        L.ref_pt_create_bunny.argtypes = [ctypes.c_char_p, ctypes.c_float]
         L.ref_pt_triangles.restype = c_int32
         L.ref_pt_triangles.argtypes = [c_void_p] * 4
         L._pt_ready = True
@@ -326,6 +328,17 @@ class RefPathTracer:
             self.h = L.ref_pt_create(_p(v), len(v), _p(i), _p(n), _p(uv), _p(m), len(m), _p(mats), len(mats),
                                      _p(cam))
             self.ordinals = len(m)
+
+    @classmethod
+    def bunny(cls, asset_root, width, height):
+        """The reference's own "bunny" scene (make_stanford_bunny_scene with the
+        CLI camera), loaded from `asset_root`/models/stanford-bunny.obj."""
+        L = _ref_pt_lib()
+        self = cls.__new__(cls)
+        self._keep = None
+        self.h = L.ref_pt_create_bunny(str(asset_root).encode(), float(np.float32(width) / np.float32(height)))
+        self.ordinals = L.ref_pt_triangles(self.h, None, None, None)
+        return self
 
     def triangles(self):
         """scene_triangle_view per scene ordinal: world corners (T, 9), world

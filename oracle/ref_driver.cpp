@@ -186,6 +186,7 @@ int64_t ref_overlap_cross_brute(const float* lboxes, int32_t nl, const float* rb
 #include "scene/scene_build.h"
 #include "scene/scene_edit.h"
 #include "scene/scene_light_sampler.h"
+#include "scene/stanford_bunny.h"
 
 extern "C" {
 
@@ -280,6 +281,17 @@ ref_pt* ref_pt_create_instanced(const float* vertices, const int32_t* vert_off, 
     }
     scene.camera = make_pinhole_camera(vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]),
                                        camera[6], camera[7]);
+    ref_pt* r = new ref_pt();
+    r->prepared = std::make_unique<PreparedScene>(std::move(scene));
+    r->prepared->prepare_host_bvh();
+    return r;
+}
+
+// The reference's own "bunny" scene: make_stanford_bunny_scene(asset_root)
+// with the camera of make_bunny_scene_from_config (src/app/render_runner.cpp:148-154).
+ref_pt* ref_pt_create_bunny(const char* asset_root, float aspect) {
+    Scene scene = make_stanford_bunny_scene(std::filesystem::path(asset_root));
+    scene.camera = make_pinhole_camera(vec3(0.0f, 0.3f, 0.5f), vec3(-0.02f, 0.11f, 0.0f), 40.0f, aspect);
     ref_pt* r = new ref_pt();
     r->prepared = std::make_unique<PreparedScene>(std::move(scene));
     r->prepared->prepare_host_bvh();

@@ -23,9 +23,13 @@ Cases:
                   world-space triangle views (scene_triangle_view per ordinal)
   pt_instanced_opaque  the same without the transparent sheet (every hit
                   blocks a shadow ray), 40x40, 8 spp, depth 5, seed 12
+  pt_bunny        the reference's own Stanford-bunny scene (OBJ import, a
+                  placed instance, a ThinLambert bunny), 64x64, 16 spp,
+                  depth 4, seed 2: film + 2048 samples
 Each stores the scene arrays, the camera and light tables the reference derived.
 
     python tests/golden/make_golden_pt.py instanced   # only the instanced cases
+    python tests/golden/make_golden_pt.py bunny       # only pt_bunny
 """
 
 import sys
@@ -64,6 +68,33 @@ def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
     print(f"{name}: {w}x{h} spp={spp} depth={depth} tris={len(scene['indices'])} mean={film.mean() / spp:.5f}")
 
 
+def bunny_case(w=64, h=64, spp=16, depth=4, seed=2):
+    """pt_bunny: the reference's own bunny scene (make_stanford_bunny_scene,
+    assets/models/stanford-bunny.obj) rendered by its CPU path tracer. The
+    scene itself is not stored: tests rebuild it from the OBJ fixture
+    (tests/golden/obj/stanford-bunny.obj.xz) through the OBJ import; a SHA-256
+    of the reference's world-space triangle view pins that rebuild."""
+    import hashlib
+
+    r = oracle.RefPathTracer.bunny("/root/reference/assets", w, h)
+    film = r.render(w, h, spp, depth, seed=seed)
+    rng = np.random.default_rng(9)
+    px = rng.integers(0, w * h, 2048)
+    sm = rng.integers(0, spp, 2048)
+    t = r.triangles()
+    digest = hashlib.sha256(b"".join(np.ascontiguousarray(t[k]).tobytes()
+                                     for k in ("corners", "normals", "material"))).hexdigest()
+    data = dict(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
+                seed=np.uint64(seed), frame=np.int32(0), film_sum=film, camera_vectors=r.camera(),
+                sample_pixels=px.astype(np.int32), sample_index=sm.astype(np.int32),
+                sample_radiance=r.samples(w, h, depth, px, sm, seed=seed),
+                triangle_view_sha256=np.array(digest), ordinals=np.int32(r.ordinals))
+    for k, v in r.lights().items():
+        data["light_" + k] = v
+    np.savez_compressed(OUT / "pt_bunny.npz", **data)
+    print(f"pt_bunny: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
+
+
 def instanced_cases():
     rng = np.random.default_rng(8)
     px = rng.integers(0, 40 * 40, 2048)
@@ -77,6 +108,9 @@ def main(only=None):
         raise SystemExit("oracle/_ref/libluxref.so missing: run `make -f oracle/ref.mk` where the reference exists")
     if only == "instanced":
         instanced_cases()
+        return
+    if only == "bunny":
+        bunny_case()
         return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
@@ -96,6 +130,7 @@ def main(only=None):
     case("pt_mixed_d0", mixed, 40, 40, 2, 0, 3)
     case("pt_mixed_d1", mixed, 40, 40, 2, 1, 3)
     instanced_cases()
+    bunny_case()
 
 
 if __name__ == "__main__":

@@ -139,3 +139,31 @@ def test_import_errors(tmp_path):
     assert len(m["indices"]) == 0 and m["skipped_degenerate"] == 1 and m["appended_default_material"] == 0
     with pytest.raises(abi.LuxError):  # a binding to a material the scene does not have
         obj.load(GOLDEN / "obj" / "shells.obj", scene_material_count=1, material_bindings={"a": 0, "b": 1})
+
+
+def bunny_dict(tmp_path, width=64, height=64):
+    from lux_amd import scenes
+
+    return scenes.bunny(obj_path("stanford-bunny.obj.xz", tmp_path), width, height)
+
+
+def test_bunny_scene_rebuild_matches_reference_scene(tmp_path):
+    """The bunny scene assembled from the imported mesh (scenes.bunny) has the
+    world-space triangle view, camera and light table of the reference's own
+    make_stanford_bunny_scene (hash and tables in tests/golden/pt_bunny.npz),
+    as the reference's scene code sees it."""
+    import hashlib
+
+    import oracle
+
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    g = load_golden("pt_bunny")
+    r = oracle.RefPathTracer(bunny_dict(tmp_path))
+    t = r.triangles()
+    digest = hashlib.sha256(b"".join(np.ascontiguousarray(t[k]).tobytes()
+                                     for k in ("corners", "normals", "material"))).hexdigest()
+    assert digest == str(g["triangle_view_sha256"])
+    assert np.array_equal(r.camera().view(np.uint32), g["camera_vectors"].view(np.uint32))
+    for k, v in r.lights().items():
+        assert np.array_equal(np.asarray(v).view(np.uint32), np.asarray(g["light_" + k]).view(np.uint32)), k
