@@ -51,8 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
-    ap.add_argument("--render", default="c3,instanced,c4,c5",
-                    help="render workloads to add to the line: any of c3,instanced,c4,c5 (comma-separated) or none")
+    ap.add_argument("--render", default="c3,instanced,bunny,c4,c5",
+                    help="render workloads to add to the line: any of c3,instanced,bunny,c4,c5 (comma-separated) "
+                         "or none")
     ap.add_argument("--two-level", default="on", choices=["on", "off"],
                     help="add the instanced (two-level TLAS/BLAS) workload to the line")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule baselines of c3/c4")
@@ -230,6 +231,74 @@ def bench_instanced(args):
                 "gpu_vs_reference_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6)}
     sc.close()
     fsc.close()
+    film.free()
+    return out
+
+
+def bench_bunny(args):
+    """The reference CLI's bunny render (`-scene bunny -gpu` at its defaults:
+    256x256, 64 spp, depth 4, seed 0): the Stanford bunny (69 451 triangles)
+    imported from the OBJ fixture, placed by its instance over a floor under a
+    quad light. Reported: OBJ import time, scene upload/build, render rate; CPU
+    baseline: the reference's trace_path on a band of rows, one thread."""
+    import lzma
+    import tempfile
+
+    from lux_amd import lbvh, obj, render, scenes
+
+    W = H = 256
+    spp, depth = 64, 4
+    src = Path(__file__).resolve().parent / "tests" / "golden" / "obj" / "stanford-bunny.obj.xz"
+    with tempfile.TemporaryDirectory() as tmp:
+        path = Path(tmp) / "stanford-bunny.obj"
+        path.write_bytes(lzma.decompress(src.read_bytes()))
+        t0 = time.perf_counter()
+        obj.load(path, scene_material_count=2, ground_to_y=True)
+        t1 = time.perf_counter()
+        d = scenes.bunny(path, W, H)
+    t2 = time.perf_counter()
+    sc = render.Scene(d)
+    t3 = time.perf_counter()
+    film = lbvh.DeviceArray(W * H * 12)
+    sc.render_device(film, W, H, 1, depth)
+    lbvh.synchronize()
+    best = None
+    for _ in range(5):
+        s0 = time.perf_counter()
+        sc.render_device(film, W, H, spp, depth)
+        lbvh.synchronize()
+        sec = time.perf_counter() - s0
+        best = sec if best is None else min(best, sec)
+    ext, sh = sc.rays()
+    gpu = film.to_numpy(np.float32, (H, W, 3))
+    out = {"workload": f"Stanford bunny ({sc.triangle_count} scene triangles), {W}x{H}, {spp} spp, depth {depth}",
+           "obj_import_s": round(t1 - t0, 3), "scene_upload_build_s": round(t3 - t2, 3),
+           "render_s": round(best, 4), "renders_per_s": round(1 / best, 2),
+           "Mpaths_per_s": round(W * H * spp / best / 1e6, 2), **_render_rate(ext, sh, W * H, best)}
+    _log(f"bunny render: {best * 1e3:.1f} ms")
+    if args.cpu_render_rows > 0:
+        import oracle
+
+        if oracle.ref_available():
+            nrows = max(1, args.cpu_render_rows // 2)
+            r0 = H // 2 - nrows // 2
+            paths = W * nrows * spp
+            ref = oracle.RefPathTracer(d)
+            pix = np.repeat(np.arange(r0 * W, (r0 + nrows) * W, dtype=np.int32), spp)
+            smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
+            s0 = time.perf_counter()
+            rad = ref.samples(W, H, depth, pix, smp)
+            t_ref = time.perf_counter() - s0
+            band = np.zeros((nrows * W, 3), np.float32)
+            for k in range(spp):
+                band += rad[k::spp]
+            out["cpu_baseline"] = {
+                "value": round(paths / t_ref / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "reference",
+                "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+                "host_cpu": _cpu_name(),
+                "gpu_vs_reference_band_relrmse": round(
+                    render.rel_rmse(gpu[r0:r0 + nrows].reshape(-1, 3) / spp, band / spp), 8)}
+    sc.close()
     film.free()
     return out
 
@@ -660,6 +729,8 @@ def main():
         out["render"]["c3_single_scene"] = bench_c3(args)
     if "instanced" in wanted:
         out["render"]["instanced_scene"] = bench_instanced(args)
+    if "bunny" in wanted:
+        out["render"]["bunny"] = bench_bunny(args)
     if "c4" in wanted or "c5" in wanted:
         c4, scs, jobs, dicts = bench_c4(args)
         if "c4" in wanted:
