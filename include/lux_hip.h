@@ -249,6 +249,12 @@ typedef struct lux_scene_desc {
 } lux_scene_desc;
 
 enum { LUX_RENDER_WAVEFRONT = 0, LUX_RENDER_PLAIN = 1 };
+/* RenderSettings::light_sampler_kind (src/light/light_distribution.h:27-31;
+ * default Power): POWER picks an area light with the power alias table, UNIFORM
+ * uniformly, BVH by descending the light BVH with the importance power /
+ * distance^2 at the shading point (sample_light_bvh,
+ * src/light/light_sampler.h:142-195). */
+enum { LUX_LIGHT_SAMPLER_POWER = 0, LUX_LIGHT_SAMPLER_UNIFORM = 1, LUX_LIGHT_SAMPLER_BVH = 2 };
 /* RenderSettings (src/integrator/render_context.h:16-24) + film size. mode
  * PLAIN reproduces the reference schedule (batch_size paths per batch, one
  * sample per batch, full-grid launches, a host sync per batch); WAVEFRONT packs
@@ -260,6 +266,7 @@ typedef struct lux_render_settings {
     int32_t frame_index, sample_offset;
     int32_t batch_size;
     int32_t mode;
+    int32_t light_sampler; /* LUX_LIGHT_SAMPLER_* */
 } lux_render_settings;
 
 typedef struct lux_hip_scene lux_hip_scene;
@@ -280,6 +287,13 @@ int lux_hip_scene_set_camera(lux_hip_scene* scene, const float* v12);
  * light_of_prim gets one entry per triangle (-1 = not an emitter). */
 int lux_hip_scene_lights(lux_hip_scene* scene, int32_t* count, int32_t* prim, float* area, float* prob,
                          int32_t* alias, float* pmf, int32_t* light_of_prim);
+/* The BVH light sampler's tree (build_light_bvh, src/light/light_distribution.cpp:68-161,
+ * nodes in build order): node count, and if the arrays are non-NULL (sized by
+ * the counts) each node's bounds {min xyz, max xyz}, power and {left, right,
+ * parent, light index}, the leaf node of each light, and the root (-1: no
+ * valid tree, the sampler then falls back to power selection). */
+int lux_hip_scene_light_bvh(lux_hip_scene* scene, int32_t* node_count, float* bounds6, float* power,
+                            int32_t* links4, int32_t* leaf_of_light, int32_t* root);
 /* The LBVH of the scene's triangles (for the traversal entry points). */
 int lux_hip_scene_bvh(lux_hip_scene* scene, lux_hip_bvh** out_borrowed);
 

@@ -109,6 +109,7 @@ _SIGNATURES = {
     "lux_hip_scene_create_instanced": [_P, POINTER(c_void_p)],
     "lux_hip_scene_set_instances": [_P, _P, c_int32],
     "lux_hip_scene_triangles": [_P, _P, _P, _P],
+    "lux_hip_scene_light_bvh": [_P, POINTER(c_int32), _P, _P, _P, _P, POINTER(c_int32)],
     "lux_hip_batch_render": [_P, _P, _P, c_int32, c_int32],
     "lux_hip_batch_render_device": [_P, _P, _P, c_int32, c_int32, _P],
     "lux_hip_batch_rays": [_P, POINTER(c_uint64), POINTER(c_uint64)],

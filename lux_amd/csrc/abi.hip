@@ -1009,6 +1009,8 @@ void check_settings(const lux_render_settings* s) {
     if (s->mode != LUX_RENDER_PLAIN && s->mode != LUX_RENDER_WAVEFRONT)
         throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "unknown render mode");
     if (s->batch_size < 0) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "negative batch size");
+    if (s->light_sampler < LUX_LIGHT_SAMPLER_POWER || s->light_sampler > LUX_LIGHT_SAMPLER_BVH)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "unknown light sampler");
 }
 
 // Reads and clears the device check bits of a scene (caller synchronised):
@@ -1286,6 +1288,34 @@ int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float
         }
         if (light_of_prim)
             std::memcpy(light_of_prim, sc->dev.host_light_of_prim.data(), 4 * sc->dev.host_light_of_prim.size());
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_light_bvh(lux_hip_scene* sc, int32_t* node_count, float* bounds6, float* power, int32_t* links4,
+                            int32_t* leaf_of_light, int32_t* root) {
+    return guarded([&] {
+        if (!sc || !node_count) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const pt::SceneDev& d = sc->dev;
+        *node_count = int32_t(d.host_light_nodes.size());
+        if (root) *root = d.light_node_count > 0 ? d.light_root : -1;
+        for (size_t i = 0; i < d.host_light_nodes.size(); ++i) {
+            const lux::pt::DevLightNode& n = d.host_light_nodes[i];
+            if (bounds6)
+                for (int a = 0; a < 3; ++a) {
+                    bounds6[6 * i + a] = n.lo[a];
+                    bounds6[6 * i + 3 + a] = n.hi[a];
+                }
+            if (power) power[i] = n.power;
+            if (links4) {
+                links4[4 * i] = n.left;
+                links4[4 * i + 1] = n.right;
+                links4[4 * i + 2] = n.parent;
+                links4[4 * i + 3] = n.light;
+            }
+        }
+        if (leaf_of_light)
+            for (size_t i = 0; i < d.host_light_leaf.size(); ++i) leaf_of_light[i] = d.host_light_leaf[i];
         return int(LUX_OK);
     });
 }

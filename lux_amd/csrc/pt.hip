@@ -323,6 +323,141 @@ __global__ __launch_bounds__(B) void k_trace(const BatchDesc* __restrict__ desc,
     }
 }
 
+// ---- light selection (LightSamplerKind) ---------------------------------------------
+constexpr int kLightPower = 0, kLightUniform = 1, kLightBvh = 2;
+
+// light_bvh_node_importance (src/light/light_sampler.h:91-115): power over the
+// squared distance from the shading point to the node's box (at least eps^2).
+__device__ __forceinline__ float light_node_importance(const SceneView& sv, int i, const vec& p) {
+    if (i < 0 || i >= sv.light_node_count) return 0.0f;
+    const DevLightNode& n = sv.light_nodes[i];
+    if (n.power <= 0 || n.lo[0] > n.hi[0] || n.lo[1] > n.hi[1] || n.lo[2] > n.hi[2]) return 0.0f;
+    const float q[3] = {p.x, p.y, p.z};
+    float d2 = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float delta = 0;
+        if (q[a] < n.lo[a])
+            delta = n.lo[a] - q[a];
+        else if (q[a] > n.hi[a])
+            delta = q[a] - n.hi[a];
+        d2 += delta * delta;
+    }
+    d2 = fmaxf(d2, kEps * kEps);
+    return n.power / d2;
+}
+
+// light_bvh_child_probability (src/light/light_sampler.h:117-140).
+__device__ __forceinline__ float light_child_probability(const SceneView& sv, int parent, int child,
+                                                         const vec& p) {
+    if (parent < 0 || parent >= sv.light_node_count || child < 0 || child >= sv.light_node_count) return 0.0f;
+    const DevLightNode& n = sv.light_nodes[parent];
+    if (child != n.left && child != n.right) return 0.0f;
+    const float li = light_node_importance(sv, n.left, p);
+    const float ri = light_node_importance(sv, n.right, p);
+    const float total = li + ri;
+    if (total <= 0) return 0.0f;
+    const float lp = li / total;
+    return child == n.left ? lp : 1.0f - lp;
+}
+
+// sample_light_bvh (src/light/light_sampler.h:142-195): descend from the root,
+// choosing a child by its importance at the shading point and rescaling u.
+__device__ __forceinline__ int sample_light_bvh(const SceneView& sv, const vec& p, float select_u, float& pmf) {
+    pmf = 0;
+    int node = sv.light_root;
+    if (node < 0 || node >= sv.light_node_count) return -1;
+    float u = fminf(select_u, 0.99999994f);
+    pmf = 1;
+    for (int depth = 0; depth < sv.light_node_count; ++depth) {
+        const DevLightNode& n = sv.light_nodes[node];
+        if (n.light >= 0) return n.light;
+        const float lp = light_child_probability(sv, node, n.left, p);
+        if (lp <= 0) {
+            node = n.right;
+        } else if (lp >= 1) {
+            node = n.left;
+        } else if (u < lp) {
+            pmf *= lp;
+            u /= lp;
+            node = n.left;
+        } else {
+            const float rp = 1.0f - lp;
+            pmf *= rp;
+            u = (u - lp) / rp;
+            node = n.right;
+        }
+        if (node < 0 || node >= sv.light_node_count) {
+            pmf = 0;
+            return -1;
+        }
+    }
+    pmf = 0;
+    return -1;
+}
+
+// light_bvh_pmf (src/light/light_sampler.h:197-223): the product of the child
+// probabilities from the light's leaf up to the root, at point p.
+__device__ __forceinline__ float light_bvh_pmf(const SceneView& sv, int light, const vec& p) {
+    if (light < 0 || light >= sv.light_count) return 0.0f;
+    int node = sv.light_leaf[light];
+    if (node < 0 || node >= sv.light_node_count) return 0.0f;
+    float pmf = 1;
+    for (int depth = 0; depth < sv.light_node_count; ++depth) {
+        const int parent = sv.light_nodes[node].parent;
+        if (parent < 0) return pmf;
+        const float prob = light_child_probability(sv, parent, node, p);
+        if (prob <= 0) return 0.0f;
+        pmf *= prob;
+        node = parent;
+        if (node < 0 || node >= sv.light_node_count) return 0.0f;
+    }
+    return 0.0f;
+}
+
+// sample_light (src/scene/scene_light_sampler.h:99-152) for area lights: the
+// light index and its selection pmf. Uniform: u * count; BVH: the tree descent
+// at the shading point (power selection when the scene has no valid tree);
+// power: the alias table.
+template <int kLight>
+__device__ __forceinline__ int choose_light(const SceneView& sv, const vec& p, float select_u, float& pmf) {
+    const int nl = sv.light_count;
+    pmf = 0;
+    if (nl <= 0) return -1;
+    if constexpr (kLight == kLightUniform) {
+        int idx = int(fminf(select_u, 0.99999994f) * float(nl));
+        if (idx >= nl) idx = nl - 1;
+        pmf = 1.0f / float(nl);
+        return idx;
+    } else {
+        if (kLight == kLightBvh && sv.light_root >= 0) {
+            const int idx = sample_light_bvh(sv, p, select_u, pmf);
+            if (idx >= 0 && !(pmf > 0)) pmf = light_bvh_pmf(sv, idx, p);  // a descent pmf that underflowed
+            return idx;
+        }
+        const float scaled = fminf(select_u, 0.99999994f) * float(nl);
+        int col = int(scaled);
+        if (col >= nl) col = nl - 1;
+        const float thr = scaled - float(col);
+        const int idx = thr < sv.lights[col].prob ? col : sv.lights[col].alias;
+        if (idx < 0 || idx >= nl) return -1;
+        pmf = sv.lights[idx].pmf;
+        return idx;
+    }
+}
+
+// light_pmf (src/scene/scene_light_sampler.h:158-203) with the reference point
+// of the context (the BVH pmf depends on it).
+template <int kLight>
+__device__ __forceinline__ float light_choice_pmf(const SceneView& sv, int light, const vec& ref) {
+    if constexpr (kLight == kLightUniform) {
+        return 1.0f / float(sv.light_count);
+    } else {
+        if (kLight == kLightBvh && sv.light_root >= 0) return light_bvh_pmf(sv, light, ref);
+        return sv.lights[light].pmf;
+    }
+}
+
 // ---- shade -----------------------------------------------------------------------
 // Shading key of a queue entry: 0 = miss, 1 + material type for a hit, and
 // kShadeKeys - 1 for the idle lanes past the end of the queue.
@@ -339,7 +474,7 @@ constexpr int kShadeKeys = kMaterialTypes + 2;
 // result.
 // Capped at 128 VGPRs (4 waves per SIMD; 12 bytes spill), which measured
 // faster than 134 VGPRs at 3 waves.
-template <int B>
+template <int B, int kLight>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int in,
                                                   int bounce, int filter) {
     constexpr int kWaves = B / 64;
@@ -452,7 +587,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
                 float lpdf = 0;
                 if (li >= 0 && li < sv.light_count && sv.lights[li].area > 0) {
                     const DevLight& lt = sv.lights[li];
-                    lpdf = lt.pmf * pdf_solid_angle(p, ng, lt.two_sided, 1.0f / lt.area, prev_pos);
+                    lpdf = light_choice_pmf<kLight>(sv, li, prev_pos) *
+                           pdf_solid_angle(p, ng, lt.two_sided, 1.0f / lt.area, prev_pos);
                 }
                 L = L + T * Le * power_heuristic(prev_pdf, lpdf);
             }
@@ -474,15 +610,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
                     rng.get1();  // light component (unused by area lights)
                     float lu0 = rng.get1();
                     float lu1 = rng.get1();
-                    // sample_light_li: power alias pick, area sample (src/scene/scene_light_sampler.h:210-250)
-                    int nl = sv.light_count;
+                    // sample_light_li: light pick, area sample (src/scene/scene_light_sampler.h:210-250)
+                    const int nl = sv.light_count;
                     if (nl > 0) {
-                        float scaled = fminf(sel, 0.99999994f) * float(nl);
-                        int col = int(scaled);
-                        if (col >= nl) col = nl - 1;
-                        float thr = scaled - float(col);
-                        int idx = thr < sv.lights[col].prob ? col : sv.lights[col].alias;
-                        if (idx >= 0 && idx < nl && sv.lights[idx].pmf > 0) {
+                        float sel_pmf = 0;
+                        const int idx = choose_light<kLight>(sv, p, sel, sel_pmf);
+                        if (idx >= 0 && idx < nl && sel_pmf > 0) {
                             const DevLight& lt = sv.lights[idx];
                             const lux_triangle lt_tri = sv.soup[lt.prim];
                             float su = sqrtf(lu0);
@@ -496,7 +629,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
                                 float dist = sqrtf(d2);
                                 vec wi = to / dist;
                                 vec rad = eval_emission(mk(lt.er, lt.eg, lt.eb), lt.two_sided, ln, -wi);
-                                float lpdf = lt.pmf * pdf_solid_angle(lp, ln, lt.two_sided, pdf_area, p);
+                                float lpdf = sel_pmf * pdf_solid_angle(lp, ln, lt.two_sided, pdf_area, p);
                                 if (max_component(rad) > 0 && lpdf > 0) {
                                     // estimate_unoccluded_direct_light_li (src/light/light_sampler.h:420-454)
                                     vec f = eval_bsdf(m, it, wi);
@@ -667,6 +800,8 @@ Segment make_segment(const Request& rq, uint64_t req_begin, uint32_t slot_begin,
 
 }  // namespace
 
+int request_kind(const Request& rq) { return (rq.sv.instanced ? 1 : 0) | (rq.rs.light_sampler << 1); }
+
 uint64_t request_paths(const lux_render_settings& rs) {
     return uint64_t(rs.width) * uint64_t(rs.height) * uint64_t(rs.spp);
 }
@@ -693,7 +828,7 @@ void Renderer::set_tuning(const Tuning& t) {
 void Renderer::release_graph() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     graph_ = nullptr;
-    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = graph_opaque_ = graph_two_level_ = -1;
+    graph_depth_ = graph_grid_ = graph_capacity_ = graph_block_ = graph_opaque_ = graph_two_level_ = graph_light_ = -1;
 }
 
 void Renderer::ensure_capacity(int paths) {
@@ -763,7 +898,7 @@ void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* sha
 namespace {
 // The wavefront launch sequence of one batch (captured or launched directly).
 // Raw launches (no probe events) when captured.
-template <int B, bool kProbe, bool kOpaque, bool kTwoLevel>
+template <int B, bool kProbe, bool kOpaque, bool kTwoLevel, int kLight>
 void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
@@ -782,7 +917,7 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
         const int in = b & 1;
         PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1,
                   2 + (b & 1));
-        PT_LAUNCH("pt_shade", k_shade<B>, d, ps, qs, in, b, -1);
+        PT_LAUNCH("pt_shade", (k_shade<B, kLight>), d, ps, qs, in, b, -1);
     }
     PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
     PT_LAUNCH("pt_flush", k_flush<B>, d, ps);
@@ -791,46 +926,65 @@ void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t str
 }  // namespace
 
 namespace {
-template <bool kProbe, bool kOpaque, bool kTwoLevel>
+template <bool kProbe, bool kOpaque, bool kTwoLevel, int kLight>
 void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
     switch (block) {
-        case 128: wavefront_sequence<128, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
-        case 512: wavefront_sequence<512, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
-        default: wavefront_sequence<256, kProbe, kOpaque, kTwoLevel>(buf, depth, grid, stream); break;
+        case 128: wavefront_sequence<128, kProbe, kOpaque, kTwoLevel, kLight>(buf, depth, grid, stream); break;
+        case 512: wavefront_sequence<512, kProbe, kOpaque, kTwoLevel, kLight>(buf, depth, grid, stream); break;
+        default: wavefront_sequence<256, kProbe, kOpaque, kTwoLevel, kLight>(buf, depth, grid, stream); break;
     }
 }
 
+template <bool kProbe, bool kOpaque, bool kTwoLevel>
+void wavefront_sequence_l(int light, int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+    switch (light) {
+        case kLightUniform:
+            wavefront_sequence_b<kProbe, kOpaque, kTwoLevel, kLightUniform>(block, buf, depth, grid, stream);
+            break;
+        case kLightBvh:
+            wavefront_sequence_b<kProbe, kOpaque, kTwoLevel, kLightBvh>(block, buf, depth, grid, stream);
+            break;
+        default:
+            wavefront_sequence_b<kProbe, kOpaque, kTwoLevel, kLightPower>(block, buf, depth, grid, stream);
+            break;
+    }
+}
+
+// The launch sequence for one batch kind: shadow-ray walk (any-hit when no
+// scene of the batch passes light), traversal (one- or two-level) and light
+// selection are template choices made here, per batch.
 template <bool kProbe>
-void wavefront_sequence_o(bool opaque, bool two_level, int block, const PathBuf& buf, int depth, int grid,
-                          hipStream_t stream) {
+void wavefront_sequence_o(bool opaque, bool two_level, int light, int block, const PathBuf& buf, int depth,
+                          int grid, hipStream_t stream) {
     if (two_level) {
         if (opaque)
-            wavefront_sequence_b<kProbe, true, true>(block, buf, depth, grid, stream);
+            wavefront_sequence_l<kProbe, true, true>(light, block, buf, depth, grid, stream);
         else
-            wavefront_sequence_b<kProbe, false, true>(block, buf, depth, grid, stream);
+            wavefront_sequence_l<kProbe, false, true>(light, block, buf, depth, grid, stream);
     } else {
         if (opaque)
-            wavefront_sequence_b<kProbe, true, false>(block, buf, depth, grid, stream);
+            wavefront_sequence_l<kProbe, true, false>(light, block, buf, depth, grid, stream);
         else
-            wavefront_sequence_b<kProbe, false, false>(block, buf, depth, grid, stream);
+            wavefront_sequence_l<kProbe, false, false>(light, block, buf, depth, grid, stream);
     }
 }
 }  // namespace
 
-void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, hipStream_t stream) {
+void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, int light, hipStream_t stream) {
     const int block = tune_.block;
     const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
     if (!tune_.use_graph) {
-        wavefront_sequence_o<true>(opaque, two_level, block, buf_, depth, grid, stream);
+        wavefront_sequence_o<true>(opaque, two_level, light, block, buf_, depth, grid, stream);
         return;
     }
     if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity ||
-        graph_block_ != block || graph_opaque_ != int(opaque) || graph_two_level_ != int(two_level)) {
+        graph_block_ != block || graph_opaque_ != int(opaque) || graph_two_level_ != int(two_level) ||
+        graph_light_ != light) {
         release_graph();
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {
-            wavefront_sequence_o<false>(opaque, two_level, block, buf_, depth, grid, stream);
+            wavefront_sequence_o<false>(opaque, two_level, light, block, buf_, depth, grid, stream);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -846,12 +1000,13 @@ void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, hipStrea
         graph_block_ = block;
         graph_opaque_ = int(opaque);
         graph_two_level_ = int(two_level);
+        graph_light_ = light;
     }
     hip_check(hipGraphLaunch(graph_, stream), "hipGraphLaunch");
 }
 
 namespace {
-template <bool kTwoLevel>
+template <bool kTwoLevel, int kLight>
 void plain_sequence(const PathBuf& buf, int depth, uint32_t count, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
@@ -863,7 +1018,7 @@ void plain_sequence(const PathBuf& buf, int depth, uint32_t count, hipStream_t s
         LUX_LAUNCH("pt_trace", (k_trace<kBlock, false, kTwoLevel>), grid, kBlock, 0, stream, d, ps, qs, in, -1,
                    in ^ 1, 2 + (b & 1));
         for (int type = 0; type < kMaterialTypes; ++type)
-            LUX_LAUNCH("pt_shade", k_shade<kBlock>, grid, kBlock, 0, stream, d, ps, qs, in, b, type);
+            LUX_LAUNCH("pt_shade", (k_shade<kBlock, kLight>), grid, kBlock, 0, stream, d, ps, qs, in, b, type);
         LUX_LAUNCH("pt_shadow", (k_trace<kBlock, false, kTwoLevel>), grid, kBlock, 0, stream, d, ps, qs, -1,
                    2 + (b & 1), -1, -1);
     }
@@ -871,11 +1026,22 @@ void plain_sequence(const PathBuf& buf, int depth, uint32_t count, hipStream_t s
 }
 }  // namespace
 
-void Renderer::launch_plain(int depth, uint32_t count, bool two_level, hipStream_t stream) {
+namespace {
+template <bool kTwoLevel>
+void plain_sequence_l(int light, const PathBuf& buf, int depth, uint32_t count, hipStream_t stream) {
+    switch (light) {
+        case kLightUniform: plain_sequence<kTwoLevel, kLightUniform>(buf, depth, count, stream); break;
+        case kLightBvh: plain_sequence<kTwoLevel, kLightBvh>(buf, depth, count, stream); break;
+        default: plain_sequence<kTwoLevel, kLightPower>(buf, depth, count, stream); break;
+    }
+}
+}  // namespace
+
+void Renderer::launch_plain(int depth, uint32_t count, bool two_level, int light, hipStream_t stream) {
     if (two_level)
-        plain_sequence<true>(buf_, depth, count, stream);
+        plain_sequence_l<true>(light, buf_, depth, count, stream);
     else
-        plain_sequence<false>(buf_, depth, count, stream);
+        plain_sequence_l<false>(light, buf_, depth, count, stream);
 }
 
 uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
@@ -886,15 +1052,16 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     uint64_t used = 0;
     uint32_t nseg = 0;
     bool opaque = true;  // every scene of the batch: shadow rays may use the any-hit walk
-    // One traversal kind per batch (the kernels are built for one): the kind of
-    // the first request with paths left; requests of the other kind wait for a
-    // later batch and keep their place in `active`.
+    // One kind per batch (traversal level and light sampler: the kernels are
+    // built for one): the kind of the first request with paths left; requests
+    // of another kind wait for a later batch and keep their place in `active`.
     int kind = -1;
     for (Request* rq : active) {
         if (used >= cap || nseg >= uint32_t(kMaxSegments)) break;
         if (rq->issued >= rq->total) continue;
-        if (kind < 0) kind = rq->sv.instanced;
-        if (rq->sv.instanced != kind) continue;
+        const int k = request_kind(*rq);
+        if (kind < 0) kind = k;
+        if (k != kind) continue;
         uint64_t take = std::min(rq->total - rq->issued, cap - used);
         if (slice_spp > 0)
             take = std::min(take, uint64_t(slice_spp) * uint64_t(rq->rs.width) * uint64_t(rq->rs.height));
@@ -909,7 +1076,7 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
     slot->count = uint32_t(used);
     slot->nseg = nseg;
     upload(slot, stream);
-    launch_wavefront(max_depth, opaque, kind == 1, stream);
+    launch_wavefront(max_depth, opaque, (kind & 1) != 0, kind >> 1, stream);
     ++batch_counter_;
     return used;
 }
@@ -936,7 +1103,7 @@ void Renderer::render_all(std::vector<Request*>& reqs, int mode, hipStream_t str
                     slot->nseg = 1;
                     slot->seg[0] = make_segment(*rq, uint64_t(s) * npix + p0, 0, n);
                     upload(slot, stream);
-                    launch_plain(rq->rs.max_depth, n, rq->sv.instanced != 0, stream);
+                    launch_plain(rq->rs.max_depth, n, rq->sv.instanced != 0, rq->rs.light_sampler, stream);
                     hip_check(hipMemcpyAsync(host_counts, buf_.counts, sizeof(host_counts), hipMemcpyDeviceToHost,
                                              stream),
                               "D2H counts");

@@ -71,6 +71,17 @@ struct DevLight {
     int pad[3];
 };
 
+// Light-BVH node (LightBvhNode, src/light/light_distribution.h:33-40): bounds,
+// power, light index (leaf) or -1, children, parent; three 16-byte loads.
+struct DevLightNode {
+    float lo[3];
+    float power;
+    float hi[3];
+    int light;
+    int left, right, parent, pad;
+};
+static_assert(sizeof(DevLightNode) == 48, "light node record");
+
 struct DevCamera {
     float ox, oy, oz;
     float lx, ly, lz;  // lower left

@@ -46,6 +46,14 @@ struct SceneDev {
     DevCamera camera{};
     std::vector<DevLight> host_lights;
     std::vector<int32_t> host_light_of_prim;
+    // the BVH light sampler's tree over the lights (rebuilt with the light table)
+    std::vector<float> host_light_power;
+    std::vector<lux_triangle> host_light_tri;
+    std::vector<DevLightNode> host_light_nodes;
+    std::vector<int32_t> host_light_leaf;
+    DevLightNode* light_nodes = nullptr;
+    int32_t* light_leaf = nullptr;
+    int light_node_count = 0, light_node_capacity = 0, light_leaf_capacity = 0, light_root = -1;
     // emissive triangles in triangle order, with their vertex indices and
     // materials (the light table is rebuilt from these after a vertex update)
     std::vector<int32_t> emitters;
@@ -79,6 +87,9 @@ struct SceneView {
     const int32_t* light_of_prim;
     DevInfo* info;
     int32_t node_count, tri_count, mat_count, light_count;
+    const DevLightNode* light_nodes;  // BVH light sampler: tree and leaf of each light
+    const int32_t* light_leaf;
+    int32_t light_node_count, light_root;  // root -1: no valid tree (power selection is used)
     int32_t opaque;     // 1: every hit blocks a shadow ray (no pass-through material)
     int32_t instanced;  // 1: traverse `tl` (two-level) instead of `nodes`
     DevCamera cam;
@@ -169,8 +180,8 @@ public:
     // request gives at most `slice_spp` samples of its image, 0 = no limit).
     // Returns the number of paths placed (0 = nothing left). Sets last_batch of
     // requests whose final path is in this batch to the batch's index. A batch
-    // holds one traversal kind (one-mesh or instanced scenes): the first
-    // unfinished request's; the others wait for a later batch.
+    // holds one kind (traversal level, light sampler): the first unfinished
+    // request's; the others wait for a later batch.
     uint64_t enqueue_batch(std::vector<Request*>& active, int slice_spp, int max_depth, hipStream_t stream,
                            int64_t* batch_index);
 
@@ -185,8 +196,8 @@ private:
     void ensure_capacity(int paths);
     BatchDesc* next_slot();
     void upload(BatchDesc* slot, hipStream_t stream);
-    void launch_wavefront(int depth, bool opaque, bool two_level, hipStream_t stream);
-    void launch_plain(int depth, uint32_t count, bool two_level, hipStream_t stream);
+    void launch_wavefront(int depth, bool opaque, bool two_level, int light, hipStream_t stream);
+    void launch_plain(int depth, uint32_t count, bool two_level, int light, hipStream_t stream);
     void release_graph();
 
     Tuning tune_;
@@ -198,7 +209,7 @@ private:
     int ring_next_ = 0;
     hipGraphExec_t graph_ = nullptr;
     int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1, graph_opaque_ = -1,
-        graph_two_level_ = -1;
+        graph_two_level_ = -1, graph_light_ = -1;
     int64_t batch_counter_ = 0;
 };
 
@@ -215,6 +226,8 @@ void scene_set_instances(SceneDev& s, const lux_instance* instances, int instanc
 void scene_release(SceneDev& s);
 SceneView scene_view(const SceneDev& s);
 uint64_t request_paths(const lux_render_settings& rs);
+// Batch kind of a request: bit 0 two-level traversal, bits 1.. the light sampler.
+int request_kind(const Request& rq);
 
 }  // namespace pt
 }  // namespace lux

@@ -130,9 +130,12 @@ lux_vec3 world_point(const float* m, const lux_vec3& p) {
 // their current vertices; the other triangles contribute nothing to the table.
 void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std::vector<lux_triangle>& tris,
                   const std::vector<lux_material>& emitter_mats, std::vector<DevLight>& lights,
-                  std::vector<int32_t>& light_of_prim) {
+                  std::vector<int32_t>& light_of_prim, std::vector<float>& light_power,
+                  std::vector<lux_triangle>& light_tri) {
     light_of_prim.assign(tri_count, -1);
     lights.clear();
+    light_power.clear();
+    light_tri.clear();
     std::vector<float> cdf;
     float total = 0;
     for (size_t e = 0; e < emitters.size(); ++e) {
@@ -158,6 +161,8 @@ void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std
         L.two_sided = m.emission_two_sided;
         L.area = area;
         lights.push_back(L);
+        light_power.push_back(power);
+        light_tri.push_back(tris[e]);
         cdf.push_back(total);
         int32_t& stored = light_of_prim[t];
         stored = stored == -1 ? int32_t(lights.size()) - 1 : -2;
@@ -165,6 +170,8 @@ void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std
     int count = int(lights.size());
     if (count == 0 || !(total > 0)) {
         lights.clear();
+        light_power.clear();
+        light_tri.clear();
         return;
     }
     std::vector<float> w(count);
@@ -205,6 +212,85 @@ void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std
     }
 }
 
+// The light BVH of the BVH light sampler: build_light_bvh
+// (src/light/light_distribution.cpp:68-161) over one item per light (its
+// world triangle's bounds, their centre, its power), split at the median
+// centroid of the widest centroid axis with std::nth_element, nodes in the
+// order the recursion creates them (preorder), interior power and bounds the
+// sums / unions of the two children. Items are the lights in table order.
+void build_light_bvh(const std::vector<float>& power, const std::vector<lux_triangle>& tri,
+                     std::vector<DevLightNode>& nodes, std::vector<int32_t>& leaf_of_light, int& root) {
+    nodes.clear();
+    leaf_of_light.clear();
+    root = -1;
+    struct Item {
+        float lo[3], hi[3], c[3];
+        int light;
+        float power;
+    };
+    std::vector<Item> items(power.size());
+    for (size_t i = 0; i < power.size(); ++i) {
+        const lux_triangle& t = tri[i];
+        const float xs[3] = {t.v0.x, t.v1.x, t.v2.x}, ys[3] = {t.v0.y, t.v1.y, t.v2.y},
+                    zs[3] = {t.v0.z, t.v1.z, t.v2.z};
+        const float* axes[3] = {xs, ys, zs};
+        Item& it = items[i];
+        for (int a = 0; a < 3; ++a) {
+            it.lo[a] = fminf(fminf(axes[a][0], axes[a][1]), axes[a][2]);
+            it.hi[a] = fmaxf(fmaxf(axes[a][0], axes[a][1]), axes[a][2]);
+            it.c[a] = (it.lo[a] + it.hi[a]) * 0.5f;
+        }
+        it.light = int(i);
+        it.power = power[i];
+        if (!(it.power > 0) || it.lo[0] > it.hi[0] || it.lo[1] > it.hi[1] || it.lo[2] > it.hi[2]) return;
+    }
+    if (items.empty()) return;
+    nodes.reserve(2 * items.size() - 1);
+    leaf_of_light.assign(items.size(), -1);
+    auto build = [&](auto&& self, int begin, int end, int parent) -> int {
+        const int index = int(nodes.size());
+        nodes.push_back(DevLightNode{});
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = begin; i < end; ++i)
+            for (int a = 0; a < 3; ++a) {
+                clo[a] = fminf(clo[a], items[i].c[a]);
+                chi[a] = fmaxf(chi[a], items[i].c[a]);
+            }
+        nodes[index].parent = parent;
+        if (end - begin == 1) {
+            const Item& it = items[begin];
+            DevLightNode& n = nodes[index];
+            for (int a = 0; a < 3; ++a) {
+                n.lo[a] = it.lo[a];
+                n.hi[a] = it.hi[a];
+            }
+            n.power = 0.0f + it.power;
+            n.light = it.light;
+            n.left = n.right = -1;
+            leaf_of_light[it.light] = index;
+            return index;
+        }
+        const float dx = chi[0] - clo[0], dy = chi[1] - clo[1], dz = chi[2] - clo[2];
+        const int axis = (dx > dy && dx > dz) ? 0 : (dy > dz) ? 1 : 2;  // AABB::max_extent_axis
+        const int mid = begin + (end - begin) / 2;
+        std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
+                         [axis](const Item& a, const Item& b) { return a.c[axis] < b.c[axis]; });
+        const int left = self(self, begin, mid, index);
+        const int right = self(self, mid, end, index);
+        DevLightNode& n = nodes[index];
+        n.left = left;
+        n.right = right;
+        n.light = -1;
+        n.power = nodes[left].power + nodes[right].power;
+        for (int a = 0; a < 3; ++a) {
+            n.lo[a] = fminf(nodes[left].lo[a], nodes[right].lo[a]);
+            n.hi[a] = fmaxf(nodes[left].hi[a], nodes[right].hi[a]);
+        }
+        return index;
+    };
+    root = build(build, 0, int(items.size()), -1);
+}
+
 }  // namespace host
 
 namespace {
@@ -217,6 +303,24 @@ void emitter_triangles(const SceneDev& s, const lux_vec3* vertices, std::vector<
 }
 
 void upload_lights(SceneDev& s, hipStream_t stream) {
+    host::build_light_bvh(s.host_light_power, s.host_light_tri, s.host_light_nodes, s.host_light_leaf, s.light_root);
+    const size_t nn = s.host_light_nodes.size(), nl = s.host_light_leaf.size();
+    if (!s.light_nodes || s.light_node_capacity < int(nn) || s.light_leaf_capacity < int(nl)) {
+        if (s.light_nodes) (void)hipFree(s.light_nodes);
+        if (s.light_leaf) (void)hipFree(s.light_leaf);
+        s.light_node_capacity = int(std::max<size_t>(nn, 1));
+        s.light_leaf_capacity = int(std::max<size_t>(nl, 1));
+        s.light_nodes = dalloc<DevLightNode>(size_t(s.light_node_capacity));
+        s.light_leaf = dalloc<int32_t>(size_t(s.light_leaf_capacity));
+    }
+    s.light_node_count = int(nn);
+    if (nn)
+        hip_check(hipMemcpyAsync(s.light_nodes, s.host_light_nodes.data(), sizeof(DevLightNode) * nn,
+                                 hipMemcpyHostToDevice, stream),
+                  "H2D light BVH");
+    if (nl)
+        hip_check(hipMemcpyAsync(s.light_leaf, s.host_light_leaf.data(), 4 * nl, hipMemcpyHostToDevice, stream),
+                  "H2D light leaves");
     if (!s.lights || s.light_count < int(s.host_lights.size())) {
         if (s.lights) (void)hipFree(s.lights);
         s.lights = dalloc<DevLight>(std::max<size_t>(s.host_lights.size(), 1));
@@ -301,7 +405,8 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
     emitter_triangles(s, desc.vertices, etris);
     std::vector<lux_material> emats(s.emitters.size());
     for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
-    host::build_lights(nt, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+    host::build_lights(nt, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
+                      s.host_light_tri);
 
     s.soup = dalloc<lux_triangle>(nt);
     s.vertices = dalloc<lux_vec3>(nv);
@@ -346,7 +451,8 @@ void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_cou
     emitter_triangles(s, vertices, etris);
     std::vector<lux_material> emats(s.emitters.size());
     for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
-    host::build_lights(s.tri_count, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+    host::build_lights(s.tri_count, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
+                      s.host_light_tri);
     // stream order: renders enqueued before this update read the old geometry
     hip_check(hipMemcpyAsync(s.vertices, vertices, sizeof(lux_vec3) * size_t(vertex_count), hipMemcpyHostToDevice,
                              stream),
@@ -446,7 +552,8 @@ void flatten_instances(SceneDev& s, const lux_instance* instances, int count, hi
     }
     std::vector<lux_material> emats(emitters.size());
     for (size_t e = 0; e < emitters.size(); ++e) emats[e] = s.host_mats[emitter_mat[e]];
-    host::build_lights(T, emitters, etris, emats, s.host_lights, s.host_light_of_prim);
+    host::build_lights(T, emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
+                      s.host_light_tri);
     s.emitters = emitters;
     s.emitter_mat = emitter_mat;
 
@@ -586,7 +693,8 @@ void scene_set_instances(SceneDev& s, const lux_instance* instances, int count, 
 }
 
 void scene_release(SceneDev& s) {
-    void* ptrs[] = {s.soup, s.vertices, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights};
+    void* ptrs[] = {s.soup, s.vertices, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights,
+                    s.light_nodes, s.light_leaf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     bvh_release(s.bvh);
@@ -616,6 +724,10 @@ SceneView scene_view(const SceneDev& s) {
     v.lights = s.lights;
     v.light_count = s.light_count;
     v.light_of_prim = s.light_of_prim;
+    v.light_nodes = s.light_nodes;
+    v.light_leaf = s.light_leaf;
+    v.light_node_count = s.light_node_count;
+    v.light_root = (s.light_root >= 0 && s.light_node_count > 0 && !s.host_light_leaf.empty()) ? s.light_root : -1;
     v.opaque = s.opaque ? 1 : 0;
     v.cam = s.camera;
     v.info = s.instanced ? s.tl.tlas.info : s.bvh.info;

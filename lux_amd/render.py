@@ -44,7 +44,10 @@ class InstancedSceneDesc(ctypes.Structure):
 class RenderSettings(ctypes.Structure):
     _fields_ = [("width", c_int32), ("height", c_int32), ("spp", c_int32), ("max_depth", c_int32),
                 ("seed", c_uint64), ("frame_index", c_int32), ("sample_offset", c_int32), ("batch_size", c_int32),
-                ("mode", c_int32)]
+                ("mode", c_int32), ("light_sampler", c_int32)]
+
+
+LIGHT_SAMPLERS = {"power": 0, "uniform": 1, "bvh": 2}
 
 
 class Tuning(ctypes.Structure):
@@ -76,8 +79,10 @@ def _register():
     return lib
 
 
-def settings(width, height, spp, depth, seed=0, frame=0, sample_offset=0, batch_size=0, mode=WAVEFRONT):
-    return RenderSettings(width, height, spp, depth, seed, frame, sample_offset, batch_size, mode)
+def settings(width, height, spp, depth, seed=0, frame=0, sample_offset=0, batch_size=0, mode=WAVEFRONT,
+             light_sampler="power"):
+    ls = LIGHT_SAMPLERS[light_sampler] if isinstance(light_sampler, str) else int(light_sampler)
+    return RenderSettings(width, height, spp, depth, seed, frame, sample_offset, batch_size, mode, ls)
 
 
 class Scene:
@@ -159,6 +164,20 @@ class Scene:
         call("lux_hip_scene_set_instances", self.handle, inst.ctypes.data, len(inst))
         self._inst = inst
         self.triangle_count = self.info()[0]
+
+    def light_bvh(self):
+        """The BVH light sampler's tree: bounds (N, 6), power (N,), links (N, 4:
+        left, right, parent, light), leaf node per light, root."""
+        n, root = c_int32(), c_int32()
+        call("lux_hip_scene_light_bvh", self.handle, ctypes.byref(n), None, None, None, None, ctypes.byref(root))
+        k = n.value
+        nl = len(self.lights()["prim"])
+        out = {"bounds": np.zeros((k, 6), np.float32), "power": np.zeros(k, np.float32),
+               "links": np.zeros((k, 4), np.int32), "leaf": np.zeros(nl if k else 0, np.int32)}
+        call("lux_hip_scene_light_bvh", self.handle, ctypes.byref(n), ptr(out["bounds"]), ptr(out["power"]),
+             ptr(out["links"]), ptr(out["leaf"]) if k else None, ctypes.byref(root))
+        out["root"] = root.value
+        return out
 
     def triangles(self):
         """The renderer's shading view per scene ordinal: world corners (T, 9),

@@ -139,6 +139,71 @@ def mixed_materials(width=40, height=40):
     return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
 
 
+def many_lights(width=40, height=40, n_lights=256, seed=0x4C5561):
+    """A many-light scene for the light samplers: the Cornell room without its
+    ceiling lamp, a Lambert box, a GGX box and a glass box, lit by n_lights
+    small emissive triangles (one-sided or two-sided, emission log-uniform
+    over two decades in random colours) scattered under the ceiling and along
+    the back wall."""
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    materials = [
+        material(LAMBERT, (0.78, 0.78, 0.74)),
+        material(LAMBERT, (0.65, 0.05, 0.05)),
+        material(LAMBERT, (0.12, 0.45, 0.15)),
+        material(GGX, (0.9, 0.7, 0.4), roughness=f(0.25), metallic=f(1.0)),
+        material(DIELECTRIC, (1.0, 1.0, 1.0), roughness=0.0, eta=f(1.5)),
+    ]
+    tris, mats = [], []
+
+    def quad(p0, p1, p2, p3, m):
+        tris.append([*p0, *p1, *p2])
+        tris.append([*p0, *p2, *p3])
+        mats.extend([m, m])
+
+    def box(lo, hi, m):
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        p000, p001, p010, p011 = (x0, y0, z0), (x0, y0, z1), (x0, y1, z0), (x0, y1, z1)
+        p100, p101, p110, p111 = (x1, y0, z0), (x1, y0, z1), (x1, y1, z0), (x1, y1, z1)
+        quad(p000, p100, p110, p010, m)
+        quad(p101, p001, p011, p111, m)
+        quad(p001, p000, p010, p011, m)
+        quad(p100, p101, p111, p110, m)
+        quad(p010, p110, p111, p011, m)
+        quad(p001, p101, p100, p000, m)
+
+    quad((-1, 0, 2), (1, 0, 2), (1, 0, 0), (-1, 0, 0), 0)
+    quad((-1, 2, 0), (1, 2, 0), (1, 2, 2), (-1, 2, 2), 0)
+    quad((-1, 0, 0), (1, 0, 0), (1, 2, 0), (-1, 2, 0), 0)
+    quad((-1, 0, 2), (-1, 0, 0), (-1, 2, 0), (-1, 2, 2), 1)
+    quad((1, 0, 0), (1, 0, 2), (1, 2, 2), (1, 2, 0), 2)
+    box((f(-0.65), 0, f(0.35)), (f(-0.15), f(0.8), f(0.9)), 0)
+    box((f(0.25), 0, f(0.75)), (f(0.75), f(0.6), f(1.35)), 3)
+    box((f(-0.1), 0, f(1.2)), (f(0.2), f(0.3), f(1.5)), 4)
+    for k in range(n_lights):
+        on_ceiling = k % 3 != 0
+        if on_ceiling:  # facing down, just under the ceiling
+            c = np.array([rng.uniform(-0.9, 0.9), 1.98, rng.uniform(0.1, 1.9)])
+            e1, e2 = np.array([1.0, 0, 0]), np.array([0, 0, 1.0])
+        else:  # facing the camera, just in front of the back wall
+            c = np.array([rng.uniform(-0.9, 0.9), rng.uniform(0.2, 1.8), 0.02])
+            e1, e2 = np.array([0, 1.0, 0]), np.array([1.0, 0, 0])
+        s = rng.uniform(0.02, 0.08)
+        a = rng.uniform(0, 2 * np.pi)
+        d1 = np.cos(a) * e1 + np.sin(a) * e2
+        d2 = -np.sin(a) * e1 + np.cos(a) * e2
+        p0, p1, p2 = c - s * d1 - s * d2, c + s * d1 - s * d2, c + s * d2
+        tris.append([*p0, *p1, *p2])
+        color = rng.uniform(0.2, 1.0, 3)
+        power = np.exp(rng.uniform(np.log(2.0), np.log(200.0)))
+        materials.append(material(LAMBERT, (0, 0, 0), emission=tuple(color * power),
+                                  two_sided=int(rng.uniform() < 0.25)))
+        mats.append(len(materials) - 1)
+    aspect = np.float32(width) / np.float32(height)
+    return _pack(np.asarray(tris, np.float32), mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
+
+
 # ---- instanced scenes ------------------------------------------------------------
 
 def instanced_room(width=40, height=40, pass_through=True):

@@ -279,9 +279,11 @@ def _ref_pt_lib():
         L.ref_pt_lights.restype = c_int32
         L.ref_pt_lights.argtypes = [c_void_p] + [c_void_p] * 8
         L.ref_pt_render.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
-                                    c_void_p]
+                                    c_int32, c_void_p]
         L.ref_pt_samples.argtypes = [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_int32, c_int32,
-                                     c_void_p, c_void_p, c_int32, c_void_p]
+                                     c_int32, c_void_p, c_void_p, c_int32, c_void_p]
+        L.ref_pt_light_bvh.restype = c_int32
+        L.ref_pt_light_bvh.argtypes = [c_void_p] * 6
         L.ref_pt_create_instanced.restype = c_void_p
         L.ref_pt_create_instanced.argtypes = [c_void_p] * 7 + [c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                                                c_void_p, c_int32, c_void_p]
@@ -371,17 +373,37 @@ class RefPathTracer:
                         _p(out["pmf"]), _p(out["primitive_light_index"]), _p(out["total_weight"]))
         return out
 
-    def render(self, width, height, spp, depth, seed=0, frame=0, sample_offset=0):
+    # LightSamplerKind values (src/light/light_distribution.h:27-31)
+    LIGHT_SAMPLERS = {"uniform": 0, "power": 1, "bvh": 2}
+
+    def render(self, width, height, spp, depth, seed=0, frame=0, sample_offset=0, light_sampler="power"):
         film = np.zeros((height, width, 3), np.float32)
-        _ref_pt_lib().ref_pt_render(self.h, width, height, spp, depth, seed, frame, sample_offset, _p(film))
+        _ref_pt_lib().ref_pt_render(self.h, width, height, spp, depth, seed, frame, sample_offset,
+                                    self.LIGHT_SAMPLERS[light_sampler], _p(film))
         return film
 
-    def samples(self, width, height, depth, pixels, samples, seed=0, frame=0, sample_offset=0):
+    def samples(self, width, height, depth, pixels, samples, seed=0, frame=0, sample_offset=0,
+                light_sampler="power"):
         px = np.ascontiguousarray(pixels, np.int32)
         sm = np.ascontiguousarray(samples, np.int32)
         out = np.zeros((len(px), 3), np.float32)
-        _ref_pt_lib().ref_pt_samples(self.h, width, height, depth, seed, frame, sample_offset, _p(px), _p(sm), len(px),
-                                     _p(out))
+        _ref_pt_lib().ref_pt_samples(self.h, width, height, depth, seed, frame, sample_offset,
+                                     self.LIGHT_SAMPLERS[light_sampler], _p(px), _p(sm), len(px), _p(out))
+        return out
+
+    def light_bvh(self):
+        """The reference's LightBvhTable: bounds (N, 6), power (N,), links (N, 4:
+        left, right, parent, light), leaf node per light, root (-1 if invalid)."""
+        L = _ref_pt_lib()
+        n = L.ref_pt_light_bvh(self.h, None, None, None, None, None)
+        nl = len(self.lights()["prim"])
+        out = {"bounds": np.zeros((n, 6), np.float32), "power": np.zeros(n, np.float32),
+               "links": np.zeros((n, 4), np.int32), "leaf": np.zeros(nl if n else 0, np.int32)}
+        root = np.zeros(1, np.int32)
+        if n:
+            L.ref_pt_light_bvh(self.h, _p(out["bounds"]), _p(out["power"]), _p(out["links"]), _p(out["leaf"]),
+                               _p(root))
+        out["root"] = int(root[0]) if n else -1
         return out
 
 

@@ -361,23 +361,49 @@ int32_t ref_pt_lights(const ref_pt* r, int32_t* prim, float* area, float* cdf, f
     return count;
 }
 
-static RenderSettings make_settings(int32_t depth, uint64_t seed, int32_t frame, int32_t sample_offset) {
+// light_sampler: LightSamplerKind as an int (0 Uniform, 1 Power, 2 Bvh).
+static RenderSettings make_settings(int32_t depth, uint64_t seed, int32_t frame, int32_t sample_offset,
+                                    int32_t light_sampler) {
     RenderSettings s;
     s.max_depth = depth;
     s.seed = seed;
     s.frame_index = frame;
     s.sample_offset = sample_offset;
     s.sampler_kind = SamplerKind::Independent;
-    s.light_sampler_kind = LightSamplerKind::Power;
+    s.light_sampler_kind = static_cast<LightSamplerKind>(light_sampler);
     return s;
+}
+
+// The light BVH of the scene's light sampling (LightBvhTable): node count, and
+// per node {min xyz, max xyz}, power, {left, right, parent, light index}; the
+// leaf node of each light; the root.
+int32_t ref_pt_light_bvh(const ref_pt* r, float* bounds6, float* power, int32_t* links4, int32_t* leaf,
+                         int32_t* root) {
+    const LightBvhTable& t = r->prepared->light_sampling().light_bvh;
+    const int n = int(t.nodes.size());
+    if (!bounds6) return n;
+    for (int i = 0; i < n; ++i) {
+        const LightBvhNode& x = t.nodes[i];
+        const float b[6] = {x.bounds.p_min.x, x.bounds.p_min.y, x.bounds.p_min.z,
+                            x.bounds.p_max.x, x.bounds.p_max.y, x.bounds.p_max.z};
+        for (int k = 0; k < 6; ++k) bounds6[6 * i + k] = b[k];
+        power[i] = x.power;
+        links4[4 * i] = x.left;
+        links4[4 * i + 1] = x.right;
+        links4[4 * i + 2] = x.parent;
+        links4[4 * i + 3] = x.light_index;
+    }
+    for (size_t i = 0; i < t.leaf_nodes_by_light_index.size(); ++i) leaf[i] = t.leaf_nodes_by_light_index[i];
+    *root = t.valid() ? t.root : -1;
+    return n;
 }
 
 // render_cpu_path_tracer itself; film_sum receives the per-pixel SUM over spp
 // (Film::pixels, row-major, y = 0 at the top). Its row printout is silenced.
 void ref_pt_render(const ref_pt* r, int32_t w, int32_t h, int32_t spp, int32_t depth, uint64_t seed,
-                   int32_t frame, int32_t sample_offset, float* film_sum) {
+                   int32_t frame, int32_t sample_offset, int32_t light_sampler, float* film_sum) {
     Film film(w, h, spp);
-    RenderSettings s = make_settings(depth, seed, frame, sample_offset);
+    RenderSettings s = make_settings(depth, seed, frame, sample_offset, light_sampler);
     std::fflush(stdout);
     int saved = dup(1);
     FILE* devnull = std::fopen("/dev/null", "w");
@@ -400,9 +426,10 @@ void ref_pt_render(const ref_pt* r, int32_t w, int32_t h, int32_t spp, int32_t d
 // trace_path radiance of individual (pixel, sample) pairs, with the camera
 // sample drawn exactly as render_cpu_path_tracer draws it.
 void ref_pt_samples(const ref_pt* r, int32_t w, int32_t h, int32_t depth, uint64_t seed, int32_t frame,
-                    int32_t sample_offset, const int32_t* pixels, const int32_t* samples, int32_t n, float* out) {
+                    int32_t sample_offset, int32_t light_sampler, const int32_t* pixels, const int32_t* samples,
+                    int32_t n, float* out) {
     const Scene& scene = r->prepared->host_scene();
-    RenderSettings s = make_settings(depth, seed, frame, sample_offset);
+    RenderSettings s = make_settings(depth, seed, frame, sample_offset, light_sampler);
     SceneLightSampler lights =
         make_scene_light_sampler(scene, r->prepared->light_sampling(), s.light_sampler_kind);
     for (int i = 0; i < n; ++i) {

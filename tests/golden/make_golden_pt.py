@@ -26,10 +26,16 @@ Cases:
   pt_bunny        the reference's own Stanford-bunny scene (OBJ import, a
                   placed instance, a ThinLambert bunny), 64x64, 16 spp,
                   depth 4, seed 2: film + 2048 samples
+  pt_lights_bvh   a many-light scene (scenes.many_lights: 256 small emitters,
+                  GGX and glass boxes) with the BVH light sampler, 40x40,
+                  8 spp, depth 5, seed 4: film + 2048 samples + the light BVH
+  pt_lights_uniform  the same scene with the uniform light sampler
+  pt_instanced_bvh   the instanced room with the BVH light sampler, seed 13
 Each stores the scene arrays, the camera and light tables the reference derived.
 
     python tests/golden/make_golden_pt.py instanced   # only the instanced cases
     python tests/golden/make_golden_pt.py bunny       # only pt_bunny
+    python tests/golden/make_golden_pt.py lights      # only the light-sampler cases
 """
 
 import sys
@@ -46,9 +52,9 @@ from lux_amd import scenes  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
-def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
+def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0, light_sampler="power"):
     r = oracle.RefPathTracer(scene)
-    film = r.render(w, h, spp, depth, seed=seed, frame=frame)
+    film = r.render(w, h, spp, depth, seed=seed, frame=frame, light_sampler=light_sampler)
     keys = ["vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera"]
     if "inst_mesh" in scene:
         keys += ["vertex_offsets", "triangle_offsets", "inst_mesh", "inst_m", "inst_override"]
@@ -63,7 +69,10 @@ def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0):
         px, sm = sample_pixels
         data["sample_pixels"] = px.astype(np.int32)
         data["sample_index"] = sm.astype(np.int32)
-        data["sample_radiance"] = r.samples(w, h, depth, px, sm, seed=seed, frame=frame)
+        data["sample_radiance"] = r.samples(w, h, depth, px, sm, seed=seed, frame=frame, light_sampler=light_sampler)
+    data["light_sampler"] = np.array(light_sampler)
+    for k, v in r.light_bvh().items():
+        data["light_bvh_" + k] = np.asarray(v)
     np.savez_compressed(OUT / f"{name}.npz", **data)
     print(f"{name}: {w}x{h} spp={spp} depth={depth} tris={len(scene['indices'])} mean={film.mean() / spp:.5f}")
 
@@ -95,6 +104,16 @@ def bunny_case(w=64, h=64, spp=16, depth=4, seed=2):
     print(f"pt_bunny: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
 
 
+def light_sampler_cases():
+    rng = np.random.default_rng(10)
+    px = rng.integers(0, 40 * 40, 2048)
+    sm = rng.integers(0, 8, 2048)
+    lights = scenes.many_lights(40, 40)
+    case("pt_lights_bvh", lights, 40, 40, 8, 5, 4, (px, sm), light_sampler="bvh")
+    case("pt_lights_uniform", lights, 40, 40, 8, 5, 4, (px, sm), light_sampler="uniform")
+    case("pt_instanced_bvh", scenes.instanced_room(40, 40), 40, 40, 8, 6, 13, (px, sm), light_sampler="bvh")
+
+
 def instanced_cases():
     rng = np.random.default_rng(8)
     px = rng.integers(0, 40 * 40, 2048)
@@ -111,6 +130,9 @@ def main(only=None):
         return
     if only == "bunny":
         bunny_case()
+        return
+    if only == "lights":
+        light_sampler_cases()
         return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
@@ -131,6 +153,7 @@ def main(only=None):
     case("pt_mixed_d1", mixed, 40, 40, 2, 1, 3)
     instanced_cases()
     bunny_case()
+    light_sampler_cases()
 
 
 if __name__ == "__main__":
