@@ -230,6 +230,23 @@ typedef struct lux_material {
     float roughness, metallic, eta;
 } lux_material;
 
+/* Scene lights besides the emissive triangles (Scene::lights,
+ * src/light/light.h:66-99, added by add_point_light / add_distant_light /
+ * add_uniform_infinite_light / add_image_infinite_light,
+ * src/scene/scene_build.h:9-90). They join the light table after the area
+ * lights, in this order, with the reference's power (max(1e-6, luminance) of
+ * intensity or radiance; an image's luminance x sin(theta) weight sum). Point
+ * lights enter the light BVH as points; the other kinds are the BVH sampler's
+ * non-BVH group. Infinite lights are seen by rays that leave the scene. */
+enum { LUX_LIGHT_POINT = 1, LUX_LIGHT_DISTANT = 2, LUX_LIGHT_UNIFORM_INFINITE = 3, LUX_LIGHT_IMAGE_INFINITE = 4 };
+typedef struct lux_light {
+    int32_t kind;                 /* LUX_LIGHT_* */
+    lux_vec3 vector;              /* point: position; distant: direction to the light (normalised here) */
+    lux_vec3 color;               /* point: intensity; distant, uniform infinite: radiance; image: scale */
+    int32_t image_width, image_height;
+    const lux_vec3* image_pixels; /* image infinite: width x height, row 0 at theta = 0 (+y) */
+} lux_light;
+
 /* One mesh under an identity instance (SceneMesh + SceneInstance,
  * src/scene/scene_data.h:35-58) and a pinhole camera given as
  * make_pinhole_camera's arguments {origin xyz, look_at xyz, vertical fov in
@@ -246,6 +263,8 @@ typedef struct lux_scene_desc {
     const lux_material* materials;
     int32_t material_count;
     float camera[8];
+    const lux_light* lights; /* scene lights (may be NULL) */
+    int32_t light_count;
 } lux_scene_desc;
 
 enum { LUX_RENDER_WAVEFRONT = 0, LUX_RENDER_PLAIN = 1 };
@@ -283,7 +302,8 @@ int lux_hip_scene_camera(lux_hip_scene* scene, float* out12);
  * pinhole: lens_radius 0), e.g. the Camera of a PreparedScene's Scene. */
 int lux_hip_scene_set_camera(lux_hip_scene* scene, const float* v12);
 /* Light table: count, and (if the arrays are non-NULL, sized by count) each
- * light's triangle, area, alias probability, alias index and pmf;
+ * light's triangle (-1 and area 0 for scene lights), area, alias probability,
+ * alias index and pmf;
  * light_of_prim gets one entry per triangle (-1 = not an emitter). */
 int lux_hip_scene_lights(lux_hip_scene* scene, int32_t* count, int32_t* prim, float* area, float* prob,
                          int32_t* alias, float* pmf, int32_t* light_of_prim);
@@ -492,6 +512,8 @@ typedef struct lux_instanced_scene_desc {
     const lux_material* materials;
     int32_t material_count;
     float camera[8];
+    const lux_light* lights; /* scene lights (may be NULL) */
+    int32_t light_count;
 } lux_instanced_scene_desc;
 /* PreparedScene(scene) + prepare_gpu() for an instanced scene. Scene
  * validation (src/scene/scene_validate.h:95-147) rejects with

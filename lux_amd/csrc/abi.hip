@@ -1280,8 +1280,9 @@ int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float
         const auto& L = sc->dev.host_lights;
         *count = int32_t(L.size());
         for (size_t i = 0; i < L.size(); ++i) {
-            if (prim) prim[i] = L[i].prim;
-            if (area) area[i] = L[i].area;
+            const bool is_area = L[i].kind == lux::pt::kLightArea;  // scene lights: no triangle, no area
+            if (prim) prim[i] = is_area ? L[i].prim : -1;
+            if (area) area[i] = is_area ? L[i].area : 0.0f;
             if (prob) prob[i] = L[i].prob;
             if (alias) alias[i] = L[i].alias;
             if (pmf) pmf[i] = L[i].pmf;

@@ -415,10 +415,70 @@ __device__ __forceinline__ float light_bvh_pmf(const SceneView& sv, int light, c
     return 0.0f;
 }
 
-// sample_light (src/scene/scene_light_sampler.h:99-152) for area lights: the
-// light index and its selection pmf. Uniform: u * count; BVH: the tree descent
-// at the shading point (power selection when the scene has no valid tree);
-// power: the alias table.
+// Alias-table draw over `count` entries (sample_alias_table_index,
+// src/light/light_sampler.h:71-82).
+__device__ __forceinline__ int alias_draw(const DevAlias* t, int count, float u) {
+    const float scaled = fminf(u, 0.99999994f) * float(count);
+    int col = int(scaled);
+    if (col >= count) col = count - 1;
+    const float thr = scaled - float(col);
+    return thr < t[col].prob ? col : t[col].alias;
+}
+
+// The BVH sampler with its non-BVH group (sample_light_bvh_mixture,
+// src/light/light_sampler.h:263-316): the tree's root power against the
+// group's total weight picks the tree or the group's alias table.
+__device__ __forceinline__ int sample_light_mixture(const SceneView& sv, const vec& p, float select_u, float& pmf) {
+    pmf = 0;
+    const float bvh_power = sv.light_nodes[sv.light_root].power;
+    if (bvh_power <= 0) return -1;
+    const float nb_power = sv.nb_count > 0 ? fmaxf(0.0f, sv.nb_total) : 0.0f;
+    const float total = bvh_power + nb_power;
+    if (total <= 0) return -1;
+    const float group = bvh_power / total;
+    if (nb_power <= 0 || group >= 1) return sample_light_bvh(sv, p, select_u, pmf);
+    auto draw_group = [&](float u, float& gp) -> int {
+        gp = 0;
+        if (sv.nb_count <= 0 || nb_power <= 0) return -1;
+        const int slot = alias_draw(sv.nb_table, sv.nb_count, u);
+        if (slot < 0 || slot >= sv.nb_count) return -1;
+        gp = sv.nb_table[slot].pmf;
+        return gp > 0 ? sv.nb_table[slot].light : -1;
+    };
+    if (group <= 0) return draw_group(select_u, pmf);
+    const float u = fminf(select_u, 0.99999994f);
+    if (u < group) {
+        float bp = 0;
+        const int idx = sample_light_bvh(sv, p, u / group, bp);
+        pmf = group * bp;
+        return pmf > 0 ? idx : -1;
+    }
+    const float other = 1.0f - group;
+    float gp = 0;
+    const int idx = draw_group((u - group) / other, gp);
+    pmf = other * gp;
+    return pmf > 0 ? idx : -1;
+}
+
+// light_bvh_mixture_pmf (src/light/light_sampler.h:318-358).
+__device__ __forceinline__ float light_mixture_pmf(const SceneView& sv, int light, const vec& p) {
+    if (light < 0) return 0.0f;
+    const float bvh_power = sv.light_nodes[sv.light_root].power;
+    if (bvh_power <= 0) return 0.0f;
+    const float nb_power = sv.nb_count > 0 ? fmaxf(0.0f, sv.nb_total) : 0.0f;
+    const float total = bvh_power + nb_power;
+    if (total <= 0) return 0.0f;
+    const float group = bvh_power / total;
+    if (light < sv.light_count && sv.light_leaf[light] >= 0) return group * light_bvh_pmf(sv, light, p);
+    const int slot = light < sv.light_count ? sv.lights[light].nb_slot : -1;
+    if (slot < 0 || nb_power <= 0) return 0.0f;
+    return (1.0f - group) * sv.nb_table[slot].pmf;
+}
+
+// sample_light (src/scene/scene_light_sampler.h:99-152): the light index and
+// its selection pmf. Uniform: u * count; BVH: the mixture of the tree descent
+// at the shading point and the non-BVH group (power selection when the scene
+// has no valid tree); power: the alias table.
 template <int kLight>
 __device__ __forceinline__ int choose_light(const SceneView& sv, const vec& p, float select_u, float& pmf) {
     const int nl = sv.light_count;
@@ -431,8 +491,8 @@ __device__ __forceinline__ int choose_light(const SceneView& sv, const vec& p, f
         return idx;
     } else {
         if (kLight == kLightBvh && sv.light_root >= 0) {
-            const int idx = sample_light_bvh(sv, p, select_u, pmf);
-            if (idx >= 0 && !(pmf > 0)) pmf = light_bvh_pmf(sv, idx, p);  // a descent pmf that underflowed
+            const int idx = sample_light_mixture(sv, p, select_u, pmf);
+            if (idx >= 0 && !(pmf > 0)) pmf = light_mixture_pmf(sv, idx, p);  // a descent pmf that underflowed
             return idx;
         }
         const float scaled = fminf(select_u, 0.99999994f) * float(nl);
@@ -453,9 +513,149 @@ __device__ __forceinline__ float light_choice_pmf(const SceneView& sv, int light
     if constexpr (kLight == kLightUniform) {
         return 1.0f / float(sv.light_count);
     } else {
-        if (kLight == kLightBvh && sv.light_root >= 0) return light_bvh_pmf(sv, light, ref);
+        if (kLight == kLightBvh && sv.light_root >= 0) return light_mixture_pmf(sv, light, ref);
         return sv.lights[light].pmf;
     }
+}
+
+// ---- scene lights (point, distant, uniform and image infinite) ------------------------
+// uniform_sphere_direction, the lat-long maps and image_env_lookup
+// (src/light/light.h:137-166).
+__device__ __forceinline__ vec uniform_sphere(float ux, float uy) {
+    const float z = 1.0f - 2.0f * ux;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 2.0f * kPi * uy;
+    return mk(r * cosf(phi), z, r * sinf(phi));
+}
+
+__device__ __forceinline__ void latlong_uv(const vec& wi, float& u, float& v) {
+    const vec w = normalize(wi);
+    float phi = atan2f(w.z, w.x);
+    if (phi < 0) phi += 2.0f * kPi;
+    const float theta = acosf(fmaxf(-1.0f, fminf(1.0f, w.y)));
+    u = phi * (0.5f * kInvPi);
+    v = theta * kInvPi;
+}
+
+__device__ __forceinline__ vec latlong_direction(float u, float v) {
+    const float phi = 2.0f * kPi * u;
+    const float theta = kPi * v;
+    const float st = sinf(theta);
+    return mk(cosf(phi) * st, cosf(theta), sinf(phi) * st);
+}
+
+__device__ __forceinline__ int image_texel(const DevLight& lt, float u, float v) {
+    const int w = lt.two_sided, h = lt.height;
+    const int x = int(fminf(float(w - 1), floorf(u * float(w))));
+    const int y = int(fminf(float(h - 1), floorf(v * float(h))));
+    return y * w + x;
+}
+
+__device__ __forceinline__ vec image_lookup(const SceneView& sv, const DevLight& lt, const vec& wi) {
+    if (lt.two_sided <= 0 || lt.height <= 0) return splat(0);
+    float u, v;
+    latlong_uv(wi, u, v);
+    const float* px = sv.env_pixels + 3 * size_t(lt.prim + image_texel(lt, u, v));
+    return mk(px[0], px[1], px[2]) * mk(lt.er, lt.eg, lt.eb);
+}
+
+// image_infinite_pdf_for_texel (src/light/light.h:205-222).
+__device__ __forceinline__ float image_texel_pdf(const SceneView& sv, const DevLight& lt, int index, float v) {
+    const int w = lt.two_sided, h = lt.height;
+    const float total = lt.area;
+    if (w <= 0 || h <= 0 || total <= 0) return 0.0f;
+    const int count = w * h;
+    if (index < 0 || index >= count) return 0.0f;
+    const float* cdf = sv.env_cdf + lt.prim;
+    const float weight = index == 0 ? cdf[0] : cdf[index] - cdf[index - 1];
+    if (weight <= 0) return 0.0f;
+    const float sin_theta = fmaxf(1e-6f, sinf(kPi * v));
+    const float uv_pdf = (weight / total) * float(count);
+    return uv_pdf / (2.0f * kPi * kPi * sin_theta);
+}
+
+// Radiance arriving along a ray that leaves the scene from one infinite light
+// (infinite_light_le) and its solid-angle pdf for that direction (light_pdf_li).
+__device__ __forceinline__ vec env_le(const SceneView& sv, const DevLight& lt, const vec& d) {
+    if (lt.kind == kLightUniformInf) return mk(lt.er, lt.eg, lt.eb);
+    if (lt.kind == kLightImageInf) return image_lookup(sv, lt, d);
+    return splat(0);
+}
+
+__device__ __forceinline__ float env_pdf(const SceneView& sv, const DevLight& lt, const vec& d) {
+    if (lt.kind == kLightUniformInf) return 1.0f / (4.0f * kPi);
+    if (lt.kind == kLightImageInf) {
+        if (lt.two_sided <= 0 || lt.height <= 0 || lt.area <= 0) return 0.0f;
+        float u, v;
+        latlong_uv(d, u, v);
+        return image_texel_pdf(sv, lt, image_texel(lt, u, v), v);
+    }
+    return 0.0f;
+}
+
+// A direct-light sample of a scene light (sample_light_li, src/light/light.h:
+// 166-193, 224-253): direction, distance (INFINITY for distant and infinite
+// lights), radiance, pdf before the selection pmf, delta flag, light position.
+struct SceneLightSample {
+    vec wi, rad, pos;
+    float dist, pdf;
+    bool delta, valid;
+};
+
+__device__ __forceinline__ SceneLightSample sample_scene_light(const SceneView& sv, const DevLight& lt, const vec& p,
+                                                               float comp_u, float lu0, float lu1) {
+    SceneLightSample r{};
+    r.valid = false;
+    if (lt.kind == kLightPoint) {
+        r.pos = mk(lt.vx, lt.vy, lt.vz);
+        const vec to = r.pos - p;
+        const float d2 = length2(to);
+        if (d2 <= kEps * kEps) return r;
+        r.dist = sqrtf(d2);
+        r.wi = to / r.dist;
+        r.rad = mk(lt.er, lt.eg, lt.eb) / d2;
+        r.pdf = 1;
+        r.delta = true;
+        r.valid = max_component(r.rad) > 0;
+    } else if (lt.kind == kLightDistant) {
+        r.wi = normalize(mk(lt.vx, lt.vy, lt.vz));
+        r.dist = INFINITY;
+        r.rad = mk(lt.er, lt.eg, lt.eb);
+        r.pdf = 1;
+        r.delta = true;
+        r.valid = max_component(r.rad) > 0;
+    } else if (lt.kind == kLightUniformInf) {
+        r.wi = uniform_sphere(lu0, lu1);
+        r.dist = INFINITY;
+        r.rad = mk(lt.er, lt.eg, lt.eb);
+        r.pdf = 1.0f / (4.0f * kPi);
+        r.delta = false;
+        r.valid = max_component(r.rad) > 0;
+    } else if (lt.kind == kLightImageInf) {
+        const int w = lt.two_sided, h = lt.height;
+        if (w <= 0 || h <= 0 || lt.area <= 0) return r;
+        const int count = w * h;
+        const float target = fminf(comp_u, 0.99999994f) * lt.area;
+        const float* cdf = sv.env_cdf + lt.prim;
+        int lo = 0, hi = count - 1;  // first texel with target <= cdf (the cdf never decreases)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (target <= cdf[mid])
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int index = lo;
+        const int x = index % w, y = index / w;
+        const float u = (float(x) + lu0) / float(w), v = (float(y) + lu1) / float(h);
+        r.wi = latlong_direction(u, v);
+        r.dist = INFINITY;
+        r.rad = image_lookup(sv, lt, r.wi);
+        r.pdf = image_texel_pdf(sv, lt, index, v);
+        r.delta = false;
+        r.valid = max_component(r.rad) > 0 && r.pdf > 0;
+    }
+    return r;
 }
 
 // ---- shade -----------------------------------------------------------------------
@@ -607,7 +807,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
                 Interaction it{p, ng, ns, wo};
                 if (direct_lobe(m)) {
                     float sel = rng.get1();
-                    rng.get1();  // light component (unused by area lights)
+                    float comp_u = rng.get1();  // light component (image lights pick their texel with it)
                     float lu0 = rng.get1();
                     float lu1 = rng.get1();
                     // sample_light_li: light pick, area sample (src/scene/scene_light_sampler.h:210-250)
@@ -617,39 +817,62 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
                         const int idx = choose_light<kLight>(sv, p, sel, sel_pmf);
                         if (idx >= 0 && idx < nl && sel_pmf > 0) {
                             const DevLight& lt = sv.lights[idx];
-                            const lux_triangle lt_tri = sv.soup[lt.prim];
-                            float su = sqrtf(lu0);
-                            float b0 = 1.0f - su, b1 = su * (1.0f - lu1), b2 = su * lu1;
-                            vec lp = b0 * tri_vertex(lt_tri, 0) + b1 * tri_vertex(lt_tri, 1) + b2 * tri_vertex(lt_tri, 2);
-                            vec ln = tri_normal(lt_tri);
-                            float pdf_area = lt.area > 0 ? 1.0f / lt.area : 0.0f;
-                            vec to = lp - p;
-                            float d2 = length2(to);
-                            if (d2 > kEps * kEps) {
-                                float dist = sqrtf(d2);
-                                vec wi = to / dist;
-                                vec rad = eval_emission(mk(lt.er, lt.eg, lt.eb), lt.two_sided, ln, -wi);
-                                float lpdf = sel_pmf * pdf_solid_angle(lp, ln, lt.two_sided, pdf_area, p);
-                                if (max_component(rad) > 0 && lpdf > 0) {
-                                    // estimate_unoccluded_direct_light_li (src/light/light_sampler.h:420-454)
-                                    vec f = eval_bsdf(m, it, wi);
-                                    float cs = abs_cos(ns, wi);
-                                    float pdf_b = pdf_bsdf(m, it, wi);
-                                    if (max_component(f) > 0 && cs > 0) {
-                                        vec off = dot(wi, ng) >= 0 ? ng : -ng;
-                                        vec so = p + off * kEps;
-                                        vec tl = lp - so;
+                            vec l_wi = splat(0), l_rad = splat(0), l_target = splat(0);
+                            float l_dist = 0, lpdf = 0;
+                            bool l_delta = false, l_ok = false;
+                            if (lt.kind == kLightArea) {
+                                const lux_triangle lt_tri = sv.soup[lt.prim];
+                                float su = sqrtf(lu0);
+                                float b0 = 1.0f - su, b1 = su * (1.0f - lu1), b2 = su * lu1;
+                                vec lp = b0 * tri_vertex(lt_tri, 0) + b1 * tri_vertex(lt_tri, 1) +
+                                         b2 * tri_vertex(lt_tri, 2);
+                                vec ln = tri_normal(lt_tri);
+                                float pdf_area = lt.area > 0 ? 1.0f / lt.area : 0.0f;
+                                vec to = lp - p;
+                                float d2 = length2(to);
+                                if (d2 > kEps * kEps) {
+                                    l_dist = sqrtf(d2);
+                                    l_wi = to / l_dist;
+                                    l_rad = eval_emission(mk(lt.er, lt.eg, lt.eb), lt.two_sided, ln, -l_wi);
+                                    lpdf = sel_pmf * pdf_solid_angle(lp, ln, lt.two_sided, pdf_area, p);
+                                    l_target = lp;
+                                    l_ok = max_component(l_rad) > 0 && lpdf > 0;
+                                }
+                            } else {
+                                const SceneLightSample ls = sample_scene_light(sv, lt, p, comp_u, lu0, lu1);
+                                l_wi = ls.wi;
+                                l_rad = ls.rad;
+                                l_dist = ls.dist;
+                                l_target = ls.pos;
+                                l_delta = ls.delta;
+                                lpdf = ls.pdf * sel_pmf;
+                                l_ok = ls.valid && lpdf > 0;
+                            }
+                            if (l_ok) {
+                                // estimate_unoccluded_direct_light_li (src/light/light_sampler.h:420-454)
+                                vec f = eval_bsdf(m, it, l_wi);
+                                float cs = abs_cos(ns, l_wi);
+                                float pdf_b = pdf_bsdf(m, it, l_wi);
+                                if (max_component(f) > 0 && cs > 0) {
+                                    vec off = dot(l_wi, ng) >= 0 ? ng : -ng;
+                                    vec so = p + off * kEps;
+                                    if (l_dist < INFINITY) {  // trace_shadow to the light point
+                                        vec tl = l_target - so;
                                         float sdist = length(tl);
                                         if (sdist > kEps) {
-                                            sh_o = so;
                                             sh_d = tl / sdist;
                                             sh_tmax = sdist - kEps;
-                                            sh_c = f * rad * (cs / lpdf);
-                                            sh_mis = power_heuristic(lpdf, pdf_b);
-                                            sh_T = T;
                                             push_shadow = true;
                                         }
+                                    } else {  // trace_shadow_ray along the light direction
+                                        sh_d = l_wi;
+                                        sh_tmax = INFINITY;
+                                        push_shadow = true;
                                     }
+                                    sh_o = so;
+                                    sh_c = f * l_rad * (cs / lpdf);
+                                    sh_mis = l_delta ? 1.0f : power_heuristic(lpdf, pdf_b);
+                                    sh_T = T;
                                 }
                             }
                         }
@@ -691,6 +914,34 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_s
             ps.ro[path] = pack(o, oq.w);
             ps.rd[path] = pack(d, dq.w);
             ps.rng[path] = make_ulonglong2(rng.state, rng.inc);
+        }
+        if (live && prim < 0 && sg->sv.env_count > 0) {
+            // a ray that left the scene: infinite lights (trace_path's miss branch,
+            // src/integrator/cpu_path_tracer.cpp:416-427)
+            const SceneView& sv = sg->sv;
+            const float4 Tq = ps.T[path], Lq = ps.L[path], pq = ps.prev[path], dq = ps.rd[path];
+            const vec d = xyz(dq);
+            vec Le = splat(0);
+            for (int k = 0; k < sv.env_count; ++k) Le = Le + env_le(sv, sv.lights[sv.env_lights[k]], d);
+            if (max_component(Le) > 0) {
+                const uint32_t pbits = __float_as_uint(pq.w);
+                const unsigned prev_flags = pbits & 0xffu;
+                const int prev_measure = int((pbits >> 8) & 0xffu);
+                const int depth = int(pbits >> 16);
+                const float prev_pdf = Lq.w;
+                vec L = xyz(Lq);
+                if (depth == 0 || is_delta(prev_flags)) {
+                    L = L + xyz(Tq) * Le;
+                } else if (prev_measure == kSolidAngle && prev_pdf > 0) {
+                    float lpdf = 0;  // infinite_lights_pdf_li: selection pmf (at the origin) x direction pdf
+                    for (int k = 0; k < sv.env_count; ++k) {
+                        const int li = sv.env_lights[k];
+                        lpdf += light_choice_pmf<kLight>(sv, li, splat(0)) * env_pdf(sv, sv.lights[li], d);
+                    }
+                    L = L + xyz(Tq) * Le * power_heuristic(prev_pdf, lpdf);
+                }
+                ps.L[path] = pack(L, prev_pdf);
+            }
         }
         uint32_t slot = wave_append(&qs.counts[in ^ 1], push_ray);
         if (push_ray) qs.q[in ^ 1][slot] = path;

@@ -60,15 +60,32 @@ struct DevMaterial {
     int pad;
 };
 
+// Light kinds of the light table (LightKind, src/light/light.h:22-33).
+constexpr int kLightArea = 0, kLightPoint = 1, kLightDistant = 2, kLightUniformInf = 3, kLightImageInf = 4;
+
+// One entry of the light table (64 bytes).
 struct DevLight {
-    int prim;
-    float er, eg, eb;
-    int two_sided;
-    float area;
-    float prob;  // alias-table column probability
+    int prim;          // area: triangle ordinal; image: first texel of its image
+    float er, eg, eb;  // area: emission; point: intensity; distant, uniform: radiance; image: scale
+    int two_sided;     // area: double-sided emission; image: width
+    float area;        // area: area; image: total texel weight
+    float prob;        // power alias table: column probability, alias, pmf
     int alias;
     float pmf;
-    int pad[3];
+    int kind;          // kLight*
+    int height;        // image: height
+    int nb_slot;       // slot in the BVH sampler's non-BVH table (-1: in the tree, or no tree)
+    float vx, vy, vz;  // point: position; distant: direction to the light
+    int pad;
+};
+static_assert(sizeof(DevLight) == 64, "light record");
+
+// Alias-table entry of the BVH sampler's non-BVH group (lights without bounds).
+struct DevAlias {
+    float prob;
+    int alias;
+    float pmf;
+    int light;  // its index in the light table
 };
 
 // Light-BVH node (LightBvhNode, src/light/light_distribution.h:33-40): bounds,

@@ -46,8 +46,22 @@ struct SceneDev {
     DevCamera camera{};
     std::vector<DevLight> host_lights;
     std::vector<int32_t> host_light_of_prim;
+    // scene lights besides the emissive triangles, and the texels of image lights
+    std::vector<lux_light> host_scene_lights;
+    std::vector<float> host_env_pixels, host_env_cdf;
+    std::vector<int32_t> host_env_first;  // per scene light: first texel (images), else -1
+    std::vector<float> host_env_total;    // per scene light: total texel weight (images)
+    float* env_pixels = nullptr;
+    float* env_cdf = nullptr;
     // the BVH light sampler's tree over the lights (rebuilt with the light table)
     std::vector<float> host_light_power;
+    std::vector<uint8_t> host_light_in_bvh;
+    std::vector<DevAlias> host_nb_table;
+    float host_nb_total = 0;
+    std::vector<int32_t> host_env_lights;
+    DevAlias* nb_table = nullptr;
+    int32_t* env_light_list = nullptr;
+    int nb_capacity = 0, env_list_capacity = 0;
     std::vector<lux_triangle> host_light_tri;
     std::vector<DevLightNode> host_light_nodes;
     std::vector<int32_t> host_light_leaf;
@@ -90,6 +104,13 @@ struct SceneView {
     const DevLightNode* light_nodes;  // BVH light sampler: tree and leaf of each light
     const int32_t* light_leaf;
     int32_t light_node_count, light_root;  // root -1: no valid tree (power selection is used)
+    const DevAlias* nb_table;              // the BVH sampler's non-BVH group
+    int32_t nb_count;
+    float nb_total;
+    const int32_t* env_lights;             // infinite lights, in light-table order
+    int32_t env_count;
+    const float* env_pixels;               // image texels (RGB), all images back to back
+    const float* env_cdf;                  // per-texel cumulative weights
     int32_t opaque;     // 1: every hit blocks a shadow ray (no pass-through material)
     int32_t instanced;  // 1: traverse `tl` (two-level) instead of `nodes`
     DevCamera cam;

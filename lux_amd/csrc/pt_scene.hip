@@ -123,19 +123,70 @@ lux_vec3 world_point(const float* m, const lux_vec3& p) {
     return lux_vec3{x / w, y / w, z / w};
 }
 
+// build_alias_table_from_cdf (src/light/light_distribution.cpp:8-64, 163-175):
+// weights are the cdf differences; the same small/large stack order, so the
+// same aliases.
+void alias_from_cdf(const std::vector<float>& cdf, std::vector<float>& prob, std::vector<int>& alias,
+                    std::vector<float>& pmf_out) {
+    const int count = int(cdf.size());
+    prob.assign(count, 1.0f);
+    alias.resize(count);
+    pmf_out.assign(count, 0.0f);
+    std::vector<float> w(count);
+    float prev = 0;
+    for (int i = 0; i < count; ++i) {
+        w[i] = cdf[i] - prev;
+        prev = cdf[i];
+    }
+    float sum = 0;
+    for (float x : w) sum += fmaxf(0.0f, x);
+    std::vector<float> scaled(count);
+    std::vector<int> small, large;
+    for (int i = 0; i < count; ++i) {
+        const float pmf = fmaxf(0.0f, w[i]) / sum;
+        pmf_out[i] = pmf;
+        alias[i] = i;
+        prob[i] = 1;
+        scaled[i] = pmf * float(count);
+        (scaled[i] < 1.0f ? small : large).push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        const int l = small.back();
+        small.pop_back();
+        const int g = large.back();
+        large.pop_back();
+        prob[l] = scaled[l];
+        alias[l] = g;
+        scaled[g] = (scaled[g] + scaled[l]) - 1.0f;
+        (scaled[g] < 1.0f ? small : large).push_back(g);
+    }
+    for (int i : large) {
+        prob[i] = 1;
+        alias[i] = i;
+    }
+    for (int i : small) {
+        prob[i] = 1;
+        alias[i] = i;
+    }
+}
+
 // Power light selection with its alias table: prepare_scene_light_sampling
 // (src/scene/scene_prepare.h:268-387) and build_alias_table_from_weights
 // (src/light/light_distribution.cpp:8-64; same stack order, so the same aliases).
 // `emitters` lists the triangles whose material emits, in triangle order, with
 // their current vertices; the other triangles contribute nothing to the table.
-void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std::vector<lux_triangle>& tris,
-                  const std::vector<lux_material>& emitter_mats, std::vector<DevLight>& lights,
-                  std::vector<int32_t>& light_of_prim, std::vector<float>& light_power,
-                  std::vector<lux_triangle>& light_tri) {
+void build_lights(SceneDev& s, int tri_count, const std::vector<int32_t>& emitters,
+                  const std::vector<lux_triangle>& tris, const std::vector<lux_material>& emitter_mats) {
+    std::vector<DevLight>& lights = s.host_lights;
+    std::vector<int32_t>& light_of_prim = s.host_light_of_prim;
+    std::vector<float>& light_power = s.host_light_power;
+    std::vector<lux_triangle>& light_tri = s.host_light_tri;
+    std::vector<uint8_t>& in_bvh = s.host_light_in_bvh;
     light_of_prim.assign(tri_count, -1);
     lights.clear();
     light_power.clear();
     light_tri.clear();
+    in_bvh.clear();
     std::vector<float> cdf;
     float total = 0;
     for (size_t e = 0; e < emitters.size(); ++e) {
@@ -160,55 +211,63 @@ void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std
         L.eb = m.emission.z;
         L.two_sided = m.emission_two_sided;
         L.area = area;
+        L.kind = kLightArea;
+        L.nb_slot = -1;
         lights.push_back(L);
         light_power.push_back(power);
         light_tri.push_back(tris[e]);
+        in_bvh.push_back(1);
         cdf.push_back(total);
         int32_t& stored = light_of_prim[t];
         stored = stored == -1 ? int32_t(lights.size()) - 1 : -2;
+    }
+    // scene lights, in order (prepare_scene_light_sampling's second loop)
+    for (size_t k = 0; k < s.host_scene_lights.size(); ++k) {
+        const lux_light& x = s.host_scene_lights[k];
+        DevLight L{};
+        L.kind = x.kind;
+        L.nb_slot = -1;
+        L.prim = -1;
+        L.er = x.color.x;
+        L.eg = x.color.y;
+        L.eb = x.color.z;
+        L.vx = x.vector.x;
+        L.vy = x.vector.y;
+        L.vz = x.vector.z;
+        const float lum = float(0.2126) * x.color.x + float(0.7152) * x.color.y + float(0.0722) * x.color.z;
+        float power = fmaxf(1e-6f, lum);
+        lux_triangle pt{};
+        if (x.kind == kLightImageInf) {
+            L.prim = s.host_env_first[k];
+            L.two_sided = x.image_width;
+            L.height = x.image_height;
+            L.area = s.host_env_total[k];
+            power = fmaxf(1e-6f, s.host_env_total[k]);
+        }
+        if (x.kind == kLightPoint) pt = lux_triangle{x.vector, x.vector, x.vector};  // AABB(position)
+        if (!(power > 0)) continue;
+        total += power;
+        lights.push_back(L);
+        light_power.push_back(power);
+        light_tri.push_back(pt);
+        in_bvh.push_back(x.kind == kLightPoint ? 1 : 0);
+        cdf.push_back(total);
     }
     int count = int(lights.size());
     if (count == 0 || !(total > 0)) {
         lights.clear();
         light_power.clear();
         light_tri.clear();
+        in_bvh.clear();
         return;
     }
-    std::vector<float> w(count);
-    float prev = 0;
+    std::vector<float> prob, pmf;
+    std::vector<int> alias;
+    alias_from_cdf(cdf, prob, alias, pmf);
     for (int i = 0; i < count; ++i) {
-        w[i] = cdf[i] - prev;
-        prev = cdf[i];
-    }
-    float sum = 0;
-    for (float x : w) sum += fmaxf(0.0f, x);
-    std::vector<float> scaled(count);
-    std::vector<int> small, large;
-    for (int i = 0; i < count; ++i) {
-        float pmf = fmaxf(0.0f, w[i]) / sum;
-        lights[i].pmf = pmf;
-        lights[i].alias = i;
-        lights[i].prob = 1;
-        scaled[i] = pmf * float(count);
-        (scaled[i] < 1.0f ? small : large).push_back(i);
-    }
-    while (!small.empty() && !large.empty()) {
-        int l = small.back();
-        small.pop_back();
-        int g = large.back();
-        large.pop_back();
-        lights[l].prob = scaled[l];
-        lights[l].alias = g;
-        scaled[g] = (scaled[g] + scaled[l]) - 1.0f;
-        (scaled[g] < 1.0f ? small : large).push_back(g);
-    }
-    for (int i : large) {
-        lights[i].prob = 1;
-        lights[i].alias = i;
-    }
-    for (int i : small) {
-        lights[i].prob = 1;
-        lights[i].alias = i;
+        lights[i].prob = prob[i];
+        lights[i].alias = alias[i];
+        lights[i].pmf = pmf[i];
     }
 }
 
@@ -218,23 +277,39 @@ void build_lights(int tri_count, const std::vector<int32_t>& emitters, const std
 // centroid of the widest centroid axis with std::nth_element, nodes in the
 // order the recursion creates them (preorder), interior power and bounds the
 // sums / unions of the two children. Items are the lights in table order.
-void build_light_bvh(const std::vector<float>& power, const std::vector<lux_triangle>& tri,
-                     std::vector<DevLightNode>& nodes, std::vector<int32_t>& leaf_of_light, int& root) {
+void build_light_bvh(SceneDev& s) {
+    const std::vector<float>& power = s.host_light_power;
+    const std::vector<lux_triangle>& tri = s.host_light_tri;
+    std::vector<DevLightNode>& nodes = s.host_light_nodes;
+    std::vector<int32_t>& leaf_of_light = s.host_light_leaf;
+    int& root = s.light_root;
     nodes.clear();
     leaf_of_light.clear();
     root = -1;
+    s.host_nb_table.clear();
+    s.host_nb_total = 0;
+    s.host_env_lights.clear();
+    for (size_t i = 0; i < s.host_lights.size(); ++i) {
+        s.host_lights[i].nb_slot = -1;
+        const int kind = s.host_lights[i].kind;
+        if (kind == kLightUniformInf || kind == kLightImageInf) s.host_env_lights.push_back(int32_t(i));
+    }
     struct Item {
         float lo[3], hi[3], c[3];
         int light;
         float power;
     };
-    std::vector<Item> items(power.size());
-    for (size_t i = 0; i < power.size(); ++i) {
+    std::vector<size_t> members;  // lights with bounds (area, point), in light order
+    for (size_t i = 0; i < power.size(); ++i)
+        if (s.host_light_in_bvh[i]) members.push_back(i);
+    std::vector<Item> items(members.size());
+    for (size_t j = 0; j < members.size(); ++j) {
+        const size_t i = members[j];
         const lux_triangle& t = tri[i];
         const float xs[3] = {t.v0.x, t.v1.x, t.v2.x}, ys[3] = {t.v0.y, t.v1.y, t.v2.y},
                     zs[3] = {t.v0.z, t.v1.z, t.v2.z};
         const float* axes[3] = {xs, ys, zs};
-        Item& it = items[i];
+        Item& it = items[j];
         for (int a = 0; a < 3; ++a) {
             it.lo[a] = fminf(fminf(axes[a][0], axes[a][1]), axes[a][2]);
             it.hi[a] = fmaxf(fmaxf(axes[a][0], axes[a][1]), axes[a][2]);
@@ -242,11 +317,50 @@ void build_light_bvh(const std::vector<float>& power, const std::vector<lux_tria
         }
         it.light = int(i);
         it.power = power[i];
-        if (!(it.power > 0) || it.lo[0] > it.hi[0] || it.lo[1] > it.hi[1] || it.lo[2] > it.hi[2]) return;
+        if (!(it.power > 0) || it.lo[0] > it.hi[0] || it.lo[1] > it.hi[1] || it.lo[2] > it.hi[2]) {
+            s.host_nb_table.clear();
+            s.host_nb_total = 0;
+            for (DevLight& L : s.host_lights) L.nb_slot = -1;
+            return;
+        }
     }
-    if (items.empty()) return;
+    if (items.empty() || power.empty()) return;
+    // the non-BVH group: the other lights' weights (power-cdf differences) and
+    // their own alias table (build_light_bvh, src/light/light_distribution.cpp:90-106)
+    {
+        std::vector<float> cdf(power.size());
+        float total = 0;
+        for (size_t i = 0; i < power.size(); ++i) cdf[i] = (total += power[i]);
+        std::vector<float> nb_cdf;
+        std::vector<int32_t> nb_light;
+        float nb_weight = 0;
+        for (size_t i = 0; i < power.size(); ++i) {
+            if (s.host_light_in_bvh[i]) continue;
+            const float previous = i == 0 ? 0.0f : cdf[i - 1];
+            const float weight = cdf[i] - previous;
+            if (weight <= 0) continue;
+            nb_weight += weight;
+            nb_light.push_back(int32_t(i));
+            nb_cdf.push_back(nb_weight);
+        }
+        s.host_nb_total = nb_weight;
+        if (!nb_cdf.empty() && nb_weight > 0) {
+            std::vector<float> prob, pmf;
+            std::vector<int> alias;
+            alias_from_cdf(nb_cdf, prob, alias, pmf);
+            for (size_t k = 0; k < nb_cdf.size(); ++k) {
+                s.host_nb_table.push_back(DevAlias{prob[k], alias[k], pmf[k], nb_light[k]});
+                s.host_lights[size_t(nb_light[k])].nb_slot = int(k);
+            }
+        } else {
+            for (size_t k = 0; k < nb_cdf.size(); ++k) {
+                s.host_nb_table.push_back(DevAlias{1.0f, int(k), 0.0f, nb_light[k]});
+                s.host_lights[size_t(nb_light[k])].nb_slot = int(k);
+            }
+        }
+    }
     nodes.reserve(2 * items.size() - 1);
-    leaf_of_light.assign(items.size(), -1);
+    leaf_of_light.assign(power.size(), -1);
     auto build = [&](auto&& self, int begin, int end, int parent) -> int {
         const int index = int(nodes.size());
         nodes.push_back(DevLightNode{});
@@ -303,7 +417,7 @@ void emitter_triangles(const SceneDev& s, const lux_vec3* vertices, std::vector<
 }
 
 void upload_lights(SceneDev& s, hipStream_t stream) {
-    host::build_light_bvh(s.host_light_power, s.host_light_tri, s.host_light_nodes, s.host_light_leaf, s.light_root);
+    host::build_light_bvh(s);
     const size_t nn = s.host_light_nodes.size(), nl = s.host_light_leaf.size();
     if (!s.light_nodes || s.light_node_capacity < int(nn) || s.light_leaf_capacity < int(nl)) {
         if (s.light_nodes) (void)hipFree(s.light_nodes);
@@ -333,6 +447,93 @@ void upload_lights(SceneDev& s, hipStream_t stream) {
     hip_check(hipMemcpyAsync(s.light_of_prim, s.host_light_of_prim.data(), 4 * size_t(s.tri_count),
                              hipMemcpyHostToDevice, stream),
               "H2D light map");
+    const size_t nb = s.host_nb_table.size(), ne = s.host_env_lights.size();
+    if (!s.nb_table || s.nb_capacity < int(nb)) {
+        if (s.nb_table) (void)hipFree(s.nb_table);
+        s.nb_capacity = int(std::max<size_t>(nb, 1));
+        s.nb_table = dalloc<DevAlias>(size_t(s.nb_capacity));
+    }
+    if (!s.env_light_list || s.env_list_capacity < int(ne)) {
+        if (s.env_light_list) (void)hipFree(s.env_light_list);
+        s.env_list_capacity = int(std::max<size_t>(ne, 1));
+        s.env_light_list = dalloc<int32_t>(size_t(s.env_list_capacity));
+    }
+    if (nb)
+        hip_check(hipMemcpyAsync(s.nb_table, s.host_nb_table.data(), sizeof(DevAlias) * nb, hipMemcpyHostToDevice,
+                                 stream),
+                  "H2D non-BVH lights");
+    if (ne)
+        hip_check(hipMemcpyAsync(s.env_light_list, s.host_env_lights.data(), 4 * ne, hipMemcpyHostToDevice, stream),
+                  "H2D infinite lights");
+}
+
+// Scene lights of a descriptor: checked as validate_scene_shape checks them
+// (src/scene/scene_validate.h:149-190), kept on the host, image texels and
+// their weights (make_image_light_asset, src/scene/scene_build.h:39-64: luminance
+// of pixel x scale times sin(theta) at the row centre, host sinf) uploaded once.
+void prepare_scene_lights(SceneDev& s, const lux_light* lights, int count, hipStream_t stream) {
+    if (count < 0 || (count > 0 && !lights)) throw Error(LUX_ERR_INVALID_ARGUMENT, "bad scene light array");
+    auto finite3 = [](const lux_vec3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); };
+    auto nonneg3 = [](const lux_vec3& v) { return v.x >= 0 && v.y >= 0 && v.z >= 0; };
+    s.host_scene_lights.clear();
+    s.host_env_pixels.clear();
+    s.host_env_cdf.clear();
+    s.host_env_first.clear();
+    s.host_env_total.clear();
+    for (int k = 0; k < count; ++k) {
+        lux_light x = lights[k];
+        const std::string which = "scene light " + std::to_string(k);
+        if (x.kind < kLightPoint || x.kind > kLightImageInf) throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": unknown kind");
+        if (!finite3(x.color) || !nonneg3(x.color))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": colour must be finite and non-negative");
+        if ((x.kind == kLightPoint || x.kind == kLightDistant) && !finite3(x.vector))
+            throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": position / direction must be finite");
+        int32_t first = -1;
+        float total = 0;
+        if (x.kind == kLightDistant) {  // add_distant_light normalises the direction
+            host::V d = host::mkv(x.vector.x, x.vector.y, x.vector.z);
+            if (host::vdot(d, d) == 0) throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": direction must be non-zero");
+            d = host::vnorm(d);
+            x.vector = lux_vec3{d.x, d.y, d.z};
+        }
+        if (x.kind == kLightImageInf) {
+            const int w = x.image_width, h = x.image_height;
+            if (w <= 0 || h <= 0 || !x.image_pixels)
+                throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": image needs width, height and pixels");
+            first = int32_t(s.host_env_cdf.size());
+            float sum = 0;
+            for (int y = 0; y < h; ++y) {
+                const float theta = kPi * ((float(y) + 0.5f) / float(h));
+                const float sin_theta = fmaxf(0.0f, sinf(theta));
+                for (int xx = 0; xx < w; ++xx) {
+                    const lux_vec3& px = x.image_pixels[size_t(y) * w + xx];
+                    if (!finite3(px)) throw Error(LUX_ERR_INVALID_ARGUMENT, which + ": non-finite texel");
+                    const float r = px.x * x.color.x, g = px.y * x.color.y, b = px.z * x.color.z;
+                    const float lum = float(0.2126) * r + float(0.7152) * g + float(0.0722) * b;
+                    sum += fmaxf(0.0f, lum) * sin_theta;
+                    s.host_env_cdf.push_back(sum);
+                    s.host_env_pixels.insert(s.host_env_pixels.end(), {px.x, px.y, px.z});
+                }
+            }
+            total = sum;
+            x.image_pixels = nullptr;  // the texels now live in host_env_pixels
+        }
+        s.host_scene_lights.push_back(x);
+        s.host_env_first.push_back(first);
+        s.host_env_total.push_back(total);
+    }
+    if (s.env_pixels) (void)hipFree(s.env_pixels);
+    if (s.env_cdf) (void)hipFree(s.env_cdf);
+    s.env_pixels = dalloc<float>(std::max<size_t>(s.host_env_pixels.size(), 1));
+    s.env_cdf = dalloc<float>(std::max<size_t>(s.host_env_cdf.size(), 1));
+    if (!s.host_env_cdf.empty()) {
+        hip_check(hipMemcpyAsync(s.env_pixels, s.host_env_pixels.data(), 4 * s.host_env_pixels.size(),
+                                 hipMemcpyHostToDevice, stream),
+                  "H2D environment texels");
+        hip_check(hipMemcpyAsync(s.env_cdf, s.host_env_cdf.data(), 4 * s.host_env_cdf.size(), hipMemcpyHostToDevice,
+                                 stream),
+                  "H2D environment weights");
+    }
 }
 
 size_t bvh_bytes(int n) {
@@ -405,8 +606,8 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
     emitter_triangles(s, desc.vertices, etris);
     std::vector<lux_material> emats(s.emitters.size());
     for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
-    host::build_lights(nt, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
-                      s.host_light_tri);
+    prepare_scene_lights(s, desc.lights, desc.light_count, stream);
+    host::build_lights(s, nt, s.emitters, etris, emats);
 
     s.soup = dalloc<lux_triangle>(nt);
     s.vertices = dalloc<lux_vec3>(nv);
@@ -451,8 +652,7 @@ void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_cou
     emitter_triangles(s, vertices, etris);
     std::vector<lux_material> emats(s.emitters.size());
     for (size_t e = 0; e < s.emitters.size(); ++e) emats[e] = s.host_mats[s.emitter_mat[e]];
-    host::build_lights(s.tri_count, s.emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
-                      s.host_light_tri);
+    host::build_lights(s, s.tri_count, s.emitters, etris, emats);
     // stream order: renders enqueued before this update read the old geometry
     hip_check(hipMemcpyAsync(s.vertices, vertices, sizeof(lux_vec3) * size_t(vertex_count), hipMemcpyHostToDevice,
                              stream),
@@ -552,8 +752,7 @@ void flatten_instances(SceneDev& s, const lux_instance* instances, int count, hi
     }
     std::vector<lux_material> emats(emitters.size());
     for (size_t e = 0; e < emitters.size(); ++e) emats[e] = s.host_mats[emitter_mat[e]];
-    host::build_lights(T, emitters, etris, emats, s.host_lights, s.host_light_of_prim, s.host_light_power,
-                      s.host_light_tri);
+    host::build_lights(s, T, emitters, etris, emats);
     s.emitters = emitters;
     s.emitter_mat = emitter_mat;
 
@@ -653,6 +852,7 @@ void scene_upload_instanced(SceneDev& s, const lux_instanced_scene_desc& d, hipS
             obj[t] = lux_triangle{V[ix[0]], V[ix[1]], V[ix[2]]};
         }
     try {
+        prepare_scene_lights(n, d.lights, d.light_count, stream);
         two_level_build(n.tl, obj.data(), d.triangle_offsets, nm, inst.data(), d.instance_count, stream);
         n.mats = dalloc<DevMaterial>(nmat);
         hip_check(hipMemcpyAsync(n.mats, mats.data(), sizeof(DevMaterial) * nmat, hipMemcpyHostToDevice, stream),
@@ -694,7 +894,7 @@ void scene_set_instances(SceneDev& s, const lux_instance* instances, int count, 
 
 void scene_release(SceneDev& s) {
     void* ptrs[] = {s.soup, s.vertices, s.indices, s.normals, s.mat_id, s.mats, s.light_of_prim, s.lights,
-                    s.light_nodes, s.light_leaf};
+                    s.light_nodes, s.light_leaf, s.nb_table, s.env_light_list, s.env_pixels, s.env_cdf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     bvh_release(s.bvh);
@@ -728,6 +928,13 @@ SceneView scene_view(const SceneDev& s) {
     v.light_leaf = s.light_leaf;
     v.light_node_count = s.light_node_count;
     v.light_root = (s.light_root >= 0 && s.light_node_count > 0 && !s.host_light_leaf.empty()) ? s.light_root : -1;
+    v.nb_table = s.nb_table;
+    v.nb_count = int32_t(s.host_nb_table.size());
+    v.nb_total = s.host_nb_total;
+    v.env_lights = s.env_light_list;
+    v.env_count = int32_t(s.host_env_lights.size());
+    v.env_pixels = s.env_pixels;
+    v.env_cdf = s.env_cdf;
     v.opaque = s.opaque ? 1 : 0;
     v.cam = s.camera;
     v.info = s.instanced ? s.tl.tlas.info : s.bvh.info;

@@ -28,17 +28,45 @@ class Material(ctypes.Structure):
                 ("roughness", c_float), ("metallic", c_float), ("eta", c_float)]
 
 
+class LuxLight(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("vector", Vec3), ("color", Vec3), ("image_width", c_int32),
+                ("image_height", c_int32), ("image_pixels", c_void_p)]
+
+
+LIGHT_KINDS = {"point": 1, "distant": 2, "uniform_infinite": 3, "image_infinite": 4}
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("vertices", c_void_p), ("vertex_count", c_int32), ("indices", c_void_p), ("normals", c_void_p),
                 ("material_ids", c_void_p), ("triangle_count", c_int32), ("materials", c_void_p),
-                ("material_count", c_int32), ("camera", c_float * 8)]
+                ("material_count", c_int32), ("camera", c_float * 8), ("lights", c_void_p), ("light_count", c_int32)]
 
 
 class InstancedSceneDesc(ctypes.Structure):
     _fields_ = [("vertices", c_void_p), ("normals", c_void_p), ("vertex_offsets", c_void_p), ("indices", c_void_p),
                 ("material_ids", c_void_p), ("triangle_offsets", c_void_p), ("mesh_count", c_int32),
                 ("instances", c_void_p), ("instance_count", c_int32), ("materials", c_void_p),
-                ("material_count", c_int32), ("camera", c_float * 8)]
+                ("material_count", c_int32), ("camera", c_float * 8), ("lights", c_void_p), ("light_count", c_int32)]
+
+
+def _scene_lights(lights):
+    """Scene-light dicts ({"kind": "point" | "distant" | "uniform_infinite" |
+    "image_infinite", "vector": position or direction, "color": intensity,
+    radiance or image scale, "image": (H, W, 3) texels}) as a lux_light array
+    and the arrays it points to."""
+    lights = lights or []
+    arr = (LuxLight * max(len(lights), 1))()
+    keep = [arr]
+    for k, x in enumerate(lights):
+        arr[k].kind = LIGHT_KINDS[x["kind"]]
+        arr[k].vector = Vec3(*[float(c) for c in x.get("vector", (0, 0, 0))])
+        arr[k].color = Vec3(*[float(c) for c in x["color"]])
+        if x["kind"] == "image_infinite":
+            img = np.ascontiguousarray(x["image"], np.float32)
+            arr[k].image_height, arr[k].image_width = img.shape[0], img.shape[1]
+            arr[k].image_pixels = img.ctypes.data
+            keep.append(img)
+    return arr, len(lights), keep
 
 
 class RenderSettings(ctypes.Structure):
@@ -119,6 +147,9 @@ class Scene:
         cam = np.asarray(scene["camera"], np.float32)
         for k in range(8):
             desc.camera[k] = float(cam[k])
+        arr, n, self._light_keep = _scene_lights(scene.get("lights"))
+        desc.lights = ctypes.addressof(arr) if n else None
+        desc.light_count = n
         h = c_void_p()
         call("lux_hip_scene_create", ctypes.byref(desc), ctypes.byref(h))
         self.handle = h
@@ -148,6 +179,9 @@ class Scene:
         cam = np.asarray(scene["camera"], np.float32)
         for k in range(8):
             desc.camera[k] = float(cam[k])
+        arr, n, self._light_keep = _scene_lights(scene.get("lights"))
+        desc.lights = ctypes.addressof(arr) if n else None
+        desc.light_count = n
         h = c_void_p()
         call("lux_hip_scene_create_instanced", ctypes.byref(desc), ctypes.byref(h))
         self.handle = h

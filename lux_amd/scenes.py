@@ -139,6 +139,32 @@ def mixed_materials(width=40, height=40):
     return _pack(tris, mats, materials, [0, 1, f(3.35), 0, 1, f(0.9), 40.0, aspect])
 
 
+def probe(kind, width=256, height=256):
+    """The reference CLI's light demo scenes (make_point_light_scene,
+    make_distant_light_scene, make_uniform_env_scene, make_image_env_scene,
+    src/app/render_runner.cpp:99-138): a 2.4 x 2.4 grey Lambert quad at y = 0
+    seen from (0, 0.8, 2.8), lit by one scene light: "point" (25 at
+    (0, 1.6, 0.2)), "distant" (1.2 from (-0.4, 1, 0.3)), "uniform_env"
+    (radiance (0.15, 0.18, 0.25)) or "image_env" (a 2x2 lat-long image scaled
+    by 0.35)."""
+    f = np.float32
+    materials = [material(LAMBERT, (0.75, 0.75, 0.75))]
+    q = [(f(-1.2), 0, f(1.2)), (f(1.2), 0, f(1.2)), (f(1.2), 0, f(-1.2)), (f(-1.2), 0, f(-1.2))]
+    tris = [[*q[0], *q[1], *q[2]], [*q[0], *q[2], *q[3]]]
+    aspect = np.float32(width) / np.float32(height)
+    d = _pack(tris, [0, 0], materials, [0, f(0.8), f(2.8), 0, 0, 0, 35.0, aspect])
+    lights = {
+        "point": {"kind": "point", "vector": (0.0, 1.6, 0.2), "color": (25.0, 25.0, 25.0)},
+        "distant": {"kind": "distant", "vector": (-0.4, 1.0, 0.3), "color": (1.2, 1.2, 1.2)},
+        "uniform_env": {"kind": "uniform_infinite", "color": (0.15, 0.18, 0.25)},
+        "image_env": {"kind": "image_infinite", "color": (0.35, 0.35, 0.35),
+                      "image": np.asarray([[[1.4, 1.0, 0.7], [0.2, 0.3, 0.8]],
+                                           [[0.1, 0.2, 0.5], [0.8, 1.1, 1.5]]], np.float32)},
+    }
+    d["lights"] = [lights[kind]]
+    return d
+
+
 def many_lights(width=40, height=40, n_lights=256, seed=0x4C5561):
     """A many-light scene for the light samplers: the Cornell room without its
     ceiling lamp, a Lambert box, a GGX box and a glass box, lit by n_lights

@@ -287,6 +287,7 @@ def _ref_pt_lib():
         L.ref_pt_create_instanced.restype = c_void_p
         L.ref_pt_create_instanced.argtypes = [c_void_p] * 7 + [c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                                                c_void_p, c_int32, c_void_p]
+        L.ref_pt_set_scene_lights.argtypes = [c_int32] + [c_void_p] * 5
         L.ref_pt_create_bunny.restype = c_void_p
         L.ref_pt_create_bunny.argtypes = [ctypes.c_char_p, ctypes.c_float]
         L.ref_pt_triangles.restype = c_int32
@@ -306,6 +307,30 @@ def _scene_args(scene):
     return (v, i, n, uv, m, mats, cam)
 
 
+LIGHT_KINDS = {"point": 1, "distant": 2, "uniform_infinite": 3, "image_infinite": 4}
+
+
+def scene_light_arrays(lights):
+    """Scene-light dicts ({"kind", "vector", "color", "image"}) as the arrays the
+    shim takes: kinds, vectors (n, 3), colours (n, 3), image sizes (n, 2), texels."""
+    n = max(len(lights), 1)
+    kind = np.zeros(n, np.int32)
+    vec = np.zeros((n, 3), np.float32)
+    col = np.zeros((n, 3), np.float32)
+    wh = np.zeros((n, 2), np.int32)
+    px = []
+    for k, x in enumerate(lights):
+        kind[k] = LIGHT_KINDS[x["kind"]]
+        vec[k] = x.get("vector", (0, 0, 0))
+        col[k] = x["color"]
+        if x["kind"] == "image_infinite":
+            img = np.asarray(x["image"], np.float32)
+            wh[k] = (img.shape[1], img.shape[0])
+            px.append(img.reshape(-1, 3))
+    texels = np.ascontiguousarray(np.concatenate(px) if px else np.zeros((1, 3), np.float32), np.float32)
+    return kind, vec, col, wh, texels
+
+
 class RefPathTracer:
     """render_cpu_path_tracer / trace_path of the reference on a scene dict (an
     instanced one when the dict has "inst_mesh": meshes + affine instances,
@@ -315,6 +340,9 @@ class RefPathTracer:
         L = _ref_pt_lib()
         self._keep = _scene_args(scene)
         v, i, n, uv, m, mats, cam = self._keep
+        lights = scene.get("lights") or []
+        self._lights = scene_light_arrays(lights)
+        L.ref_pt_set_scene_lights(len(lights), *[_p(a) for a in self._lights])
         if "inst_mesh" in scene:
             vo = np.ascontiguousarray(scene["vertex_offsets"], np.int32)
             to = np.ascontiguousarray(scene["triangle_offsets"], np.int32)

@@ -177,6 +177,7 @@ int64_t ref_overlap_cross_brute(const float* lboxes, int32_t nl, const float* rb
 #include <unistd.h>
 
 #include <cstdio>
+#include <functional>
 #include <memory>
 
 #include "camera/pinhole_camera.h"
@@ -193,6 +194,41 @@ extern "C" {
 struct ref_pt {
     std::unique_ptr<PreparedScene> prepared;
 };
+
+// Scene lights for the next ref_pt_create* call (kind 1 point, 2 distant, 3
+// uniform infinite, 4 image infinite; vec = position / direction, color =
+// intensity / radiance / image scale; image texels back to back), added with
+// add_point_light / add_distant_light / add_uniform_infinite_light /
+// add_image_infinite_light.
+static std::vector<std::function<void(Scene&)>> g_pending_lights;
+
+void ref_pt_set_scene_lights(int32_t n, const int32_t* kind, const float* vec, const float* color,
+                             const int32_t* image_wh, const float* pixels) {
+    g_pending_lights.clear();
+    size_t texel = 0;
+    for (int i = 0; i < n; ++i) {
+        const vec3 v(vec[3 * i], vec[3 * i + 1], vec[3 * i + 2]);
+        const vec3 c(color[3 * i], color[3 * i + 1], color[3 * i + 2]);
+        switch (kind[i]) {
+            case 1: g_pending_lights.push_back([v, c](Scene& s) { add_point_light(s, v, c); }); break;
+            case 2: g_pending_lights.push_back([v, c](Scene& s) { add_distant_light(s, v, c); }); break;
+            case 3: g_pending_lights.push_back([c](Scene& s) { add_uniform_infinite_light(s, c); }); break;
+            case 4: {
+                const int w = image_wh[2 * i], h = image_wh[2 * i + 1];
+                std::vector<vec3> px(size_t(w) * h);
+                for (size_t k = 0; k < px.size(); ++k, ++texel)
+                    px[k] = vec3(pixels[3 * texel], pixels[3 * texel + 1], pixels[3 * texel + 2]);
+                g_pending_lights.push_back([px, w, h, c](Scene& s) { add_image_infinite_light(s, px, w, h, c); });
+                break;
+            }
+        }
+    }
+}
+
+static void add_pending_lights(Scene& scene) {
+    for (auto& f : g_pending_lights) f(scene);
+    g_pending_lights.clear();
+}
 
 // materials: m x 11 floats {type, albedo rgb, emission rgb, two_sided, roughness,
 // metallic, eta}; camera: {origin xyz, look_at xyz, vertical fov degrees, aspect}
@@ -225,6 +261,7 @@ ref_pt* ref_pt_create(const float* vertices, int32_t nv, const uint32_t* indices
     }
     int mesh_id = add_scene_mesh(scene, std::move(mesh));
     add_scene_instance(scene, mesh_id, Transform{});
+    add_pending_lights(scene);
     scene.camera = make_pinhole_camera(vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]),
                                        camera[6], camera[7]);
     ref_pt* r = new ref_pt();
@@ -279,6 +316,7 @@ ref_pt* ref_pt_create_instanced(const float* vertices, const int32_t* vert_off, 
             for (int c = 0; c < 4; ++c) M.m[r][c] = inst_m[16 * i + 4 * r + c];
         add_scene_instance(scene, inst_mesh[i], make_transform(M), inst_override[i]);
     }
+    add_pending_lights(scene);
     scene.camera = make_pinhole_camera(vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]),
                                        camera[6], camera[7]);
     ref_pt* r = new ref_pt();

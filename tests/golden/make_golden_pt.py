@@ -31,11 +31,17 @@ Cases:
                   8 spp, depth 5, seed 4: film + 2048 samples + the light BVH
   pt_lights_uniform  the same scene with the uniform light sampler
   pt_instanced_bvh   the instanced room with the BVH light sampler, seed 13
+  pt_probe_*      the reference CLI's point-light, distant-light, uniform-env
+                  and image-env scenes, 40x40, 8 spp, depth 4
+  pt_all_lights_bvh / _power   the Cornell box with a point, a distant, a
+                  uniform and an image infinite light besides its area light,
+                  BVH sampler (tree + non-BVH group) and power sampler
 Each stores the scene arrays, the camera and light tables the reference derived.
 
     python tests/golden/make_golden_pt.py instanced   # only the instanced cases
     python tests/golden/make_golden_pt.py bunny       # only pt_bunny
     python tests/golden/make_golden_pt.py lights      # only the light-sampler cases
+    python tests/golden/make_golden_pt.py scene_lights  # only the scene-light cases
 """
 
 import sys
@@ -56,6 +62,11 @@ def case(name, scene, w, h, spp, depth, seed, sample_pixels=None, frame=0, light
     r = oracle.RefPathTracer(scene)
     film = r.render(w, h, spp, depth, seed=seed, frame=frame, light_sampler=light_sampler)
     keys = ["vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera"]
+    if scene.get("lights"):
+        for k, v in zip(("kinds", "vectors", "colors", "image_wh", "texels"),
+                        oracle.scene_light_arrays(scene["lights"])):
+            scene = dict(scene, **{"scene_light_" + k: v})
+            keys.append("scene_light_" + k)
     if "inst_mesh" in scene:
         keys += ["vertex_offsets", "triangle_offsets", "inst_mesh", "inst_m", "inst_override"]
     data = {k: scene[k] for k in keys}
@@ -104,6 +115,20 @@ def bunny_case(w=64, h=64, spp=16, depth=4, seed=2):
     print(f"pt_bunny: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
 
 
+def scene_light_cases():
+    """The reference CLI's light demo scenes and a Cornell box with every light
+    kind at once (BVH and power samplers)."""
+    rng = np.random.default_rng(12)
+    px = rng.integers(0, 40 * 40, 2048)
+    sm = rng.integers(0, 8, 2048)
+    for seed, kind in enumerate(("point", "distant", "uniform_env", "image_env")):
+        case(f"pt_probe_{kind}", scenes.probe(kind, 40, 40), 40, 40, 8, 4, 20 + seed, (px, sm))
+    box = scenes.cornell(40, 40)
+    box["lights"] = [scenes.probe(k)["lights"][0] for k in ("point", "distant", "uniform_env", "image_env")]
+    case("pt_all_lights_bvh", box, 40, 40, 8, 5, 31, (px, sm), light_sampler="bvh")
+    case("pt_all_lights_power", box, 40, 40, 8, 5, 32, (px, sm))
+
+
 def light_sampler_cases():
     rng = np.random.default_rng(10)
     px = rng.integers(0, 40 * 40, 2048)
@@ -134,6 +159,9 @@ def main(only=None):
     if only == "lights":
         light_sampler_cases()
         return
+    if only == "scene_lights":
+        scene_light_cases()
+        return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
     sm = np.tile(np.arange(4), w * h)
@@ -154,6 +182,7 @@ def main(only=None):
     instanced_cases()
     bunny_case()
     light_sampler_cases()
+    scene_light_cases()
 
 
 if __name__ == "__main__":

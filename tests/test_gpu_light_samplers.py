@@ -117,3 +117,85 @@ def test_samplers_differ_and_mixed_batches_equal_standalone(rd):
     assert e.value.status == abi.LUX_ERR_INVALID_ARGUMENT
     for s in (sc, inst, cb):
         s.close()
+
+
+# ---- scene lights: point, distant, uniform and image infinite --------------------------
+LIGHT_NAMES = {1: "point", 2: "distant", 3: "uniform_infinite", 4: "image_infinite"}
+SCENE_LIGHT_CASES = ["pt_probe_point", "pt_probe_distant", "pt_probe_uniform_env", "pt_probe_image_env",
+                     "pt_all_lights_bvh", "pt_all_lights_power"]
+
+
+def with_scene_lights(g):
+    """The scene dict of a golden case, scene lights rebuilt from its arrays."""
+    d = scene_of(g)
+    kinds, vecs, cols, wh, texels = (g["scene_light_" + k] for k in ("kinds", "vectors", "colors", "image_wh",
+                                                                     "texels"))
+    lights, t = [], 0
+    for k, kind in enumerate(kinds):
+        x = {"kind": LIGHT_NAMES[int(kind)], "vector": vecs[k], "color": cols[k]}
+        if int(kind) == 4:
+            w, h = int(wh[k][0]), int(wh[k][1])
+            x["image"] = texels[t:t + w * h].reshape(h, w, 3)
+            t += w * h
+        lights.append(x)
+    d["lights"] = lights
+    return d
+
+
+@pytest.mark.parametrize("case", SCENE_LIGHT_CASES)
+def test_scene_light_tables_bit_exact(rd, case):
+    print("case: scene light tables", case, flush=True)
+    g = load_golden(case)
+    sc = rd.Scene(with_scene_lights(g))
+    lt = sc.lights()
+    for k in ("prim", "area", "prob", "alias", "pmf", "primitive_light_index"):
+        assert np.array_equal(np.asarray(lt[k]).view(np.uint32), np.asarray(g["light_" + k]).view(np.uint32)), k
+    t = sc.light_bvh()
+    assert t["root"] == int(g["light_bvh_root"])
+    for k in ("bounds", "power", "links", "leaf"):
+        assert np.array_equal(bits(t[k]), bits(g["light_bvh_" + k])), k
+    sc.close()
+
+
+@pytest.mark.parametrize("case", SCENE_LIGHT_CASES)
+def test_scene_light_render_matches_reference(rd, case):
+    """Point and distant lights (delta: MIS weight 1, shadow ray to the point or
+    to infinity), uniform and image infinite lights (seen by escaping rays with
+    MIS, sampled uniformly over the sphere or by texel weight), alone and mixed
+    with area lights under the BVH sampler's tree + non-BVH group."""
+    print("case: scene light render", case, flush=True)
+    g = load_golden(case)
+    sc = rd.Scene(with_scene_lights(g))
+    w, h, spp, depth, seed = int(g["width"]), int(g["height"]), int(g["spp"]), int(g["depth"]), int(g["seed"])
+    ls = str(g["light_sampler"])
+    films = [sc.render(w, h, spp, depth, seed=seed, mode=mode, light_sampler=ls) for mode in (rd.WAVEFRONT, rd.PLAIN)]
+    for film in films:
+        a, c = film / spp, g["film_sum"] / spp
+        err = rd.rel_rmse(a, c)
+        off = float((np.abs(a - c).max(axis=2) > 1e-3).mean())
+        print(f"relRMSE={err:.3e} frac>1e-3={off:.4f}", flush=True)
+        assert err < 2e-2 and off < FRAC_PIXELS_OFF
+    assert np.array_equal(bits(films[0]), bits(films[1]))
+    px, sm, ref = g["sample_pixels"], g["sample_index"], g["sample_radiance"]
+    close = 0
+    for k in range(spp):
+        film = sc.render(w, h, 1, depth, seed=seed, sample_offset=k, light_sampler=ls).reshape(-1, 3)
+        sel = sm == k
+        close += int(np.all(np.abs(film[px[sel]] - ref[sel]) <= 1e-4 * np.maximum(np.abs(ref[sel]), 1e-2),
+                            axis=1).sum())
+    print(f"samples within 1e-4: {close / len(px):.4f}", flush=True)
+    assert close / len(px) > SAMPLES_CLOSE
+    sc.close()
+
+
+def test_scene_light_validation(rd):
+    from lux_amd import abi
+
+    g = load_golden("pt_probe_distant")
+    for bad in ({"kind": "distant", "vector": (0, 0, 0), "color": (1, 1, 1)},
+                {"kind": "point", "vector": (0, 1, 0), "color": (-1, 1, 1)},
+                {"kind": "image_infinite", "color": (1, 1, 1), "image": np.zeros((0, 2, 3), np.float32)}):
+        d = dict(scene_of(g), lights=[bad])
+        with pytest.raises(abi.LuxError) as e:
+            rd.Scene(d)
+        assert e.value.status == abi.LUX_ERR_INVALID_ARGUMENT
