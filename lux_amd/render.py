@@ -154,6 +154,8 @@ class Scene:
         call("lux_hip_scene_create", ctypes.byref(desc), ctypes.byref(h))
         self.handle = h
         self.triangle_count = len(self._m)
+        if "camera_vectors" in scene:
+            self.set_camera(scene["camera_vectors"])
 
     def _create_instanced(self, scene, material_count):
         from . import two_level as TL
@@ -162,7 +164,8 @@ class Scene:
         self._voff = np.ascontiguousarray(scene["vertex_offsets"], np.int32)
         self._toff = np.ascontiguousarray(scene["triangle_offsets"], np.int32)
         self._inst = np.ascontiguousarray(TL.make_instances(
-            scene["inst_mesh"], scene["inst_m"], world_bounds=np.zeros((len(scene["inst_mesh"]), 6), np.float32),
+            scene["inst_mesh"], scene["inst_m"], inverses=scene.get("inst_inv"),
+            world_bounds=np.zeros((len(scene["inst_mesh"]), 6), np.float32),
             material_override=scene.get("inst_override")), TL.INSTANCE)
         desc = InstancedSceneDesc()
         desc.vertices = self._v.ctypes.data
@@ -186,6 +189,13 @@ class Scene:
         call("lux_hip_scene_create_instanced", ctypes.byref(desc), ctypes.byref(h))
         self.handle = h
         self.triangle_count = self.info()[0]
+        if "camera_vectors" in scene:
+            self.set_camera(scene["camera_vectors"])
+
+    def set_camera(self, vectors12):
+        """Built camera vectors {origin, lower_left, horizontal, vertical}."""
+        v = np.ascontiguousarray(vectors12, np.float32)[:12]
+        call("lux_hip_scene_set_camera", self.handle, ptr(v))
 
     def set_instances(self, inst_mesh, inst_m, inst_override=None):
         """Move the instances of an instanced scene (same count): TLAS rebuild,

@@ -418,6 +418,7 @@ def bunny(obj_path, width=512, height=512, loader=None):
         "triangle_offsets": np.array([0, len(mesh0_i), len(mesh0_i) + len(m["indices"])], np.int32),
         "inst_mesh": np.array([0, 1], np.int32),
         "inst_m": np.stack([np.eye(4, dtype=np.float32), m["object_to_world"]]).astype(np.float32),
+        "inst_inv": np.stack([np.eye(4, dtype=np.float32), m["world_to_object"]]).astype(np.float32),
         "inst_override": np.array([-1, -1], np.int32),
     }
 

@@ -288,6 +288,9 @@ def _ref_pt_lib():
         L.ref_pt_create_instanced.argtypes = [c_void_p] * 7 + [c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                                                c_void_p, c_int32, c_void_p]
         L.ref_pt_set_scene_lights.argtypes = [c_int32] + [c_void_p] * 5
+        L.ref_pt_create_builtin.restype = c_void_p
+        L.ref_pt_create_builtin.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_float]
+        L.ref_pt_export.argtypes = [c_void_p] * 14
         L.ref_pt_create_bunny.restype = c_void_p
         L.ref_pt_create_bunny.argtypes = [ctypes.c_char_p, ctypes.c_float]
         L.ref_pt_triangles.restype = c_int32
@@ -369,6 +372,39 @@ class RefPathTracer:
         self.h = L.ref_pt_create_bunny(str(asset_root).encode(), float(np.float32(width) / np.float32(height)))
         self.ordinals = L.ref_pt_triangles(self.h, None, None, None)
         return self
+
+    @classmethod
+    def builtin(cls, name, asset_root, width, height):
+        """One of the reference CLI's built-in scenes ("direct-light",
+        "transparent-filter", "bunny", "lux-cover") with its CLI camera."""
+        L = _ref_pt_lib()
+        self = cls.__new__(cls)
+        self._keep = None
+        L.ref_pt_set_scene_lights(0, None, None, None, None, None)
+        self.h = L.ref_pt_create_builtin(name.encode(), str(asset_root).encode(),
+                                         float(np.float32(width) / np.float32(height)))
+        if not self.h:
+            raise ValueError(f"unknown built-in scene {name}")
+        self.ordinals = L.ref_pt_triangles(self.h, None, None, None)
+        return self
+
+    def export(self):
+        """The scene as an instanced scene dict (meshes, instances with their
+        Transform m and inv, materials, built camera vectors)."""
+        L = _ref_pt_lib()
+        c = np.zeros(5, np.int32)
+        L.ref_pt_export(self.h, _p(c), *([None] * 12))
+        nv, nt, nm, ni, nmat = map(int, c)
+        d = {"vertices": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
+             "vertex_offsets": np.zeros(nm + 1, np.int32), "indices": np.zeros((nt, 3), np.uint32),
+             "material_ids": np.zeros(nt, np.int32), "triangle_offsets": np.zeros(nm + 1, np.int32),
+             "inst_mesh": np.zeros(ni, np.int32), "inst_m": np.zeros((ni, 4, 4), np.float32),
+             "inst_inv": np.zeros((ni, 4, 4), np.float32), "inst_override": np.zeros(ni, np.int32),
+             "materials": np.zeros((nmat, 11), np.float32), "camera_vectors": np.zeros(12, np.float32)}
+        L.ref_pt_export(self.h, _p(c), *[_p(v) for v in d.values()])
+        d["uvs"] = np.zeros((nv, 2), np.float32)
+        d["camera"] = np.zeros(8, np.float32)
+        return d
 
     def triangles(self):
         """scene_triangle_view per scene ordinal: world corners (T, 9), world

@@ -187,7 +187,10 @@ int64_t ref_overlap_cross_brute(const float* lboxes, int32_t nl, const float* rb
 #include "scene/scene_build.h"
 #include "scene/scene_edit.h"
 #include "scene/scene_light_sampler.h"
+#include "scene/direct_light_test.h"
+#include "scene/lux_cover.h"
 #include "scene/stanford_bunny.h"
+#include "scene/transparent_filter.h"
 
 extern "C" {
 
@@ -334,6 +337,95 @@ ref_pt* ref_pt_create_bunny(const char* asset_root, float aspect) {
     r->prepared = std::make_unique<PreparedScene>(std::move(scene));
     r->prepared->prepare_host_bvh();
     return r;
+}
+
+// The reference CLI's built-in scenes with their CLI cameras
+// (src/app/render_runner.cpp:92-170): "direct-light", "transparent-filter",
+// "bunny", "lux-cover" (the last two read assets under asset_root).
+ref_pt* ref_pt_create_builtin(const char* name, const char* asset_root, float aspect) {
+    const std::string n(name);
+    const std::filesystem::path root(asset_root);
+    Scene scene;
+    if (n == "direct-light") {
+        scene = make_direct_light_test_scene();
+        scene.camera = make_pinhole_camera(vec3(0, 1.2f, 3.0f), vec3(0, 0, 0), 35.0f, aspect);
+    } else if (n == "transparent-filter") {
+        scene = make_transparent_filter_scene();
+        scene.camera = make_pinhole_camera(vec3(0, 0.85f, 3.2f), vec3(0, 0.65f, -0.7f), 38.0f, aspect);
+    } else if (n == "bunny") {
+        scene = make_stanford_bunny_scene(root);
+        scene.camera = make_pinhole_camera(vec3(0.0f, 0.3f, 0.5f), vec3(-0.02f, 0.11f, 0.0f), 40.0f, aspect);
+    } else if (n == "lux-cover") {
+        scene = make_lux_cover_scene(root);
+        scene.camera = make_pinhole_camera(vec3(0.0f, 0.52f, -3.0f), vec3(0.0f, 0.36f, 0.0f), 32.0f, aspect);
+    } else {
+        return nullptr;
+    }
+    ref_pt* r = new ref_pt();
+    r->prepared = std::make_unique<PreparedScene>(std::move(scene));
+    r->prepared->prepare_host_bvh();
+    return r;
+}
+
+// The scene as arrays, meshes and instances as the reference holds them:
+// counts {vertices, triangles, meshes, instances, materials}; per vertex xyz
+// and normal; mesh vertex / triangle offsets; mesh-local indices, material ids;
+// per instance mesh, Transform m and inv (row-major), override; materials as
+// 11 floats {type, albedo, emission, two_sided, roughness, metallic, eta};
+// camera {origin, lower_left, horizontal, vertical}. Call with null arrays
+// first for the counts.
+void ref_pt_export(const ref_pt* r, int32_t* counts5, float* vertices, float* normals, int32_t* vert_off,
+                   uint32_t* indices, int32_t* mat_ids, int32_t* tri_off, int32_t* inst_mesh, float* inst_m,
+                   float* inst_inv, int32_t* inst_override, float* materials, float* camera12) {
+    const Scene& sc = r->prepared->host_scene();
+    int nv = 0, nt = 0;
+    for (const SceneMesh& m : sc.mesh_assets) {
+        nv += int(m.vertices.size());
+        nt += int(m.indices.size());
+    }
+    counts5[0] = nv;
+    counts5[1] = nt;
+    counts5[2] = int(sc.mesh_assets.size());
+    counts5[3] = int(sc.instances.size());
+    counts5[4] = int(sc.materials.size());
+    if (!vertices) return;
+    int v = 0, t = 0;
+    vert_off[0] = tri_off[0] = 0;
+    for (size_t k = 0; k < sc.mesh_assets.size(); ++k) {
+        const SceneMesh& m = sc.mesh_assets[k];
+        for (size_t i = 0; i < m.vertices.size(); ++i, ++v) {
+            std::memcpy(vertices + 3 * v, &m.vertices[i], 12);
+            std::memcpy(normals + 3 * v, &m.normals[i], 12);
+        }
+        for (size_t i = 0; i < m.indices.size(); ++i, ++t) {
+            indices[3 * t] = m.indices[i].v0;
+            indices[3 * t + 1] = m.indices[i].v1;
+            indices[3 * t + 2] = m.indices[i].v2;
+            mat_ids[t] = m.material_ids[i];
+        }
+        vert_off[k + 1] = v;
+        tri_off[k + 1] = t;
+    }
+    for (size_t i = 0; i < sc.instances.size(); ++i) {
+        const SceneInstance& in = sc.instances[i];
+        inst_mesh[i] = in.mesh_id;
+        inst_override[i] = in.material_override;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                inst_m[16 * i + 4 * a + b] = in.object_to_world.m.m[a][b];
+                inst_inv[16 * i + 4 * a + b] = in.object_to_world.inv.m[a][b];
+            }
+    }
+    for (size_t i = 0; i < sc.materials.size(); ++i) {
+        const Material& m = sc.materials[i];
+        const float row[11] = {float(int(m.type)), m.albedo.x, m.albedo.y, m.albedo.z, m.emission.x, m.emission.y,
+                               m.emission.z, m.emission_sidedness == EmissionSidedness::DoubleSided ? 1.0f : 0.0f,
+                               m.roughness, m.metallic, m.eta};
+        std::memcpy(materials + 11 * i, row, sizeof(row));
+    }
+    const PerspectiveCamera& c = sc.camera.perspective;
+    const vec3* cv[4] = {&c.origin, &c.lower_left, &c.horizontal, &c.vertical};
+    for (int k = 0; k < 4; ++k) std::memcpy(camera12 + 3 * k, cv[k], 12);
 }
 
 // scene_triangle_view of every scene ordinal: world corners (9 floats), world

@@ -33,6 +33,9 @@ Cases:
   pt_instanced_bvh   the instanced room with the BVH light sampler, seed 13
   pt_probe_*      the reference CLI's point-light, distant-light, uniform-env
                   and image-env scenes, 40x40, 8 spp, depth 4
+  pt_builtin_*    the reference CLI's direct-light, transparent-filter and
+                  lux-cover scenes (exported from the reference's scene code),
+                  48x48, 8 spp, depth 4
   pt_all_lights_bvh / _power   the Cornell box with a point, a distant, a
                   uniform and an image infinite light besides its area light,
                   BVH sampler (tree + non-BVH group) and power sampler
@@ -42,6 +45,7 @@ Each stores the scene arrays, the camera and light tables the reference derived.
     python tests/golden/make_golden_pt.py bunny       # only pt_bunny
     python tests/golden/make_golden_pt.py lights      # only the light-sampler cases
     python tests/golden/make_golden_pt.py scene_lights  # only the scene-light cases
+    python tests/golden/make_golden_pt.py builtin     # only the built-in CLI scenes
 """
 
 import sys
@@ -115,6 +119,35 @@ def bunny_case(w=64, h=64, spp=16, depth=4, seed=2):
     print(f"pt_bunny: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
 
 
+def builtin_cases(w=48, h=48, spp=8, depth=4):
+    """pt_builtin_*: the reference CLI's built-in "direct-light",
+    "transparent-filter" and "lux-cover" scenes, built by the reference's own
+    scene code and exported as arrays (meshes, instances with Transform m and
+    inv, materials, camera vectors), rendered by its CPU path tracer at the
+    CLI's depth 4."""
+    import hashlib
+
+    rng = np.random.default_rng(14)
+    px = rng.integers(0, w * h, 2048)
+    sm = rng.integers(0, spp, 2048)
+    for k, name in enumerate(("direct-light", "transparent-filter", "lux-cover")):
+        r = oracle.RefPathTracer.builtin(name, "/root/reference/assets", w, h)
+        data = r.export()
+        film = r.render(w, h, spp, depth, seed=40 + k)
+        t = r.triangles()
+        data.update(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
+                    seed=np.uint64(40 + k), frame=np.int32(0), film_sum=film, camera_vectors=r.camera(),
+                    sample_pixels=px.astype(np.int32), sample_index=sm.astype(np.int32),
+                    sample_radiance=r.samples(w, h, depth, px, sm, seed=40 + k),
+                    triangle_view_sha256=np.array(hashlib.sha256(b"".join(
+                        np.ascontiguousarray(t[x]).tobytes() for x in ("corners", "normals", "material"))).hexdigest()))
+        for kk, v in r.lights().items():
+            data["light_" + kk] = v
+        out = "pt_builtin_" + name.replace("-", "_")
+        np.savez_compressed(OUT / f"{out}.npz", **data)
+        print(f"{out}: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
+
+
 def scene_light_cases():
     """The reference CLI's light demo scenes and a Cornell box with every light
     kind at once (BVH and power samplers)."""
@@ -162,6 +195,9 @@ def main(only=None):
     if only == "scene_lights":
         scene_light_cases()
         return
+    if only == "builtin":
+        builtin_cases()
+        return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
     sm = np.tile(np.arange(4), w * h)
@@ -183,6 +219,7 @@ def main(only=None):
     bunny_case()
     light_sampler_cases()
     scene_light_cases()
+    builtin_cases()
 
 
 if __name__ == "__main__":

@@ -320,3 +320,38 @@ def test_bunny_scene_matches_reference(rd, tmp_path):
     print(f"samples within 1e-4: {close / len(px):.4f}", flush=True)
     assert close / len(px) > 0.95
     sc.close()
+
+
+@pytest.mark.parametrize("case", ["pt_builtin_direct_light", "pt_builtin_transparent_filter",
+                                  "pt_builtin_lux_cover"])
+def test_builtin_cli_scene_matches_reference(rd, case):
+    """The reference CLI's built-in scenes as its own scene code builds them
+    (exported meshes, instances, materials, camera): identical triangle view,
+    light table and camera, and a render that matches its CPU path tracer."""
+    import hashlib
+
+    print("case: built-in scene", case, flush=True)
+    g = load_golden(case)
+    keys = ("vertices", "normals", "uvs", "vertex_offsets", "indices", "material_ids", "triangle_offsets",
+            "inst_mesh", "inst_m", "inst_inv", "inst_override", "materials", "camera", "camera_vectors")
+    sc = rd.Scene({k: g[k] for k in keys})
+    t = sc.triangles()
+    digest = hashlib.sha256(b"".join(np.ascontiguousarray(t[k]).tobytes()
+                                     for k in ("corners", "normals", "material"))).hexdigest()
+    assert digest == str(g["triangle_view_sha256"])
+    assert np.array_equal(bits(sc.camera()), bits(g["camera_vectors"][:12]))
+    lt = sc.lights()
+    for k in ("prim", "area", "prob", "alias", "pmf", "primitive_light_index"):
+        assert np.array_equal(np.asarray(lt[k]).view(np.uint32), np.asarray(g["light_" + k]).view(np.uint32)), k
+    w, h, spp, depth, seed = int(g["width"]), int(g["height"]), int(g["spp"]), int(g["depth"]), int(g["seed"])
+    check_film(rd, sc.render(w, h, spp, depth, seed=seed), g["film_sum"], spp, RELRMSE_SMALL)
+    px, sm, ref = g["sample_pixels"], g["sample_index"], g["sample_radiance"]
+    close = 0
+    for k in range(spp):
+        film = sc.render(w, h, 1, depth, seed=seed, sample_offset=k).reshape(-1, 3)
+        sel = sm == k
+        close += int(np.all(np.abs(film[px[sel]] - ref[sel]) <= 1e-4 * np.maximum(np.abs(ref[sel]), 1e-2),
+                            axis=1).sum())
+    print(f"samples within 1e-4: {close / len(px):.4f}", flush=True)
+    assert close / len(px) > 0.95
+    sc.close()

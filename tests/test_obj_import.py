@@ -167,3 +167,21 @@ def test_bunny_scene_rebuild_matches_reference_scene(tmp_path):
     assert np.array_equal(r.camera().view(np.uint32), g["camera_vectors"].view(np.uint32))
     for k, v in r.lights().items():
         assert np.array_equal(np.asarray(v).view(np.uint32), np.asarray(g["light_" + k]).view(np.uint32)), k
+
+
+def test_lux_cover_mesh_equals_reference_scene(tmp_path):
+    """The logo mesh and instance of the reference's own lux-cover scene
+    (make_lux_cover_scene, exported in tests/golden/pt_builtin_lux_cover.npz)
+    equal the OBJ import of LUX.obj with that scene's options, bit for bit."""
+    from lux_amd import obj
+
+    g = load_golden("pt_builtin_lux_cover")
+    m = obj.load(GOLDEN / "obj" / "LUX.obj", scene_material_count=5, normal_mode="generate_missing_smooth",
+                 boundary_topology="welded_positions", scale_mode="x_extent", target_extent=1.85, center_xz=True,
+                 ground_to_y=True, material_bindings={"L_lambert": 2, "U_glass": 3, "X_metal": 4})
+    vo, to = g["vertex_offsets"], g["triangle_offsets"]
+    assert list(g["inst_mesh"]) == [0, 1]
+    for got, ref in ((m["vertices"], g["vertices"][vo[1]:vo[2]]), (m["normals"], g["normals"][vo[1]:vo[2]]),
+                     (m["indices"], g["indices"][to[1]:to[2]]), (m["material_ids"], g["material_ids"][to[1]:to[2]]),
+                     (m["object_to_world"], g["inst_m"][1]), (m["world_to_object"], g["inst_inv"][1])):
+        assert np.array_equal(np.ascontiguousarray(got).view(np.uint8), np.ascontiguousarray(ref).view(np.uint8))
