@@ -782,7 +782,7 @@ void bvh_release(BvhDev& d) {
                     d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
                     d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
                     d.rtop_int,  d.rtop_leaf,   d.wide,         d.wsrc,       d.wq[0],      d.wq[1],
-                    d.wcounts,   d.pairs,       d.pair_pos};
+                    d.wcounts,   d.pairs,       d.pair_pos,     d.slot_pre};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     BvhDev empty;
@@ -824,9 +824,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_layout(int n, const uint32_t* _
 // Copies every node to its slot: boxes and leaf ids as they are, internal
 // nodes point at their right child's slot, escapes are translated to slots.
 // Slot 1 (the root's unused sibling) gets an empty box that no walk reaches.
+// slot_pre records each slot's preorder position (the tie order of the
+// ordered walk).
 __global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node* __restrict__ nodes,
                                                       const uint32_t* __restrict__ pair_pos,
-                                                      lux_bvh_node* __restrict__ pairs, DevInfo* info) {
+                                                      lux_bvh_node* __restrict__ pairs,
+                                                      uint32_t* __restrict__ slot_pre, DevInfo* info) {
     const int node_count = 2 * n - 1;
     const int slots = 2 * n;
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < node_count; k += gridDim.x * kBlock) {
@@ -853,6 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node*
             nd.escape = int32_t(e);
         }
         pairs[dst] = nd;
+        slot_pre[dst] = uint32_t(k);
     }
     if (n >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
         lux_bvh_node pad;
@@ -860,6 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node*
         pad.primitive_id = -1;
         pad.escape = -1;
         pairs[1] = pad;
+        slot_pre[1] = 0xffffffffu;
     }
 }
 
@@ -868,7 +873,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_fill(int n, const lux_bvh_node*
 void pair_fill(BvhDev& d, hipStream_t s) {
     if (d.n <= 0) return;
     LUX_LAUNCH("lbvh_pair_fill", k_pair_fill, max(1, min(2048, div_up(2 * d.n - 1, kBlock))), kBlock, 0, s, d.n,
-               (const lux_bvh_node*)d.nodes, (const uint32_t*)d.pair_pos, d.pairs, d.info);
+               (const lux_bvh_node*)d.nodes, (const uint32_t*)d.pair_pos, d.pairs, d.slot_pre, d.info);
 }
 
 void pair_build(BvhDev& d, hipStream_t s) {
@@ -957,6 +962,7 @@ void bvh_allocate(BvhDev& d, int n) {
     d.wide_capacity = 0;  // the wide tree is allocated by its first build
     d.pairs = dalloc<lux_bvh_node>(2 * size_t(cap));
     d.pair_pos = dalloc<uint32_t>(2 * size_t(cap));
+    d.slot_pre = dalloc<uint32_t>(2 * size_t(cap));
     // One-time clears at allocation: look-back words (epoch 0 is never used),
     // tickets, arrival counters (even = idle), info, frontier count.
     hip_check(hipMemset(d.status, 0, sizeof(uint64_t) * d.status_words), "hipMemset status");

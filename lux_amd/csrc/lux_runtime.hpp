@@ -88,9 +88,11 @@ struct BvhDev {
     // traversal copy of `nodes` in sibling-pair order: slot 0 = root, the
     // children of the r-th internal node (in preorder) at slots 2r + 2 (right)
     // and 2r + 3 (left); an internal node's primitive_id holds ~(right child
-    // slot), escapes are slots. pair_pos maps preorder position -> slot.
+    // slot), escapes are slots. pair_pos maps preorder position -> slot,
+    // slot_pre slot -> preorder position (0xffffffff for the unused slot 1).
     lux_bvh_node* pairs = nullptr;
     uint32_t* pair_pos = nullptr;
+    uint32_t* slot_pre = nullptr;
     uint32_t* hist = nullptr;
     uint64_t* status = nullptr;
     uint32_t* tickets = nullptr;

@@ -33,6 +33,7 @@
 #include <cstring>
 #include <vector>
 
+#include "ordered.hpp"
 #include "pt_device.hpp"
 #include "pt_runtime.hpp"
 
@@ -133,12 +134,60 @@ __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, co
     return best;
 }
 
+// The same closest hit by the near-child-first walk (ordered.hpp): both children
+// of a node tested together, the farther one kept on this lane's LDS stack
+// (entries `stride` words apart), exact-t ties kept by preorder position. A
+// ray whose stack overflows takes closest_walk.
+template <bool kParallel>
+__device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
+                                               float t_min, float t_max0, int ignore, float& t_hit, float& u_hit,
+                                               float& v_hit, uint32_t* stack, int stride) {
+    float t_max = t_max0;
+    int best = -1;
+    uint32_t best_slot = 0, visits = 0;
+    bool bad = false;
+    const lux_triangle* soup = sv.soup;
+    const uint32_t* slot_pre = sv.slot_pre;
+    auto leaf = [&](uint32_t slot, int prim) {
+        if (prim == ignore) return false;
+        if (uint32_t(prim) >= uint32_t(sv.tri_count)) {
+            set_error(sv.info, kErrPrimRange);
+            return false;
+        }
+        const float* p = reinterpret_cast<const float*>(soup + prim);
+        f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
+        float t, u, v;
+        if (tri_hit(O, D, t_min, tie_bound(t_max, best >= 0), v0, v1, v2, t, u, v) &&
+            (t < t_max || slot_pre[slot] < slot_pre[best_slot])) {
+            best = prim;
+            best_slot = slot;
+            t_max = t;
+            u_hit = u;
+            v_hit = v;
+        }
+        return false;
+    };
+    const bool done =
+        ordered_walk<kParallel>(sv.nodes, uint32_t(sv.node_count), sr, t_max, stack, stride, visits, bad, leaf);
+    if (bad) set_error(sv.info, kErrWalkOrder);
+    if (!done) return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
+    t_hit = t_max;
+    return best;
+}
+
 __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
-                                       int ignore, float& t_hit, float& u_hit, float& v_hit) {
+                                       int ignore, float& t_hit, float& u_hit, float& v_hit, uint32_t* stack,
+                                       int stride) {
     const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
     const SlabRay sr = make_slab_ray(O, D, t_min);
-    if (wave_any_parallel(sr))
+    const bool ordered = sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
+    if (wave_any_parallel(sr)) {
+        if (ordered)
+            return closest_ordered<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
         return closest_walk<true, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+    }
+    if (ordered)
+        return closest_ordered<false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
     return closest_walk<false, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
 }
 
@@ -160,7 +209,8 @@ __device__ __forceinline__ bool occluded(const SceneView& sv, const vec& o, cons
 // scene ordinal. kTwoLevel is chosen per batch on the host, like kAnyShadow.
 template <bool kTwoLevel>
 __device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, const vec& d, float t_min,
-                                             float t_max, int ignore, float& t_hit, float& u_hit, float& v_hit) {
+                                             float t_max, int ignore, float& t_hit, float& u_hit, float& v_hit,
+                                             uint32_t* stack, int stride) {
     if constexpr (kTwoLevel) {
         const TwoLevelHit h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
                                                     ignore);
@@ -173,7 +223,7 @@ __device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, 
         }
         return h.prim;
     } else {
-        return closest(sv, o, d, t_min, t_max, ignore, t_hit, u_hit, v_hit);
+        return closest(sv, o, d, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
     }
 }
 
@@ -258,7 +308,8 @@ __global__ __launch_bounds__(B) void k_generate(const BatchDesc* __restrict__ de
 // same transmittance (0 or 1) as the closest-hit loop. The choice is made per
 // batch on the host; reading the scene's flag per ray instead measured slower.
 template <bool kAnyShadow, bool kTwoLevel>
-__device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, const Paths& ps, uint32_t i) {
+__device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, const Paths& ps, uint32_t i,
+                                           uint32_t* stack, int stride) {
     float4 so = ps.so[i], sd = ps.sd[i], sc = ps.sc[i], st = ps.st[i];
     const uint32_t path = __float_as_uint(sc.w);
     const SceneView& sv = desc->seg[segment_of_path(ps, path)].sv;
@@ -269,7 +320,7 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
     if (kAnyShadow && scene_occluded<kTwoLevel>(sv, o, d, t_min, t_max, ignore)) tr = splat(0);
     for (int guard = 0; !kAnyShadow && guard < 1 << 16; ++guard) {
         float t = 0, u = 0, v = 0;
-        int prim = scene_closest<kTwoLevel>(sv, o, d, t_min, t_max, ignore, t, u, v);
+        int prim = scene_closest<kTwoLevel>(sv, o, d, t_min, t_max, ignore, t, u, v, stack, stride);
         if (prim < 0) break;
         const DevMaterial& m = sv.mats[sv.mat_id[prim]];
         if (!passthrough(m)) {
@@ -312,6 +363,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void k_t
         qs.rays[0] += n_ext;
         qs.rays[1] += n_sh;
     }
+    // per-lane stacks of the ordered closest-hit walk (the one-level walk only)
+    __shared__ uint32_t lds_stack[kTwoLevel ? 1 : kOrderedStack * B];
+    uint32_t* stack = lds_stack + (kTwoLevel ? 0 : threadIdx.x);
     const uint32_t total = n_ext + n_sh;
     for (uint32_t i = xcd_block() * B + threadIdx.x; i < total; i += gridDim.x * B) {
         if (i < n_ext) {
@@ -319,10 +373,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void k_t
             const float4 o = ps.ro[path], d = ps.rd[path];
             const SceneView& sv = desc->seg[__float_as_uint(d.w)].sv;
             float t = INFINITY, u = 0, v = 0;
-            int prim = scene_closest<kTwoLevel>(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v);
+            int prim = scene_closest<kTwoLevel>(sv, xyz(o), xyz(d), kEps, INFINITY, -1, t, u, v, stack, B);
             ps.hit[path] = make_float4(__int_as_float(prim), t, u, v);
         } else {
-            shadow_ray<kAnyShadow, kTwoLevel>(desc, ps, i - n_ext);
+            shadow_ray<kAnyShadow, kTwoLevel>(desc, ps, i - n_ext, stack, B);
         }
     }
 }

@@ -92,6 +92,7 @@ struct SceneDev {
 // POD view of one scene handed to kernels.
 struct SceneView {
     const lux_bvh_node* nodes;
+    const uint32_t* slot_pre;  // preorder position of each pair slot (tie order of the ordered walk)
     const lux_triangle* soup;
     const uint32_t* indices;
     const float* normals;

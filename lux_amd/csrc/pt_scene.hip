@@ -538,8 +538,10 @@ void prepare_scene_lights(SceneDev& s, const lux_light* lights, int count, hipSt
 
 size_t bvh_bytes(int n) {
     // boxes, ids, 2x keys/vals, 5 tree links, leaf parents, arrivals, 2 slots,
-    // 2 layout maps, 2 node records, 2 frontier entries, 2 climb words
-    return size_t(std::max(n, 1)) * (24 + 4 + 16 + 20 + 4 + 4 + 48 + 8 + 64 + 8 + 8) + (size_t(n) / 2048 + 1) * 2048;
+    // 2 layout maps, 2 node records, 2 frontier entries, 2 climb words, 2
+    // sibling-pair records with their two slot maps
+    return size_t(std::max(n, 1)) * (24 + 4 + 16 + 20 + 4 + 4 + 48 + 8 + 64 + 8 + 8 + 64 + 16) +
+           (size_t(n) / 2048 + 1) * 2048;
 }
 }  // namespace
 
@@ -906,11 +908,13 @@ SceneView scene_view(const SceneDev& s) {
     SceneView v{};
     if (s.instanced) {
         v.nodes = nullptr;
+        v.slot_pre = nullptr;
         v.node_count = 0;
         v.tl = two_level_view(s.tl);
         v.instanced = 1;
     } else {
         v.nodes = s.bvh.pairs;  // sibling-pair traversal copy
+        v.slot_pre = s.bvh.slot_pre;
         v.node_count = s.bvh.n > 1 ? 2 * s.bvh.n : (s.bvh.n > 0 ? 1 : 0);
         v.instanced = 0;
     }

@@ -158,7 +158,8 @@ void upload_instances(TwoLevelDev& s, const lux_instance* instances, int count, 
 size_t bvh_device_bytes(const BvhDev& d) {
     const size_t cap = size_t(d.capacity);
     // boxes, ids, keys/vals, tree links, slots, preorder maps, nodes, pairs
-    return cap * (24 + 4 + 16 + 28 + 48 + 8 + 64 + 64 + 8);
+    // and their two slot maps
+    return cap * (24 + 4 + 16 + 28 + 48 + 8 + 64 + 64 + 16);
 }
 
 }  // namespace

@@ -189,6 +189,92 @@ def test_repeated_builds_reuse_handle(gpu):
         assert np.array_equal(bvh.nodes().view(np.uint32), o.nodes.view(np.uint32)), (n, seed)
 
 
+def check_closest_against_oracle(gpu, tris, ids, rays):
+    """GPU closest hit == the oracle's stackless preorder walk == the oracle's
+    host stack traversal, ids and t bit for bit."""
+    boxes = W.triangle_boxes(tris)
+    bvh = gpu.build_gpu_lbvh(boxes, ids)
+    tri_by_id = np.zeros((int(ids.max()) + 1, 9), np.float32)
+    tri_by_id[ids] = tris.reshape(-1, 9)
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, rays)
+    o = oracle.OracleBvh(boxes, ids)
+    pre_ids, pre_t = o.trace_stackless(tri_by_id, rays)
+    host = o.trace_closest(tri_by_id, rays)
+    assert host["overflows"] == 0
+    assert np.array_equal(pre_ids, host["ids"]) and np.array_equal(bits(pre_t), bits(host["t"]))
+    assert np.array_equal(hits["primitive_id"], pre_ids)
+    assert np.array_equal(bits(hits["t"]), bits(pre_t))
+    return hits, o
+
+
+def test_closest_hit_exact_ties(gpu):
+    """Rays that hit shared edges and vertices of a triangulated grid, every
+    triangle present twice under two ids: several triangles at exactly the same
+    t. The closest-hit walk must keep the one the right-first preorder walk
+    reaches first."""
+    print_case("closest ties")
+    k = 48
+    g = np.arange(k + 1, dtype=np.float32) / k
+    tris = []
+    for i in range(k):
+        for j in range(k):
+            a = (g[i], g[j], 0.5)
+            b = (g[i + 1], g[j], 0.5)
+            c = (g[i + 1], g[j + 1], 0.5)
+            d = (g[i], g[j + 1], 0.5)
+            tris += [(a, b, c), (a, c, d)]
+    tris = np.asarray(tris, np.float32).reshape(-1, 9)
+    tris = np.concatenate([tris, tris])  # exact duplicates under other ids
+    ids = np.random.default_rng(5).permutation(len(tris)).astype(np.int32)
+    # origins on grid vertices, edge midpoints and cell diagonals, straight up
+    pts = np.arange(2 * k + 1, dtype=np.float32) / (2 * k)
+    px, py = np.meshgrid(pts, pts)
+    n = px.size
+    rays = np.zeros((n, 9), np.float32)
+    rays[:, 0], rays[:, 1] = px.ravel(), py.ravel()
+    rays[:, 5] = 1.0
+    rays[:, 7] = np.inf
+    hits, _ = check_closest_against_oracle(gpu, tris, ids, rays)
+    assert (hits["primitive_id"] >= 0).mean() > 0.99
+
+
+def test_closest_hit_deep_stack_falls_back_to_preorder(gpu):
+    """2^18 parallel triangles stacked along x, every one crossing the rays'
+    line: the near-child-first walk keeps one pending sibling per level with two
+    internal children, more than its 16-entry stack holds on the way to the
+    nearest leaf, so those rays are walked again in preorder. Results equal the
+    oracle's either way."""
+    print_case("closest deep stack")
+    m = 1 << 18
+    x = ((np.arange(m) + 0.5) / m).astype(np.float32)
+    tris = np.zeros((m, 9), np.float32)
+    tris[:, 0] = tris[:, 3] = tris[:, 6] = x
+    tris[:, 4] = 2.0
+    tris[:, 8] = 2.0
+    ids = np.arange(m, dtype=np.int32)
+    rng = np.random.default_rng(11)
+    k = 64
+    rays = np.zeros((2 * k, 9), np.float32)
+    rays[:, 1:3] = rng.uniform(0.3, 0.7, (2 * k, 2))
+    rays[:k, 0], rays[:k, 3] = -0.5, 1.0  # +x: the nearest leaf is the leftmost
+    rays[k:, 0], rays[k:, 3] = 1.5, -1.0  # -x: the nearest leaf is the rightmost
+    rays[:, 7] = np.inf
+    hits, o = check_closest_against_oracle(gpu, tris, ids, rays)
+    assert np.array_equal(hits["primitive_id"][:k], np.zeros(k, np.int32))
+    assert np.array_equal(hits["primitive_id"][k:], np.full(k, m - 1, np.int32))
+    # levels with two internal children on the path to the leftmost leaf: the
+    # stack depth a +x ray reaches there
+    left, right = o.left.astype(np.int64), o.right.astype(np.int64)
+    node, both = 0, 0
+    while True:
+        li, ri = (left[node] & 0x80000000) == 0, (right[node] & 0x80000000) == 0
+        both += int(li and ri)
+        if not li:
+            break
+        node = int(left[node])
+    assert both > 16
+
+
 def test_full_size_microbench_inputs(gpu):
     """1M-triangle soup, 4M rays: the tree equals the oracle's bit for bit; a
     65536-ray sample matches the oracle traversal; all rays satisfy any-hit ==
