@@ -212,8 +212,10 @@ __device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, 
                                              float t_max, int ignore, float& t_hit, float& u_hit, float& v_hit,
                                              uint32_t* stack, int stride) {
     if constexpr (kTwoLevel) {
-        const TwoLevelHit h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
-                                                    ignore);
+        TwoLevelHit h;
+        if (!two_level_closest_ordered(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore, stack,
+                                       stride, h))
+            h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore);
         if (h.prim >= 0) {
             t_hit = h.t;
             u_hit = h.u;
@@ -348,12 +350,12 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
 // counted by counts[sh_in] (-1: none), in one grid-stride pass. Block 0 clears
 // the counters the following shade launch appends to (zero_a, zero_b; never
 // the ones read here) and adds the queue sizes to the ray totals.
-// At least 8 waves per SIMD (64 VGPRs): the one-level variants fit as they
-// are; the two-level walk spills a little and still measured 8 % faster on an
-// instanced render than at its natural 75-82 VGPRs (5-6 waves), since the walk
-// waits on dependent node loads.
+// One-level variants: at least 8 waves per SIMD (64 VGPRs, 65 uncapped). C3
+// measured 0.079 s at 8 waves against 0.096 s at 7. Two-level variants keep
+// their natural 81-100 VGPRs (4-5 waves): capped at 8 waves they spill 116-168
+// bytes per lane and the instanced render measured 94.8 ms against 92.1 ms.
 template <int B, bool kAnyShadow, bool kTwoLevel>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kTwoLevel ? 1 : 8))) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
                                                   int sh_in, int zero_a, int zero_b) {
     const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
     const uint32_t n_sh = sh_in >= 0 ? qs.counts[sh_in] : 0u;
@@ -363,9 +365,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void k_t
         qs.rays[0] += n_ext;
         qs.rays[1] += n_sh;
     }
-    // per-lane stacks of the ordered closest-hit walk (the one-level walk only)
-    __shared__ uint32_t lds_stack[kTwoLevel ? 1 : kOrderedStack * B];
-    uint32_t* stack = lds_stack + (kTwoLevel ? 0 : threadIdx.x);
+    // per-lane stacks of the ordered closest-hit walks
+    __shared__ uint32_t lds_stack[kOrderedStack * B];
+    uint32_t* stack = lds_stack + threadIdx.x;
     const uint32_t total = n_ext + n_sh;
     for (uint32_t i = xcd_block() * B + threadIdx.x; i < total; i += gridDim.x * B) {
         if (i < n_ext) {

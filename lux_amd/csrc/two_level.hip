@@ -16,9 +16,13 @@ __global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, co
                                                               lux_scene_hit* __restrict__ hits) {
     const int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
+    __shared__ uint32_t stack[kOrderedStack * kBlock];
     const lux_ray r = rays[i];
-    const TwoLevelHit h =
-        two_level_walk<false>(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ignore ? ignore[i] : -1);
+    const int ign = ignore ? ignore[i] : -1;
+    TwoLevelHit h;
+    if (!two_level_closest_ordered(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ign, stack + threadIdx.x,
+                                   kBlock, h))
+        h = two_level_walk<false>(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ign);
     lux_scene_hit out;
     out.primitive_id = h.prim;
     out.t = h.prim >= 0 ? h.t : INFINITY;
@@ -273,6 +277,7 @@ void two_level_build(TwoLevelDev& s, const lux_triangle* tris, const int32_t* of
         lbvh_build(s.blas[m], d_boxes.p, d_ids.p, n, stream);
         sync_and_check(s.blas[m], stream, "BLAS build");
         meshes[m].nodes = s.blas[m].pairs;
+        meshes[m].slot_pre = s.blas[m].slot_pre;
         meshes[m].tris = s.soup + offsets[m];
         meshes[m].node_count = n > 1 ? 2 * n : n;
         meshes[m].tri_count = n;
@@ -299,6 +304,7 @@ void two_level_set_instances(TwoLevelDev& s, const lux_instance* instances, int 
 TwoLevelView two_level_view(const TwoLevelDev& s) {
     TwoLevelView v;
     v.tlas_nodes = s.tlas.pairs;
+    v.tlas_slot_pre = s.tlas.slot_pre;
     v.instances = s.instances;
     v.meshes = s.meshes;
     v.info = s.tlas.info;

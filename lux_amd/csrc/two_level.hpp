@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "lux_runtime.hpp"
+#include "ordered.hpp"
 
 namespace lux {
 
@@ -15,7 +16,8 @@ namespace lux {
 struct DevMesh {
     const lux_bvh_node* nodes;
     const lux_triangle* tris;
-    int32_t node_count;  // sibling-pair slots (0: empty mesh)
+    const uint32_t* slot_pre;  // preorder position of each pair slot (tie order)
+    int32_t node_count;        // sibling-pair slots (0: empty mesh)
     int32_t tri_count;
 };
 
@@ -32,6 +34,7 @@ static_assert(sizeof(DevInstance) == 80, "instance record");
 // What the walk kernels take by value.
 struct TwoLevelView {
     const lux_bvh_node* tlas_nodes;
+    const uint32_t* tlas_slot_pre;
     const DevInstance* instances;
     const DevMesh* meshes;
     DevInfo* info;
@@ -169,6 +172,192 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
         }
     }
     return best;
+}
+
+// Closest hit of the same two-level scene, near child first at both levels
+// (ordered.hpp). A TLAS step tests both children of a node; each child that
+// passes is an internal node (its pair slot) or an instance leaf (its slot
+// tagged with kInstanceItem). The nearer is taken next and the other pushed on
+// the lane's LDS stack; taking an instance leaf moves the ray to object space
+// and walks that BLAS near child first on the same stack above the entry
+// depth, returning to the world ray when the stack falls back to it.
+// The preorder walk above visits triangles in the order (TLAS preorder of the
+// instance leaf, BLAS preorder of the triangle leaf) and keeps the first of
+// equal t; this walk keeps, of equal t, the one smallest in that order
+// (slot_pre of both levels, read only on an exact tie), so both return the
+// same hit. Returns false when the stack would overflow or the arrays are
+// malformed (error bits set for the latter); the caller then runs
+// two_level_walk<false> for the ray.
+constexpr uint32_t kInstanceItem = 0x80000000u;
+
+__device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s, const f3& WO, const f3& WD,
+                                                          float t_min, float t_max, int ignore, uint32_t* stack,
+                                                          int stride, TwoLevelHit& best) {
+    best = TwoLevelHit{};
+    const uint32_t tlas_nc = uint32_t(s.tlas_node_count);
+    if (tlas_nc == 0u) return true;
+    const SlabRay wsr = make_slab_ray(WO, WD, t_min);
+    SlabRay sr = wsr;
+    f3 O = WO, D = WD;
+    const lux_bvh_node* nodes = s.tlas_nodes;
+    uint32_t nc = tlas_nc;
+    bool in_blas = false;
+    const lux_triangle* tris = nullptr;
+    const uint32_t* blas_pre = nullptr;
+    int tcount = 0, first = -1, inst_cur = -1, ignore_local = -1;
+    uint32_t inst_slot = 0, best_inst_slot = 0, best_slot = 0;
+    int sp = 0, sp0 = 0;
+    uint32_t steps = s.step_limit;
+    uint32_t item;
+    {
+        float4 a, b;
+        load_node32(nodes, 0, a, b);
+        float tn;
+        const bool par = wave_any_parallel(sr);
+        const bool hit = par ? slab_near<true>(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn)
+                             : slab_near<false>(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn);
+        if (!hit) return true;
+        const int p = __float_as_int(b.z);
+        item = p >= 0 ? kInstanceItem : ~uint32_t(p);
+    }
+    // Tests BLAS leaf `slot` (local triangle prim) of the current instance.
+    auto leaf = [&](uint32_t slot, int prim) {
+        if (prim == ignore_local) return;  // should_skip_primitive
+        if (prim >= tcount) {
+            set_error(s.info, kErrPrimRange);
+            return;
+        }
+        const float* p = reinterpret_cast<const float*>(tris + prim);
+        const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
+        float t, u, v;
+        if (!tri_hit(O, D, t_min, tie_bound(t_max, best.prim >= 0), v0, v1, v2, t, u, v)) return;
+        if (!(t < t_max) && (inst_slot != best_inst_slot ? s.tlas_slot_pre[inst_slot] >= s.tlas_slot_pre[best_inst_slot]
+                                                         : blas_pre[slot] >= blas_pre[best_slot]))
+            return;
+        t_max = t;
+        best.prim = first + prim;
+        best.instance = inst_cur;
+        best.local = prim;
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        best_inst_slot = inst_slot;
+        best_slot = slot;
+    };
+    for (;;) {
+        if (steps-- == 0u) {
+            set_error(s.info, kErrWalkOrder);
+            return false;
+        }
+        bool descend = false;
+        if (item & kInstanceItem) {
+            // enter the instance of TLAS leaf slot `ls`, if it has a BLAS
+            const uint32_t ls = item & ~kInstanceItem;
+            if (ls >= tlas_nc) {
+                set_error(s.info, kErrWalkOrder);
+                return false;
+            }
+            float4 a, b;
+            load_node32(s.tlas_nodes, ls, a, b);
+            const int prim = __float_as_int(b.z);
+            const DevInstance* in = (prim >= 0 && prim < s.instance_count) ? &s.instances[prim] : nullptr;
+            const bool ok = in && in->mesh >= 0 && in->mesh < s.mesh_count;
+            const DevMesh mesh = ok ? s.meshes[in->mesh] : DevMesh{};
+            if (ok && mesh.node_count > 0 && in->first >= 0 && uint32_t(mesh.node_count) <= kOffset32Nodes) {
+                float inv[16];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float4 row = reinterpret_cast<const float4*>(in->inv)[i];
+                    inv[4 * i] = row.x;
+                    inv[4 * i + 1] = row.y;
+                    inv[4 * i + 2] = row.z;
+                    inv[4 * i + 3] = row.w;
+                }
+                in_blas = true;
+                sp0 = sp;
+                nodes = mesh.nodes;
+                nc = uint32_t(mesh.node_count);
+                tris = mesh.tris;
+                blas_pre = mesh.slot_pre;
+                tcount = mesh.tri_count;
+                first = in->first;
+                inst_cur = prim;
+                inst_slot = ls;
+                ignore_local = (ignore >= first && ignore - first < tcount) ? ignore - first : -1;
+                O = xform_point(inv, WO);
+                D = xform_vector(inv, WD);
+                sr = make_slab_ray(O, D, t_min);
+                load_node32(nodes, 0, a, b);
+                float tn;
+                const bool par = wave_any_parallel(sr);
+                const bool hit = par ? slab_near<true>(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn)
+                                     : slab_near<false>(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn);
+                const int root = __float_as_int(b.z);
+                if (hit && root >= 0) leaf(0u, root);
+                if (hit && root < 0) {
+                    item = ~uint32_t(root);
+                    descend = true;
+                }
+            } else if (ok && mesh.node_count > 0 && in->first >= 0) {
+                return false;  // a BLAS beyond 32-bit node offsets: the preorder walk takes the ray
+            }
+        } else {
+            const uint32_t c = item;
+            if (c + 1 >= nc) {
+                set_error(s.info, kErrWalkOrder);
+                return false;
+            }
+            float4 a0, b0, a1, b1;
+            load_node32(nodes, c, a0, b0);
+            load_node32(nodes, c + 1, a1, b1);
+            float tn0, tn1;
+            const bool par = wave_any_parallel(sr);
+            const bool h0 = par ? slab_near<true>(sr, t_max, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0)
+                                : slab_near<false>(sr, t_max, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
+            const bool h1 = par ? slab_near<true>(sr, t_max, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1)
+                                : slab_near<false>(sr, t_max, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
+            const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
+            bool take0, take1;
+            uint32_t i0, i1;
+            if (in_blas) {
+                if (h0 && p0 >= 0) leaf(c, p0);
+                if (h1 && p1 >= 0) leaf(c + 1, p1);
+                take0 = h0 && p0 < 0;
+                take1 = h1 && p1 < 0;
+                i0 = ~uint32_t(p0);
+                i1 = ~uint32_t(p1);
+            } else {
+                take0 = h0;
+                take1 = h1;
+                i0 = p0 >= 0 ? (c | kInstanceItem) : ~uint32_t(p0);
+                i1 = p1 >= 0 ? ((c + 1) | kInstanceItem) : ~uint32_t(p1);
+            }
+            if (take0 && take1) {
+                if (sp == kOrderedStack) return false;
+                const bool far0 = tn1 < tn0;
+                stack[sp * stride] = far0 ? i0 : i1;
+                ++sp;
+                item = far0 ? i1 : i0;
+                descend = true;
+            } else if (take0 || take1) {
+                item = take0 ? i0 : i1;
+                descend = true;
+            }
+        }
+        if (descend) continue;
+        // pop: leave the instance once its part of the stack is used up
+        if (in_blas && sp == sp0) {
+            in_blas = false;
+            nodes = s.tlas_nodes;
+            nc = tlas_nc;
+            sr = wsr;
+            O = WO;
+            D = WD;
+        }
+        if (sp == 0) return true;
+        --sp;
+        item = stack[sp * stride];
+    }
 }
 
 
