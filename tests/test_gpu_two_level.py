@@ -150,3 +150,61 @@ def test_instanced_microbench_inputs_match_oracle(gpu):
     assert np.array_equal(s.occluded(inp["rays"]).astype(bool), h["primitive_id"] >= 0)
     assert (h["primitive_id"] >= 0).mean() > 0.2
     s.close()
+
+
+def test_two_level_ordered_walk_ties_and_deep_stack(gpu):
+    """Closest hits that stress the near-child-first two-level walk: two
+    instances of one triangulated grid under the same transform (every hit is
+    an exact t tie between instances, and the grid's shared edges tie within
+    one), and a mesh of 2^18 parallel triangles stacked along the rays, deep
+    enough that a lane's stack overflows and the ray is walked again in
+    preorder. Ids, instances and t equal the oracle's walk."""
+    print_case("two-level ties and deep stack")
+    m = 1 << 18
+    x = ((np.arange(m) + 0.5) / m).astype(np.float32)
+    stack_tris = np.zeros((m, 9), np.float32)
+    stack_tris[:, 0] = stack_tris[:, 3] = stack_tris[:, 6] = x
+    stack_tris[:, 4] = 2.0
+    stack_tris[:, 8] = 2.0
+    k = 16
+    g1 = np.arange(k + 1, dtype=np.float32) / k
+    grid = []
+    for i in range(k):
+        for j in range(k):
+            a, b = (g1[i], g1[j], 0.5), (g1[i + 1], g1[j], 0.5)
+            c, d = (g1[i + 1], g1[j + 1], 0.5), (g1[i], g1[j + 1], 0.5)
+            grid += [(a, b, c), (a, c, d)]
+    grid = np.asarray(grid, np.float32).reshape(-1, 9)
+    tris = np.concatenate([stack_tris, grid])
+    off = np.array([0, m, m + len(grid)], np.int32)
+    eye = np.eye(4, dtype=np.float32)
+    shifted_x, raised = eye.copy(), eye.copy()
+    shifted_x[0, 3] = 2.0
+    raised[2, 3] = 5.0
+    inst_mesh = np.array([0, 1, 0, 1], np.int32)
+    inst_m = np.stack([eye, raised, shifted_x, raised])
+    rec = TL.make_instances(inst_mesh, inst_m, tris=tris, mesh_offsets=off)
+    rng = np.random.default_rng(12)
+    n = 64
+    rays = np.zeros((3 * n + (2 * k + 1) ** 2, 9), np.float32)
+    rays[:, 7] = np.inf
+    rays[: 2 * n, 1:3] = rng.uniform(0.3, 0.7, (2 * n, 2))
+    rays[:n, 0], rays[:n, 3] = -0.5, 1.0  # +x into the first stack: nearest is its leftmost triangle
+    rays[n:2 * n, 0], rays[n:2 * n, 3] = 3.5, -1.0  # -x into the shifted stack: its rightmost
+    rays[2 * n:3 * n, 0:2] = rng.uniform(0.1, 0.9, (n, 2))
+    rays[2 * n:3 * n, 2], rays[2 * n:3 * n, 5] = 4.0, 1.0  # up into the two coincident grids
+    pts = np.arange(2 * k + 1, dtype=np.float32) / (2 * k)
+    px, py = np.meshgrid(pts, pts)
+    rays[3 * n:, 0], rays[3 * n:, 1] = px.ravel(), py.ravel()
+    rays[3 * n:, 2], rays[3 * n:, 5] = 4.0, 1.0  # grid vertices and edge midpoints
+    s = TL.TwoLevelScene(tris, off, rec)
+    h = s.intersect(rays)
+    o = oracle.OracleTwoLevel(tris, off, inst_mesh, rec["world_to_object"], rec["world_bounds"]).trace(rays)
+    assert o["overflows"] == 0
+    assert np.array_equal(h["primitive_id"], o["prim"])
+    assert np.array_equal(h["instance_id"], o["instance"])
+    assert np.array_equal(bits(h["t"]), bits(o["t"]))
+    assert np.array_equal(h["local_primitive_id"][:n], np.zeros(n, np.int32))
+    assert np.array_equal(h["local_primitive_id"][n:2 * n], np.full(n, m - 1, np.int32))
+    assert (h["primitive_id"][2 * n:] >= 0).mean() > 0.99
+    s.close()
