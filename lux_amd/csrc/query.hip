@@ -363,9 +363,9 @@ void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_tria
     if (n <= 0) return;
     int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
     // The near-child-first walk touches the nodes near each ray's origin first,
-    // which neighbouring rays share: tracing in origin order measured 5.35 ms
-    // against 5.60 ms in input order on 2^22 random rays over 2^20 triangles,
-    // sort included.
+    // which neighbouring rays share: tracing in ray order (octant, then origin)
+    // measured 5.12 ms against 5.60 ms in input order on 2^22 random rays over
+    // 2^20 triangles, sort included.
     const uint32_t* ord = maybe_order(d, order, rays, n, s);
     if (wide)
         LUX_LAUNCH("trace_closest", (k_trace_closest<true, false>), div_up(n, kBlock), kBlock, 0, s,
