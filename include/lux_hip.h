@@ -549,8 +549,7 @@ int lux_hip_scene_triangles(lux_hip_scene* scene, float* corners, float* normals
  * default material, the boundary-edge mask per triangle, and the instance
  * transform object_to_world * placement (make_obj_placement_transform). The
  * mesh is object space; it enters a scene through lux_instanced_scene_desc
- * with that transform. Boundary-hole capping is not built. Host code: needs no
- * GPU and no lux_hip_init. */
+ * with that transform. Host code: needs no GPU and no lux_hip_init. */
 enum { LUX_OBJ_NORMALS_SOURCE_OR_GEOMETRY = 0, LUX_OBJ_NORMALS_GENERATE_MISSING_SMOOTH = 1,
        LUX_OBJ_NORMALS_FACE_ONLY = 2 };
 enum { LUX_OBJ_SCALE_NONE = 0, LUX_OBJ_SCALE_MAX_EXTENT = 1, LUX_OBJ_SCALE_X_EXTENT = 2 };
@@ -575,6 +574,8 @@ typedef struct lux_obj_options {
     int32_t binding_count;         /* usemtl name -> scene material id */
     const char* const* binding_names;
     const int32_t* binding_material_ids;
+    int32_t cap_boundary_holes;    /* fan-fill closed boundary loops (ObjLoadOptions::cap_boundary_holes) */
+    int32_t cap_boundary_topology; /* LUX_OBJ_TOPOLOGY_* the caps are traced over (default: welded) */
 } lux_obj_options;
 typedef struct lux_obj_mesh lux_obj_mesh;
 
@@ -589,6 +590,14 @@ int lux_obj_mesh_free(lux_obj_mesh* mesh);
 int lux_obj_mesh_counts(const lux_obj_mesh* mesh, int32_t* vertex_count, int32_t* triangle_count,
                         int32_t* appended_default_material, int32_t* skipped_invalid, int32_t* skipped_nonfinite,
                         int32_t* skipped_degenerate);
+/* Boundary-hole capping counts (ObjLoadResult, src/util/obj_loader.h:80-88):
+ * cap triangles appended, boundary edges of the capping topology, boundary
+ * vertices that are not on exactly one incoming and one outgoing edge (any
+ * makes the loader skip capping), open loops, loops under 3 vertices and
+ * cap triangles of area <= 1e-12 skipped. Any pointer may be NULL. */
+int lux_obj_mesh_cap_counts(const lux_obj_mesh* mesh, int32_t* cap_triangles, int32_t* boundary_edges,
+                            int32_t* non_simple_vertices, int32_t* incomplete_loops, int32_t* degenerate_loops,
+                            int32_t* degenerate_triangles);
 /* The mesh arrays (sized by the counts; any may be NULL) and the instance
  * transform (16 + 16 floats, row-major). */
 int lux_obj_mesh_arrays(const lux_obj_mesh* mesh, lux_vec3* vertices, lux_vec3* normals, float* uvs,

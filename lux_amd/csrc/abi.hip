@@ -1199,6 +1199,22 @@ int lux_obj_mesh_counts(const lux_obj_mesh* m, int32_t* vertex_count, int32_t* t
     });
 }
 
+int lux_obj_mesh_cap_counts(const lux_obj_mesh* m, int32_t* cap_triangles, int32_t* boundary_edges,
+                            int32_t* non_simple_vertices, int32_t* incomplete_loops, int32_t* degenerate_loops,
+                            int32_t* degenerate_triangles) {
+    return guarded([&] {
+        if (!m) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        const lux::obj::Mesh& x = m->mesh;
+        if (cap_triangles) *cap_triangles = x.cap_triangles;
+        if (boundary_edges) *boundary_edges = x.cap_boundary_edges;
+        if (non_simple_vertices) *non_simple_vertices = x.cap_non_simple_vertices;
+        if (incomplete_loops) *incomplete_loops = x.cap_incomplete_loops;
+        if (degenerate_loops) *degenerate_loops = x.cap_degenerate_loops;
+        if (degenerate_triangles) *degenerate_triangles = x.cap_degenerate_triangles;
+        return int(LUX_OK);
+    });
+}
+
 int lux_obj_mesh_arrays(const lux_obj_mesh* m, lux_vec3* vertices, lux_vec3* normals, float* uvs, uint32_t* indices,
                         int32_t* material_ids, uint8_t* boundary_edges, float* object_to_world,
                         float* world_to_object) {

@@ -163,6 +163,9 @@ struct Mesh {
     int appended_default_material = 0;
     int skipped_invalid = 0, skipped_nonfinite = 0, skipped_degenerate = 0, nonfinite_vertices = 0;
     int invalid_normal_refs = 0, nonfinite_normal_refs = 0, zero_normal_refs = 0;
+    // boundary-hole capping (ObjLoadResult's cap_* counts)
+    int cap_triangles = 0, cap_boundary_edges = 0, cap_non_simple_vertices = 0, cap_incomplete_loops = 0,
+        cap_degenerate_loops = 0, cap_degenerate_triangles = 0;
 };
 void load(const std::string& path, const lux_obj_options& options, Mesh& out);
 void default_options(lux_obj_options& options);

@@ -13,7 +13,8 @@
 // p (indices checked, primitives dropped), usemtl, mtllib (ignored), o, g, s,
 // comments, \n and \r\n line ends. Triangulation as the parser's: quads split
 // along the shorter diagonal, larger polygons by ear clipping in the plane of
-// the two dominant axes. Not built: boundary-hole capping.
+// the two dominant axes. Boundary holes are capped as the loader caps them
+// (src/util/obj_loader.cpp:1119-1254) when asked.
 #include <algorithm>
 #include <cerrno>
 #include <charconv>
@@ -507,7 +508,11 @@ void load(const std::string& path, const lux_obj_options& o, Mesh& out) {
         throw Error(LUX_ERR_INVALID_ARGUMENT, "placement target y must be finite");
     if (!instance_transform_valid(o.object_to_world, o.world_to_object))
         throw Error(LUX_ERR_INVALID_ARGUMENT, "object_to_world must be affine-invertible with a consistent inverse");
-    if (o.boundary_topology == 1 && (!std::isfinite(o.position_weld_epsilon) || o.position_weld_epsilon <= 0))
+    if (o.cap_boundary_holes && (o.cap_boundary_topology < 0 || o.cap_boundary_topology > 1))
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "unknown cap boundary topology");
+    // validate_obj_weld_options (src/util/obj_loader.cpp:356-371)
+    const bool welded = o.boundary_topology == 1 || (o.cap_boundary_holes && o.cap_boundary_topology == 1);
+    if (welded && (!std::isfinite(o.position_weld_epsilon) || o.position_weld_epsilon <= 0))
         throw Error(LUX_ERR_INVALID_ARGUMENT, "weld epsilon must be finite and positive");
 
     std::ifstream f(path, std::ios::binary);
@@ -614,12 +619,21 @@ void load(const std::string& path, const lux_obj_options& o, Mesh& out) {
     if (faces.empty()) return;  // nothing appended, no material added
     m.appended_default_material = append_default ? 1 : 0;
 
-    // boundary edges: OBJ vertex ids, or positions welded on a grid of epsilon
-    std::vector<int> topo(static_cast<size_t>(nv), -1);
-    if (o.boundary_topology == 0) {
-        for (int i = 0; i < nv; ++i)
-            if (v_finite[i]) topo[i] = i;
-    } else {
+    // topology vertices (make_topology_vertices, src/util/obj_loader.cpp:476-519):
+    // OBJ vertex ids, or positions welded on a grid of epsilon, each with the
+    // position of its first vertex
+    auto make_topology = [&](int topology, std::vector<int>& ids, std::vector<V3>& positions) {
+        ids.assign(static_cast<size_t>(nv), -1);
+        positions.clear();
+        if (topology == 0) {
+            positions.assign(static_cast<size_t>(nv), V3{0, 0, 0});
+            for (int i = 0; i < nv; ++i)
+                if (v_finite[i]) {
+                    ids[i] = i;
+                    positions[size_t(i)] = pos(i);
+                }
+            return;
+        }
         struct Key {
             int64_t x, y, z;
             bool operator==(const Key& k) const { return x == k.x && y == k.y && z == k.z; }
@@ -632,38 +646,59 @@ void load(const std::string& path, const lux_obj_options& o, Mesh& out) {
                 return size_t(h);
             }
         };
-        std::unordered_map<Key, int, KeyHash> welded;
+        std::unordered_map<Key, int, KeyHash> welded_ids;
         const double inv_eps = 1.0 / double(o.position_weld_epsilon);
-        int next = 0;
         for (int i = 0; i < nv; ++i) {
             if (!v_finite[i]) continue;
             const V3 q = pos(i);
             const Key k{std::llround(double(q.x) * inv_eps), std::llround(double(q.y) * inv_eps),
                         std::llround(double(q.z) * inv_eps)};
-            auto it = welded.find(k);
-            if (it != welded.end()) {
-                topo[i] = it->second;
+            auto it = welded_ids.find(k);
+            if (it != welded_ids.end()) {
+                ids[i] = it->second;
             } else {
-                welded.emplace(k, next);
-                topo[i] = next++;
+                welded_ids.emplace(k, int(positions.size()));
+                ids[i] = int(positions.size());
+                positions.push_back(q);
             }
         }
-    }
+    };
     auto edge_key = [](int a, int b) {
         return (uint64_t(uint32_t(std::min(a, b))) << 32) | uint32_t(std::max(a, b));
     };
-    std::unordered_map<uint64_t, int> edges;
-    for (const Face& fc : faces) {
-        const int a = topo[fc.idx[0].v], b = topo[fc.idx[1].v], c = topo[fc.idx[2].v];
-        if (a != b) ++edges[edge_key(a, b)];
-        if (b != c) ++edges[edge_key(b, c)];
-        if (c != a) ++edges[edge_key(c, a)];
-    }
-    auto boundary = [&](int a, int b) {
-        if (a < 0 || b < 0 || a == b) return false;
-        auto it = edges.find(edge_key(a, b));
-        return it != edges.end() && it->second == 1;
+    auto count_edges = [&](const std::vector<int>& ids, std::unordered_map<uint64_t, int>& counts) {
+        for (const Face& fc : faces) {
+            const int a = ids[fc.idx[0].v], b = ids[fc.idx[1].v], c = ids[fc.idx[2].v];
+            if (a != b) ++counts[edge_key(a, b)];
+            if (b != c) ++counts[edge_key(b, c)];
+            if (c != a) ++counts[edge_key(c, a)];
+        }
     };
+    auto is_boundary = [&](const std::unordered_map<uint64_t, int>& counts, int a, int b) {
+        if (a < 0 || b < 0 || a == b) return false;
+        auto it = counts.find(edge_key(a, b));
+        return it != counts.end() && it->second == 1;
+    };
+    // boundary-edge masks over the chosen topology
+    std::vector<int> topo;
+    std::vector<V3> topo_pos;
+    make_topology(o.boundary_topology, topo, topo_pos);
+    std::unordered_map<uint64_t, int> edges;
+    count_edges(topo, edges);
+    // the capping topology and its edge counts
+    std::vector<int> cap_topo;
+    std::vector<V3> cap_pos;
+    std::unordered_map<uint64_t, int> cap_edges;
+    if (o.cap_boundary_holes) {
+        make_topology(o.cap_boundary_topology, cap_topo, cap_pos);
+        count_edges(cap_topo, cap_edges);
+        for (const auto& e : cap_edges) m.cap_boundary_edges += e.second == 1 ? 1 : 0;
+    }
+    struct CapEdge {
+        int from, to, material;
+    };
+    std::vector<CapEdge> cap_edge_list;
+    auto boundary = [&](int a, int b) { return is_boundary(edges, a, b); };
 
     // vertices: one per distinct (position, used normal, used texcoord), first use first
     auto normal_used = [&](const Idx& i) { return o.normal_mode != 2 && normal_nonzero(i); };
@@ -723,6 +758,117 @@ void load(const std::string& path, const lux_obj_options& o, Mesh& out) {
         m.indices.insert(m.indices.end(), {i0, i1, i2});
         m.material_ids.push_back(fc.material);
         m.boundary_edges.push_back(mask);
+        if (o.cap_boundary_holes) {
+            const int ca[3] = {cap_topo[fc.idx[0].v], cap_topo[fc.idx[1].v], cap_topo[fc.idx[2].v]};
+            for (int k = 0; k < 3; ++k)
+                if (is_boundary(cap_edges, ca[k], ca[(k + 1) % 3]))
+                    cap_edge_list.push_back(CapEdge{ca[k], ca[(k + 1) % 3], fc.material});
+        }
+    }
+
+    // boundary-hole caps (src/util/obj_loader.cpp:1119-1254): with every
+    // boundary vertex on exactly one outgoing and one incoming boundary edge,
+    // each closed loop of 3 or more vertices is fanned from its centroid, the
+    // fan turned to face away from the mesh centre (then flipped again for
+    // reverse_orientation); each cap triangle gets three new vertices with its
+    // face normal and uv 0
+    if (o.cap_boundary_holes && !cap_edge_list.empty()) {
+        // topology_center_for_faces: mean of the distinct topology vertices in first-use order
+        V3 mesh_center{0, 0, 0};
+        {
+            int count = 0;
+            std::unordered_set<int> seen;
+            for (const Face& fc : faces)
+                for (int k = 0; k < 3; ++k) {
+                    const int id = cap_topo[fc.idx[k].v];
+                    if (id < 0 || id >= int(cap_pos.size()) || !seen.insert(id).second) continue;
+                    const V3 q = cap_pos[size_t(id)];
+                    mesh_center = V3{mesh_center.x + q.x, mesh_center.y + q.y, mesh_center.z + q.z};
+                    ++count;
+                }
+            if (count > 0) {
+                const float inv = 1 / float(count);
+                mesh_center = V3{mesh_center.x * inv, mesh_center.y * inv, mesh_center.z * inv};
+            } else {
+                mesh_center = V3{0, 0, 0};
+            }
+        }
+        std::unordered_map<int, int> outgoing, incoming;
+        for (const CapEdge& e : cap_edge_list) {
+            ++outgoing[e.from];
+            ++incoming[e.to];
+        }
+        for (const auto& e : outgoing) {
+            auto it = incoming.find(e.first);
+            if (e.second != 1 || it == incoming.end() || it->second != 1) ++m.cap_non_simple_vertices;
+        }
+        for (const auto& e : incoming)
+            if (outgoing.find(e.first) == outgoing.end()) ++m.cap_non_simple_vertices;
+        std::unordered_map<int, CapEdge> next_edge;
+        if (m.cap_non_simple_vertices == 0)
+            for (const CapEdge& e : cap_edge_list) next_edge.emplace(e.from, e);
+        std::unordered_set<int> visited;
+        for (const CapEdge& first_edge : cap_edge_list) {
+            if (m.cap_non_simple_vertices != 0) break;
+            if (visited.count(first_edge.from)) continue;
+            std::vector<int> loop;
+            int material = first_edge.material;
+            const int start = first_edge.from;
+            int current = start;
+            while (!visited.count(current)) {
+                auto it = next_edge.find(current);
+                if (it == next_edge.end()) break;
+                visited.insert(current);
+                loop.push_back(current);
+                material = it->second.material;
+                current = it->second.to;
+                if (current == start) break;
+            }
+            if (current != start) {
+                ++m.cap_incomplete_loops;
+                continue;
+            }
+            if (loop.size() < 3) {
+                ++m.cap_degenerate_loops;
+                continue;
+            }
+            V3 center{0, 0, 0};
+            for (int id : loop) {
+                const V3 q = cap_pos[size_t(id)];
+                center = V3{center.x + q.x, center.y + q.y, center.z + q.z};
+            }
+            {
+                const float inv = 1 / float(loop.size());
+                center = V3{center.x * inv, center.y * inv, center.z * inv};
+            }
+            for (size_t i = 0; i < loop.size(); ++i) {
+                V3 t0 = center, t1 = cap_pos[size_t(loop[i])], t2 = cap_pos[size_t(loop[(i + 1) % loop.size()])];
+                const V3 cr = cross(sub(t1, t0), sub(t2, t0));
+                V3 normal = normalize(cr);
+                if (0.5f * std::sqrt(dot(cr, cr)) <= 1e-12f) {
+                    ++m.cap_degenerate_triangles;
+                    continue;
+                }
+                if (dot(normal, sub(center, mesh_center)) < 0) {
+                    std::swap(t1, t2);
+                    normal = V3{-normal.x, -normal.y, -normal.z};
+                }
+                if (o.reverse_orientation) {
+                    std::swap(t1, t2);
+                    normal = V3{-normal.x, -normal.y, -normal.z};
+                }
+                const uint32_t first = uint32_t(m.vertices.size() / 3);
+                for (const V3& q : {t0, t1, t2}) {
+                    m.vertices.insert(m.vertices.end(), {q.x, q.y, q.z});
+                    m.normals.insert(m.normals.end(), {normal.x, normal.y, normal.z});
+                    m.uvs.insert(m.uvs.end(), {0.0f, 0.0f});
+                }
+                m.indices.insert(m.indices.end(), {first, first + 1, first + 2});
+                m.material_ids.push_back(material);
+                m.boundary_edges.push_back(0);
+                ++m.cap_triangles;
+            }
+        }
     }
 
     // mesh bounds (union of the triangle bounds) and the placement transform
@@ -775,6 +921,8 @@ void default_options(lux_obj_options& o) {
     o.normal_mode = 0;
     o.boundary_topology = 0;
     o.position_weld_epsilon = 1e-6f;
+    o.cap_boundary_holes = 0;
+    o.cap_boundary_topology = 1;
     o.default_material_id = -1;
     o.placement_target_extent = 1.0f;
     for (int k = 0; k < 16; ++k) o.object_to_world[k] = o.world_to_object[k] = (k % 5 == 0) ? 1.0f : 0.0f;

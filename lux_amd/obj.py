@@ -16,6 +16,8 @@ from .abi import call
 NORMAL_MODES = {"source_or_geometry": 0, "generate_missing_smooth": 1, "face_only": 2}
 SCALE_MODES = {"none": 0, "max_extent": 1, "x_extent": 2}
 TOPOLOGIES = {"source_vertices": 0, "welded_positions": 1}
+CAP_COUNTS = ("cap_triangles", "cap_boundary_edges", "cap_non_simple_vertices", "cap_incomplete_loops",
+              "cap_degenerate_loops", "cap_degenerate_triangles")
 
 
 class Options(ctypes.Structure):
@@ -26,7 +28,8 @@ class Options(ctypes.Structure):
                 ("placement_center_xz", c_int32), ("placement_ground_to_y", c_int32),
                 ("placement_target_y", c_float), ("object_to_world", c_float * 16),
                 ("world_to_object", c_float * 16), ("binding_count", c_int32),
-                ("binding_names", POINTER(c_char_p)), ("binding_material_ids", POINTER(c_int32))]
+                ("binding_names", POINTER(c_char_p)), ("binding_material_ids", POINTER(c_int32)),
+                ("cap_boundary_holes", c_int32), ("cap_boundary_topology", c_int32)]
 
 
 def _register():
@@ -36,6 +39,7 @@ def _register():
         "lux_obj_load": [c_char_p, POINTER(Options), POINTER(c_void_p)],
         "lux_obj_mesh_free": [c_void_p],
         "lux_obj_mesh_counts": [c_void_p] + [POINTER(c_int32)] * 6,
+        "lux_obj_mesh_cap_counts": [c_void_p] + [POINTER(c_int32)] * 6,
         "lux_obj_mesh_arrays": [c_void_p] * 9,
     }
     for name, args in sigs.items():
@@ -52,7 +56,8 @@ def _code(table, value):
 def options(normal_mode="source_or_geometry", reverse_orientation=False, flip_texcoord_v=False,
             boundary_topology="source_vertices", position_weld_epsilon=1e-6, default_material_id=-1,
             scene_material_count=0, scale_mode="none", target_extent=1.0, center_xz=False, ground_to_y=False,
-            target_y=0.0, object_to_world=None, material_bindings=None):
+            target_y=0.0, object_to_world=None, material_bindings=None, cap_boundary_holes=False,
+            cap_boundary_topology="welded_positions"):
     """An Options record (the reference's defaults unless given; enum options
     by name or by their ObjLoadOptions value) and the objects it points to
     (keep them alive while it is used)."""
@@ -71,6 +76,8 @@ def options(normal_mode="source_or_geometry", reverse_orientation=False, flip_te
     o.placement_center_xz = int(center_xz)
     o.placement_ground_to_y = int(ground_to_y)
     o.placement_target_y = target_y
+    o.cap_boundary_holes = int(cap_boundary_holes)
+    o.cap_boundary_topology = _code(TOPOLOGIES, cap_boundary_topology)
     keep = []
     if object_to_world is not None:
         from .two_level import inverse_affine
@@ -93,8 +100,8 @@ def options(normal_mode="source_or_geometry", reverse_orientation=False, flip_te
 def load(path, **kw):
     """The mesh load_obj_into_scene appends: dict of vertices (V, 3), normals
     (V, 3), uvs (V, 2), indices (T, 3), material_ids (T,), boundary_edges (T,),
-    object_to_world / world_to_object (4, 4), appended_default_material and the
-    skipped-face counts."""
+    object_to_world / world_to_object (4, 4), appended_default_material, the
+    skipped-face counts and the boundary-capping counts (CAP_COUNTS)."""
     o, keep = options(**kw)
     h = c_void_p()
     call("lux_obj_load", str(path).encode(), ctypes.byref(o), ctypes.byref(h))
@@ -109,6 +116,9 @@ def load(path, **kw):
         call("lux_obj_mesh_arrays", h, *[v.ctypes.data for v in out.values()])
         out.update(appended_default_material=c[2].value, skipped_invalid=c[3].value,
                    skipped_nonfinite=c[4].value, skipped_degenerate=c[5].value)
+        cc = [c_int32() for _ in range(6)]
+        call("lux_obj_mesh_cap_counts", h, *[ctypes.byref(x) for x in cc])
+        out.update(zip(CAP_COUNTS, (x.value for x in cc)))
         return out
     finally:
         call("lux_obj_mesh_free", h)

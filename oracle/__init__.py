@@ -475,10 +475,10 @@ class RefPathTracer:
 def ref_load_obj(path, normal_mode=0, reverse_orientation=False, flip_texcoord_v=False, boundary_topology=0,
                  position_weld_epsilon=1e-6, default_material_id=-1, scene_material_count=0, scale_mode=0,
                  target_extent=1.0, center_xz=False, ground_to_y=False, target_y=0.0, object_to_world=None,
-                 material_bindings=None):
+                 material_bindings=None, cap_boundary_holes=False, cap_boundary_topology=1):
     """load_obj_into_scene_with_result into a scene holding
-    `scene_material_count` materials: ok flag, the appended mesh arrays and
-    the created instance's transform (m, inv)."""
+    `scene_material_count` materials: ok flag, the appended mesh arrays, the
+    created instance's transform (m, inv) and the boundary-capping counts."""
     L = ref()
     if not hasattr(L, "_obj_ready"):
         L.ref_obj_load.restype = c_void_p
@@ -488,7 +488,10 @@ def ref_load_obj(path, normal_mode=0, reverse_orientation=False, flip_texcoord_v
         L.ref_obj_free.argtypes = [c_void_p]
         L.ref_obj_counts.argtypes = [c_void_p, c_void_p]
         L.ref_obj_arrays.argtypes = [c_void_p] * 9
+        L.ref_obj_set_cap.argtypes = [c_int32, c_int32]
+        L.ref_obj_cap_counts.argtypes = [c_void_p, c_void_p]
         L._obj_ready = True
+    L.ref_obj_set_cap(int(cap_boundary_holes), int(cap_boundary_topology))
     m = None if object_to_world is None else np.ascontiguousarray(object_to_world, np.float32).reshape(16)
     b = material_bindings or {}
     names = (ctypes.c_char_p * max(len(b), 1))(*[k.encode() for k in b])
@@ -508,6 +511,10 @@ def ref_load_obj(path, normal_mode=0, reverse_orientation=False, flip_texcoord_v
         L.ref_obj_arrays(h, *[_p(v) for v in out.values()])
         out.update(ok=bool(c[0]), appended_default_material=int(c[3]), skipped_invalid=int(c[4]),
                    skipped_nonfinite=int(c[5]), skipped_degenerate=int(c[6]))
+        cc = np.zeros(6, np.int32)
+        L.ref_obj_cap_counts(h, _p(cc))
+        out.update(zip(("cap_triangles", "cap_boundary_edges", "cap_non_simple_vertices", "cap_incomplete_loops",
+                        "cap_degenerate_loops", "cap_degenerate_triangles"), (int(x) for x in cc)))
         return out
     finally:
         L.ref_obj_free(h)

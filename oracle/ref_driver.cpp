@@ -719,6 +719,13 @@ struct ref_obj {
     bool ok = false;
 };
 
+// Capping options for the next ref_obj_load (kept out of its argument list).
+static int32_t g_obj_cap = 0, g_obj_cap_topology = 1;
+void ref_obj_set_cap(int32_t cap, int32_t topology) {
+    g_obj_cap = cap;
+    g_obj_cap_topology = topology;
+}
+
 ref_obj* ref_obj_load(const char* path, int32_t normal_mode, int32_t reverse, int32_t flip_v,
                       int32_t boundary_topology, float weld_epsilon, int32_t default_material_id,
                       int32_t scene_material_count, int32_t scale_mode, float target_extent, int32_t center_xz,
@@ -745,6 +752,8 @@ ref_obj* ref_obj_load(const char* path, int32_t normal_mode, int32_t reverse, in
         o.object_to_world = make_transform(M);
     }
     for (int b = 0; b < binding_count; ++b) o.material_bindings[binding_names[b]] = binding_ids[b];
+    o.cap_boundary_holes = g_obj_cap != 0;
+    o.cap_boundary_topology = static_cast<ObjBoundaryTopology>(g_obj_cap_topology);
     std::fflush(stderr);
     r->ok = load_obj_into_scene_with_result(path, r->scene, o, r->result);
     return r;
@@ -762,6 +771,18 @@ void ref_obj_counts(const ref_obj* r, int32_t* out7) {
     out7[4] = x.skipped_invalid_triangle_count;
     out7[5] = x.skipped_nonfinite_triangle_count;
     out7[6] = x.skipped_degenerate_triangle_count;
+}
+
+// cap triangles, cap boundary edges, non-simple vertices, incomplete loops,
+// degenerate loops, degenerate cap triangles
+void ref_obj_cap_counts(const ref_obj* r, int32_t* out6) {
+    const ObjLoadResult& x = r->result;
+    out6[0] = x.cap_triangle_count;
+    out6[1] = x.cap_boundary_edge_count;
+    out6[2] = x.cap_non_simple_boundary_vertex_count;
+    out6[3] = x.cap_incomplete_loop_count;
+    out6[4] = x.cap_degenerate_loop_count;
+    out6[5] = x.cap_degenerate_triangle_count;
 }
 
 void ref_obj_arrays(const ref_obj* r, float* vertices, float* normals, float* uvs, uint32_t* indices,

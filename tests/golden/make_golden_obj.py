@@ -5,10 +5,10 @@ tinyobjloader it bundles) through oracle/_ref.
     python tests/golden/make_golden_obj.py
 
 Inputs: small OBJ files written here (tests/golden/obj/*.obj) that exercise the
-parser and the import rules, plus the reference's own assets
+parser, the import rules and boundary-hole capping, plus the reference's own assets
 (assets/models/LUX.obj as is, stanford-bunny.obj xz-compressed). Each case
 stores the options and the loaded mesh (vertices, normals, uvs, indices,
-material ids, boundary-edge masks, instance transform, counts) in
+material ids, boundary-edge masks, instance transform, counts, capping counts) in
 tests/golden/obj_<case>.npz; a case the reference rejects stores ok = False.
 """
 
@@ -129,6 +129,62 @@ f 3 4 9
 f 4 1 9
 """
 
+# Boundary holes for the capping option: a box without its top face, its faces
+# not sharing vertex ids (welded positions join them; source ids leave six
+# separate quads); an open octagonal tube (two holes); two triangles touching at
+# one vertex (a non-simple boundary: capping is skipped).
+OPEN_BOX = """v 0 0 0
+v 1 0 0
+v 1 0 1
+v 0 0 1
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 1 0 0
+v 1 0 1
+v 1 1 1
+v 1 1 0
+v 1 0 1
+v 0 0 1
+v 0 1 1
+v 1 1 1
+v 0 0 1
+v 0 0 0
+v 0 1 0
+v 0 1 1
+usemtl bottom
+f 1 2 3 4
+usemtl sides
+f 5 8 7 6
+f 9 12 11 10
+f 13 16 15 14
+f 17 20 19 18
+"""
+
+
+def tube_obj(sides=8):
+    lines = []
+    for ring, y in ((0, -0.5), (1, 0.75)):
+        for k in range(sides):
+            a = 2 * np.pi * k / sides
+            lines.append(f"v {np.cos(a):.6f} {y} {np.sin(a):.6f}")
+    lines.append("usemtl wall")
+    for k in range(sides):
+        a, b = k + 1, (k + 1) % sides + 1
+        lines.append(f"f {a} {a + sides} {b + sides} {b}")
+    return "\n".join(lines) + "\n"
+
+
+BOWTIE = """v 0 0 0
+v 1 0 0
+v 1 1 0
+v -1 0 0
+v -1 -1 0
+f 1 2 3
+f 1 4 5
+"""
+
 CASES = {
     # name: (file, options)
     "polygons_default": ("polygons.obj", dict(scene_material_count=2)),
@@ -150,6 +206,22 @@ CASES = {
                             target_extent=1.85, center_xz=True, ground_to_y=True,
                             material_bindings={"L_lambert": 3, "U_glass": 4, "X_metal": 5})),
     "bunny": ("stanford-bunny.obj.xz", dict(scene_material_count=2, ground_to_y=True)),
+    "box_cap_welded": ("open_box.obj", dict(scene_material_count=2, material_bindings={"bottom": 0, "sides": 1},
+                                           cap_boundary_holes=True)),
+    "box_cap_source": ("open_box.obj", dict(scene_material_count=2, material_bindings={"bottom": 0, "sides": 1},
+                                           cap_boundary_holes=True, cap_boundary_topology=0, normal_mode=1)),
+    "tube_cap_reverse": ("tube.obj", dict(scene_material_count=1, material_bindings={"wall": 0},
+                                          cap_boundary_holes=True, reverse_orientation=True, scale_mode=1,
+                                          target_extent=2.0, ground_to_y=True)),
+    "bowtie_cap": ("bowtie.obj", dict(scene_material_count=1, cap_boundary_holes=True)),
+    "shells_cap": ("shells.obj", dict(scene_material_count=3, material_bindings={"a": 0, "b": 2},
+                                      cap_boundary_holes=True, boundary_topology=1)),
+    "edges_cap": ("edges.obj", dict(scene_material_count=1, cap_boundary_holes=True)),
+    "lux_cap": ("LUX.obj", dict(scene_material_count=6, normal_mode=1, boundary_topology=1, scale_mode=2,
+                                target_extent=1.85, center_xz=True, ground_to_y=True, cap_boundary_holes=True,
+                                material_bindings={"L_lambert": 3, "U_glass": 4, "X_metal": 5})),
+    "bad_cap_epsilon": ("open_box.obj", dict(scene_material_count=2, material_bindings={"bottom": 0, "sides": 1},
+                                             cap_boundary_holes=True, position_weld_epsilon=0.0)),
 }
 
 
@@ -172,19 +244,27 @@ def main():
     (OBJ / "polygons.obj").write_bytes(POLYGONS.encode())
     (OBJ / "edges.obj").write_text(EDGES)
     (OBJ / "shells.obj").write_text(SHELLS)
+    (OBJ / "open_box.obj").write_text(OPEN_BOX)
+    (OBJ / "tube.obj").write_text(tube_obj())
+    (OBJ / "bowtie.obj").write_text(BOWTIE)
     shutil.copyfile(ASSETS / "LUX.obj", OBJ / "LUX.obj")
     (OBJ / "stanford-bunny.obj.xz").write_bytes(lzma.compress((ASSETS / "stanford-bunny.obj").read_bytes(),
                                                               preset=9))
     for case, (fname, opts) in CASES.items():
         r = oracle.ref_load_obj(materialise(fname), **opts)
         data = {k: v for k, v in r.items() if isinstance(v, np.ndarray)}
-        for k in ("ok", "appended_default_material", "skipped_invalid", "skipped_nonfinite", "skipped_degenerate"):
+        for k in ("ok", "appended_default_material", "skipped_invalid", "skipped_nonfinite", "skipped_degenerate",
+                  "cap_triangles", "cap_boundary_edges", "cap_non_simple_vertices", "cap_incomplete_loops",
+                  "cap_degenerate_loops", "cap_degenerate_triangles"):
             data[k] = np.int32(r[k])
         data["obj_file"] = np.array(fname)
         data["options"] = np.array(json.dumps(opts))
         np.savez_compressed(OUT / f"obj_{case}.npz", **data)
         print(f"{case}: ok={r['ok']} vertices={len(r['vertices'])} triangles={len(r['indices'])} "
-              f"skipped={r['skipped_invalid']}/{r['skipped_nonfinite']}/{r['skipped_degenerate']}")
+              f"skipped={r['skipped_invalid']}/{r['skipped_nonfinite']}/{r['skipped_degenerate']} "
+              f"caps={r['cap_triangles']} (edges {r['cap_boundary_edges']}, non-simple "
+              f"{r['cap_non_simple_vertices']}, open {r['cap_incomplete_loops']}, degenerate "
+              f"{r['cap_degenerate_loops']}/{r['cap_degenerate_triangles']})")
     shutil.rmtree(OBJ / "_expanded", ignore_errors=True)
 
 

@@ -18,7 +18,13 @@ import pytest
 from conftest import GOLDEN, load_golden
 
 CASES = ["polygons_default", "polygons_bound", "edges_default", "edges_smooth", "edges_face_only",
-         "shells_welded", "shells_reverse", "shells_unbound", "lux", "bunny"]
+         "shells_welded", "shells_reverse", "shells_unbound", "lux", "bunny",
+         # boundary-hole capping: closed loops over welded and source topology, reverse
+         # orientation with placement, non-simple boundaries (capping skipped), a bad epsilon
+         "box_cap_welded", "box_cap_source", "tube_cap_reverse", "bowtie_cap", "shells_cap", "edges_cap",
+         "lux_cap", "bad_cap_epsilon"]
+CAP_COUNTS = ("cap_triangles", "cap_boundary_edges", "cap_non_simple_vertices", "cap_incomplete_loops",
+              "cap_degenerate_loops", "cap_degenerate_triangles")
 ARRAYS = ("vertices", "normals", "uvs", "indices", "material_ids", "boundary_edges", "object_to_world",
           "world_to_object")
 
@@ -56,6 +62,43 @@ def test_obj_import_matches_reference_loader(case, tmp_path):
         assert np.array_equal(np.ascontiguousarray(m[k]).view(np.uint8), np.ascontiguousarray(g[k]).view(np.uint8)), k
     for k in ("appended_default_material", "skipped_invalid", "skipped_nonfinite", "skipped_degenerate"):
         assert m[k] == int(g[k]), k
+    for k in CAP_COUNTS:
+        if k in g:
+            assert m[k] == int(g[k]), k
+
+
+def test_caps_close_the_box_and_the_tube(tmp_path):
+    """What the geometry fixes, apart from the reference's files: capping the
+    open box adds one fan of 4 triangles over the unit-square hole at y = 1,
+    with normals +y (away from the box centre) and total area 1; capping the
+    tube adds two fans of 8 whose areas are the regular octagon's, with normals
+    pointing out of the tube ends, flipped under reverse_orientation."""
+    g, opts, run = load_case("box_cap_welded", tmp_path)
+    m = run()
+    caps = m["indices"][-m["cap_triangles"]:]
+    assert m["cap_triangles"] == 4
+    v = m["vertices"][caps].astype(np.float64)
+    assert np.allclose(v[:, :, 1], 1.0)
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    assert np.isclose(area.sum(), 1.0, rtol=1e-6)
+    assert np.allclose(m["normals"][caps.ravel()], [0, 1, 0])
+    face_n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    assert np.all(face_n[:, 1] > 0)  # wound to match the outward normal
+    _, opts, run = load_case("tube_cap_reverse", tmp_path)
+    t = run()
+    assert t["cap_triangles"] == 16
+    caps = t["indices"][-16:]
+    v = t["vertices"][caps].astype(np.float64)
+    n = t["normals"][caps[:, 0]].astype(np.float64)
+    centre_y = t["vertices"][:, 1].astype(np.float64).mean()
+    end = np.sign(v[:, 0, 1] - centre_y)  # +1 top cap, -1 bottom cap
+    assert sorted(end.tolist()) == [-1.0] * 8 + [1.0] * 8
+    assert np.allclose(n[:, 1], -end)  # outward normals, flipped by reverse_orientation
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    octagon = 2 * np.sqrt(2)  # unit circumradius
+    scale = t["object_to_world"][0, 0]  # the placement's uniform scale is not in the object-space mesh
+    for side in (-1, 1):
+        assert np.isclose(area[end == side].sum(), octagon, rtol=1e-5), scale
 
 
 def polygon_area(p):
