@@ -9,7 +9,9 @@
 namespace lux {
 namespace {
 
-constexpr int kBlock = 256;
+// 128 threads per block: closest 20.2 -> 19.3 ms and occlusion 16.1 -> 15.7 ms
+// against 256 on 4.19 M rays over 64 instances.
+constexpr int kBlock = 128;
 
 __global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, const lux_ray* __restrict__ rays,
                                                               const int32_t* __restrict__ ignore, int n,
