@@ -51,7 +51,7 @@ def build_oracle():
 
 def build_reference_shim():
     """oracle/_ref/libluxref.so from the reference's own host sources (only where
-    they are mounted; the GPU box uses the prebuilt file)."""
+    they are mounted; elsewhere the prebuilt file is used)."""
     if not REFERENCE_SRC.exists() or shutil.which("make") is None:
         return None
     _run(["make", "-s", "-B", "-f", "oracle/ref.mk", f"REF_SRC={REFERENCE_SRC}"])
