@@ -357,14 +357,17 @@ int lux_hip_batch_rays(lux_stream stream, uint64_t* extension_rays, uint64_t* sh
 
 /* Launch geometry of the wavefront schedule: path slots per batch, workgroups
  * of the grid-stride kernels, threads per workgroup (128, 256 or 512), graph
- * replay on/off. Applies to renders started after the call. No reference
- * counterpart (the reference's capacity is RenderSettings::batch_size,
- * src/integrator/render_context.h:16-24). */
+ * replay on/off, and preorder_walk: 1 makes extension rays take only the
+ * right-first preorder walk (the host LBVH's visit order) instead of the
+ * near-child-first walk, for checking that both give the same hits. Applies to
+ * renders started after the call. No reference counterpart (the reference's
+ * capacity is RenderSettings::batch_size, src/integrator/render_context.h:16-24). */
 typedef struct lux_render_tuning {
     int32_t capacity;
     int32_t grid;
     int32_t block;
     int32_t use_graph;
+    int32_t preorder_walk;
 } lux_render_tuning;
 int lux_hip_set_render_tuning(const lux_render_tuning* tuning);
 int lux_hip_get_render_tuning(lux_render_tuning* tuning);

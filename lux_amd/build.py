@@ -4,6 +4,7 @@ sources exist) the reference shim. Run from anywhere; outputs are in-tree."""
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 from . import LIB_PATH, PACKAGE_DIR, REPO_ROOT
@@ -21,7 +22,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 ORACLE_SRCS = [REPO_ROOT / "oracle" / "lbvh_oracle.cpp", REPO_ROOT / "oracle" / "pt_oracle.cpp",
-               REPO_ROOT / "oracle" / "two_level_oracle.cpp", REPO_ROOT / "oracle" / "film_oracle.cpp"]
+               REPO_ROOT / "oracle" / "two_level_oracle.cpp", REPO_ROOT / "oracle" / "film_oracle.cpp",
+               REPO_ROOT / "oracle" / "ordered_model.cpp"]
 ORACLE_LIB = REPO_ROOT / "oracle" / "_build" / "liblux_oracle.so"
 
 
@@ -33,10 +35,23 @@ def _run(cmd, cwd=REPO_ROOT):
 
 
 def build_hip():
-    srcs = [str(PACKAGE_DIR / "csrc" / s) for s in HIP_SOURCES]
+    """Each source compiles to its own object in parallel (separate device code
+    objects, as one multi-source hipcc line makes them), then one link."""
+    obj_dir = PACKAGE_DIR / "_obj"
+    obj_dir.mkdir(exist_ok=True)
+    inc = [f"-I{REPO_ROOT / 'include'}", f"-I{PACKAGE_DIR / 'csrc'}"]
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = obj_dir / (Path(src).stem + ".o")
+        _run([HIPCC, *compile_flags, *inc, "-c", str(PACKAGE_DIR / "csrc" / src), "-o", str(obj)])
+        return str(obj)
+
+    jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(compile_one, HIP_SOURCES))
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [HIPCC, *HIP_FLAGS, f"-I{REPO_ROOT / 'include'}", f"-I{PACKAGE_DIR / 'csrc'}", *srcs, "-o", str(tmp)]
-    _run(cmd)
+    _run([HIPCC, f"--offload-arch={GPU_ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -1551,6 +1551,7 @@ int lux_hip_set_render_tuning(const lux_render_tuning* t) {
         g_tuning.grid = t->grid;
         g_tuning.block = t->block;
         g_tuning.use_graph = t->use_graph ? 1 : 0;
+        g_tuning.preorder_walk = t->preorder_walk ? 1 : 0;
         return int(LUX_OK);
     });
 }
@@ -1562,6 +1563,7 @@ int lux_hip_get_render_tuning(lux_render_tuning* t) {
         t->grid = g_tuning.grid;
         t->block = g_tuning.block;
         t->use_graph = g_tuning.use_graph;
+        t->preorder_walk = g_tuning.preorder_walk;
         return int(LUX_OK);
     });
 }

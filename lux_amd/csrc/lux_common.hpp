@@ -179,9 +179,9 @@ __device__ __forceinline__ bool wave_any_parallel(const SlabRay& r) {
     return __ballot(r.par_x || r.par_y || r.par_z) != 0;
 }
 
-// Moller-Trumbore, operation order of src/geometry/triangle.h:33-52.
-__device__ __forceinline__ bool tri_hit(const f3& o, const f3& d, float t_min, float t_max,
-                                        const f3& v0, const f3& v1, const f3& v2, float& t,
+// Moller-Trumbore, operation order of src/geometry/triangle.h:33-52, without
+// the final range test: true when u and v pass, with t computed.
+__device__ __forceinline__ bool tri_uvt(const f3& o, const f3& d, const f3& v0, const f3& v1, const f3& v2, float& t,
                                         float& u, float& v) {
     f3 e1 = sub3(v1, v0);
     f3 e2 = sub3(v2, v0);
@@ -196,7 +196,14 @@ __device__ __forceinline__ bool tri_hit(const f3& o, const f3& d, float t_min, f
     v = f * dot3(d, q);
     if (v < 0.0f || u + v > 1.0f) return false;
     t = f * dot3(e2, q);
-    return t > t_min && t < t_max;
+    return true;
+}
+
+// intersect_triangle: a hit in the open range (t_min, t_max).
+__device__ __forceinline__ bool tri_hit(const f3& o, const f3& d, float t_min, float t_max,
+                                        const f3& v0, const f3& v1, const f3& v2, float& t,
+                                        float& u, float& v) {
+    return tri_uvt(o, d, v0, v1, v2, t, u, v) && t > t_min && t < t_max;
 }
 
 // Inclusive AABB overlap (src/core/aabb.cuh:68-72).

@@ -137,18 +137,19 @@ __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, co
 // The same closest hit by the near-child-first walk (ordered.hpp): both children
 // of a node tested together, the farther one kept on this lane's LDS stack
 // (entries `stride` words apart), exact-t ties kept by preorder position. A
-// ray whose stack overflows takes closest_walk.
+// ray whose stack overflows, or whose walk the ambiguity check flags, takes
+// closest_walk.
 template <bool kParallel>
 __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
                                                float t_min, float t_max0, int ignore, float& t_hit, float& u_hit,
                                                float& v_hit, uint32_t* stack, int stride) {
-    float t_max = t_max0;
+    OrderedBest ob(t_max0);
     int best = -1;
     uint32_t best_slot = 0, visits = 0;
     bool bad = false;
     const lux_triangle* soup = sv.soup;
     const uint32_t* slot_pre = sv.slot_pre;
-    auto leaf = [&](uint32_t slot, int prim) {
+    auto leaf = [&](uint32_t slot, int prim, float tn) {
         if (prim == ignore) return false;
         if (uint32_t(prim) >= uint32_t(sv.tri_count)) {
             set_error(sv.info, kErrPrimRange);
@@ -157,21 +158,25 @@ __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O,
         const float* p = reinterpret_cast<const float*>(soup + prim);
         f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
         float t, u, v;
-        if (tri_hit(O, D, t_min, tie_bound(t_max, best >= 0), v0, v1, v2, t, u, v) &&
-            (t < t_max || slot_pre[slot] < slot_pre[best_slot])) {
-            best = prim;
-            best_slot = slot;
-            t_max = t;
-            u_hit = u;
-            v_hit = v;
+        if (tri_uvt(O, D, v0, v1, v2, t, u, v) && t > t_min) {
+            if (t < ob.t_max || (best >= 0 && t == ob.t_max && slot_pre[slot] < slot_pre[best_slot])) {
+                ob.take(t, tn);
+                best = prim;
+                best_slot = slot;
+                u_hit = u;
+                v_hit = v;
+            } else {
+                ob.pass_over(t);
+            }
         }
         return false;
     };
-    const bool done =
-        ordered_walk<kParallel>(sv.nodes, uint32_t(sv.node_count), sr, t_max, stack, stride, visits, bad, leaf);
+    const bool done = ordered_walk<kParallel>(sv.nodes, uint32_t(sv.node_count), sr, ob, stack, stride, visits, bad,
+                                              leaf);
     if (bad) set_error(sv.info, kErrWalkOrder);
-    if (!done) return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
-    t_hit = t_max;
+    if (!done || ob.ambiguous())
+        return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
+    t_hit = ob.t_max;
     return best;
 }
 
@@ -180,7 +185,7 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
                                        int stride) {
     const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
     const SlabRay sr = make_slab_ray(O, D, t_min);
-    const bool ordered = sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
+    const bool ordered = !sv.preorder && sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
     if (wave_any_parallel(sr)) {
         if (ordered)
             return closest_ordered<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
@@ -213,8 +218,8 @@ __device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, 
                                              uint32_t* stack, int stride) {
     if constexpr (kTwoLevel) {
         TwoLevelHit h;
-        if (!two_level_closest_ordered(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore, stack,
-                                       stride, h))
+        if (sv.preorder || !two_level_closest_ordered(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
+                                                      ignore, stack, stride, h))
             h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore);
         if (h.prim >= 0) {
             t_hit = h.t;
@@ -1377,7 +1382,8 @@ uint64_t Renderer::enqueue_batch(std::vector<Request*>& active, int slice_spp, i
         if (slice_spp > 0)
             take = std::min(take, uint64_t(slice_spp) * uint64_t(rq->rs.width) * uint64_t(rq->rs.height));
         opaque = opaque && rq->sv.opaque != 0;
-        slot->seg[nseg++] = make_segment(*rq, rq->issued, uint32_t(used), uint32_t(take));
+        slot->seg[nseg] = make_segment(*rq, rq->issued, uint32_t(used), uint32_t(take));
+        slot->seg[nseg++].sv.preorder = tune_.preorder_walk;
         rq->issued += take;
         used += take;
         if (rq->issued == rq->total) rq->last_batch = batch_counter_;
@@ -1413,6 +1419,7 @@ void Renderer::render_all(std::vector<Request*>& reqs, int mode, hipStream_t str
                     slot->count = n;
                     slot->nseg = 1;
                     slot->seg[0] = make_segment(*rq, uint64_t(s) * npix + p0, 0, n);
+                    slot->seg[0].sv.preorder = tune_.preorder_walk;
                     upload(slot, stream);
                     launch_plain(rq->rs.max_depth, n, rq->sv.instanced != 0, rq->rs.light_sampler, stream);
                     hip_check(hipMemcpyAsync(host_counts, buf_.counts, sizeof(host_counts), hipMemcpyDeviceToHost,

@@ -114,6 +114,7 @@ struct SceneView {
     const float* env_cdf;                  // per-texel cumulative weights
     int32_t opaque;     // 1: every hit blocks a shadow ray (no pass-through material)
     int32_t instanced;  // 1: traverse `tl` (two-level) instead of `nodes`
+    int32_t preorder;   // 1: closest hits by the right-first preorder walk only
     DevCamera cam;
     TwoLevelView tl;
 };
@@ -179,6 +180,7 @@ struct Tuning {
     int grid = 2048;         // workgroups of the grid-stride kernels
     int block = 256;         // threads per workgroup (128, 256 or 512)
     int use_graph = 1;       // replay each batch's bounce loop as one hipGraph
+    int preorder_walk = 0;   // 1: extension rays take the right-first preorder walk only
 };
 
 // The batch engine: a path buffer, a pinned ring of batch descriptors and the

@@ -113,18 +113,19 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
 
 // Closest hit by the near-child-first walk (ordered.hpp), equal to walk<false>.
 // stack: this lane's LDS stack (entries kBlock words apart). A ray whose stack
-// overflows is walked again in right-first preorder.
+// overflows, or whose walk the ambiguity check flags, is walked again in
+// right-first preorder.
 template <bool kParallel>
 __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nodes, int node_count,
                                             const uint32_t* __restrict__ slot_pre,
                                             const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
                                             const SlabRay& sr, float& t_best, uint32_t* stack, DevInfo* info,
                                             uint32_t& visits, uint32_t& tests) {
-    float t_max = __builtin_canonicalizef(ray.t_max);
+    OrderedBest ob(__builtin_canonicalizef(ray.t_max));
     int best = -1;
     uint32_t best_slot = 0;
     bool bad = false, bad_prim = false;
-    auto leaf = [&](uint32_t slot, int prim) {
+    auto leaf = [&](uint32_t slot, int prim, float tn) {
         if (prim >= tri_count) {
             bad_prim = true;
             return false;
@@ -133,22 +134,24 @@ __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nod
         load_tri(tris, prim, v0, v1, v2);
         ++tests;
         float t, u, v;
-        if (tri_hit(ray.o, ray.d, ray.t_min, tie_bound(t_max, best >= 0), v0, v1, v2, t, u, v) &&
-            (t < t_max || slot_pre[slot] < slot_pre[best_slot])) {
-            best = prim;
-            best_slot = slot;
-            t_max = t;
+        if (tri_uvt(ray.o, ray.d, v0, v1, v2, t, u, v) && t > ray.t_min) {
+            if (t < ob.t_max || (best >= 0 && t == ob.t_max && slot_pre[slot] < slot_pre[best_slot])) {
+                ob.take(t, tn);
+                best = prim;
+                best_slot = slot;
+            } else {
+                ob.pass_over(t);
+            }
         }
         return false;
     };
-    const bool done = ordered_walk<kParallel>(nodes, uint32_t(node_count), sr, t_max, stack, kBlock, visits, bad,
-                                              leaf);
+    const bool done = ordered_walk<kParallel>(nodes, uint32_t(node_count), sr, ob, stack, kBlock, visits, bad, leaf);
     if (bad_prim) set_error(info, kErrPrimRange);
     if (bad) set_error(info, kErrWalkOrder);
-    if (!done)
+    if (!done || ob.ambiguous())
         return walk_impl<false, kParallel, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits,
                                                  tests);
-    t_best = t_max;
+    t_best = ob.t_max;
     return best;
 }
 

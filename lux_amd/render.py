@@ -79,7 +79,8 @@ LIGHT_SAMPLERS = {"power": 0, "uniform": 1, "bvh": 2}
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("capacity", c_int32), ("grid", c_int32), ("block", c_int32), ("use_graph", c_int32)]
+    _fields_ = [("capacity", c_int32), ("grid", c_int32), ("block", c_int32), ("use_graph", c_int32),
+                ("preorder_walk", c_int32)]
 
 
 assert ctypes.sizeof(Material) == 44
@@ -324,14 +325,18 @@ def get_tuning():
     lbvh.init()
     t = Tuning()
     call("lux_hip_get_render_tuning", ctypes.byref(t))
-    return {"capacity": t.capacity, "grid": t.grid, "block": t.block, "use_graph": t.use_graph}
+    return {"capacity": t.capacity, "grid": t.grid, "block": t.block, "use_graph": t.use_graph,
+            "preorder_walk": t.preorder_walk}
 
 
-def set_tuning(capacity=None, grid=None, block=None, use_graph=None):
-    """Launch geometry of the wavefront schedule for renders started afterwards."""
+def set_tuning(capacity=None, grid=None, block=None, use_graph=None, preorder_walk=None):
+    """Launch geometry of the wavefront schedule for renders started afterwards;
+    preorder_walk=True makes extension rays take the right-first preorder walk
+    only (the host LBVH's visit order)."""
     cur = get_tuning()
     t = Tuning(capacity or cur["capacity"], grid or cur["grid"], block or cur["block"],
-               cur["use_graph"] if use_graph is None else int(use_graph))
+               cur["use_graph"] if use_graph is None else int(use_graph),
+               cur["preorder_walk"] if preorder_walk is None else int(preorder_walk))
     call("lux_hip_set_render_tuning", ctypes.byref(t))
 
 

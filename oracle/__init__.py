@@ -22,6 +22,11 @@ REF_LIB = ORACLE_DIR / "_ref" / "libluxref.so"
 _lib = None
 _ref = None
 
+# Relative window of the near-child-first walk's ambiguity check (the library
+# uses the same value): boxes culled with an entry within L * 2^-11 of the
+# best hit's bound send the ray back to the preorder walk.
+ORDERED_WINDOW_LOG2 = 11
+
 
 def _p(a):
     if a is None:
@@ -57,6 +62,9 @@ def lib():
         L.orc_overlap_tree.restype = c_int64
         L.orc_overlap_tree.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_int64]
         L.orc_triangle_bounds.argtypes = [c_void_p, c_int32, c_void_p]
+        L.orc_trace_ordered.restype = c_int32
+        L.orc_trace_ordered.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p,
+                                        c_void_p, c_void_p]
         _lib = L
     return _lib
 
@@ -140,6 +148,20 @@ class OracleBvh:
         th = np.zeros(len(r), np.float32)
         lib().orc_trace_stackless(self.h, _p(t), _p(r), len(r), _p(ids), _p(th))
         return ids, th
+
+    def trace_ordered(self, tris, rays, rule, window_log2=ORDERED_WINDOW_LOG2):
+        """Model of the library's near-child-first closest-hit walk
+        (ordered_model.cpp): rule 0 = plain walk, rule 1 = with the ambiguity
+        check. Returns ids, t and per-ray flags (bit 0 stack overflow, bit 1
+        check fired; either means the ray was walked again in preorder)."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)
+        ids = np.zeros(len(r), np.int32)
+        th = np.zeros(len(r), np.float32)
+        flags = np.zeros(len(r), np.uint8)
+        lib().orc_trace_ordered(self.h, _p(t), _p(r), len(r), int(rule), int(window_log2), _p(ids), _p(th),
+                                _p(flags))
+        return ids, th, flags
 
     def overlap_tree(self, qboxes, qids, self_mode, cap=1 << 22):
         q = np.ascontiguousarray(qboxes, np.float32).reshape(-1, 6)

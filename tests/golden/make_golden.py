@@ -97,9 +97,90 @@ def overlap_case(name, boxes, qboxes):
     print(f"overlap {name}: n={n} q={q} self={cs} cross={cc}")
 
 
+def _tri_box(tri):
+    t = tri.reshape(3, 3)
+    return np.concatenate([t.min(0), t.max(0)]).astype(np.float32)
+
+
+def rounding_scene(seed=5, max_trials=100000):
+    """Four triangles and one ray on which the near-child-first walk, without
+    its ambiguity check, returns another hit than the right-first preorder
+    walk of the host LBVH (oracle.OracleBvh.trace_ordered models that walk).
+
+    B is tiny and the ray meets it next to a vertex, where its rounded
+    Moller-Trumbore t lies >= 3 ulps below the rounded slab entry of its own
+    box. A lies in a z = const plane (its t equals its box's z entry) at a t
+    strictly between the two. C (missed, before B on the ray) pulls the entry
+    of B's subtree in front of A's; D (far) pairs with A. The preorder walk
+    takes A's subtree first, holds t_A and culls B's leaf box; the
+    near-child-first walk enters B's subtree first, finds B and then culls A.
+    The reference returns A (nearer in exact arithmetic is B)."""
+    f32 = np.float32
+    rng = np.random.default_rng(seed)
+    o = np.array([0.05, 0.05, 0.05], f32)
+    for _ in range(max_trials):
+        d = np.abs(rng.normal(size=3))
+        d = (d / np.linalg.norm(d)).astype(f32)
+        p_hit = (o + d * f32(0.5 + 0.2 * rng.random())).astype(f32)
+        dirs = rng.normal(size=(2, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        lens = 0.001 + 0.009 * rng.random(2)
+        a = np.ldexp(rng.random(), -int(30 * rng.random()))
+        b = np.ldexp(rng.random(), -int(30 * rng.random()))
+        e1, e2 = dirs[0] * lens[0], dirs[1] * lens[1]
+        v0 = p_hit - e1 * a - e2 * b
+        tri_b = np.concatenate([v0, v0 + e1, v0 + e2]).astype(f32)
+        ray = np.array([[*o, *d, 0.0, np.inf, 0.0]], f32)
+        hb = oracle.brute_closest(tri_b[None], ray)
+        if hb[0][0] != 0:
+            continue
+        t_b = hb[1][0]
+        # slab entry of B's own box (all direction components are positive)
+        box_b = _tri_box(tri_b)
+        inv = f32(1) / d
+        t_nb = max(f32(0), *[f32((box_b[i] - o[i]) * inv[i]) for i in range(3)])
+        if int(t_nb.view(np.int32)) - int(t_b.view(np.int32)) < 3:
+            continue
+        z_t = f32(o[2] + d[2] * (t_b + (t_nb - t_b) * f32(0.5)))
+        tri_a = None
+        for k in range(400):
+            z = z_t
+            step = f32(np.inf) if k % 2 else f32(-np.inf)
+            for _ in range(k // 2):
+                z = np.nextafter(z, step, dtype=f32)
+            tz = (z - o[2]) * inv[2]
+            px, py = o[0] + d[0] * tz, o[1] + d[1] * tz
+            cand = np.array([px - 0.01, py - 0.01, z, px + 0.6, py - 0.01, z, px - 0.01, py + 0.6, z], f32)
+            ha = oracle.brute_closest(cand[None], ray)
+            if ha[0][0] == 0 and t_b < ha[1][0] < t_nb:
+                tri_a = cand
+                break
+        if tri_a is None:
+            continue
+        side = np.cross(d, [0.0, 0.0, 1.0])
+        side /= np.linalg.norm(side)
+        pc = o + d * (t_b * 0.85) + side * 0.004
+        tri_c = np.concatenate([pc, pc + [0.001, 0, 0], pc + [0, 0.001, 0]]).astype(f32)
+        pd = np.array([px + 0.5, py + 0.5, z + 0.2], f32)
+        tri_d = np.concatenate([pd, pd + [0.001, 0, 0], pd + [0, 0.001, 0]]).astype(f32)
+        tris = np.stack([tri_a, tri_b, tri_c, tri_d]).astype(f32)
+        tree = oracle.OracleBvh(oracle.triangle_bounds(tris))
+        pre = tree.trace_closest(tris, ray)["ids"][0]
+        plain = tree.trace_ordered(tris, ray, 0)[0][0]
+        if pre != plain:
+            return tris, ray
+    raise RuntimeError("no rounding scene found")
+
+
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libluxref.so missing: run `make -f oracle/ref.mk` where the reference exists")
+    only = set(sys.argv[1:])
+    if only:
+        if "rounding4" in only:
+            rounding()
+        return
+    rounding()
     soup = W.soup(2000, seed=1)
     reference_case("soup2000", soup, mixed_rays(soup, 4096, 100))
 
@@ -131,6 +212,12 @@ def main():
     dense = squeeze(W.soup(2000, seed=31, edge=(0.002, 0.02)), 0.15)
     q = squeeze(W.soup(300, seed=32, edge=(0.002, 0.02)), 0.15)
     overlap_case("dense2000", W.triangle_boxes(dense), W.triangle_boxes(q))
+
+
+def rounding():
+    tris, ray = rounding_scene()
+    # the divergent ray, then rays at random points of the four triangles
+    reference_case("rounding4", tris, np.concatenate([ray, targeted_rays(tris, 63, 700)]))
 
 
 if __name__ == "__main__":

@@ -262,3 +262,24 @@ def test_vertex_update_refits(rd):
     short = np.ascontiguousarray(moved["vertices"][:-1])
     with pytest.raises(abi.LuxError):
         abi.call("lux_hip_scene_update_vertices", sc.handle, abi.ptr(short), len(short))
+
+
+@pytest.mark.parametrize("case", ["pt_icosphere", "pt_mixed", "pt_cornell_32"])
+def test_near_first_walk_equals_preorder_walk_in_renders(rd, case):
+    """Extension rays walked near child first (with the ambiguity check) and in
+    the host LBVH's right-first preorder give bit-identical films and the same
+    ray totals."""
+    print("case: walk order", case, flush=True)
+    g = load_golden(case)
+    sc = rd.Scene(scene_of(g))
+    saved = rd.get_tuning()
+    try:
+        a = sc.render(48, 40, 8, 8, seed=3)
+        ra = sc.rays()
+        rd.set_tuning(preorder_walk=True)
+        b = sc.render(48, 40, 8, 8, seed=3)
+        rb = sc.rays()
+    finally:
+        rd.set_tuning(**saved)
+    assert np.array_equal(bits(a), bits(b))
+    assert ra == rb

@@ -208,3 +208,20 @@ def test_two_level_ordered_walk_ties_and_deep_stack(gpu):
     assert np.array_equal(h["local_primitive_id"][n:2 * n], np.full(n, m - 1, np.int32))
     assert (h["primitive_id"][2 * n:] >= 0).mean() > 0.99
     s.close()
+
+
+def test_two_level_rounding_case_matches_reference(gpu):
+    """The rounding4 case (a triangle whose hit t lies below its own leaf box's
+    entry) as one mesh under an identity instance: the TLAS leaf adds nothing,
+    so the two-level walk must return the reference host LBVH's hits, which the
+    near-child-first walk alone would not (tests/test_ordered_model.py)."""
+    print_case("two-level rounding4")
+    g = load_golden("lbvh_rounding4")
+    tris = np.ascontiguousarray(g["tri_by_id"], np.float32)
+    off = np.array([0, len(tris)], np.int32)
+    rec = TL.make_instances([0], np.eye(4, dtype=np.float32)[None], tris=tris, mesh_offsets=off)
+    s = TL.TwoLevelScene(tris, off, rec)
+    h = s.intersect(g["rays"])
+    assert np.array_equal(h["primitive_id"], g["hit_ids"])
+    assert np.array_equal(bits(h["t"]), bits(g["hit_t"]))
+    s.close()
