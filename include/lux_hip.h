@@ -133,21 +133,12 @@ int lux_hip_bvh_refit_triangles_device(lux_hip_bvh* bvh,
                                        const lux_triangle* d_triangles, int32_t n,
                                        lux_stream stream);
 
-/* Any-hit calls with at least 65 536 rays trace them in the order of their
- * origins' Morton codes (2-pass radix sort of the top 16 bits); occlusion
- * flags still land at each ray's own index and are identical either way.
- * enable = 0 traces in input order. Closest-hit calls always trace in input
- * order. No reference counterpart. */
+/* Closest-hit and any-hit calls with at least 65 536 rays trace them sorted by
+ * direction octant, then by the Morton code of their origins (2-pass radix
+ * sort of 16 key bits); hits and occlusion flags still land at each ray's own
+ * index and are identical either way. On by default; enable = 0 traces in
+ * input order. No reference counterpart. */
 int lux_hip_bvh_set_ray_order(lux_hip_bvh* bvh, int32_t enable);
-
-/* enable != 0: closest-hit and any-hit calls on this handle walk a 4-wide
- * traversal tree collapsed from the LBVH (128-byte records of up to 4 exact
- * child boxes in right-first preorder; built now and with every later build
- * or refit of the handle); results are bit-identical to the default binary
- * right-first walk. Off by default: on MI355X the binary walk measured faster
- * (12.5 vs 15.6 ms for 2^22 random rays over 2^20 triangles). No reference
- * counterpart. */
-int lux_hip_bvh_set_wide_walk(lux_hip_bvh* bvh, int32_t enable);
 
 /* GpuBvh::is_valid / primitive_count / root_bounds (src/bvh/gpu_bvh.cuh:35-47).
  * Synchronises with outstanding device work on the handle. */
@@ -637,9 +628,8 @@ int lux_hip_ppm_thresholds(float* out255);
 int lux_hip_probe_begin(const char* kernel_name);
 int lux_hip_probe_end(double* total_ms, int64_t* launches);
 /* Runs the closest-hit and any-hit walks of the given device rays once more,
- * counting work: stats[6] = {closest: nodes loaded (32-byte binary nodes, or
- * 128-byte records with lux_hip_bvh_set_wide_walk), separately loaded leaf
- * boxes (0 in both layouts), triangle tests; any-hit: the same three}.
+ * counting work: stats[6] = {closest: 32-byte nodes loaded, 0, triangle
+ * tests; any-hit: the same three}.
  * Synchronous; no reference counterpart. Used to state the bytes a traversal
  * touches (plus 36 B per triangle test). */
 int lux_hip_trace_stats_device(lux_hip_bvh* bvh, const lux_triangle* d_triangles,

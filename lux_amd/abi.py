@@ -79,7 +79,6 @@ _SIGNATURES = {
     "lux_hip_bvh_refit_device": [_P, _P, c_int32, _P],
     "lux_hip_bvh_refit_triangles_device": [_P, _P, c_int32, _P],
     "lux_hip_bvh_set_ray_order": [_P, c_int32],
-    "lux_hip_bvh_set_wide_walk": [_P, c_int32],
     "lux_hip_bvh_info": [_P, POINTER(c_int32), POINTER(c_int32), POINTER(Aabb)],
     "lux_hip_bvh_view": [_P, POINTER(BvhView)],
     "lux_hip_bvh_download_nodes": [_P, _P, c_int32],

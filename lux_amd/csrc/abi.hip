@@ -26,7 +26,6 @@ struct lux_hip_bvh {
     hipStream_t last_stream = nullptr;
     lux::RaySortDev order;           // ray-order scratch of the traversal calls
     bool order_rays = true;          // trace large any-hit ray sets in origin order
-    bool wide_walk = false;          // traverse the wide tree (default: the binary array)
 };
 
 namespace lux {
@@ -98,7 +97,6 @@ void check_device_flags(lux_hip_bvh* h) {
     if (info.error & kErrSpinLimit) what += " radix-sort look-back spin limit;";
     if (info.error & kErrWalkOrder) what += " stackless walk did not advance;";
     if (info.error & kErrClimbLimit) what += " bottom-up pass exceeded n levels;";
-    if (info.error & kErrWideBuild) what += " wide traversal tree build failed;";
     if (info.error & kErrNegativeId) h->valid = false;
     throw Error(LUX_ERR_DEVICE_CHECK, "device check failed:" + what);
 }
@@ -297,20 +295,6 @@ int lux_hip_bvh_destroy(lux_hip_bvh* bvh) {
         bvh_release(bvh->dev);
         raysort_release(bvh->order);
         delete bvh;
-        return int(LUX_OK);
-    });
-}
-
-int lux_hip_bvh_set_wide_walk(lux_hip_bvh* bvh, int32_t enable) {
-    return guarded([&] {
-        require_handle(bvh);
-        bvh->wide_walk = enable != 0;
-        const bool had = bvh->dev.want_wide;
-        bvh->dev.want_wide = enable != 0;
-        if (enable && !had && bvh->dev.n > 0) {  // collapse the current tree now
-            wide_build(bvh->dev, bvh->last_stream ? bvh->last_stream : g_stream);
-            sync_handle(bvh);
-        }
         return int(LUX_OK);
     });
 }
@@ -548,7 +532,8 @@ int lux_hip_trace_closest(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tr
         hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
         hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
         bvh->last_stream = g_stream;
-        trace_closest(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, bvh->wide_walk, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        trace_closest(bvh->dev, closest_order(bvh->order, bvh->order_rays, bvh->dev, dr.p, n, g_stream), dt.p, tri_count,
+                      dr.p, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(hits, dh.p, sizeof(lux_hit) * n, hipMemcpyDeviceToHost, g_stream), "D2H hits");
         sync_handle(bvh);
         return int(LUX_OK);
@@ -566,7 +551,8 @@ int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, i
         hipStream_t s = pick(stream);
         if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
         bvh->last_stream = s;
-        trace_closest(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, bvh->wide_walk, d_tris, tri_count, d_rays, n, d_hits, s);
+        trace_closest(bvh->dev, closest_order(bvh->order, bvh->order_rays, bvh->dev, d_rays, n, s), d_tris, tri_count,
+                      d_rays, n, d_hits, s);
         return int(LUX_OK);
     });
 }
@@ -588,7 +574,7 @@ int lux_hip_trace_any(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tri_co
         hip_check(hipMemcpyAsync(dt.p, tris, sizeof(lux_triangle) * tri_count, hipMemcpyHostToDevice, g_stream), "H2D tris");
         hip_check(hipMemcpyAsync(dr.p, rays, sizeof(lux_ray) * n, hipMemcpyHostToDevice, g_stream), "H2D rays");
         bvh->last_stream = g_stream;
-        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, bvh->wide_walk, dt.p, tri_count, dr.p, n, dh.p, g_stream);
+        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, dt.p, tri_count, dr.p, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(occluded, dh.p, size_t(n), hipMemcpyDeviceToHost, g_stream), "D2H occluded");
         sync_handle(bvh);
         return int(LUX_OK);
@@ -606,7 +592,7 @@ int lux_hip_trace_any_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32
         hipStream_t s = pick(stream);
         if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
         bvh->last_stream = s;
-        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, bvh->wide_walk, d_tris, tri_count, d_rays, n, d_occluded, s);
+        trace_any(bvh->dev, bvh->order_rays ? &bvh->order : nullptr, d_tris, tri_count, d_rays, n, d_occluded, s);
         return int(LUX_OK);
     });
 }
@@ -685,7 +671,7 @@ int lux_hip_trace_stats_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int
         sync_handle(bvh);
         DeviceBuffer<unsigned long long> ds(kStats);
         bvh->last_stream = g_stream;
-        trace_stats(bvh->dev, bvh->wide_walk, d_tris, tri_count, d_rays, n, ds.p, g_stream);
+        trace_stats(bvh->dev, d_tris, tri_count, d_rays, n, ds.p, g_stream);
         unsigned long long h[kStats];
         hip_check(hipMemcpyAsync(h, ds.p, sizeof(h), hipMemcpyDeviceToHost, g_stream), "D2H stats");
         sync_handle(bvh);

@@ -29,7 +29,6 @@
 // stable key order src/bvh/cpu_lbvh_builder.h:212-216, Karras 2012 range/split
 // with merged keys :63-131, internal box = union of children :139-159.
 #include "lux_runtime.hpp"
-#include "wide.hpp"
 
 namespace lux {
 namespace {
@@ -790,8 +789,7 @@ void bvh_release(BvhDev& d) {
                     d.left,      d.right,       d.parent,       d.first,      d.last,       d.leaf_parent,
                     d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
                     d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
-                    d.rtop_int,  d.rtop_leaf,   d.wide,         d.wsrc,       d.wq[0],      d.wq[1],
-                    d.wcounts,   d.pairs,       d.pair_pos,     d.slot_pre};
+                    d.rtop_int,  d.rtop_leaf,   d.pairs,        d.pair_pos,   d.slot_pre};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     BvhDev empty;
@@ -903,6 +901,30 @@ void raysort_release(RaySortDev& r) {
     r = RaySortDev{};
 }
 
+size_t ray_order_status_words(int n) { return size_t(sort_tiles(n)) * kRadix; }
+
+const uint32_t* ray_order_into(const RayOrderBufs& b, uint32_t first_epoch, const BvhDev& tree, const lux_ray* rays,
+                               int n, hipStream_t s) {
+    LUX_LAUNCH("ray_order_zero", k_zero_u32, 1, kBlock, 0, s, b.hist, 2 * kRadix);
+    LUX_LAUNCH("ray_order_keys", k_ray_keys, max(1, min(1024, div_up(n, kBlock * 16))), kBlock, 0, s, rays, n,
+               (const DevInfo*)tree.info, b.keys[0], b.vals[0], b.hist);
+    const int tiles = sort_tiles(n);
+    for (int p = 0; p < 2; ++p) {  // digits 14..21, then 22..29: the top 16 bits
+        const int src = p & 1, dst = src ^ 1;
+        LUX_LAUNCH("ray_order_sort", k_sort_pass, tiles, kBlock, 0, s, (const uint32_t*)b.keys[src],
+                   (const uint32_t*)b.vals[src], b.keys[dst], b.vals[dst], n, 14 + 8 * p,
+                   (const uint32_t*)(b.hist + p * kRadix), b.status, b.tickets, first_epoch + uint32_t(p), tiles,
+                   tree.info);
+    }
+    return b.vals[0];
+}
+
+void zero_words(uint32_t* p, size_t count, hipStream_t s) {
+    if (count == 0) return;
+    LUX_LAUNCH("zero_words", k_zero_u32, max(1, min(1024, div_up(int64_t(count), kBlock))), kBlock, 0, s, p,
+               int(count));
+}
+
 const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays, int n, hipStream_t s) {
     if (n > r.capacity) {
         if (r.capacity) hip_check(hipDeviceSynchronize(), "synchronize before ray-sort growth");
@@ -913,23 +935,15 @@ const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays
             r.vals[b] = dalloc<uint32_t>(n);
         }
         r.hist = dalloc<uint32_t>(2 * kRadix);
-        r.status = dalloc<uint64_t>(size_t(sort_tiles(n)) * kRadix);
+        r.status = dalloc<uint64_t>(ray_order_status_words(n));
         r.tickets = dalloc<uint32_t>(4);
-        hip_check(hipMemset(r.status, 0, sizeof(uint64_t) * size_t(sort_tiles(n)) * kRadix), "hipMemset status");
+        hip_check(hipMemset(r.status, 0, sizeof(uint64_t) * ray_order_status_words(n)), "hipMemset status");
         hip_check(hipMemset(r.tickets, 0, sizeof(uint32_t) * 4), "hipMemset tickets");
     }
-    LUX_LAUNCH("ray_order_zero", k_zero_u32, 1, kBlock, 0, s, r.hist, 2 * kRadix);
-    LUX_LAUNCH("ray_order_keys", k_ray_keys, max(1, min(1024, div_up(n, kBlock * 16))), kBlock, 0, s, rays, n,
-               (const DevInfo*)tree.info, r.keys[0], r.vals[0], r.hist);
-    const int tiles = sort_tiles(n);
-    for (int p = 0; p < 2; ++p) {  // digits 14..21, then 22..29: the top 16 bits
-        const uint32_t epoch = ++r.epoch;
-        const int src = p & 1, dst = src ^ 1;
-        LUX_LAUNCH("ray_order_sort", k_sort_pass, tiles, kBlock, 0, s, (const uint32_t*)r.keys[src],
-                   (const uint32_t*)r.vals[src], r.keys[dst], r.vals[dst], n, 14 + 8 * p,
-                   (const uint32_t*)(r.hist + p * kRadix), r.status, r.tickets, epoch, tiles, tree.info);
-    }
-    return r.vals[0];
+    const RayOrderBufs b{{r.keys[0], r.keys[1]}, {r.vals[0], r.vals[1]}, r.hist, r.status, r.tickets};
+    const uint32_t first = r.epoch + 1;
+    r.epoch += 2;
+    return ray_order_into(b, first, tree, rays, n, s);
 }
 
 void bvh_allocate(BvhDev& d, int n) {
@@ -968,7 +982,6 @@ void bvh_allocate(BvhDev& d, int n) {
     d.partials = dalloc<uint8_t>(sizeof(BoxIds) * kReduceBlocksMax);
     d.info = dalloc<DevInfo>(1);
     d.counter = dalloc<int32_t>(4);
-    d.wide_capacity = 0;  // the wide tree is allocated by its first build
     d.pairs = dalloc<lux_bvh_node>(2 * size_t(cap));
     d.pair_pos = dalloc<uint32_t>(2 * size_t(cap));
     d.slot_pre = dalloc<uint32_t>(2 * size_t(cap));
@@ -984,10 +997,7 @@ void bvh_allocate(BvhDev& d, int n) {
 
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s) {
     bvh_allocate(d, n);
-    if (n <= 0) {
-        if (d.want_wide) wide_build(d, s);  // clears the record count
-        return;
-    }
+    if (n <= 0) return;
     int copy = (in_boxes != d.boxes || in_ids != d.ids) ? 1 : 0;
     int rb = reduce_blocks(n);
     uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
@@ -1000,7 +1010,6 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
                    d.keys[0], d.vals[0], d.info, 1);
         // root box from the one leaf (the reduction result is identical)
         pair_build(d, s);
-        if (d.want_wide) wide_build(d, s);
         return;
     }
     int mh_blocks = max(1, min(1024, div_up(n, kBlock * 16)));
@@ -1029,7 +1038,6 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
     pair_build(d, s);
-    if (d.want_wide) wide_build(d, s);
 }
 
 namespace {
@@ -1042,7 +1050,6 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
         LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, in_boxes, tris, (const int32_t*)d.ids, d.boxes,
                    wb, d.nodes, d.pre_leaf, d.keys[0], d.vals[0], d.info, 0);
         pair_fill(d, s);
-        if (d.want_wide && d.wide_capacity) wide_refit(d, s);
         return;
     }
     LUX_LAUNCH(kMode == 2 ? "lbvh_local_tris" : "lbvh_local_boxes", k_local<kMode>, div_up(n, kLocalLeaves),
@@ -1054,7 +1061,6 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
                (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
                (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
     pair_fill(d, s);
-    if (d.want_wide && d.wide_capacity) wide_refit(d, s);
 }
 }  // namespace
 

@@ -25,7 +25,6 @@ enum DeviceError : uint32_t {
     kErrClimbLimit = 1u << 2,     // bottom-up pass climbed more than n levels
     kErrPrimRange = 1u << 3,      // a leaf id is outside the triangle array
     kErrNegativeId = 1u << 4,     // an input primitive id is negative
-    kErrWideBuild = 1u << 5,      // the wide traversal tree could not be built
 };
 
 // Per-handle device scratch that survives across kernels of one build.
@@ -35,8 +34,7 @@ struct DevInfo {
     uint32_t error;        // DeviceError bits
     uint32_t max_depth;    // deepest leaf (root = depth 0)
     uint32_t done_blocks;  // last-block-done counter of the root reduction
-    uint32_t wide_count;   // records of the wide traversal tree (0 = none)
-    uint32_t pad[2];
+    uint32_t pad[3];
 };
 
 struct f3 {
@@ -233,6 +231,19 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t l = lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+// Appends one entry per pushing lane of the wave with one atomic add: returns
+// this lane's slot (meaningful only where push is true).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
+    uint64_t m = ballot64(push);
+    uint32_t base = 0;
+    if (m) {
+        uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+        if (lane_id() == leader)
+            base = __hip_atomic_fetch_add(count, uint32_t(__popcll(m)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        base = __shfl(base, int(leader), 64);
+    }
+    return base + uint32_t(__popcll(m & lanemask_lt()));
 }
 __device__ __forceinline__ void set_error(DevInfo* info, uint32_t bit) {
     __hip_atomic_fetch_or(&info->error, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -102,14 +102,6 @@ struct BvhDev {
     int32_t* counter = nullptr;
     uint32_t epoch = 0;
     int status_words = 0;
-    // 8-wide traversal tree (wide.hpp): records, the binary positions of each
-    // record's children (for refits), two level queues and 4 counters
-    WideNode* wide = nullptr;
-    uint32_t* wsrc = nullptr;
-    uint4* wq[2] = {nullptr, nullptr};
-    uint32_t* wcounts = nullptr;
-    int wide_capacity = 0;
-    bool want_wide = false;  // build/refit the wide tree with the binary one
 };
 
 void bvh_allocate(BvhDev& d, int n);
@@ -133,14 +125,29 @@ void raysort_release(RaySortDev& r);
 // Tracing in this order keeps the rays in flight on one XCD close in origin
 // and heading, so their nodes stay in L2.
 const uint32_t* ray_order(RaySortDev& r, const BvhDev& tree, const lux_ray* rays, int n, hipStream_t s);
+// The same sort on caller-provided buffers: keys/vals n words each, hist 512
+// words, status ray_order_status_words(n) 64-bit words, tickets 4 words. The
+// look-back words of pass p carry epoch first_epoch + p; status must hold no
+// word of those epochs from an earlier sort (zeroed words never do) and
+// tickets must be zero. Enqueues no allocation.
+struct RayOrderBufs {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;
+    uint64_t* status;
+    uint32_t* tickets;
+};
+size_t ray_order_status_words(int n);
+const uint32_t* ray_order_into(const RayOrderBufs& b, uint32_t first_epoch, const BvhDev& tree, const lux_ray* rays,
+                               int n, hipStream_t s);
+// Fills count 32-bit words with zero (a kernel: safe inside captured graphs).
+void zero_words(uint32_t* p, size_t count, hipStream_t s);
+// Below this many rays the traversal entry points trace in input order.
+constexpr int kOrderMinRays = 1 << 16;
 
 // Enqueue a full build of n primitives; in_boxes/in_ids are device arrays
 // (may alias d.boxes/d.ids).
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s);
-// Enqueue the collapse of the built binary tree into the wide traversal tree
-// (allocating it on first use), and its refresh after a refit; build and refit
-// call them when `want_wide` is set.
-void wide_build(BvhDev& d, hipStream_t s);
 
 // Film output (film.hip): write_ppm's pixel bytes (3 per pixel, film order),
 // write_pfm's float body (rows bottom to top), both from a device film of
@@ -170,20 +177,22 @@ struct Mesh {
 void load(const std::string& path, const lux_obj_options& options, Mesh& out);
 void default_options(lux_obj_options& options);
 }  // namespace obj
-void wide_refit(BvhDev& d, hipStream_t s);
 // Enqueue a refit with new boxes (input order) or triangles (input order).
 void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s);
 void lbvh_refit_triangles(BvhDev& d, const lux_triangle* tris, hipStream_t s);
 
-// Queries (all enqueue only).
-// With `order` non-null and n large enough, any-hit rays are traced in origin
-// order (results still land at each ray's own index). `wide` selects the wide
-// tree walk (default) or the binary right-first walk; results are identical.
-void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
+// Queries (all enqueue only). Rays may be traced in origin order (results
+// still land at each ray's own index).
+//
+// The order to trace n rays in: sorted when `sort` is set and n >=
+// kOrderMinRays (r holds the sort's buffers), else null (input order).
+const uint32_t* closest_order(RaySortDev& r, bool sort, const BvhDev& d, const lux_ray* rays, int n,
+                              hipStream_t s);
+void trace_closest(const BvhDev& d, const uint32_t* order, const lux_triangle* tris, int tri_count,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s);
-void trace_any(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
-               const lux_ray* rays, int n, uint8_t* occluded, hipStream_t s);
-void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+               int n, uint8_t* occluded, hipStream_t s);
+void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s);
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,

@@ -23,20 +23,25 @@
 // lie a few hundred ulps above its triangle's Moller-Trumbore t (a hit at a
 // vertex or edge on the face the ray enters through). The preorder walk then
 // culls that triangle if it already holds a t between the two, and this walk
-// may not (or the other way round). OrderedBest records what decides this:
-//   L       max(t_best, entry of the best triangle's leaf box),
-//   m_hit   the smallest hit t of any other triangle tested,
-//   m_cull  the smallest entry of a box culled only because it lay beyond the
-//           held t_max (its ray segment met the box).
-// With no triangle whose t sits more than L * kAmbiguityWindow below its leaf
-// box's entry, the preorder walk returns this walk's hit unless m_hit < L (it
-// may hold that t when it reaches the best triangle's path) or m_cull <=
-// L + L * kAmbiguityWindow (a culled box may hold a triangle it reaches). Such
-// a ray is walked again in preorder, as is a ray whose stack overflows. (Entries
-// grow down a path: a child box lies inside its parent and rounding is
-// monotone, so the leaf box holds the largest entry of the path.) The observed
-// gap between a hit and its leaf box's entry is below 2^-14.7 relative; on
-// 2^20 random triangles about 0.4 % of rays take the second walk.
+// may not (or the other way round). So this walk
+//   * enters boxes up to t_cull = t_max + |t_max| * kSoftWindow once it holds
+//     a hit (up to the ray's own t_max before), so every triangle whose t is
+//     at most the final t_best gets tested, as long as no triangle's t lies
+//     more than that window below its own leaf box's entry;
+//   * records in m the smallest t <= t_max of a tested triangle whose t lies
+//     below its own leaf box's entry (the largest entry on its path: a child
+//     box lies inside its parent and rounding is monotone);
+//   * hands the ray to the preorder walk when it ends holding a hit with
+//     m <= t_best.
+// Otherwise the preorder walk returns the same hit: it accepts a triangle iff
+// the triangle's leaf box passes (entry <= held t) and t < held t, in
+// preorder; every hit with t <= t_best was tested here and lies at or beyond
+// its leaf box's entry, so the preorder walk reaches the winner (t_best,
+// smallest preorder position) holding a larger t, accepts it, and nothing
+// after it can win. The observed gap between a hit and its leaf box's entry is
+// below 2^-14.7 relative (sampled hits next to vertices and edges); on 2^18
+// random triangles and on a displaced icosphere no random or secondary ray
+// needed the second walk.
 //
 // A lane whose stack would exceed kOrderedStack entries stops and reports
 // overflow; its caller then runs the preorder walk for that ray. The stack
@@ -50,42 +55,38 @@ namespace lux {
 
 constexpr int kOrderedStack = 16;
 
-// Relative window of the ambiguity check (see the file comment).
-constexpr float kAmbiguityWindow = 0x1p-11f;
+// Relative margin past t_max up to which boxes are still entered (see above).
+constexpr float kSoftWindow = 0x1p-11f;
 
-// Closest-hit state of an ordered walk: the held bound and the ambiguity record.
+// Closest-hit state of an ordered walk: the held bound, the box-entry bound
+// and the record of tested hits below their own leaf box's entry.
 struct OrderedBest {
-    float t_max;             // the ray's t_max, then the best hit's t
-    float L = -INFINITY;     // max(t_best, entry of the best triangle's leaf box)
-    float m_hit = INFINITY;  // smallest hit t among the other tested triangles
-    float m_cull = INFINITY; // smallest entry among boxes culled by t_max alone
-    bool has = false;
+    float t_max;         // the ray's t_max, then the best hit's t
+    float t_cull;        // boxes are entered up to here
+    float m = INFINITY;  // smallest t <= t_max of a tested hit below its leaf entry
 
-    __device__ __forceinline__ explicit OrderedBest(float t) : t_max(t) {}
-    // A hit t (u, v passed, t > t_min) the caller accepts; leaf_tn = entry of
-    // its leaf box (the largest of the path's entries).
-    __device__ __forceinline__ void take(float t, float leaf_tn) {
-        if (has) m_hit = fminf(m_hit, t_max);
+    __device__ __forceinline__ explicit OrderedBest(float t) : t_max(t), t_cull(t) {}
+    // Every tested hit t (u, v passed, t > t_min), before the caller decides
+    // to take it; path_tn = the largest box entry on its path.
+    __device__ __forceinline__ void note(float t, float path_tn) {
+        if (t <= t_max && path_tn > t) m = fminf(m, t);
+    }
+    // A hit the caller accepts.
+    __device__ __forceinline__ void take(float t) {
         t_max = t;
-        L = fmaxf(t, leaf_tn);
-        has = true;
+        t_cull = t + fabsf(t) * kSoftWindow;
     }
-    // A hit t the caller does not accept.
-    __device__ __forceinline__ void pass_over(float t) { m_hit = fminf(m_hit, t); }
-    // True when the preorder walk might return another hit.
-    __device__ __forceinline__ bool ambiguous() const {
-        return has && (m_hit < L || m_cull <= L + fabsf(L) * kAmbiguityWindow);
-    }
+    // With a hit held: true when the preorder walk might return another one.
+    __device__ __forceinline__ bool ambiguous() const { return m <= t_max; }
 };
 
-// Slab test (slab_hit / slab_hit_np) that also returns the entry distance
-// t_near and records in m_cull the entry of a box the ray's segment meets but
-// that lies beyond t_max. The box passes exactly when AABB::intersect passes:
-// the min over t_max and the three exits equals the min over the exits
-// compared with t_max separately.
+// Slab test (slab_hit / slab_hit_np) against [t_min, t_bound] that also
+// returns the entry distance t_near; with t_bound = t_max it equals
+// AABB::intersect's boolean (the min over t_max and the three exits compared
+// with t_near is the two comparisons below).
 template <bool kParallel>
-__device__ __forceinline__ bool slab_near(const SlabRay& r, float t_max, float lox, float loy, float loz, float hix,
-                                          float hiy, float hiz, float& t_near, float& m_cull) {
+__device__ __forceinline__ bool slab_near(const SlabRay& r, float t_bound, float lox, float loy, float loz, float hix,
+                                          float hiy, float hiz, float& t_near) {
     bool geo;
     float tn;
     if (!kParallel) {
@@ -135,15 +136,13 @@ __device__ __forceinline__ bool slab_near(const SlabRay& r, float t_max, float l
         geo = !miss && !(tn > tf);
     }
     t_near = tn;
-    const bool beyond = tn > t_max;
-    if (geo && beyond) m_cull = fminf(m_cull, tn);
-    return geo && !beyond;
+    return geo && !(tn > t_bound);
 }
 
-// Walks the pair array from the root. leaf(slot, prim, tn) tests one leaf the
-// ray's box test passed (tn = its box's entry) and may lower best.t_max (it
-// holds the tie rule above); it returns true to end the walk (an any-hit
-// query). stack points at this lane's first LDS entry; entries are `stride`
+// Walks the pair array from the root, entering boxes up to best.t_cull.
+// leaf(slot, prim, tn) tests one leaf whose box passed (tn = its box's entry)
+// and may lower best.t_max (it holds the tie rule above); it returns true to
+// end the walk (an any-hit query). stack points at this lane's first LDS entry; entries are `stride`
 // words apart. Returns false when the stack overflowed (the walk stopped
 // early); *bad is set on a malformed array (child slot out of range, a pair
 // entered twice).
@@ -155,7 +154,7 @@ __device__ __forceinline__ bool ordered_walk(const lux_bvh_node* __restrict__ no
     load_node32(nodes, 0, a, b);
     ++visits;
     float tn;
-    if (!slab_near<kParallel>(sr, best.t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn, best.m_cull)) return true;
+    if (!slab_near<kParallel>(sr, best.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)) return true;
     const int root = __float_as_int(b.z);
     if (root >= 0) {
         (void)leaf(0u, root, tn);
@@ -174,8 +173,8 @@ __device__ __forceinline__ bool ordered_walk(const lux_bvh_node* __restrict__ no
         load_node32(nodes, c + 1, a1, b1);
         visits += 2;
         float tn0, tn1;
-        const bool h0 = slab_near<kParallel>(sr, best.t_max, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0, best.m_cull);
-        const bool h1 = slab_near<kParallel>(sr, best.t_max, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1, best.m_cull);
+        const bool h0 = slab_near<kParallel>(sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
+        const bool h1 = slab_near<kParallel>(sr, best.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
         const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
         if (h0 && p0 >= 0 && leaf(c, p0, tn0)) return true;
         if (h1 && p1 >= 0 && leaf(c + 1, p1, tn1)) return true;

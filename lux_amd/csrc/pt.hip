@@ -66,18 +66,6 @@ struct Paths {
     float4* st;     // throughput at the vertex xyz, bits: ignored primitive
 };
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* count, bool push) {
-    uint64_t m = ballot64(push);
-    uint32_t base = 0;
-    if (m) {
-        uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
-        if (lane_id() == leader)
-            base = __hip_atomic_fetch_add(count, uint32_t(__popcll(m)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        base = __shfl(base, int(leader), 64);
-    }
-    return base + uint32_t(__popcll(m & lanemask_lt()));
-}
-
 // Closest hit / occlusion walk over the sibling-pair copy of the right-first
 // stackless LBVH (same visits and order as the preorder walk); `ignore` skips
 // one leaf's triangle test (the source primitive of a shadow ray). kParallel =
@@ -137,8 +125,8 @@ __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, co
 // The same closest hit by the near-child-first walk (ordered.hpp): both children
 // of a node tested together, the farther one kept on this lane's LDS stack
 // (entries `stride` words apart), exact-t ties kept by preorder position. A
-// ray whose stack overflows, or whose walk the ambiguity check flags, takes
-// closest_walk.
+// ray whose stack overflows, or whose walk the check of ordered.hpp flags,
+// takes closest_walk.
 template <bool kParallel>
 __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
                                                float t_min, float t_max0, int ignore, float& t_hit, float& u_hit,
@@ -159,14 +147,11 @@ __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O,
         f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
         float t, u, v;
         if (tri_uvt(O, D, v0, v1, v2, t, u, v) && t > t_min) {
+            ob.note(t, tn);
             if (t < ob.t_max || (best >= 0 && t == ob.t_max && slot_pre[slot] < slot_pre[best_slot])) {
-                ob.take(t, tn);
+                ob.take(t);
                 best = prim;
                 best_slot = slot;
-                u_hit = u;
-                v_hit = v;
-            } else {
-                ob.pass_over(t);
             }
         }
         return false;
@@ -174,18 +159,28 @@ __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O,
     const bool done = ordered_walk<kParallel>(sv.nodes, uint32_t(sv.node_count), sr, ob, stack, stride, visits, bad,
                                               leaf);
     if (bad) set_error(sv.info, kErrWalkOrder);
-    if (!done || ob.ambiguous())
+    if (!done || (best >= 0 && ob.ambiguous()))
         return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
     t_hit = ob.t_max;
+    if (best >= 0) {
+        // u, v of the winner, recomputed once rather than carried through the
+        // walk (the same operations give the same bits)
+        const float* p = reinterpret_cast<const float*>(soup + best);
+        float t;
+        (void)tri_uvt(O, D, make3(p[0], p[1], p[2]), make3(p[3], p[4], p[5]), make3(p[6], p[7], p[8]), t, u_hit,
+                      v_hit);
+    }
     return best;
 }
 
+// stack == nullptr: the preorder walk only.
 __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const vec& d, float t_min, float t_max,
                                        int ignore, float& t_hit, float& u_hit, float& v_hit, uint32_t* stack,
                                        int stride) {
     const f3 O = make3(o.x, o.y, o.z), D = make3(d.x, d.y, d.z);
     const SlabRay sr = make_slab_ray(O, D, t_min);
-    const bool ordered = !sv.preorder && sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
+    const bool ordered =
+        stack && !sv.preorder && sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
     if (wave_any_parallel(sr)) {
         if (ordered)
             return closest_ordered<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
@@ -212,14 +207,16 @@ __device__ __forceinline__ bool occluded(const SceneView& sv, const vec& o, cons
 // The same two queries on an instanced scene: the two-level walk (TLAS, then
 // the BLAS of each instance the ray reaches, in object space), returning the
 // scene ordinal. kTwoLevel is chosen per batch on the host, like kAnyShadow.
+// stack as for closest().
 template <bool kTwoLevel>
 __device__ __forceinline__ int scene_closest(const SceneView& sv, const vec& o, const vec& d, float t_min,
                                              float t_max, int ignore, float& t_hit, float& u_hit, float& v_hit,
                                              uint32_t* stack, int stride) {
     if constexpr (kTwoLevel) {
         TwoLevelHit h;
-        if (sv.preorder || !two_level_closest_ordered(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
-                                                      ignore, stack, stride, h))
+        const bool ordered = stack && !sv.preorder;
+        if (!ordered || !two_level_closest_ordered(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max,
+                                                   ignore, stack, stride, h))
             h = two_level_walk<false>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore);
         if (h.prim >= 0) {
             t_hit = h.t;
@@ -327,7 +324,9 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
     if (kAnyShadow && scene_occluded<kTwoLevel>(sv, o, d, t_min, t_max, ignore)) tr = splat(0);
     for (int guard = 0; !kAnyShadow && guard < 1 << 16; ++guard) {
         float t = 0, u = 0, v = 0;
-        int prim = scene_closest<kTwoLevel>(sv, o, d, t_min, t_max, ignore, t, u, v, stack, stride);
+        // the preorder walk (no stack): with the near-child-first walk inlined a
+        // second time here the one-level kernel spilled 36 VGPRs per lane
+        int prim = scene_closest<kTwoLevel>(sv, o, d, t_min, t_max, ignore, t, u, v, nullptr, 0);
         if (prim < 0) break;
         const DevMaterial& m = sv.mats[sv.mat_id[prim]];
         if (!passthrough(m)) {

@@ -15,7 +15,6 @@
 
 #include "lux_runtime.hpp"
 #include "ordered.hpp"
-#include "wide.hpp"
 
 namespace lux {
 namespace {
@@ -113,7 +112,7 @@ __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int 
 
 // Closest hit by the near-child-first walk (ordered.hpp), equal to walk<false>.
 // stack: this lane's LDS stack (entries kBlock words apart). A ray whose stack
-// overflows, or whose walk the ambiguity check flags, is walked again in
+// overflows, or whose walk the check of ordered.hpp flags, is walked again in
 // right-first preorder.
 template <bool kParallel>
 __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nodes, int node_count,
@@ -135,12 +134,11 @@ __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nod
         ++tests;
         float t, u, v;
         if (tri_uvt(ray.o, ray.d, v0, v1, v2, t, u, v) && t > ray.t_min) {
+            ob.note(t, tn);
             if (t < ob.t_max || (best >= 0 && t == ob.t_max && slot_pre[slot] < slot_pre[best_slot])) {
-                ob.take(t, tn);
+                ob.take(t);
                 best = prim;
                 best_slot = slot;
-            } else {
-                ob.pass_over(t);
             }
         }
         return false;
@@ -148,33 +146,21 @@ __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nod
     const bool done = ordered_walk<kParallel>(nodes, uint32_t(node_count), sr, ob, stack, kBlock, visits, bad, leaf);
     if (bad_prim) set_error(info, kErrPrimRange);
     if (bad) set_error(info, kErrWalkOrder);
-    if (!done || ob.ambiguous())
+    if (!done || (best >= 0 && ob.ambiguous()))
         return walk_impl<false, kParallel, true>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits,
                                                  tests);
     t_best = ob.t_max;
     return best;
 }
 
-// One ray through the wide tree (kWideWalk) or the binary right-first array;
-// both give bit-identical results (wide.hpp). Counters: records or nodes
-// loaded, exact leaf boxes loaded (wide walk only), triangle tests.
-template <bool kAny, bool kWideWalk, bool kCount>
-__device__ __forceinline__ int trace_one(const WideNode* __restrict__ wide, const lux_bvh_node* __restrict__ nodes,
-                                         int node_count, const lux_triangle* __restrict__ tris, int tri_count,
-                                         const RayIn& r, float& t, DevInfo* info, uint32_t& visits,
-                                         uint32_t& leaf_boxes, uint32_t& tests,
+// One ray: closest hit by the near-child-first walk when `stack` is given,
+// otherwise (and for any hit) the preorder walk. Counters: nodes loaded and
+// triangle tests.
+template <bool kAny>
+__device__ __forceinline__ int trace_one(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                         const lux_triangle* __restrict__ tris, int tri_count, const RayIn& r,
+                                         float& t, DevInfo* info, uint32_t& visits, uint32_t& tests,
                                          const uint32_t* __restrict__ slot_pre = nullptr, uint32_t* stack = nullptr) {
-    if (kWideWalk) {
-        WideStats st;
-        float t_max = r.t_max, u = 0.0f, v = 0.0f;
-        int id = wide_walk<kAny, kCount>(wide, info->wide_count, nodes, node_count, tris, tri_count, r.o, r.d,
-                                         r.t_min, t_max, -1, u, v, info, st);
-        t = t_max;
-        visits += st.records;
-        leaf_boxes += st.leaf_boxes;
-        tests += st.tests;
-        return id;
-    }
     if (!kAny && stack && uint32_t(node_count) <= kOffset32Nodes && node_count > 1) {
         const SlabRay sr = make_slab_ray(r.o, r.d, r.t_min);
         if (wave_any_parallel(sr))
@@ -186,13 +172,12 @@ __device__ __forceinline__ int trace_one(const WideNode* __restrict__ wide, cons
     return walk<kAny>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
 }
 
-template <bool kWideWalk, bool kOrdered>
-__global__ __launch_bounds__(kBlock) void k_trace_closest(const WideNode* __restrict__ wide,
-                                                          const lux_bvh_node* __restrict__ nodes,
-                                                          int node_count,
+// Closest hit, one ray per lane in `order` (a permutation of [0, n), or input
+// order); the hit lands at the ray's own index.
+template <bool kOrdered>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(const lux_bvh_node* __restrict__ nodes, int node_count,
                                                           const uint32_t* __restrict__ slot_pre,
-                                                          const lux_triangle* __restrict__ tris,
-                                                          int tri_count,
+                                                          const lux_triangle* __restrict__ tris, int tri_count,
                                                           const lux_ray* __restrict__ rays, int n,
                                                           const uint32_t* __restrict__ order,
                                                           lux_hit* __restrict__ hits, DevInfo* info) {
@@ -206,22 +191,19 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(const WideNode* __rest
     }
     RayIn r = load_ray(rays, ray);
     float t;
-    uint32_t visits = 0, leaf_boxes = 0, tests = 0;
-    int id = trace_one<false, kWideWalk, false>(wide, nodes, node_count, tris, tri_count, r, t, info, visits,
-                                                leaf_boxes, tests, slot_pre, kOrdered ? stack + threadIdx.x : nullptr);
+    uint32_t visits = 0, tests = 0;
+    const int id = trace_one<false>(nodes, node_count, tris, tri_count, r, t, info, visits, tests, slot_pre,
+                                    kOrdered ? stack + threadIdx.x : nullptr);
     lux_hit h;
     h.primitive_id = id;
     h.t = id >= 0 ? t : INFINITY;
     hits[ray] = h;
 }
 
-template <bool kWideWalk>
-__global__ __launch_bounds__(kBlock) void k_trace_any(const WideNode* __restrict__ wide,
-                                                      const lux_bvh_node* __restrict__ nodes,
-                                                      int node_count,
-                                                      const lux_triangle* __restrict__ tris,
-                                                      int tri_count, const lux_ray* __restrict__ rays,
-                                                      int n, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                                      const lux_triangle* __restrict__ tris, int tri_count,
+                                                      const lux_ray* __restrict__ rays, int n,
+                                                      const uint32_t* __restrict__ order,
                                                       uint8_t* __restrict__ occluded, DevInfo* info) {
     const int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -232,36 +214,33 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const WideNode* __restrict
     }
     RayIn r = load_ray(rays, ray);
     float t;
-    uint32_t visits = 0, leaf_boxes = 0, tests = 0;
-    int id = trace_one<true, kWideWalk, false>(wide, nodes, node_count, tris, tri_count, r, t, info, visits,
-                                               leaf_boxes, tests);
+    uint32_t visits = 0, tests = 0;
+    int id = trace_one<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
     occluded[ray] = id >= 0 ? 1 : 0;
 }
 
-// The walks of the handle, counting per ray the nodes loaded (32-byte binary
-// nodes, or 128-byte wide records), the exact leaf boxes loaded separately
-// (none in either layout) and the triangle tests (summed per wave, one atomic
-// per wave): stats[0..2] closest-hit, stats[3..5] any-hit. Used to state the
-// bytes a traversal actually touches.
-template <bool kWideWalk, bool kOrdered>
-__global__ __launch_bounds__(kBlock) void k_trace_stats(const WideNode* __restrict__ wide,
-                                                        const lux_bvh_node* __restrict__ nodes,
-                                                        int node_count,
+// The walks of the handle, counting per ray the 32-byte nodes loaded and the
+// triangle tests (summed per wave, one atomic per wave): stats[0..2]
+// closest-hit {nodes, 0, tests}, stats[3..5] any-hit {nodes, 0, tests}. Rays
+// the near-child-first walk hands to the preorder walk count both walks. Used
+// to state the bytes a traversal actually touches.
+template <bool kOrdered>
+__global__ __launch_bounds__(kBlock) void k_trace_stats(const lux_bvh_node* __restrict__ nodes, int node_count,
                                                         const uint32_t* __restrict__ slot_pre,
-                                                        const lux_triangle* __restrict__ tris,
-                                                        int tri_count, const lux_ray* __restrict__ rays,
-                                                        int n, unsigned long long* stats, DevInfo* info) {
+                                                        const lux_triangle* __restrict__ tris, int tri_count,
+                                                        const lux_ray* __restrict__ rays, int n,
+                                                        unsigned long long* stats, DevInfo* info) {
     __shared__ uint32_t stack[kOrdered ? kOrderedStack * kBlock : 1];
     int i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t v0 = 0, l0 = 0, t0 = 0, v1 = 0, l1 = 0, t1 = 0;
+    uint32_t v0 = 0, t0 = 0, v1 = 0, t1 = 0;
     if (i < n) {
         RayIn r = load_ray(rays, i);
         float t;
-        trace_one<false, kWideWalk, true>(wide, nodes, node_count, tris, tri_count, r, t, info, v0, l0, t0, slot_pre,
-                                          kOrdered ? stack + threadIdx.x : nullptr);
-        trace_one<true, kWideWalk, true>(wide, nodes, node_count, tris, tri_count, r, t, info, v1, l1, t1);
+        trace_one<false>(nodes, node_count, tris, tri_count, r, t, info, v0, t0, slot_pre,
+                         kOrdered ? stack + threadIdx.x : nullptr);
+        trace_one<true>(nodes, node_count, tris, tri_count, r, t, info, v1, t1);
     }
-    unsigned long long c[6] = {v0, l0, t0, v1, l1, t1};
+    unsigned long long c[6] = {v0, 0, t0, v1, 0, t1};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         unsigned long long x = c[k];
@@ -355,16 +334,20 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const lux_bvh_node* __restri
 }  // namespace
 
 namespace {
-// Below this many rays the origin sort costs more than the locality it buys.
-constexpr int kOrderMinRays = 1 << 16;
-
+// Below kOrderMinRays rays the origin sort costs more than the locality it buys.
 const uint32_t* maybe_order(const BvhDev& d, RaySortDev* order, const lux_ray* rays, int n, hipStream_t s) {
     if (!order || n < kOrderMinRays || d.n < 2) return nullptr;
     return ray_order(*order, d, rays, n, s);
 }
+
 }  // namespace
 
-void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
+const uint32_t* closest_order(RaySortDev& r, bool sort, const BvhDev& d, const lux_ray* rays, int n,
+                              hipStream_t s) {
+    return maybe_order(d, sort ? &r : nullptr, rays, n, s);
+}
+
+void trace_closest(const BvhDev& d, const uint32_t* order, const lux_triangle* tris, int tri_count,
                    const lux_ray* rays, int n, lux_hit* hits, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
@@ -372,45 +355,29 @@ void trace_closest(const BvhDev& d, RaySortDev* order, bool wide, const lux_tria
     // which neighbouring rays share: tracing in ray order (octant, then origin)
     // measured 5.12 ms against 5.60 ms in input order on 2^22 random rays over
     // 2^20 triangles, sort included.
-    const uint32_t* ord = maybe_order(d, order, rays, n, s);
-    if (wide)
-        LUX_LAUNCH("trace_closest", (k_trace_closest<true, false>), div_up(n, kBlock), kBlock, 0, s,
-                   (const WideNode*)d.wide, (const lux_bvh_node*)d.pairs, node_count, (const uint32_t*)d.slot_pre,
-                   tris, tri_count, rays, n, ord, hits, d.info);
-    else
-        LUX_LAUNCH("trace_closest", (k_trace_closest<false, true>), div_up(n, kBlock), kBlock, 0, s,
-                   (const WideNode*)d.wide, (const lux_bvh_node*)d.pairs, node_count, (const uint32_t*)d.slot_pre,
-                   tris, tri_count, rays, n, ord, hits, d.info);
+    LUX_LAUNCH("trace_closest", k_trace_closest<true>, div_up(n, kBlock), kBlock, 0, s,
+               (const lux_bvh_node*)d.pairs, node_count, (const uint32_t*)d.slot_pre, tris, tri_count, rays, n,
+               order, hits, d.info);
 }
 
-void trace_any(const BvhDev& d, RaySortDev* order, bool wide, const lux_triangle* tris, int tri_count,
-               const lux_ray* rays, int n, uint8_t* occluded, hipStream_t s) {
+void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int tri_count, const lux_ray* rays,
+               int n, uint8_t* occluded, hipStream_t s) {
     if (n <= 0) return;
     int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
     const uint32_t* ord = maybe_order(d, order, rays, n, s);
     // Any hit stays on the preorder walk: the ordered walk measured 5.84 ms
     // against 5.40 ms here (it stops at the first hit, so near-first order
     // saves little and its pair steps cost more).
-    if (wide)
-        LUX_LAUNCH("trace_any", k_trace_any<true>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
-    else
-        LUX_LAUNCH("trace_any", k_trace_any<false>, div_up(n, kBlock), kBlock, 0, s, (const WideNode*)d.wide,
-                   (const lux_bvh_node*)d.pairs, node_count, tris, tri_count, rays, n, ord, occluded, d.info);
+    LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.pairs, node_count,
+               tris, tri_count, rays, n, ord, occluded, d.info);
 }
 
-void trace_stats(const BvhDev& d, bool wide, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
+void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s) {
     LUX_LAUNCH("trace_stats_zero", k_zero_u64, 1, 64, 0, s, d_stats, 6);
     if (n <= 0 || d.n <= 0) return;
-    if (wide)
-        LUX_LAUNCH("trace_stats", (k_trace_stats<true, false>), div_up(n, kBlock), kBlock, 0, s,
-                   (const WideNode*)d.wide, (const lux_bvh_node*)d.pairs, d.n > 1 ? 2 * d.n : 1,
-                   (const uint32_t*)d.slot_pre, tris, tri_count, rays, n, d_stats, d.info);
-    else
-        LUX_LAUNCH("trace_stats", (k_trace_stats<false, true>), div_up(n, kBlock), kBlock, 0, s,
-                   (const WideNode*)d.wide, (const lux_bvh_node*)d.pairs, d.n > 1 ? 2 * d.n : 1,
-                   (const uint32_t*)d.slot_pre, tris, tri_count, rays, n, d_stats, d.info);
+    LUX_LAUNCH("trace_stats", k_trace_stats<true>, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.pairs,
+               d.n > 1 ? 2 * d.n : 1, (const uint32_t*)d.slot_pre, tris, tri_count, rays, n, d_stats, d.info);
 }
 
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s) {

@@ -13,6 +13,19 @@ namespace {
 // against 256 on 4.19 M rays over 64 instances.
 constexpr int kBlock = 128;
 
+__device__ __forceinline__ lux_scene_hit scene_hit(const TwoLevelHit& h) {
+    lux_scene_hit out;
+    out.primitive_id = h.prim;
+    out.t = h.prim >= 0 ? h.t : INFINITY;
+    out.u = h.prim >= 0 ? h.u : 0.0f;
+    out.v = h.prim >= 0 ? h.v : 0.0f;
+    out.instance_id = h.instance;
+    out.local_primitive_id = h.local;
+    return out;
+}
+
+// Closest hit near child first at both levels; a ray the walk cannot settle
+// (stack overflow or the check of ordered.hpp) takes the preorder walk.
 __global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, const lux_ray* __restrict__ rays,
                                                               const int32_t* __restrict__ ignore, int n,
                                                               lux_scene_hit* __restrict__ hits) {
@@ -25,14 +38,7 @@ __global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, co
     if (!two_level_closest_ordered(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ign, stack + threadIdx.x,
                                    kBlock, h))
         h = two_level_walk<false>(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ign);
-    lux_scene_hit out;
-    out.primitive_id = h.prim;
-    out.t = h.prim >= 0 ? h.t : INFINITY;
-    out.u = h.prim >= 0 ? h.u : 0.0f;
-    out.v = h.prim >= 0 ? h.v : 0.0f;
-    out.instance_id = h.instance;
-    out.local_primitive_id = h.local;
-    hits[i] = out;
+    hits[i] = scene_hit(h);
 }
 
 __global__ __launch_bounds__(kBlock) void k_two_level_occluded(TwoLevelView s, const lux_ray* __restrict__ rays,

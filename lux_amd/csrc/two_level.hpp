@@ -185,12 +185,12 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
 // instance leaf, BLAS preorder of the triangle leaf) and keeps the first of
 // equal t; this walk keeps, of equal t, the one smallest in that order
 // (slot_pre of both levels, read only on an exact tie), so both return the
-// same hit. The ambiguity record of ordered.hpp spans both levels: a
-// triangle's path entry L takes the larger of its BLAS leaf's entry (object
-// space) and its instance's TLAS leaf entry (world space), and boxes of both
-// levels culled beyond t_max feed m_cull. Returns false when the stack would
-// overflow, the arrays are malformed (error bits set) or the ambiguity check
-// fires; the caller then runs two_level_walk<false> for the ray.
+// same hit. The check of ordered.hpp spans both levels: boxes of both are
+// entered up to t_cull, and a triangle's largest path entry is the larger of
+// its BLAS leaf's entry (object space) and its instance's TLAS leaf entry
+// (world space). Returns false when the stack would overflow, the arrays are
+// malformed (error bits set) or the check fires; the caller then runs
+// two_level_walk<false> for the ray.
 constexpr uint32_t kInstanceItem = 0x80000000u;
 
 __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s, const f3& WO, const f3& WD,
@@ -219,8 +219,8 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         load_node32(nodes, 0, a, b);
         float tn;
         const bool par = wave_any_parallel(sr);
-        const bool hit = par ? slab_near<true>(sr, ob.t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn, ob.m_cull)
-                             : slab_near<false>(sr, ob.t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn, ob.m_cull);
+        const bool hit = par ? slab_near<true>(sr, ob.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)
+                             : slab_near<false>(sr, ob.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn);
         if (!hit) return true;
         const int p = __float_as_int(b.z);
         item = p >= 0 ? kInstanceItem : ~uint32_t(p);
@@ -238,16 +238,14 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
         float t, u, v;
         if (!tri_uvt(O, D, v0, v1, v2, t, u, v) || !(t > t_min)) return;
+        ob.note(t, fmaxf(tn, inst_tn));
         const bool take =
             t < ob.t_max ||
             (best.prim >= 0 && t == ob.t_max &&
              (inst_slot != best_inst_slot ? s.tlas_slot_pre[inst_slot] < s.tlas_slot_pre[best_inst_slot]
                                           : blas_pre[slot] < blas_pre[best_slot]));
-        if (!take) {
-            ob.pass_over(t);
-            return;
-        }
-        ob.take(t, fmaxf(tn, inst_tn));
+        if (!take) return;
+        ob.take(t);
         best.prim = first + prim;
         best.instance = inst_cur;
         best.local = prim;
@@ -288,11 +286,10 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
                 }
                 {
                     // entry of this instance's TLAS leaf box (tested when its
-                    // item was taken or pushed; not a cull record here)
-                    float cull_unused = INFINITY;
+                    // item was taken or pushed)
                     const bool wpar = wave_any_parallel(wsr);
-                    (void)(wpar ? slab_near<true>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn, cull_unused)
-                                : slab_near<false>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn, cull_unused));
+                    (void)(wpar ? slab_near<true>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn)
+                                : slab_near<false>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn));
                 }
                 in_blas = true;
                 sp0 = sp;
@@ -311,8 +308,8 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
                 load_node32(nodes, 0, a, b);
                 float tn;
                 const bool par = wave_any_parallel(sr);
-                const bool hit = par ? slab_near<true>(sr, ob.t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn, ob.m_cull)
-                                     : slab_near<false>(sr, ob.t_max, a.x, a.y, a.z, a.w, b.x, b.y, tn, ob.m_cull);
+                const bool hit = par ? slab_near<true>(sr, ob.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)
+                                     : slab_near<false>(sr, ob.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn);
                 const int root = __float_as_int(b.z);
                 if (hit && root >= 0) leaf(0u, root, tn);
                 if (hit && root < 0) {
@@ -333,10 +330,10 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             load_node32(nodes, c + 1, a1, b1);
             float tn0, tn1;
             const bool par = wave_any_parallel(sr);
-            const bool h0 = par ? slab_near<true>(sr, ob.t_max, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0, ob.m_cull)
-                                : slab_near<false>(sr, ob.t_max, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0, ob.m_cull);
-            const bool h1 = par ? slab_near<true>(sr, ob.t_max, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1, ob.m_cull)
-                                : slab_near<false>(sr, ob.t_max, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1, ob.m_cull);
+            const bool h0 = par ? slab_near<true>(sr, ob.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0)
+                                : slab_near<false>(sr, ob.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
+            const bool h1 = par ? slab_near<true>(sr, ob.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1)
+                                : slab_near<false>(sr, ob.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
             const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
             bool take0, take1;
             uint32_t i0, i1;
@@ -375,7 +372,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             O = WO;
             D = WD;
         }
-        if (sp == 0) return !ob.ambiguous();
+        if (sp == 0) return !(best.prim >= 0 && ob.ambiguous());
         --sp;
         item = stack[sp * stride];
     }

@@ -134,10 +134,6 @@ class GpuBvh:
     def refit_triangles_device(self, d_tris, n, stream=None):
         call("lux_hip_bvh_refit_triangles_device", self.handle, d_tris.ptr, int(n), stream)
 
-    def set_wide_walk(self, enable):
-        """Traverse a 4-wide tree collapsed from the LBVH (off by default); results are identical."""
-        call("lux_hip_bvh_set_wide_walk", self.handle, int(bool(enable)))
-
     def set_ray_order(self, enable):
         """Trace large ray sets in origin order (default on); results are identical."""
         call("lux_hip_bvh_set_ray_order", self.handle, int(bool(enable)))

@@ -22,9 +22,9 @@ REF_LIB = ORACLE_DIR / "_ref" / "libluxref.so"
 _lib = None
 _ref = None
 
-# Relative window of the near-child-first walk's ambiguity check (the library
-# uses the same value): boxes culled with an entry within L * 2^-11 of the
-# best hit's bound send the ray back to the preorder walk.
+# Soft window of the library's checked near-child-first walk (the library uses
+# the same value): once it holds a hit at t, boxes are entered up to
+# t + |t| * 2^-11.
 ORDERED_WINDOW_LOG2 = 11
 
 
@@ -151,8 +151,7 @@ class OracleBvh:
 
     def trace_ordered(self, tris, rays, rule, window_log2=ORDERED_WINDOW_LOG2):
         """Model of the library's near-child-first closest-hit walk
-        (ordered_model.cpp): rule 0 = plain walk, rule 1 = with the ambiguity
-        check. Returns ids, t and per-ray flags (bit 0 stack overflow, bit 1
+        (ordered_model.cpp): rule 0 = plain walk, rule 1 = the checked walk. Returns ids, t and per-ray flags (bit 0 stack overflow, bit 1
         check fired; either means the ray was walked again in preorder)."""
         t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 9)

@@ -16,16 +16,19 @@
 // the t_max held when its box is tested, and the near-child-first order holds
 // a different t_max than the preorder order. Rule 0 below is the plain
 // near-child-first walk (it can return another hit than the preorder walk);
-// rule 1 adds the library's check and, when it fires, walks the ray again in
-// preorder. The check, with L = max(t_best, entry distance of the best
-// triangle's leaf box):
-//   (a) some other triangle the walk tested has a hit t < L (the preorder walk
-//       may hold that t when it reaches the best triangle's box and cull it);
-//   (b) some box culled only because its entry exceeded the held t_max has an
-//       entry <= L + L * 2^-kWindowLog2 (it may hold a triangle whose rounded t
-//       lies below its rounded entry, which the preorder walk may reach).
-// A ray with neither walks to the same triangle in both orders, as long as no
-// triangle's t sits more than L * 2^-kWindowLog2 below its leaf box's entry.
+// rule 1 is the library's checked walk:
+//   * after the first hit, boxes are entered up to t_cull = t_max + |t_max| *
+//     2^-window_log2 instead of t_max, so every triangle whose t is <= the
+//     final t_best is tested (as long as no triangle's t lies more than that
+//     window below its own leaf box's entry);
+//   * a tested hit whose t lies below its own leaf box's entry (the largest
+//     entry on its path) is "inconsistent"; m is the smallest t of those;
+//   * the ray is walked again in preorder when m <= t_best.
+// Why this is exact: the preorder walk accepts triangle T iff its leaf box
+// passes (entry <= held t) and t_T < held t, in preorder. If every hit with
+// t <= t_best is tested and none is inconsistent, the tested minimum (t, pre)
+// is reached with a held t above its t, so the preorder walk accepts it and
+// nothing after it can win.
 
 #include "oracle_core.hpp"
 
@@ -44,7 +47,7 @@ struct SlabOut {
     float tn;
 };
 
-SlabOut slab_near(const Box& b, const Ray& r, float t_max) {
+SlabOut slab_near(const Box& b, const Ray& r, float t_cull) {
     bool miss = false;
     float tn = r.t_min, tf = INFINITY;
     for (int i = 0; i < 3; ++i) {
@@ -62,7 +65,7 @@ SlabOut slab_near(const Box& b, const Ray& r, float t_max) {
     SlabOut s;
     s.tn = tn;
     s.geo = !miss && !(tn > tf);
-    s.hit = s.geo && !(tn > t_max);
+    s.hit = s.geo && !(tn > t_cull);
     return s;
 }
 
@@ -89,9 +92,8 @@ struct Walk {
     int best = -1;
     int best_pre = 0;
     float t_max;
-    float L = -INFINITY;          // max(t_best, leaf entry of the best triangle)
-    float m_hit = INFINITY;       // smallest hit t among other tested triangles
-    float m_cull = INFINITY;      // smallest entry among boxes culled by t_max
+    float t_cull;        // boxes are entered up to here
+    float m = INFINITY;  // smallest t of a tested hit below its own leaf entry
     bool overflow = false;
 };
 
@@ -100,25 +102,23 @@ struct Walk {
 // (the right child first, as the library's sibling pair holds the right child
 // in its first slot); of two internal children the nearer is entered and the
 // farther pushed (ties enter the right child).
-void ordered_walk(const Tree& t, const Tri* tris, const Ray& r, Walk& w) {
+void ordered_walk(const Tree& t, const Tri* tris, const Ray& r, Walk& w, int rule, float window) {
     const int ni = t.n - 1;
     auto leaf = [&](int h, float tn) {
         const int prim = t.leaf_ids[h - ni];
         float tt;
         if (!tri_uv_t(r, tris[prim], tt) || !(tt > r.t_min)) return;
+        if (tt <= w.t_max && tn > tt) w.m = fminf(w.m, tt);
         const int pre = t.n == 1 ? 0 : t.pre_of[h];
         const bool accept = w.best >= 0 ? (tt < w.t_max || (tt == w.t_max && pre < w.best_pre)) : tt < w.t_max;
         if (accept) {
-            if (w.best >= 0) w.m_hit = fminf(w.m_hit, w.t_max);
             w.best = prim;
             w.best_pre = pre;
             w.t_max = tt;
-            w.L = fmaxf(tt, tn);
-        } else {
-            w.m_hit = fminf(w.m_hit, tt);
+            w.t_cull = rule == 1 ? tt + fabsf(tt) * window : tt;
         }
     };
-    SlabOut root = slab_near(t.bounds[0], r, w.t_max);
+    SlabOut root = slab_near(t.bounds[0], r, w.t_cull);
     if (!root.hit) return;
     if (t.n == 1) {
         leaf(0, root.tn);  // single leaf: host index 0 is the leaf (ni = 0)
@@ -130,10 +130,7 @@ void ordered_walk(const Tree& t, const Tri* tris, const Ray& r, Walk& w) {
     for (;;) {
         const int kids[2] = {host_index(t.right[c], ni), host_index(t.left[c], ni)};
         SlabOut s[2];
-        for (int k = 0; k < 2; ++k) {
-            s[k] = slab_near(t.bounds[kids[k]], r, w.t_max);
-            if (s[k].geo && !s[k].hit) w.m_cull = fminf(w.m_cull, s[k].tn);
-        }
+        for (int k = 0; k < 2; ++k) s[k] = slab_near(t.bounds[kids[k]], r, w.t_cull);
         for (int k = 0; k < 2; ++k)
             if (s[k].hit && kids[k] >= ni) leaf(kids[k], s[k].tn);
         const bool i0 = s[0].hit && kids[0] < ni, i1 = s[1].hit && kids[1] < ni;
@@ -159,10 +156,10 @@ void ordered_walk(const Tree& t, const Tri* tris, const Ray& r, Walk& w) {
 extern "C" {
 
 // Closest hit of each ray by the near-child-first walk. rule 0: plain walk
-// (stack overflow -> preorder walk); rule 1: also the ambiguity check above,
-// with a window of L * 2^-window_log2. flags[i] (nullable): bit 0 = stack
-// overflow, bit 1 = ambiguity check fired (both mean the ray was walked again
-// in preorder). Returns the number of rays walked again.
+// (stack overflow -> preorder walk); rule 1: the checked walk above, with a
+// soft window of |t_max| * 2^-window_log2. flags[i] (nullable): bit 0 = stack
+// overflow, bit 1 = the check fired (both mean the ray was walked again in
+// preorder). Returns the number of rays walked again.
 int32_t orc_trace_ordered(const void* h, const float* tris, const float* rays, int32_t n, int32_t rule,
                           int32_t window_log2, int32_t* ids, float* t_out, uint8_t* flags) {
     const Tree& t = *reinterpret_cast<const Tree*>(h);
@@ -173,13 +170,11 @@ int32_t orc_trace_ordered(const void* h, const float* tris, const float* rays, i
         const Ray& r = rs[i];
         Walk w;
         w.t_max = r.t_max;
+        w.t_cull = r.t_max;
         uint8_t f = 0;
-        if (t.n > 0) ordered_walk(t, tr, r, w);
+        if (t.n > 0) ordered_walk(t, tr, r, w, rule, ldexpf(1.0f, -window_log2));
         if (w.overflow) f |= 1;
-        if (rule == 1 && !w.overflow && w.best >= 0) {
-            const float win = fabsf(w.L) * ldexpf(1.0f, -window_log2);
-            if (w.m_hit < w.L || w.m_cull <= w.L + win) f |= 2;
-        }
+        if (rule == 1 && !w.overflow && w.best >= 0 && w.m <= w.t_max) f |= 2;
         if (f) {
             ++again;
             int id;

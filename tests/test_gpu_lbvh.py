@@ -59,36 +59,29 @@ def test_closest_hit_bit_exact(gpu, case):
 
 
 @pytest.mark.parametrize("case", LBVH_CASES)
-def test_wide_walk_matches_binary_walk(gpu, case):
-    """The opt-in 4-wide walk gives the golden hits bit for bit, also after a
-    refit (the wide records are refreshed with the binary tree)."""
-    print_case("wide walk", case)
+def test_closest_hit_after_refit_matches_oracle(gpu, case):
+    """Refit with jittered triangles, then closest hits equal the oracle's walk
+    of the oracle-refitted tree (same topology, new boxes), and a rebuild from
+    the new boxes gives the hits of a fresh build."""
+    print_case("closest after refit", case)
     g = load_golden(f"lbvh_{case}")
     bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
-    bvh.set_wide_walk(True)
-    hits = gpu.intersect_gpu_bvh_stackless(bvh, g["tri_by_id"], g["rays"])
-    assert np.array_equal(hits["primitive_id"], g["hit_ids"])
-    assert np.array_equal(bits(hits["t"]), bits(g["hit_t"]))
-    occ = gpu.trace_any(bvh, g["tri_by_id"], g["rays"])
-    assert np.array_equal(occ.astype(bool), g["hit_ids"] >= 0)
     moved = W.jitter(g["tris"], sigma=2e-3, seed=7)
     new_boxes = W.triangle_boxes(moved)
     assert gpu.refit_gpu_lbvh(bvh, new_boxes, g["ids"])
     tri_by_id = np.zeros_like(g["tri_by_id"])
     tri_by_id[g["ids"]] = moved
-    wide = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, g["rays"])
-    bvh.set_wide_walk(False)
-    binary = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, g["rays"])
-    assert np.array_equal(wide["primitive_id"], binary["primitive_id"])
-    assert np.array_equal(bits(wide["t"]), bits(binary["t"]))
-    # a rebuild of a wide-walk handle re-collapses the new tree
-    bvh.set_wide_walk(True)
+    hits = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, g["rays"])
+    o = oracle.OracleBvh(g["boxes"], g["ids"])
+    o.refit(new_boxes)
+    exp = o.trace_closest(tri_by_id, g["rays"])
+    assert np.array_equal(hits["primitive_id"], exp["ids"])
+    assert np.array_equal(bits(hits["t"]), bits(exp["t"]))
     bvh.build(new_boxes, g["ids"])
     rebuilt = gpu.intersect_gpu_bvh_stackless(bvh, tri_by_id, g["rays"])
-    plain = gpu.build_gpu_lbvh(new_boxes, g["ids"])
-    expect = gpu.intersect_gpu_bvh_stackless(plain, tri_by_id, g["rays"])
-    assert np.array_equal(rebuilt["primitive_id"], expect["primitive_id"])
-    assert np.array_equal(bits(rebuilt["t"]), bits(expect["t"]))
+    fresh = oracle.OracleBvh(new_boxes, g["ids"]).trace_closest(tri_by_id, g["rays"])
+    assert np.array_equal(rebuilt["primitive_id"], fresh["ids"])
+    assert np.array_equal(bits(rebuilt["t"]), bits(fresh["t"]))
 
 
 @pytest.mark.parametrize("case", ["soup2000", "clustered1500", "sparse_ids777", "tiny3", "tiny1"])
