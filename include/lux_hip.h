@@ -171,6 +171,18 @@ int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_triangl
                                  int32_t triangle_count, const lux_ray* d_rays,
                                  int32_t ray_count, lux_hit* d_hits,
                                  lux_stream stream);
+/* lux_hip_trace_closest_device without allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The only scratch of the closest-hit query, the ray-order sort, lives in a
+ * fixed 68 MiB device array that each GPU holds from library load until the
+ * process ends; ray_count above 4194304 returns LUX_ERR_INVALID_ARGUMENT. The
+ * hits equal lux_hip_trace_closest_device's. Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (any streams) race. */
+int lux_hip_trace_closest_device_noalloc(lux_hip_bvh* bvh, const lux_triangle* d_triangles,
+                                         int32_t triangle_count, const lux_ray* d_rays,
+                                         int32_t ray_count, lux_hit* d_hits,
+                                         lux_stream stream);
 /* Any-hit occlusion: occluded[i] = 1 iff some triangle is hit with
  * t_min < t < t_max. The reference has no early-exit any-hit; its shadow path
  * repeats the closest-hit walk (src/integrator/wavefront_kernels.cu:28-83), so

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "lux_hip_bvh_download_topology": [_P, _P, _P, _P, _P, _P, _P, _P],
     "lux_hip_trace_closest": [_P, _P, c_int32, _P, c_int32, _P],
     "lux_hip_trace_closest_device": [_P, _P, c_int32, _P, c_int32, _P, _P],
+    "lux_hip_trace_closest_device_noalloc": [_P, _P, c_int32, _P, c_int32, _P, _P],
     "lux_hip_trace_any": [_P, _P, c_int32, _P, c_int32, _P],
     "lux_hip_trace_any_device": [_P, _P, c_int32, _P, c_int32, _P, _P],
     "lux_hip_overlap_self": [_P, c_int32, _P, POINTER(c_int32)],

@@ -13,7 +13,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if Path("/opt/rocm/bin/hip
 GPU_ARCH = "gfx950"
 REFERENCE_SRC = Path("/root/reference/src")
 
-HIP_SOURCES = ["lbvh.hip", "query.hip", "two_level.hip", "film.hip", "obj_import.hip", "pt.hip", "pt_scene.hip", "serve.hip", "abi.hip"]
+HIP_SOURCES = ["lbvh.hip", "query.hip", "two_level.hip", "film.hip", "obj_import.hip", "pt.hip", "pt_scene.hip", "serve.hip", "noalloc.hip", "abi.hip"]
 # -ffp-contract=off keeps the reference's exact float operation order (no FMA
 # contraction) in box, triangle and Morton arithmetic; packed FP32 ops are off.
 HIP_FLAGS = [

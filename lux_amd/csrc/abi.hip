@@ -169,6 +169,7 @@ int lux_hip_init(int device) {
         if (device < 0 || device >= count)
             throw Error(LUX_ERR_INVALID_ARGUMENT, "device index out of range");
         hip_check(hipSetDevice(device), "hipSetDevice");
+        noalloc_setup(device);  // device address of the no-alloc scratch array
         hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
         g_device = device;
         g_initialized = true;
@@ -553,6 +554,25 @@ int lux_hip_trace_closest_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, i
         bvh->last_stream = s;
         trace_closest(bvh->dev, closest_order(bvh->order, bvh->order_rays, bvh->dev, d_rays, n, s), d_tris, tri_count,
                       d_rays, n, d_hits, s);
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_trace_closest_device_noalloc(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t tri_count,
+                                         const lux_ray* d_rays, int32_t n, lux_hit* d_hits, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0 || tri_count < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > kNoAllocMaxRays)
+            throw Error(LUX_ERR_INVALID_ARGUMENT,
+                        "lux_hip_trace_closest_device_noalloc: more than 4194304 rays in one call (split the batch)");
+        if (n > 0 && (!d_rays || !d_hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
+        if (!bvh->valid) throw Error(LUX_ERR_INVALID_ARGUMENT, "BVH is not valid");
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        trace_closest_noalloc(g_device, bvh->dev, bvh->order_rays, d_tris, tri_count, d_rays, n, d_hits, s);
         return int(LUX_OK);
     });
 }

@@ -195,6 +195,14 @@ void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int
 void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const lux_ray* rays, int n,
                  unsigned long long* d_stats, hipStream_t s);
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Closest hit with the ray-order scratch in a fixed per-device array
+// (noalloc.hip): no allocation. noalloc_setup takes the array's device address
+// on `device` (current) once; trace_closest_noalloc rejects n > kNoAllocMaxRays.
+constexpr int kNoAllocMaxRays = 1 << 22;
+void noalloc_setup(int device);
+size_t noalloc_scratch_bytes();
+void trace_closest_noalloc(int device, const BvhDev& d, bool order_rays, const lux_triangle* tris, int tri_count,
+                           const lux_ray* rays, int n, lux_hit* hits, hipStream_t s);
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
                    lux_pair* pairs, int32_t* counter, hipStream_t s);
 
