@@ -397,6 +397,8 @@ typedef struct lux_serve_stats {
     uint64_t batches, paths;
     uint64_t resident_bytes, peak_bytes;
     uint64_t extension_rays, shadow_rays;
+    uint64_t shared_batches;      /* batches that held paths of two or more requests */
+    uint64_t max_batch_requests;  /* most requests with paths in one batch */
 } lux_serve_stats;
 
 typedef struct lux_hip_server lux_hip_server;

@@ -1519,6 +1519,8 @@ int lux_hip_server_stats(lux_hip_server* sv, lux_serve_stats* out) {
         out->paths = st.paths;
         out->resident_bytes = st.resident_bytes;
         out->peak_bytes = st.peak_bytes;
+        out->shared_batches = st.shared_batches;
+        out->max_batch_requests = st.max_batch_requests;
         sv->server->ray_counts(&out->extension_rays, &out->shadow_rays);
         return int(LUX_OK);
     });

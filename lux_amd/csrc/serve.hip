@@ -169,11 +169,15 @@ bool Server::enqueue_one() {
     // to the back; the ones that received nothing keep their place at the front
     std::deque<Job*> keep, served;
     size_t k = 0;
+    uint64_t in_batch = 0;
     for (Job* j : issue_) {
         const uint64_t was = before[k++];
+        if (j->rq.issued > was) ++in_batch;
         if (j->rq.issued >= j->rq.total) continue;  // fully placed; completes with its last batch
         (j->rq.issued > was ? served : keep).push_back(j);
     }
+    if (in_batch >= 2) ++stats_.shared_batches;
+    stats_.max_batch_requests = std::max(stats_.max_batch_requests, in_batch);
     issue_.clear();
     for (Job* j : keep) issue_.push_back(j);
     for (Job* j : served) issue_.push_back(j);

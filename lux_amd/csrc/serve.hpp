@@ -34,6 +34,8 @@ struct ServeStats {
     uint64_t submitted = 0, admitted = 0, completed = 0, refits = 0;
     uint64_t batches = 0, paths = 0;
     uint64_t resident_bytes = 0, peak_bytes = 0;
+    uint64_t shared_batches = 0;      // batches holding paths of two or more requests
+    uint64_t max_batch_requests = 0;  // most requests with paths in one batch
 };
 
 class Server {

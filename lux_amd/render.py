@@ -353,7 +353,8 @@ class ServeConfig(ctypes.Structure):
 
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, c_uint64) for n in ("submitted", "admitted", "completed", "refits", "batches", "paths",
-                                        "resident_bytes", "peak_bytes", "extension_rays", "shadow_rays")]
+                                        "resident_bytes", "peak_bytes", "extension_rays", "shadow_rays",
+                                        "shared_batches", "max_batch_requests")]
 
 
 class Server:

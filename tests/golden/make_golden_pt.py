@@ -39,6 +39,13 @@ Cases:
   pt_all_lights_bvh / _power   the Cornell box with a point, a distant, a
                   uniform and an image infinite light besides its area light,
                   BVH sampler (tree + non-BVH group) and power sampler
+  pt_c3_band      the full-size single-scene workload (displaced icosphere,
+                  865 280 triangles, 1024x1024, 64 spp, depth 8, seed 0):
+                  rows 128..191 of the film (the lit top of the sphere; the
+                  rows below the equator see no light) as trace_path's
+                  per-sample radiance summed in sample order, 4096 of those
+                  samples, and a SHA-256 of the scene arrays (tests rebuild
+                  the scene)
 Each stores the scene arrays, the camera and light tables the reference derived.
 
     python tests/golden/make_golden_pt.py instanced   # only the instanced cases
@@ -46,6 +53,7 @@ Each stores the scene arrays, the camera and light tables the reference derived.
     python tests/golden/make_golden_pt.py lights      # only the light-sampler cases
     python tests/golden/make_golden_pt.py scene_lights  # only the scene-light cases
     python tests/golden/make_golden_pt.py builtin     # only the built-in CLI scenes
+    python tests/golden/make_golden_pt.py c3band      # only pt_c3_band
 """
 
 import sys
@@ -148,6 +156,40 @@ def builtin_cases(w=48, h=48, spp=8, depth=4):
         print(f"{out}: {w}x{h} spp={spp} depth={depth} ordinals={r.ordinals} mean={film.mean() / spp:.5f}")
 
 
+def scene_digest(scene):
+    import hashlib
+
+    h = hashlib.sha256()
+    for k in ("vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera"):
+        h.update(np.ascontiguousarray(scene[k]).tobytes())
+    return h.hexdigest()
+
+
+def c3_band_case(y0=128, y1=192, spp=64, depth=8, seed=0):
+    scene = scenes.displaced_icosphere()  # frequency 208, 1024x1024
+    w = h = 1024
+    r = oracle.RefPathTracer(scene)
+    px = np.arange(y0 * w, y1 * w)
+    band = np.zeros((len(px), 3), np.float32)
+    rng = np.random.default_rng(15)
+    pick_px = rng.integers(0, len(px), 4096)
+    pick_s = rng.integers(0, spp, 4096)
+    picked = np.zeros((4096, 3), np.float32)
+    for s in range(spp):
+        rad = r.samples(w, h, depth, px, np.full(len(px), s), seed=seed).astype(np.float32)
+        band += rad  # film order: each pixel's samples added in sample order
+        sel = pick_s == s
+        picked[sel] = rad[pick_px[sel]]
+    data = dict(width=np.int32(w), height=np.int32(h), spp=np.int32(spp), depth=np.int32(depth),
+                seed=np.uint64(seed), rows=np.array([y0, y1], np.int32), band_sum=band.reshape(y1 - y0, w, 3),
+                sample_pixels=(px[pick_px]).astype(np.int32), sample_index=pick_s.astype(np.int32),
+                sample_radiance=picked, camera_vectors=r.camera(), scene_sha256=np.array(scene_digest(scene)),
+                triangles=np.int32(len(scene["indices"])))
+    np.savez_compressed(OUT / "pt_c3_band.npz", **data)
+    print(f"pt_c3_band: rows {y0}..{y1 - 1} of {w}x{h} spp={spp} depth={depth} "
+          f"tris={len(scene['indices'])} mean={band.mean() / spp:.5f}")
+
+
 def scene_light_cases():
     """The reference CLI's light demo scenes and a Cornell box with every light
     kind at once (BVH and power samplers)."""
@@ -198,6 +240,9 @@ def main(only=None):
     if only == "builtin":
         builtin_cases()
         return
+    if only == "c3band":
+        c3_band_case()
+        return
     w = h = 32
     px = np.repeat(np.arange(w * h), 4)
     sm = np.tile(np.arange(4), w * h)
@@ -220,6 +265,7 @@ def main(only=None):
     light_sampler_cases()
     scene_light_cases()
     builtin_cases()
+    c3_band_case()
 
 
 if __name__ == "__main__":
