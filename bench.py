@@ -63,19 +63,20 @@ def parse():
     ap.add_argument("--c5-requests", type=int, default=96)
     ap.add_argument("--c5-load", type=float, default=0.75, help="arrival rate as a fraction of the c4 capacity")
     ap.add_argument("--cpu-render-rows", type=int, default=32,
-                    help="rows of the C3 image the CPU render baseline traces (centre band)")
+                    help="rows of the C3 image the CPU render baseline traces (from row 128, the lit band)")
     return ap.parse_args()
 
 
 def cpu_baseline(inp, sample_rays):
-    """The reference's own host LBVH build + stack traversal, one thread."""
+    """The host LBVH build + stack traversal of the reference, as restated by
+    the oracle (bit-identical to the reference on the golden vectors), one
+    thread."""
     import oracle
 
     boxes, tris, rays = inp["boxes"], inp["tris"], inp["rays"]
     sample = rays[:sample_rays]
-    kind = "reference" if oracle.ref_available() else "port"
     t0 = time.perf_counter()
-    tree = oracle.RefLbvh(boxes) if kind == "reference" else oracle.OracleBvh(boxes)
+    tree = oracle.OracleBvh(boxes)
     t1 = time.perf_counter()
     tree.trace_closest(tris, sample)
     t2 = time.perf_counter()
@@ -83,10 +84,10 @@ def cpu_baseline(inp, sample_rays):
         "value": round(len(sample) / (t2 - t1) / 1e6, 4),
         "unit": "Mrays/s",
         "cores": 1,
-        "kind": kind,
+        "kind": "port",
         "sample": (f"closest-hit of the first {len(sample)} of the {len(rays)} rays over the host LBVH of the "
-                   f"same {len(boxes)}-triangle soup (intersect_cpu_bvh_tree, 1 thread); host LBVH build "
-                   f"{len(boxes) / (t1 - t0) / 1e6:.2f} Mprims/s"),
+                   f"same {len(boxes)}-triangle soup (restated build_cpu_lbvh_tree + intersect_cpu_bvh_tree, "
+                   f"1 thread); host LBVH build {len(boxes) / (t1 - t0) / 1e6:.2f} Mprims/s"),
         "build_mprims_per_s": round(len(boxes) / (t1 - t0) / 1e6, 4),
         "host_cpu": _cpu_name(),
     }
@@ -156,7 +157,10 @@ def bench_c3(args):
         out["films_bit_identical"] = bool(np.array_equal(films["wavefront"].view(np.uint32),
                                                          films["plain"].view(np.uint32)))
     if args.cpu_render_rows > 0:
-        out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], args.cpu_render_rows)
+        # rows from 128: the lit top of the sphere (the rows below the equator
+        # see no light)
+        out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], args.cpu_render_rows,
+                                                  128)
     sc.close()
     return out
 
@@ -210,25 +214,19 @@ def bench_instanced(args):
     if args.cpu_render_rows > 0:
         import oracle
 
-        if oracle.ref_available():
-            nrows = max(1, args.cpu_render_rows // 4)
-            r0 = H // 2 - nrows // 2
-            paths = W * nrows * spp
-            ref = oracle.RefPathTracer(d)
-            pix = np.repeat(np.arange(r0 * W, (r0 + nrows) * W, dtype=np.int32), spp)
-            smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
-            s0 = time.perf_counter()
-            rad = ref.samples(W, H, depth, pix, smp)
-            t_ref = time.perf_counter() - s0
-            band = np.zeros((nrows * W, 3), np.float32)
-            for k in range(spp):
-                band += rad[k::spp]
-            gpu_band = films["instanced"][r0:r0 + nrows].reshape(-1, 3)
-            out["cpu_baseline"] = {
-                "value": round(paths / t_ref / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "reference",
-                "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
-                "host_cpu": _cpu_name(),
-                "gpu_vs_reference_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6)}
+        nrows = max(1, args.cpu_render_rows // 4)
+        r0 = H // 2 - nrows // 2
+        paths = W * nrows * spp
+        port = oracle.OraclePathTracer(d)
+        s0 = time.perf_counter()
+        band = port.render(W, H, spp, depth, rows=(r0, r0 + nrows), threads=1)[r0:r0 + nrows].reshape(-1, 3)
+        t_port = time.perf_counter() - s0
+        gpu_band = films["instanced"][r0:r0 + nrows].reshape(-1, 3)
+        out["cpu_baseline"] = {
+            "value": round(paths / t_port / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+            "host_cpu": _cpu_name(),
+            "gpu_vs_cpu_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6)}
     sc.close()
     fsc.close()
     film.free()
@@ -279,45 +277,35 @@ def bench_bunny(args):
     if args.cpu_render_rows > 0:
         import oracle
 
-        if oracle.ref_available():
-            nrows = max(1, args.cpu_render_rows // 2)
-            r0 = H // 2 - nrows // 2
-            paths = W * nrows * spp
-            ref = oracle.RefPathTracer(d)
-            pix = np.repeat(np.arange(r0 * W, (r0 + nrows) * W, dtype=np.int32), spp)
-            smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
-            s0 = time.perf_counter()
-            rad = ref.samples(W, H, depth, pix, smp)
-            t_ref = time.perf_counter() - s0
-            band = np.zeros((nrows * W, 3), np.float32)
-            for k in range(spp):
-                band += rad[k::spp]
-            out["cpu_baseline"] = {
-                "value": round(paths / t_ref / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "reference",
-                "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
-                "host_cpu": _cpu_name(),
-                "gpu_vs_reference_band_relrmse": round(
-                    render.rel_rmse(gpu[r0:r0 + nrows].reshape(-1, 3) / spp, band / spp), 8)}
+        nrows = max(1, args.cpu_render_rows // 2)
+        r0 = H // 2 - nrows // 2
+        paths = W * nrows * spp
+        port = oracle.OraclePathTracer(d)
+        s0 = time.perf_counter()
+        band = port.render(W, H, spp, depth, rows=(r0, r0 + nrows), threads=1)[r0:r0 + nrows].reshape(-1, 3)
+        t_port = time.perf_counter() - s0
+        out["cpu_baseline"] = {
+            "value": round(paths / t_port / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
+            "host_cpu": _cpu_name(),
+            "gpu_vs_cpu_band_relrmse": round(
+                render.rel_rmse(gpu[r0:r0 + nrows].reshape(-1, 3) / spp, band / spp), 8)}
     sc.close()
     film.free()
     return out
 
 
-def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows):
-    """A band of `nrows` rows through the image centre on the CPU:
-    - the reference's own trace_path (oracle/_ref, compiled from the LUX
-      sources), one thread, camera samples drawn as render_cpu_path_tracer
-      draws them and summed per pixel in sample order -> `value`, kind
-      "reference" (the CPU restatement stands in where oracle/_ref is absent);
-    - the CPU restatement (bit-identical to it) on one thread and on all host
-      threads (rows split across threads, per-pixel sample order unchanged)."""
+def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows, r0):
+    """A band of `nrows` rows from row r0 on the CPU, by the CPU restatement of
+    the reference path tracer (bit-identical to the reference's trace_path on
+    the golden renders): one thread (`value`, as the reference renders) and all
+    host threads (rows split across threads, per-pixel sample order unchanged)."""
     import oracle
 
-    r0 = H // 2 - nrows // 2
     rows = (r0, r0 + nrows)
     band = slice(rows[0], rows[1])
     paths = W * nrows * spp
-    out = {"unit": "Mpaths/s", "cores": 1,
+    out = {"unit": "Mpaths/s", "cores": 1, "kind": "port",
            "sample": f"rows {rows[0]}..{rows[1] - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
            "host_cpu": _cpu_name()}
     t0 = time.perf_counter()
@@ -330,28 +318,11 @@ def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows):
     s0 = time.perf_counter()
     pt.render(W, H, spp, depth, rows=rows, threads=threads)
     t_all = time.perf_counter() - s0
-    out.update(port_Mpaths_per_s=round(paths / t_port / 1e6, 4),
-               port_all_threads_Mpaths_per_s=round(paths / t_all / 1e6, 4), all_threads=threads)
     from lux_amd.render import rel_rmse
 
-    out["gpu_vs_cpu_band_relrmse"] = round(rel_rmse(gpu_film[band] / spp, port[band] / spp), 8)
-    if oracle.ref_available():
-        t0 = time.perf_counter()
-        ref = oracle.RefPathTracer(scene)
-        out["reference_scene_prepare_s"] = round(time.perf_counter() - t0, 3)
-        pix = np.repeat(np.arange(rows[0] * W, rows[1] * W, dtype=np.int32), spp)
-        smp = np.tile(np.arange(spp, dtype=np.int32), nrows * W)
-        s0 = time.perf_counter()
-        rad = ref.samples(W, H, depth, pix, smp)
-        t_ref = time.perf_counter() - s0
-        film = np.zeros((nrows * W, 3), np.float32)
-        for k in range(spp):  # per pixel, in sample order, as render_cpu_path_tracer adds them
-            film += rad[k::spp]
-        out.update(value=round(paths / t_ref / 1e6, 4), kind="reference",
-                   reference_equals_port=bool(np.array_equal(film.view(np.uint32),
-                                                             port[band].reshape(-1, 3).view(np.uint32))))
-    else:
-        out.update(value=out["port_Mpaths_per_s"], kind="port")
+    out.update(value=round(paths / t_port / 1e6, 4), all_threads_Mpaths_per_s=round(paths / t_all / 1e6, 4),
+               all_threads=threads,
+               gpu_vs_cpu_band_relrmse=round(rel_rmse(gpu_film[band] / spp, port[band] / spp), 8))
     return out
 
 

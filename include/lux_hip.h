@@ -159,6 +159,19 @@ int lux_hip_bvh_download_topology(lux_hip_bvh* bvh, uint32_t* morton_sorted,
                                   uint32_t* sorted_indices, uint32_t* left,
                                   uint32_t* right, uint32_t* parent,
                                   uint32_t* leaf_parent, lux_aabb* node_bounds);
+/* The primitives the tree holds, in input order: boxes (as last built or
+ * refit) and primitive ids, n each (GpuBvhStorage::input_bounds /
+ * primitive_ids, read by download_bvh_for_validation and
+ * validate_gpu_bvh_matches_primitives, src/bvh/gpu_bvh.cu:615-891). Either
+ * pointer may be NULL. */
+int lux_hip_bvh_download_input(lux_hip_bvh* bvh, lux_aabb* boxes, int32_t* primitive_ids);
+/* GpuBvh::overlap_bounds_device_ptr / overlap_primitive_ids_device_ptr /
+ * overlap_sorted_indices_device_ptr (src/bvh/gpu_bvh.cuh:41-43, gpu_bvh.cu:443-453):
+ * the device arrays of the input boxes and ids (input order) and of the sorted
+ * input indices, n each; NULL for an invalid tree. Valid until the next build
+ * or the handle's destruction. Any out pointer may be NULL. */
+int lux_hip_bvh_device_arrays(lux_hip_bvh* bvh, const lux_aabb** d_input_bounds,
+                              const int32_t** d_primitive_ids, const uint32_t** d_sorted_indices);
 
 /* ---- queries ----------------------------------------------------------- */
 /* intersect_gpu_bvh_stackless (src/bvh/gpu_bvh.cuh:57-59, gpu_bvh.cu:893-915):
@@ -297,6 +310,8 @@ typedef struct lux_hip_scene lux_hip_scene;
  * :61-79): light tables, camera, device arrays and the LBVH over the
  * triangles. Synchronous. */
 int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out);
+/* LUX_ERR_INVALID_ARGUMENT while a server request for the scene (submitted
+ * and not yet completed) exists. */
 int lux_hip_scene_destroy(lux_hip_scene* scene);
 /* Camera vectors {origin, lower_left, horizontal, vertical} (12 floats). */
 int lux_hip_scene_camera(lux_hip_scene* scene, float* out12);
@@ -338,7 +353,9 @@ int lux_hip_scene_info(lux_hip_scene* scene, int32_t* triangle_count, int32_t* v
  * regathered on the device and the LBVH is refit with its topology kept, as
  * refit_gpu_lbvh does for a primitive set with unchanged ids
  * (src/bvh/gpu_bvh.cu:597-613); emitter areas and the power-alias light table
- * are rebuilt. Renders enqueued earlier read the old geometry. Synchronous. */
+ * are rebuilt. Renders enqueued earlier read the old geometry. Synchronous.
+ * LUX_ERR_INVALID_ARGUMENT while a server request for the scene is unfinished
+ * (send the update with lux_hip_server_submit instead). */
 int lux_hip_scene_update_vertices(lux_hip_scene* scene, const lux_vec3* vertices, int32_t vertex_count);
 
 /* ---- multi-scene batches ------------------------------------------------

@@ -1,8 +1,7 @@
-"""Build steps: the gfx950 HIP library, the CPU oracle and (where the reference
-sources exist) the reference shim. Run from anywhere; outputs are in-tree."""
+"""Build step of the product: the gfx950 HIP library (lux_amd/liblux_hip.so).
+Run from anywhere; the output is in-tree."""
 
 import os
-import shutil
 import subprocess
 from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
@@ -11,7 +10,6 @@ from . import LIB_PATH, PACKAGE_DIR, REPO_ROOT
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if Path("/opt/rocm/bin/hipcc").exists() else "hipcc")
 GPU_ARCH = "gfx950"
-REFERENCE_SRC = Path("/root/reference/src")
 
 HIP_SOURCES = ["lbvh.hip", "query.hip", "two_level.hip", "film.hip", "obj_import.hip", "pt.hip", "pt_scene.hip", "serve.hip", "noalloc.hip", "abi.hip"]
 # -ffp-contract=off keeps the reference's exact float operation order (no FMA
@@ -21,10 +19,6 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-ORACLE_SRCS = [REPO_ROOT / "oracle" / "lbvh_oracle.cpp", REPO_ROOT / "oracle" / "pt_oracle.cpp",
-               REPO_ROOT / "oracle" / "two_level_oracle.cpp", REPO_ROOT / "oracle" / "film_oracle.cpp",
-               REPO_ROOT / "oracle" / "ordered_model.cpp"]
-ORACLE_LIB = REPO_ROOT / "oracle" / "_build" / "liblux_oracle.so"
 
 
 def _run(cmd, cwd=REPO_ROOT):
@@ -56,26 +50,5 @@ def build_hip():
     return LIB_PATH
 
 
-def build_oracle():
-    ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
-    cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
-           *map(str, ORACLE_SRCS), "-o", str(ORACLE_LIB)]
-    _run(cmd)
-    return ORACLE_LIB
-
-
-def build_reference_shim():
-    """oracle/_ref/libluxref.so from the reference's own host sources (only where
-    they are mounted; elsewhere the prebuilt file is used)."""
-    if not REFERENCE_SRC.exists() or shutil.which("make") is None:
-        return None
-    _run(["make", "-s", "-B", "-f", "oracle/ref.mk", f"REF_SRC={REFERENCE_SRC}"])
-    return REPO_ROOT / "oracle" / "_ref" / "libluxref.so"
-
-
 def build_all():
-    out = [build_hip(), build_oracle()]
-    ref = build_reference_shim()
-    if ref:
-        out.append(ref)
-    return out
+    return [build_hip()]

@@ -513,6 +513,32 @@ int lux_hip_bvh_download_topology(lux_hip_bvh* bvh, uint32_t* morton_sorted, uin
     });
 }
 
+int lux_hip_bvh_download_input(lux_hip_bvh* bvh, lux_aabb* boxes, int32_t* primitive_ids) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (!bvh->valid) return int(LUX_OK);
+        sync_handle(bvh);
+        const size_t n = size_t(bvh->dev.n);
+        if (boxes) hip_check(hipMemcpy(boxes, bvh->dev.boxes, sizeof(lux_aabb) * n, hipMemcpyDeviceToHost), "D2H boxes");
+        if (primitive_ids)
+            hip_check(hipMemcpy(primitive_ids, bvh->dev.ids, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "D2H ids");
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_bvh_device_arrays(lux_hip_bvh* bvh, const lux_aabb** d_input_bounds, const int32_t** d_primitive_ids,
+                              const uint32_t** d_sorted_indices) {
+    return guarded([&] {
+        require_handle(bvh);
+        const bool v = bvh->valid;
+        if (d_input_bounds) *d_input_bounds = v ? bvh->dev.boxes : nullptr;
+        if (d_primitive_ids) *d_primitive_ids = v ? bvh->dev.ids : nullptr;
+        if (d_sorted_indices) *d_sorted_indices = v ? bvh->dev.vals[0] : nullptr;
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_trace_closest(lux_hip_bvh* bvh, const lux_triangle* tris, int32_t tri_count, const lux_ray* rays,
                           int32_t n, lux_hit* hits) {
     return guarded([&] {
@@ -1245,6 +1271,8 @@ int lux_obj_mesh_arrays(const lux_obj_mesh* m, lux_vec3* vertices, lux_vec3* nor
 int lux_hip_scene_destroy(lux_hip_scene* sc) {
     return guarded([&] {
         if (!sc) return int(LUX_OK);
+        if (sc->dev.server_jobs > 0)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has unfinished server requests: destroy it after they complete");
         if (g_initialized) (void)hipDeviceSynchronize();
         sc->renderer.reset();
         pt::scene_release(sc->dev);
@@ -1261,6 +1289,9 @@ int lux_hip_scene_update_vertices(lux_hip_scene* sc, const lux_vec3* vertices, i
         if (!sc || !vertices) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
         if (sc->dev.instanced)
             throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: move instances with lux_hip_scene_set_instances");
+        if (sc->dev.server_jobs > 0)
+            throw Error(LUX_ERR_INVALID_ARGUMENT,
+                        "scene has unfinished server requests: submit the update with lux_hip_server_submit");
         pt::scene_update_vertices(sc->dev, vertices, vertex_count, g_stream);
         scene_check(sc);
         return int(LUX_OK);

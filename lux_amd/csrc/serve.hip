@@ -20,6 +20,7 @@ Server::Server(const ServeConfig& cfg, const Tuning& tuning, hipStream_t stream)
 Server::~Server() {
     (void)hipStreamSynchronize(stream_);
     for (Job* j : active_) --j->scene->server_jobs;  // admitted, never retired
+    for (auto& j : pending_) --j->scene->server_jobs;  // submitted, never admitted
     for (auto& f : inflight_) (void)hipEventDestroy(f.ev);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     for (auto& kv : jobs_)
@@ -51,6 +52,7 @@ uint64_t Server::submit(SceneDev* scene, const lux_render_settings& rs, const lu
     }
     job->film_bytes = sizeof(float) * 3 * size_t(rs.width) * size_t(rs.height);
     uint64_t id = job->id;
+    ++scene->server_jobs;  // from submission to completion: the scene must stay as it is
     pending_.push_back(std::move(job));
     ++stats_.submitted;
     return id;
@@ -128,7 +130,6 @@ void Server::admit() {
         j->rq.tag = j->id;
         j->admitted = now();
         ++scene_users_[j->scene];
-        ++j->scene->server_jobs;
         active_.push_back(j);
         issue_.push_back(j);
         jobs_[j->id] = std::move(*it);

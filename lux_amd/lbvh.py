@@ -185,6 +185,22 @@ class GpuBvh:
                  ptr(t["node_bounds"]))
         return t
 
+    def input(self):
+        """(boxes (n, 6), primitive ids (n,)) the tree holds, in input order."""
+        _, n, _ = self.info()
+        boxes = np.zeros((n, 6), np.float32)
+        ids = np.zeros(n, np.int32)
+        if n:
+            call("lux_hip_bvh_download_input", self.handle, ptr(boxes), ptr(ids))
+        return boxes, ids
+
+    def device_arrays(self):
+        """Device addresses of the input boxes, ids and sorted input indices
+        (GpuBvh::overlap_*_device_ptr); None for an invalid tree."""
+        b, i, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        call("lux_hip_bvh_device_arrays", self.handle, ctypes.byref(b), ctypes.byref(i), ctypes.byref(s))
+        return b.value, i.value, s.value
+
 
 def build_gpu_lbvh(boxes, ids=None):
     b = as_boxes(boxes)

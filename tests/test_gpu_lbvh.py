@@ -318,3 +318,30 @@ def test_full_size_microbench_inputs(gpu):
     s = gpu.query_aabb_overlaps_self(bvh, max_pairs=1 << 22)
     sp, sc = o.overlap_tree(o.boxes, np.arange(len(boxes), dtype=np.int32), True)
     assert s.found_count == sc and np.array_equal(key(s.pairs), key(sp))
+
+
+@pytest.mark.parametrize("case", ["soup2000", "sparse_ids777", "tiny1"])
+def test_input_and_device_arrays(gpu, case):
+    """The build input in input order (host copy and the device arrays behind
+    GpuBvh::overlap_*_device_ptr), before and after a refit."""
+    print_case("input arrays", case)
+    g = load_golden(f"lbvh_{case}")
+    n = len(g["boxes"])
+    bvh = gpu.build_gpu_lbvh(g["boxes"], g["ids"])
+    boxes, ids = bvh.input()
+    assert np.array_equal(bits(boxes), bits(g["boxes"])) and np.array_equal(ids, g["ids"])
+    db, di, ds = bvh.device_arrays()
+    assert db and di and ds
+    L = abi.load()
+    got_b = np.zeros((n, 6), np.float32)
+    got_i = np.zeros(n, np.int32)
+    got_s = np.zeros(n, np.uint32)
+    for dst, src in ((got_b, db), (got_i, di), (got_s, ds)):
+        assert L.lux_hip_memcpy_dtoh(dst.ctypes.data, src, dst.nbytes) == 0
+    assert np.array_equal(bits(got_b), bits(g["boxes"])) and np.array_equal(got_i, g["ids"])
+    assert np.array_equal(got_s, bvh.topology()["sorted_indices"])
+    moved = W.triangle_boxes(W.jitter(g["tris"], sigma=2e-3, seed=3))
+    assert gpu.refit_gpu_lbvh(bvh, moved, g["ids"])
+    assert np.array_equal(bits(bvh.input()[0]), bits(moved))
+    empty = gpu.GpuBvh()
+    assert empty.device_arrays() == (None, None, None)

@@ -139,3 +139,29 @@ def test_budget_admission(rd):
             assert np.array_equal(bits(a), bits(f))
     finally:
         rd.set_tuning(**saved)
+
+
+def test_scene_guarded_while_requests_are_unfinished(rd):
+    """A scene with a submitted, unfinished request refuses a direct vertex
+    update and destruction; both work once the request completed."""
+    print("case: server scene guards", flush=True)
+    from lux_amd.abi import LuxError
+
+    dicts, _ = make_trace()
+    sc = rd.Scene(dicts[1])
+    sv = rd.Server()
+    rid = sv.submit(sc, dict(width=32, height=32, spp=64, depth=6, seed=1))
+    with pytest.raises(LuxError):
+        sc.update_vertices(jitter(dicts[1]["vertices"], 5))
+    with pytest.raises(LuxError):
+        rd.call("lux_hip_scene_destroy", sc.handle)
+    done = []
+    for _ in range(100000):
+        done += sv.poll(block=True)
+        if rid in done:
+            break
+    assert rid in done
+    sv.result(rid)
+    sc.update_vertices(jitter(dicts[1]["vertices"], 5))
+    sv.close()
+    sc.close()
