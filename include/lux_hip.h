@@ -116,6 +116,13 @@ int lux_hip_bvh_build(lux_hip_bvh* bvh, const lux_aabb* boxes,
 int lux_hip_bvh_build_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes,
                              const int32_t* d_primitive_ids, int32_t n,
                              lux_stream stream);
+/* build_gpu_lbvh over the primitives collect_scene_mesh_bvh_build_primitives
+ * makes from a triangle list (src/scene/scene_prepare.h:420-439), with the
+ * host loop and copy of src/bvh/gpu_bvh.cu:543-563 moved onto the device: box
+ * i = triangle_bounds(d_triangles[i]) (src/geometry/triangle.h:22-28),
+ * primitive id i. Same tree as lux_hip_bvh_build with those boxes and ids. */
+int lux_hip_bvh_build_triangles_device(lux_hip_bvh* bvh, const lux_triangle* d_triangles, int32_t n,
+                                       lux_stream stream);
 
 /* refit_gpu_lbvh (src/bvh/gpu_bvh.cuh:51, src/bvh/gpu_bvh.cu:597-613): same
  * topology, new boxes. Returns LUX_ERR_MISMATCH (the reference returns false)

@@ -84,6 +84,7 @@ _SIGNATURES = {
     "lux_hip_bvh_download_nodes": [_P, _P, c_int32],
     "lux_hip_bvh_download_topology": [_P, _P, _P, _P, _P, _P, _P, _P],
     "lux_hip_bvh_download_input": [_P, _P, _P],
+    "lux_hip_bvh_build_triangles_device": [_P, _P, c_int32, _P],
     "lux_hip_bvh_device_arrays": [_P, _P, _P, _P],
     "lux_hip_trace_closest": [_P, _P, c_int32, _P, c_int32, _P],
     "lux_hip_trace_closest_device": [_P, _P, c_int32, _P, c_int32, _P, _P],

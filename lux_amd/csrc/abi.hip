@@ -364,6 +364,28 @@ int lux_hip_bvh_build_device(lux_hip_bvh* bvh, const lux_aabb* d_boxes, const in
     });
 }
 
+int lux_hip_bvh_build_triangles_device(lux_hip_bvh* bvh, const lux_triangle* d_tris, int32_t n, lux_stream stream) {
+    return guarded([&] {
+        require_init();
+        require_handle(bvh);
+        if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative primitive count");
+        if (n > 0 && !d_tris) throw Error(LUX_ERR_INVALID_ARGUMENT, "null triangle array");
+        if (n >= (1 << 29)) throw Error(LUX_ERR_INVALID_ARGUMENT, "primitive count exceeds 2^29");
+        bvh->valid = false;
+        bvh->host_ids_known = false;
+        if (n == 0) {
+            bvh->dev.n = 0;
+            return int(LUX_OK);
+        }
+        hipStream_t s = pick(stream);
+        if (bvh->last_stream && bvh->last_stream != s) hip_check(hipStreamSynchronize(bvh->last_stream), "synchronize");
+        bvh->last_stream = s;
+        lbvh_build_triangles(bvh->dev, d_tris, nullptr, nullptr, 0, n, s);
+        bvh->valid = true;
+        return int(LUX_OK);
+    });
+}
+
 int lux_hip_bvh_refit(lux_hip_bvh* bvh, const lux_aabb* boxes, const int32_t* ids, int32_t n) {
     return guarded([&] {
         require_init();

@@ -1064,6 +1064,48 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
 }
 }  // namespace
 
+namespace {
+// Build input from triangles (collect_scene_mesh_bvh_build_primitives,
+// src/scene/scene_prepare.h:420-439): box i = triangle_bounds of triangle i
+// (an empty box expanded by v0, v1, v2), id i. The triangle comes from the
+// soup `tris`, or from the indexed mesh (verts, idx; indices already checked
+// by the caller, a bad one leaves an empty box and sets kErrPrimRange).
+__global__ __launch_bounds__(kBlock) void k_tri_input(const lux_triangle* __restrict__ tris,
+                                                      const lux_vec3* __restrict__ verts,
+                                                      const uint32_t* __restrict__ idx, int nv, int n,
+                                                      lux_aabb* __restrict__ boxes, int32_t* __restrict__ ids,
+                                                      DevInfo* info) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        lux_triangle t;
+        if (tris) {
+            t = tris[i];
+        } else {
+            const uint32_t a = idx[3 * size_t(i)], b = idx[3 * size_t(i) + 1], c = idx[3 * size_t(i) + 2];
+            if (a >= uint32_t(nv) || b >= uint32_t(nv) || c >= uint32_t(nv)) {
+                set_error(info, kErrPrimRange);
+                boxes[i] = box_empty();
+                ids[i] = i;
+                continue;
+            }
+            t.v0 = verts[a];
+            t.v1 = verts[b];
+            t.v2 = verts[c];
+        }
+        boxes[i] = tri_box(t);
+        ids[i] = i;
+    }
+}
+}  // namespace
+
+void lbvh_build_triangles(BvhDev& d, const lux_triangle* tris, const lux_vec3* verts, const uint32_t* idx, int nv,
+                          int n, hipStream_t s) {
+    bvh_allocate(d, n);
+    if (n <= 0) return;
+    LUX_LAUNCH("lbvh_tri_input", k_tri_input, std::max(1, std::min(4096, div_up(n, kBlock))), kBlock, 0, s, tris,
+               verts, idx, nv, n, d.boxes, d.ids, d.info);
+    lbvh_build(d, d.boxes, d.ids, n, s);
+}
+
 void lbvh_refit_boxes(BvhDev& d, const lux_aabb* in_boxes, hipStream_t s) {
     if (d.n <= 0) return;
     refit_launch<1>(d, in_boxes, nullptr, s);

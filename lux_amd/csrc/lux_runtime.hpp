@@ -148,6 +148,11 @@ constexpr int kOrderMinRays = 1 << 16;
 // Enqueue a full build of n primitives; in_boxes/in_ids are device arrays
 // (may alias d.boxes/d.ids).
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s);
+// Enqueue a full build whose input is computed on the device from n triangles:
+// the soup `tris`, or (tris null) the indexed mesh verts[nv] / idx[3n]; box i =
+// triangle_bounds of triangle i, id i.
+void lbvh_build_triangles(BvhDev& d, const lux_triangle* tris, const lux_vec3* verts, const uint32_t* idx, int nv,
+                          int n, hipStream_t s);
 
 // Film output (film.hip): write_ppm's pixel bytes (3 per pixel, film order),
 // write_pfm's float body (rows bottom to top), both from a device film of

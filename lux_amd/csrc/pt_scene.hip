@@ -556,21 +556,6 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
         if (desc.material_ids[t] < 0 || desc.material_ids[t] >= nm)
             throw Error(LUX_ERR_INVALID_ARGUMENT, "material id out of range");
     }
-    std::vector<lux_aabb> boxes(nt);
-    std::vector<int32_t> ids(nt);
-    for (int t = 0; t < nt; ++t) {
-        const uint32_t* ix = desc.indices + 3 * size_t(t);
-        lux_aabb b;
-        b.p_min = lux_vec3{INFINITY, INFINITY, INFINITY};
-        b.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
-        for (int k = 0; k < 3; ++k) {
-            const lux_vec3& p = desc.vertices[ix[k]];
-            b.p_min = lux_vec3{fminf(b.p_min.x, p.x), fminf(b.p_min.y, p.y), fminf(b.p_min.z, p.z)};
-            b.p_max = lux_vec3{fmaxf(b.p_max.x, p.x), fmaxf(b.p_max.y, p.y), fmaxf(b.p_max.z, p.z)};
-        }
-        boxes[t] = b;
-        ids[t] = t;
-    }
     std::vector<float> normals;
     if (desc.normals) {
         normals.resize(3 * size_t(nv));
@@ -628,18 +613,12 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
     if (desc.normals)
         hip_check(hipMemcpyAsync(s.normals, normals.data(), 4 * normals.size(), hipMemcpyHostToDevice, stream),
                   "H2D normals");
-    // BLAS over the triangle boxes (ids = triangle index); the soup is gathered
-    // on the device from the indexed mesh
-    lux_aabb* d_boxes = dalloc<lux_aabb>(nt);
-    int32_t* d_ids = dalloc<int32_t>(nt);
-    hip_check(hipMemcpyAsync(d_boxes, boxes.data(), sizeof(lux_aabb) * nt, hipMemcpyHostToDevice, stream), "H2D boxes");
-    hip_check(hipMemcpyAsync(d_ids, ids.data(), 4 * size_t(nt), hipMemcpyHostToDevice, stream), "H2D ids");
-    lbvh_build(s.bvh, d_boxes, d_ids, nt, stream);
+    // BLAS over the triangle boxes (ids = triangle index), both computed on the
+    // device from the indexed mesh; the soup is gathered there too
+    lbvh_build_triangles(s.bvh, nullptr, s.vertices, s.indices, nv, nt, stream);
     LUX_LAUNCH("pt_gather_soup", k_gather_soup, std::min(div_up(nt, 256), 4096), 256, 0, stream, s.vertices,
                s.indices, nt, nv, s.soup, s.bvh.info);
     hip_check(hipStreamSynchronize(stream), "scene upload");
-    (void)hipFree(d_boxes);
-    (void)hipFree(d_ids);
     s.device_bytes = size_t(nt) * (36 + 12 + 4 + 4) + size_t(nv) * (12 + (desc.normals ? 12 : 0)) +
                      sizeof(DevMaterial) * nm + sizeof(DevLight) * std::max<size_t>(s.host_lights.size(), 1) +
                      bvh_bytes(nt);

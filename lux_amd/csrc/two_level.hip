@@ -262,27 +262,9 @@ void two_level_build(TwoLevelDev& s, const lux_triangle* tris, const int32_t* of
     std::vector<DevMesh> meshes(static_cast<size_t>(mesh_count));
     for (int m = 0; m < mesh_count; ++m) {
         const int n = offsets[m + 1] - offsets[m];
-        std::vector<lux_aabb> boxes(static_cast<size_t>(std::max(n, 0)));
-        std::vector<int32_t> ids(static_cast<size_t>(std::max(n, 0)));
-        for (int i = 0; i < n; ++i) {  // triangle_bounds of each triangle (exact min/max)
-            const lux_triangle& t = tris[offsets[m] + i];
-            boxes[i].p_min = lux_vec3{std::fmin(std::fmin(t.v0.x, t.v1.x), t.v2.x),
-                                      std::fmin(std::fmin(t.v0.y, t.v1.y), t.v2.y),
-                                      std::fmin(std::fmin(t.v0.z, t.v1.z), t.v2.z)};
-            boxes[i].p_max = lux_vec3{std::fmax(std::fmax(t.v0.x, t.v1.x), t.v2.x),
-                                      std::fmax(std::fmax(t.v0.y, t.v1.y), t.v2.y),
-                                      std::fmax(std::fmax(t.v0.z, t.v1.z), t.v2.z)};
-            ids[i] = i;
-        }
-        TempBuffer<lux_aabb> d_boxes(size_t(std::max(n, 1)));
-        TempBuffer<int32_t> d_ids(size_t(std::max(n, 1)));
-        if (n > 0) {
-            hip_check(hipMemcpyAsync(d_boxes.p, boxes.data(), sizeof(lux_aabb) * n, hipMemcpyHostToDevice, stream),
-                      "H2D mesh boxes");
-            hip_check(hipMemcpyAsync(d_ids.p, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream),
-                      "H2D mesh ids");
-        }
-        lbvh_build(s.blas[m], d_boxes.p, d_ids.p, n, stream);
+        // BLAS input (triangle_bounds, local ids) computed on the device from
+        // the soup copied above
+        lbvh_build_triangles(s.blas[m], s.soup + offsets[m], nullptr, nullptr, 0, n, stream);
         sync_and_check(s.blas[m], stream, "BLAS build");
         meshes[m].nodes = s.blas[m].pairs;
         meshes[m].slot_pre = s.blas[m].slot_pre;

@@ -113,6 +113,11 @@ class GpuBvh:
         call("lux_hip_bvh_build", self.handle, ptr(b), ptr(i), len(b))
         return self
 
+    def build_triangles_device(self, d_tris, n, stream=None):
+        """Build from a device triangle soup: box i = triangle_bounds(tri i), id i."""
+        call("lux_hip_bvh_build_triangles_device", self.handle, d_tris.ptr, int(n), stream)
+        return self
+
     def build_device(self, d_boxes, d_ids, n, stream=None):
         call("lux_hip_bvh_build_device", self.handle, d_boxes.ptr, d_ids.ptr, int(n), stream)
         return self

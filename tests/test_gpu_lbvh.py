@@ -345,3 +345,19 @@ def test_input_and_device_arrays(gpu, case):
     assert np.array_equal(bits(bvh.input()[0]), bits(moved))
     empty = gpu.GpuBvh()
     assert empty.device_arrays() == (None, None, None)
+
+
+@pytest.mark.parametrize("case", ["soup2000", "clustered1500", "flat300", "tiny1", "rounding4"])
+def test_build_from_device_triangles(gpu, case):
+    """lux_hip_bvh_build_triangles_device (box i = triangle_bounds of triangle i,
+    id i, computed on the device) builds the tree of lux_hip_bvh_build with the
+    host-computed boxes, node for node."""
+    print_case("build from device triangles", case)
+    g = load_golden(f"lbvh_{case}")
+    tris = np.ascontiguousarray(g["tris"], np.float32)
+    ref = gpu.build_gpu_lbvh(g["boxes"])  # the reference's triangle_bounds, ids 0..n-1
+    d_tris = gpu.DeviceArray.from_numpy(tris)
+    bvh = gpu.GpuBvh().build_triangles_device(d_tris, len(tris))
+    assert np.array_equal(bvh.nodes().view(np.uint32), ref.nodes().view(np.uint32))
+    assert np.array_equal(bits(bvh.input()[0]), bits(g["boxes"]))
+    d_tris.free()
