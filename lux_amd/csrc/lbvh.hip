@@ -2,32 +2,35 @@
 //
 // Build pipeline (one stream, no host round trip):
 //   root_partials  grid-stride min/max of the input boxes (+ id range) into
-//                  per-block partials; block 0 clears the histogram and the
-//                  frontier counter
+//                  per-block partials; block 0 clears the histogram
 //   morton_hist    every block folds the partials into the root box, then
 //                  30-bit Morton codes, identity payload and the four 8-bit
 //                  digit histograms
 //   sort_pass x4   stable LSD radix pass with decoupled look-back (one read and
 //                  one write of key+value per pass)
-//   karras         radix tree over the sorted codes (merged 64-bit keys); also
-//                  lists the "frontier": nodes whose leaf range lies inside one
-//                  leaf block while their parent's range does not
-//   top_layout     right-first preorder positions of the nodes above the
-//                  frontier (one thread per frontier node)
+//   karras         radix tree over the sorted codes (merged 64-bit keys)
+//   top_layout     right-first preorder positions of the nodes whose leaf
+//                  range spans more than one leaf block, and the right steps
+//                  from the "frontier" (local nodes under them) to the root
 //   local          one workgroup per block of kLocalLeaves sorted leaves: places
-//                  the leaves and the nodes inside the block, then unions boxes
-//                  bottom-up inside the block with LDS arrival counters
-//   top_climb      one workgroup unions the boxes of the nodes above the frontier
-// Refit runs only local (bounds phase) + top_climb.
+//                  the leaves and the nodes inside the block, unions boxes
+//                  bottom-up inside the block with LDS arrival counters, and
+//                  leaves prefix / suffix unions of the block's frontier nodes
+//   top_boxes      one wave per node above the frontier: its box from two of
+//                  those unions and the whole-block boxes between them
+// Refit runs only local (bounds phase) + top_boxes.
 //
-// Only the few thousand nodes above the frontier need cross-workgroup hand-off,
-// and a single workgroup handles them, so no agent-scope release/acquire (an L2
-// write-back + invalidate on this chip) is issued per node.
+// No box crosses workgroups through a counter: every hand-off between
+// workgroups is a kernel boundary, so no agent-scope release/acquire (an L2
+// write-back + invalidate on this chip) is issued per node. Every union keeps
+// its operands in leaf order (box_join is the host's order-sensitive fminf /
+// fmaxf), so each box has the bits of the host's recursive union.
 //
 // Algorithms follow the reference: Morton quantisation
 // src/bvh/cpu_lbvh_builder.h:37-61 (device twin src/bvh/gpu_bvh.cu:135-149),
 // stable key order src/bvh/cpu_lbvh_builder.h:212-216, Karras 2012 range/split
-// with merged keys :63-131, internal box = union of children :139-159.
+// with merged keys :63-131, internal box = union of children :139-159
+// (union_aabb(left, right), src/core/aabb.cuh:113-115).
 #include "lux_runtime.hpp"
 
 namespace lux {
@@ -40,7 +43,6 @@ constexpr int kRadix = 256;
 constexpr int kPasses = 4;                      // 4 x 8 bits cover the 30-bit codes
 constexpr int kReduceBlocksMax = 1024;
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr int kTopThreads = 1024;
 
 // ---------------------------------------------------------------------------
 // block-wide helpers (256 threads = 4 waves of 64)
@@ -120,7 +122,6 @@ __global__ __launch_bounds__(kBlock) void k_root_partials(const lux_aabb* __rest
                                                           int32_t* __restrict__ ids, int copy, int n,
                                                           BoxIds* __restrict__ partials,
                                                           uint32_t* __restrict__ hist,
-                                                          uint32_t* __restrict__ frontier_count,
                                                           DevInfo* __restrict__ info) {
     BoxIds acc = boxids_empty();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -138,7 +139,6 @@ __global__ __launch_bounds__(kBlock) void k_root_partials(const lux_aabb* __rest
     if (blockIdx.x == 0) {
         for (int w = threadIdx.x; w < kPasses * kRadix; w += kBlock) hist[w] = 0;
         if (threadIdx.x == 0) {
-            *frontier_count = 0;
             info->max_depth = 0;
         }
     }
@@ -416,9 +416,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint32_t* __restrict
 // Leaves are grouped into blocks of kLocalLeaves consecutive sorted leaves. A
 // node is LOCAL when its leaf range lies inside one block; a local node whose
 // parent is not local is a FRONTIER node. Every internal node is either local
-// (handled entirely by the workgroup of its block) or above the frontier.
+// (handled entirely by the workgroup of its block) or above the frontier (its
+// range [first, last] has first / kLocalLeaves != last / kLocalLeaves).
 constexpr int kLocalLeaves = 1024;
-constexpr int kLocalThreads = 256;
 
 __device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ codes, int a, int b) {
     uint64_t ka = (uint64_t(codes[a]) << 32) | uint64_t(uint32_t(a));
@@ -431,10 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ right, uint32_t* parent,
                                                    uint32_t* leaf_parent,
                                                    uint32_t* __restrict__ first_out,
-                                                   uint32_t* __restrict__ last_out,
-                                                   uint32_t* __restrict__ arrive,
-                                                   uint32_t* __restrict__ frontier,
-                                                   uint32_t* frontier_count) {
+                                                   uint32_t* __restrict__ last_out) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
     int first, last;
@@ -490,13 +487,6 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
     right[i] = rchild;
     first_out[i] = uint32_t(first);
     last_out[i] = uint32_t(last);
-    arrive[i] = 0;
-    if (first / kLocalLeaves != last / kLocalLeaves) {
-        if (first / kLocalLeaves == split / kLocalLeaves)
-            frontier[__hip_atomic_fetch_add(frontier_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = lchild;
-        if ((split + 1) / kLocalLeaves == last / kLocalLeaves)
-            frontier[__hip_atomic_fetch_add(frontier_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = rchild;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -513,68 +503,54 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
 // (its last leaf) and -- if it is above the frontier -- of exactly one frontier
 // node. Subtree size = 2 (last - first + 1) - 1 and escape = pre + size.
 //
-// top_layout: one thread per frontier node F walks to the root (only nodes above
-// the frontier), stores R(F) (| depth << 16) and places F's right chain.
-__device__ __forceinline__ void put_node(lux_bvh_node* nodes, int pos, int32_t prim, int total, int size) {
-    int32_t* rec = reinterpret_cast<int32_t*>(nodes) + size_t(pos) * 8;
-    int esc = pos + size;
-    rec[6] = prim;
-    rec[7] = esc < total ? esc : -1;
-}
-
-__global__ __launch_bounds__(kBlock) void k_top_layout(int n, const uint32_t* __restrict__ frontier,
-                                                       const uint32_t* __restrict__ frontier_count,
-                                                       const uint32_t* __restrict__ parent,
-                                                       const uint32_t* __restrict__ leaf_parent,
+// top_layout: one thread per internal node above the frontier walks to the
+// root counting right steps R and levels, stores pre(X) = 2 (n - 1 - last) + R,
+// and hands each local child F (a frontier node) R(F) = R(X) + [F is X's right
+// child] | (depth(X) + 1) << 16, which the local pass adds to the steps inside
+// the block.
+__global__ __launch_bounds__(kBlock) void k_top_layout(int n, const uint32_t* __restrict__ parent,
+                                                       const uint32_t* __restrict__ left,
+                                                       const uint32_t* __restrict__ right,
                                                        const uint32_t* __restrict__ first_of,
                                                        const uint32_t* __restrict__ last_of,
                                                        uint32_t* __restrict__ rtop_int,
                                                        uint32_t* __restrict__ rtop_leaf,
-                                                       lux_bvh_node* __restrict__ nodes,
                                                        int32_t* __restrict__ pre_internal, DevInfo* info) {
-    int f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= int(*frontier_count)) return;
     const int total = 2 * n - 1;
-    uint32_t F = frontier[f];
-    bool is_leaf = F & kLeafBit;
-    uint32_t idx = F & kIndexMask;
-    int last_f = is_leaf ? int(idx) : int(last_of[idx]);
-    uint32_t pinfo = is_leaf ? leaf_parent[idx] : parent[idx];
-    uint32_t all_right = pinfo >> 31;
-    int rights = int(all_right);
-    int chain = 0;
-    uint32_t depth = 1;
-    uint32_t p = pinfo & kIndexMask;
-    for (int guard = 0;; ++guard) {
-        chain += int(all_right);
-        if (p == 0) break;
-        if (guard > n) {
-            set_error(info, kErrClimbLimit);
-            return;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n - 1; i += gridDim.x * kBlock) {
+        const uint32_t first = first_of[i], last = last_of[i];
+        if (first / kLocalLeaves == last / kLocalLeaves) continue;
+        uint32_t rights = 0, depth = 0, p = uint32_t(i);
+        bool bad = false;
+        while (p != 0) {
+            if (depth > 64u) {  // a merged-key tree is at most 64 internal nodes deep
+                bad = true;
+                break;
+            }
+            const uint32_t pp = parent[p];
+            rights += pp >> 31;
+            p = pp & kIndexMask;
+            ++depth;
         }
-        uint32_t pp = parent[p];
-        uint32_t r = pp >> 31;
-        rights += int(r);
-        all_right &= r;
-        p = pp & kIndexMask;
-        ++depth;
-    }
-    uint32_t packed = uint32_t(rights) | (depth << 16);
-    if (is_leaf)
-        rtop_leaf[idx] = packed;
-    else
-        rtop_int[idx] = packed;
-    int pos = 2 * (n - 1 - last_f) + rights;
-    if (pos - chain < 0 || pos >= total) {
-        set_error(info, kErrClimbLimit);
-        return;
-    }
-    p = pinfo & kIndexMask;
-    for (int c = 0; c < chain; ++c) {
-        --pos;
-        put_node(nodes, pos, -1, total, 2 * (last_f - int(first_of[p]) + 1) - 1);
-        pre_internal[p] = pos;
-        if (c + 1 < chain) p = parent[p] & kIndexMask;
+        const int pos = 2 * (n - 1 - int(last)) + int(rights);
+        if (bad || pos < 0 || pos >= total) {
+            set_error(info, kErrClimbLimit);
+            continue;
+        }
+        pre_internal[i] = pos;
+        const uint32_t kids[2] = {left[i], right[i]};
+        for (uint32_t side = 0; side < 2; ++side) {
+            const uint32_t c = kids[side];
+            const uint32_t idx = c & kIndexMask;
+            const bool leaf = c & kLeafBit;
+            const uint32_t cf = leaf ? idx : first_of[idx], cl = leaf ? idx : last_of[idx];
+            if (cf / kLocalLeaves != cl / kLocalLeaves) continue;  // above the frontier too
+            const uint32_t packed = (rights + side) | ((depth + 1) << 16);
+            if (leaf)
+                rtop_leaf[idx] = packed;
+            else
+                rtop_int[idx] = packed;
+        }
     }
 }
 
@@ -584,11 +560,43 @@ __global__ __launch_bounds__(kBlock) void k_top_layout(int n, const uint32_t* __
 // kMode 0: build (layout, then boxes from the handle's input copy)
 // kMode 1: refit from boxes        kMode 2: refit from triangles
 //
+// The block's tree data sits in LDS (parent links of the internal nodes and
+// leaves with index in [lo, hi] -- every local internal node has its index in
+// its own range, so inside the block -- their leaf counts and preorder
+// positions), so the climbs below touch global memory only for the leaf input
+// and the one 32-byte write of each finished node.
+//
 // Bounds: every local internal node gets exactly two arrivals from threads of
-// this workgroup. The arrival that reads 0 from the LDS counter leaves its box in
-// the node's LDS slot and stops; the other unions both slots, writes the node and
-// climbs on. Climbing stops at the frontier; the frontier box is in the node
-// array for top_climb.
+// this workgroup; each leaves its box in its side's LDS slot and bumps the
+// node's 8-bit arrival counter. The arrival that reads 0 stops; the other
+// unions left-then-right, writes the node and climbs on. A climb stops at a
+// frontier node (its parent lies above the block); the frontier nodes are then
+// put in leaf order and folded into
+//   edge_boxes[2 first]     union from a left-child frontier node's first leaf
+//                           to the end of the block
+//   edge_boxes[2 last + 1]  union from the start of the block to a right-child
+//                           frontier node's last leaf
+//   block_boxes[block]      union of the whole block
+// which is all top_boxes needs. A block holds at most 128 frontier nodes: each
+// is a child of a node whose range crosses one of the block's two edges, and
+// the nodes over an edge lie on one root path, at most 64 long (a child's
+// common key prefix is longer than its parent's, and keys have 64 bits).
+constexpr int kMaxFrontier = 128;
+constexpr int kLocalThreads = 512;
+
+struct FrontierRec {
+    lux_aabb box;
+    uint32_t span;  // first leaf - lo | (last leaf - lo) << 10 | is_left_child << 20
+};
+
+__device__ __forceinline__ void store_node(lux_bvh_node* nodes, int pos, const lux_aabb& b, int32_t prim,
+                                           int size, int total) {
+    const int esc = pos + size;
+    float4* p = reinterpret_cast<float4*>(nodes + pos);
+    p[0] = make_float4(b.p_min.x, b.p_min.y, b.p_min.z, b.p_max.x);
+    p[1] = make_float4(b.p_max.y, b.p_max.z, __int_as_float(prim), __int_as_float(esc < total ? esc : -1));
+}
+
 template <int kMode>
 __global__ __launch_bounds__(kLocalThreads) void k_local(
     int n, const uint32_t* __restrict__ sorted_input, const int32_t* __restrict__ ids,
@@ -596,29 +604,62 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(
     lux_aabb* own_boxes, int write_back, const uint32_t* __restrict__ parent,
     const uint32_t* __restrict__ leaf_parent, const uint32_t* __restrict__ first_of,
     const uint32_t* __restrict__ last_of, const uint32_t* __restrict__ rtop_int,
-    const uint32_t* __restrict__ rtop_leaf, lux_bvh_node* nodes, int32_t* pre_internal,
-    int32_t* pre_leaf, DevInfo* info) {
-    __shared__ uint32_t s_arrive[kLocalLeaves];
+    const uint32_t* __restrict__ rtop_leaf, lux_bvh_node* __restrict__ nodes, int32_t* __restrict__ pre_internal,
+    int32_t* __restrict__ pre_leaf, lux_aabb* __restrict__ edge_boxes, lux_aabb* __restrict__ block_boxes,
+    DevInfo* info) {
     __shared__ lux_aabb s_slot[kLocalLeaves][2];
+    __shared__ uint32_t s_arrive[kLocalLeaves / 4];
+    __shared__ uint32_t s_par[kLocalLeaves];
+    __shared__ uint32_t s_lpar[kLocalLeaves];
+    __shared__ uint16_t s_leaves[kLocalLeaves];  // leaf count of a local internal node, 0 if not local
+    __shared__ int32_t s_pre_int[kLocalLeaves];
+    __shared__ int32_t s_pre_leaf[kLocalLeaves];
+    __shared__ FrontierRec s_fr[kMaxFrontier];
+    __shared__ uint8_t s_order[kMaxFrontier];
+    __shared__ uint32_t s_fcount;
     const int lo = blockIdx.x * kLocalLeaves;
     const int hi = min(lo + kLocalLeaves, n) - 1;
+    const int cnt = hi - lo + 1;
     const int total = 2 * n - 1;
-    for (int t = threadIdx.x; t < kLocalLeaves; t += kLocalThreads) s_arrive[t] = 0;
-    auto is_local = [&](uint32_t p) { return int(first_of[p]) >= lo && int(last_of[p]) <= hi; };
+    const int tid = threadIdx.x;
+
+    for (int k = tid; k < kLocalLeaves / 4; k += kLocalThreads) s_arrive[k] = 0;
+    if (tid == 0) s_fcount = 0;
+    for (int k = tid; k < cnt; k += kLocalThreads) {
+        const int i = lo + k;
+        s_lpar[k] = leaf_parent[i];
+        uint32_t leaves = 0, par = 0;
+        if (i <= n - 2) {
+            const uint32_t f = first_of[i], l = last_of[i];
+            if (int(f) >= lo && int(l) <= hi) leaves = l - f + 1;
+            par = parent[i];
+        }
+        s_leaves[k] = uint16_t(leaves);
+        s_par[k] = par;
+        if (kMode != 0) {
+            s_pre_leaf[k] = pre_leaf[i];
+            s_pre_int[k] = leaves ? pre_internal[i] : -1;
+        }
+    }
+    __syncthreads();
+    auto local_leaves = [&](uint32_t p) -> uint32_t {
+        const uint32_t r = p - uint32_t(lo);
+        return r < uint32_t(cnt) ? uint32_t(s_leaves[r]) : 0u;
+    };
 
     if (kMode == 0) {
         uint32_t max_depth = 0;
-        for (int j = lo + int(threadIdx.x); j <= hi; j += kLocalThreads) {
-            uint32_t lp = leaf_parent[j];
+        for (int k = tid; k < cnt; k += kLocalThreads) {
+            const int j = lo + k;
+            const uint32_t lp = s_lpar[k];
             uint32_t side = lp >> 31;
             uint32_t p = lp & kIndexMask;
             int rights = 0, chain = 0;
             uint32_t all_right = 1, steps = 0;
-            bool reached_root = false, f_is_leaf = true;
+            bool reached_root = false, f_is_leaf = true, bad = false;
             uint32_t f = uint32_t(j);
-            bool bad = false;
             for (int guard = 0;; ++guard) {
-                if (!is_local(p)) break;
+                if (!local_leaves(p)) break;
                 if (guard > kLocalLeaves) {
                     bad = true;
                     break;
@@ -633,121 +674,202 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(
                     reached_root = true;
                     break;
                 }
-                uint32_t pp = parent[p];
+                const uint32_t pp = s_par[p - uint32_t(lo)];
                 side = pp >> 31;
                 p = pp & kIndexMask;
             }
-            uint32_t top = reached_root ? 0u : (f_is_leaf ? rtop_leaf[f] : rtop_int[f]);
+            const uint32_t top = reached_root ? 0u : (f_is_leaf ? rtop_leaf[f] : rtop_int[f]);
             int pos = 2 * (n - 1 - j) + rights + int(top & 0xffffu);
             max_depth = max(max_depth, steps + (top >> 16));
             if (bad || pos - chain < 0 || pos >= total) {  // no early return: a barrier follows
                 set_error(info, kErrClimbLimit);
+                s_pre_leaf[k] = -1;
                 continue;
             }
-            put_node(nodes, pos, ids[sorted_input[j]], total, 1);
+            s_pre_leaf[k] = pos;
             pre_leaf[j] = pos;
             uint32_t c_node = lp & kIndexMask;
             for (int c = 0; c < chain; ++c) {
                 --pos;
-                put_node(nodes, pos, -1, total, 2 * (j - int(first_of[c_node]) + 1) - 1);
+                s_pre_int[c_node - uint32_t(lo)] = pos;
                 pre_internal[c_node] = pos;
-                if (c + 1 < chain) c_node = parent[c_node] & kIndexMask;
+                if (c + 1 < chain) c_node = s_par[c_node - uint32_t(lo)] & kIndexMask;
             }
         }
-        __hip_atomic_fetch_max(&info->max_depth, max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) max_depth = max(max_depth, uint32_t(__shfl_xor(int(max_depth), off, 64)));
+        if (lane_id() == 0)
+            __hip_atomic_fetch_max(&info->max_depth, max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();  // every preorder position of the block is in LDS
     }
-    __syncthreads();  // layout of this block's nodes and the LDS clear are visible
 
-    for (int j = lo + int(threadIdx.x); j <= hi; j += kLocalThreads) {
-        uint32_t in = sorted_input[j];
+    for (int k = tid; k < cnt; k += kLocalThreads) {
+        const int j = lo + k;
+        const uint32_t in = sorted_input[j];
         lux_aabb box = kMode == 2 ? tri_box(src_tris[in]) : src_boxes[in];
         if (write_back) own_boxes[in] = box;
-        int pre = pre_leaf[j];
+        int pre = s_pre_leaf[k];
         if (uint32_t(pre) >= uint32_t(total)) {
             set_error(info, kErrClimbLimit);
-            return;
+            continue;
         }
-        nodes[pre].bounds = box;
-        uint32_t lp = leaf_parent[j];
+        store_node(nodes, pre, box, ids[in], 1, total);
+        uint32_t lp = s_lpar[k];
+        uint32_t first_rel = uint32_t(k), last_rel = uint32_t(k);
         for (int guard = 0;; ++guard) {
-            uint32_t side = lp >> 31;
-            uint32_t p = lp & kIndexMask;
-            if (!is_local(p)) break;
-            uint32_t slot = p - uint32_t(lo);
-            if (slot >= uint32_t(kLocalLeaves) || guard > kLocalLeaves) {
-                set_error(info, kErrClimbLimit);
-                return;
+            const uint32_t side = lp >> 31;
+            const uint32_t p = lp & kIndexMask;
+            const uint32_t leaves = local_leaves(p);
+            if (!leaves) {  // frontier node: its parent lies above the block
+                const uint32_t e = __hip_atomic_fetch_add(&s_fcount, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (e < uint32_t(kMaxFrontier)) {
+                    s_fr[e].box = box;
+                    s_fr[e].span = first_rel | (last_rel << 10) | ((side ^ 1u) << 20);
+                } else {
+                    set_error(info, kErrClimbLimit);
+                }
+                break;
             }
+            if (guard > kLocalLeaves) {
+                set_error(info, kErrClimbLimit);
+                break;
+            }
+            const uint32_t slot = p - uint32_t(lo);
             s_slot[slot][side] = box;
-            uint32_t old = __hip_atomic_fetch_add(&s_arrive[slot], 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            box = box_join(s_slot[slot][0], s_slot[slot][1]);
-            pre = pre_internal[p];
+            // The hand-off is LDS only: order the slot write before the counter
+            // and the counter before the sibling's slot read, without waiting
+            // for this thread's node stores to reach memory.
+            const uint32_t sh = 8u * (slot & 3u);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            const uint32_t old = __hip_atomic_fetch_add(&s_arrive[slot >> 2], 1u << sh, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            if (((old >> sh) & 0xffu) == 0) break;
+            box = side ? box_join(s_slot[slot][0], box) : box_join(box, s_slot[slot][1]);
+            if (side)
+                first_rel = last_rel + 1 - leaves;
+            else
+                last_rel = first_rel + leaves - 1;
+            pre = s_pre_int[slot];
             if (uint32_t(pre) >= uint32_t(total)) {
                 set_error(info, kErrClimbLimit);
-                return;
+                break;
             }
-            nodes[pre].bounds = box;
+            store_node(nodes, pre, box, -1, int(2 * leaves - 1), total);
             if (p == 0) {
                 info->root = box;
                 break;
             }
-            lp = parent[p];
+            lp = s_par[slot];
+        }
+    }
+    __syncthreads();  // every frontier record is in LDS
+
+    const uint32_t fc = min(s_fcount, uint32_t(kMaxFrontier));
+    for (uint32_t e = tid; e < fc; e += kLocalThreads) {
+        const uint32_t fe = s_fr[e].span & 1023u;
+        uint32_t rank = 0;
+        for (uint32_t o = 0; o < fc; ++o) rank += (s_fr[o].span & 1023u) < fe ? 1u : 0u;
+        s_order[rank] = uint8_t(e);
+    }
+    __syncthreads();
+    if (tid == 0) {  // prefix unions, left to right
+        lux_aabb acc = box_empty();
+        for (uint32_t r = 0; r < fc; ++r) {
+            const FrontierRec& f = s_fr[s_order[r]];
+            acc = r ? box_join(acc, f.box) : f.box;
+            if (!(f.span >> 20)) edge_boxes[2 * size_t(lo + int((f.span >> 10) & 1023u)) + 1] = acc;
+        }
+        if (fc) block_boxes[blockIdx.x] = acc;
+    } else if (tid == 64) {  // suffix unions, right to left (second wave)
+        lux_aabb acc = box_empty();
+        for (uint32_t r = fc; r-- > 0;) {
+            const FrontierRec& f = s_fr[s_order[r]];
+            acc = r + 1 < fc ? box_join(f.box, acc) : f.box;
+            if (f.span >> 20) edge_boxes[2 * size_t(lo + int(f.span & 1023u))] = acc;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// 7. top climb: one workgroup unions the boxes above the frontier
+// 7. top boxes: one wave per node above the frontier
 // ---------------------------------------------------------------------------
-// Same two-arrival protocol on global counters (never reset: each pass adds two
-// per node, so the parity restarts every pass); workgroup scope suffices because
-// every arrival comes from this one workgroup.
-__global__ __launch_bounds__(kTopThreads) void k_top_climb(int n, const uint32_t* __restrict__ frontier,
-                                                           const uint32_t* __restrict__ frontier_count,
-                                                           const uint32_t* __restrict__ parent,
-                                                           const uint32_t* __restrict__ leaf_parent,
-                                                           const int32_t* __restrict__ pre_internal,
-                                                           const int32_t* __restrict__ pre_leaf,
-                                                           uint32_t* arrive, lux_aabb* slots,
-                                                           lux_bvh_node* nodes, DevInfo* info) {
-    const uint32_t count = *frontier_count;
+// A node X above the frontier covers leaves [first, last] that start inside
+// leaf block bf and end inside block bl > bf: its box is the union from
+// `first` to the end of bf (the suffix stored at its leftmost frontier node, a
+// left child), the whole blocks between, and the union from the start of bl to
+// `last` (the prefix at its rightmost frontier node, a right child). The
+// blocks between are folded in contiguous runs, lane by lane, then across the
+// wave with lower lanes on the left, so the host's left-to-right order holds.
+__device__ __forceinline__ lux_aabb shfl_down_box(const lux_aabb& b, int off) {
+    lux_aabb r;
+    r.p_min.x = __shfl_down(b.p_min.x, off, 64);
+    r.p_min.y = __shfl_down(b.p_min.y, off, 64);
+    r.p_min.z = __shfl_down(b.p_min.z, off, 64);
+    r.p_max.x = __shfl_down(b.p_max.x, off, 64);
+    r.p_max.y = __shfl_down(b.p_max.y, off, 64);
+    r.p_max.z = __shfl_down(b.p_max.z, off, 64);
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_top_boxes(int n, const uint32_t* __restrict__ first_of,
+                                                      const uint32_t* __restrict__ last_of,
+                                                      const int32_t* __restrict__ pre_internal,
+                                                      const lux_aabb* __restrict__ edge_boxes,
+                                                      const lux_aabb* __restrict__ block_boxes,
+                                                      lux_bvh_node* __restrict__ nodes, DevInfo* info) {
+    const uint32_t lane = lane_id();
+    const int waves = gridDim.x * (kBlock / 64);
     const int total = 2 * n - 1;
-    for (uint32_t f = threadIdx.x; f < count; f += kTopThreads) {
-        uint32_t F = frontier[f];
-        bool is_leaf = F & kLeafBit;
-        uint32_t idx = F & kIndexMask;
-        int pre = is_leaf ? pre_leaf[idx] : pre_internal[idx];
-        if (uint32_t(pre) >= uint32_t(total)) {
-            set_error(info, kErrClimbLimit);
-            continue;
+    // each wave scans 64 internal nodes at a time and handles the ones above
+    // the frontier (about one in a hundred) one after another, whole-wave
+    for (int base = (blockIdx.x * (kBlock / 64) + int(threadIdx.x >> 6)) * 64; base < n - 1; base += waves * 64) {
+        const int i = base + int(lane);
+        uint32_t my_first = 0, my_last = 0;
+        if (i < n - 1) {
+            my_first = first_of[i];
+            my_last = last_of[i];
         }
-        lux_aabb box = nodes[pre].bounds;
-        uint32_t pinfo = is_leaf ? leaf_parent[idx] : parent[idx];
-        for (int guard = 0;; ++guard) {
-            if (guard > n) {
-                set_error(info, kErrClimbLimit);
-                break;
+        uint64_t todo = __ballot(i < n - 1 && my_first / kLocalLeaves != my_last / kLocalLeaves);
+        while (todo) {
+            const int l = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t x = uint32_t(base + l);
+            const uint32_t first = uint32_t(__shfl(int(my_first), l, 64));
+            const uint32_t last = uint32_t(__shfl(int(my_last), l, 64));
+            const uint32_t bf = first / uint32_t(kLocalLeaves), bl = last / uint32_t(kLocalLeaves);
+            const uint32_t m = bl - bf - 1;  // whole blocks between (bl > bf)
+            const uint32_t per = (m + 63u) / 64u;
+            const uint32_t b0 = bf + 1 + lane * per;
+            const uint32_t b1 = min(b0 + per, bl);
+            lux_aabb acc = box_empty();
+            int have = 0;
+            for (uint32_t b = b0; b < b1; ++b) {
+                const lux_aabb bb = block_boxes[b];
+                acc = have ? box_join(acc, bb) : bb;
+                have = 1;
             }
-            uint32_t side = pinfo >> 31;
-            uint32_t p = pinfo & kIndexMask;
-            slots[2 * size_t(p) + side] = box;
-            uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if ((old & 1u) == 0) break;
-            lux_aabb sib = slots[2 * size_t(p) + (side ^ 1u)];
-            box = side ? box_join(sib, box) : box_join(box, sib);
-            pre = pre_internal[p];
-            if (uint32_t(pre) >= uint32_t(total)) {
-                set_error(info, kErrClimbLimit);
-                break;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const lux_aabb o = shfl_down_box(acc, off);
+                const int o_have = __shfl_down(have, off, 64);
+                if ((lane & uint32_t(2 * off - 1)) == 0 && lane + uint32_t(off) < 64u && o_have) {
+                    acc = have ? box_join(acc, o) : o;
+                    have = 1;
+                }
             }
-            nodes[pre].bounds = box;
-            if (p == 0) {
-                info->root = box;
-                break;
+            if (lane == 0) {
+                const int pre = pre_internal[x];
+                if (uint32_t(pre) >= uint32_t(total)) {
+                    set_error(info, kErrClimbLimit);
+                } else {
+                    lux_aabb box = edge_boxes[2 * size_t(first)];
+                    if (have) box = box_join(box, acc);
+                    box = box_join(box, edge_boxes[2 * size_t(last) + 1]);
+                    store_node(nodes, pre, box, -1, int(2 * (last - first + 1) - 1), total);
+                    if (x == 0) info->root = box;
+                }
             }
-            pinfo = parent[p];
         }
     }
 }
@@ -787,8 +909,8 @@ int sort_tiles(int n) { return max(1, div_up(n, kSortTile)); }
 void bvh_release(BvhDev& d) {
     void* ptrs[] = {d.boxes,     d.ids,         d.keys[0],      d.keys[1],    d.vals[0],    d.vals[1],
                     d.left,      d.right,       d.parent,       d.first,      d.last,       d.leaf_parent,
-                    d.arrive,    d.slots,       d.pre_internal, d.pre_leaf,   d.nodes,      d.hist,
-                    d.status,    d.tickets,     d.partials,     d.info,       d.counter,    d.frontier,
+                    d.edge_boxes, d.block_boxes, d.pre_internal, d.pre_leaf, d.nodes,       d.hist,
+                    d.status,    d.tickets,     d.partials,     d.info,       d.counter,
                     d.rtop_int,  d.rtop_leaf,   d.pairs,        d.pair_pos,   d.slot_pre};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -967,12 +1089,11 @@ void bvh_allocate(BvhDev& d, int n) {
     d.first = dalloc<uint32_t>(cap);
     d.last = dalloc<uint32_t>(cap);
     d.leaf_parent = dalloc<uint32_t>(cap);
-    d.arrive = dalloc<uint32_t>(cap);
-    d.slots = dalloc<lux_aabb>(2 * size_t(cap));
+    d.edge_boxes = dalloc<lux_aabb>(2 * size_t(cap));
+    d.block_boxes = dalloc<lux_aabb>(div_up(cap, kLocalLeaves));
     d.pre_internal = dalloc<int32_t>(cap);
     d.pre_leaf = dalloc<int32_t>(cap);
     d.nodes = dalloc<lux_bvh_node>(2 * size_t(cap));
-    d.frontier = dalloc<uint32_t>(2 * size_t(cap));
     d.rtop_int = dalloc<uint32_t>(cap);
     d.rtop_leaf = dalloc<uint32_t>(cap);
     d.hist = dalloc<uint32_t>(kPasses * kRadix);
@@ -986,24 +1107,31 @@ void bvh_allocate(BvhDev& d, int n) {
     d.pair_pos = dalloc<uint32_t>(2 * size_t(cap));
     d.slot_pre = dalloc<uint32_t>(2 * size_t(cap));
     // One-time clears at allocation: look-back words (epoch 0 is never used),
-    // tickets, arrival counters (even = idle), info, frontier count.
+    // tickets, info, frontier counts.
     hip_check(hipMemset(d.status, 0, sizeof(uint64_t) * d.status_words), "hipMemset status");
     hip_check(hipMemset(d.tickets, 0, sizeof(uint32_t) * 4), "hipMemset tickets");
-    hip_check(hipMemset(d.arrive, 0, sizeof(uint32_t) * cap), "hipMemset arrive");
     hip_check(hipMemset(d.info, 0, sizeof(DevInfo)), "hipMemset info");
     hip_check(hipMemset(d.counter, 0, sizeof(int32_t) * 4), "hipMemset counter");
     d.epoch = 0;
 }
+
+namespace {
+void top_boxes(BvhDev& d, hipStream_t s) {
+    if (d.n <= kLocalLeaves) return;  // one leaf block: the root is local
+    LUX_LAUNCH("lbvh_top_boxes", k_top_boxes, max(1, min(2048, div_up(d.n - 1, kBlock))), kBlock, 0, s, d.n,
+               (const uint32_t*)d.first, (const uint32_t*)d.last, (const int32_t*)d.pre_internal,
+               (const lux_aabb*)d.edge_boxes, (const lux_aabb*)d.block_boxes, d.nodes, d.info);
+}
+}  // namespace
 
 void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int n, hipStream_t s) {
     bvh_allocate(d, n);
     if (n <= 0) return;
     int copy = (in_boxes != d.boxes || in_ids != d.ids) ? 1 : 0;
     int rb = reduce_blocks(n);
-    uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
     BoxIds* partials = reinterpret_cast<BoxIds*>(d.partials);
     LUX_LAUNCH("lbvh_root_partials", k_root_partials, rb, kBlock, 0, s, in_boxes, in_ids, d.boxes, d.ids, copy,
-               n, partials, d.hist, frontier_count, d.info);
+               n, partials, d.hist, d.info);
     if (n == 1) {
         LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, (const lux_aabb*)d.boxes,
                    (const lux_triangle*)nullptr, (const int32_t*)d.ids, d.boxes, 0, d.nodes, d.pre_leaf,
@@ -1023,20 +1151,20 @@ void lbvh_build(BvhDev& d, const lux_aabb* in_boxes, const int32_t* in_ids, int 
                    (const uint32_t*)d.vals[src], d.keys[dst], d.vals[dst], n, 8 * p,
                    (const uint32_t*)(d.hist + p * kRadix), d.status, d.tickets, epoch, tiles, d.info);
     }
+    const int blocks = div_up(n, kLocalLeaves);
     LUX_LAUNCH("lbvh_karras", k_karras, div_up(n - 1, kBlock), kBlock, 0, s, (const uint32_t*)d.keys[0], n,
-               d.left, d.right, d.parent, d.leaf_parent, d.first, d.last, d.arrive, d.frontier, frontier_count);
-    LUX_LAUNCH("lbvh_top_layout", k_top_layout, div_up(2 * n, kBlock), kBlock, 0, s, n,
-               (const uint32_t*)d.frontier, (const uint32_t*)frontier_count, (const uint32_t*)d.parent,
-               (const uint32_t*)d.leaf_parent, (const uint32_t*)d.first, (const uint32_t*)d.last, d.rtop_int,
-               d.rtop_leaf, d.nodes, d.pre_internal, d.info);
-    LUX_LAUNCH("lbvh_local", k_local<0>, div_up(n, kLocalLeaves), kLocalThreads, 0, s, n,
-               (const uint32_t*)d.vals[0], (const int32_t*)d.ids, (const lux_aabb*)d.boxes,
-               (const lux_triangle*)nullptr, d.boxes, 0, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
-               (const uint32_t*)d.first, (const uint32_t*)d.last, (const uint32_t*)d.rtop_int,
-               (const uint32_t*)d.rtop_leaf, d.nodes, d.pre_internal, d.pre_leaf, d.info);
-    LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
-               (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
-               (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+               d.left, d.right, d.parent, d.leaf_parent, d.first, d.last);
+    if (blocks > 1)
+        LUX_LAUNCH("lbvh_top_layout", k_top_layout, max(1, min(2048, div_up(n - 1, kBlock))), kBlock, 0, s, n,
+                   (const uint32_t*)d.parent, (const uint32_t*)d.left, (const uint32_t*)d.right,
+                   (const uint32_t*)d.first, (const uint32_t*)d.last, d.rtop_int, d.rtop_leaf, d.pre_internal,
+                   d.info);
+    LUX_LAUNCH("lbvh_local", k_local<0>, blocks, kLocalThreads, 0, s, n, (const uint32_t*)d.vals[0],
+               (const int32_t*)d.ids, (const lux_aabb*)d.boxes, (const lux_triangle*)nullptr, d.boxes, 0,
+               (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent, (const uint32_t*)d.first,
+               (const uint32_t*)d.last, (const uint32_t*)d.rtop_int, (const uint32_t*)d.rtop_leaf, d.nodes,
+               d.pre_internal, d.pre_leaf, d.edge_boxes, d.block_boxes, d.info);
+    top_boxes(d, s);
     pair_build(d, s);
 }
 
@@ -1045,7 +1173,6 @@ template <int kMode>
 void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris, hipStream_t s) {
     int n = d.n;
     int wb = (kMode == 2 || in_boxes != d.boxes) ? 1 : 0;
-    uint32_t* frontier_count = reinterpret_cast<uint32_t*>(d.counter + 1);
     if (n == 1) {
         LUX_LAUNCH("lbvh_single_leaf", k_single_leaf, 1, 1, 0, s, in_boxes, tris, (const int32_t*)d.ids, d.boxes,
                    wb, d.nodes, d.pre_leaf, d.keys[0], d.vals[0], d.info, 0);
@@ -1056,10 +1183,8 @@ void refit_launch(BvhDev& d, const lux_aabb* in_boxes, const lux_triangle* tris,
                kLocalThreads, 0, s, n, (const uint32_t*)d.vals[0], (const int32_t*)d.ids, in_boxes, tris, d.boxes,
                wb, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent, (const uint32_t*)d.first,
                (const uint32_t*)d.last, (const uint32_t*)d.rtop_int, (const uint32_t*)d.rtop_leaf, d.nodes,
-               d.pre_internal, d.pre_leaf, d.info);
-    LUX_LAUNCH("lbvh_top_climb", k_top_climb, 1, kTopThreads, 0, s, n, (const uint32_t*)d.frontier,
-               (const uint32_t*)frontier_count, (const uint32_t*)d.parent, (const uint32_t*)d.leaf_parent,
-               (const int32_t*)d.pre_internal, (const int32_t*)d.pre_leaf, d.arrive, d.slots, d.nodes, d.info);
+               d.pre_internal, d.pre_leaf, d.edge_boxes, d.block_boxes, d.info);
+    top_boxes(d, s);
     pair_fill(d, s);
 }
 }  // namespace

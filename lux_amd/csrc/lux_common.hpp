@@ -55,21 +55,35 @@ __device__ __forceinline__ lux_aabb box_empty() {
     b.p_max = lux_vec3{-INFINITY, -INFINITY, -INFINITY};
     return b;
 }
+// Box union and expansion in the reference's host order: union_aabb and
+// AABB::expand take fminf/fmaxf component-wise (src/core/aabb.cuh:47-55,
+// :113-115; src/core/vec3.cuh:66-72), and the host libm returns the SECOND
+// operand when the two compare equal (+0 against -0) and the other operand when
+// one is NaN. Written as compare-and-select so that the sign of a zero bound
+// never depends on how the GPU's min/max instruction orders zeros. The
+// selection is associative as long as operands keep their left-to-right
+// order, so any grouping of an ordered fold gives the host's bits.
+__device__ __forceinline__ float host_fminf(float x, float y) { return (x < y || y != y) ? x : y; }
+__device__ __forceinline__ float host_fmaxf(float x, float y) { return (x > y || y != y) ? x : y; }
+
 __device__ __forceinline__ lux_aabb box_join(const lux_aabb& a, const lux_aabb& b) {
     lux_aabb r;
-    r.p_min = lux_vec3{fminf(a.p_min.x, b.p_min.x), fminf(a.p_min.y, b.p_min.y), fminf(a.p_min.z, b.p_min.z)};
-    r.p_max = lux_vec3{fmaxf(a.p_max.x, b.p_max.x), fmaxf(a.p_max.y, b.p_max.y), fmaxf(a.p_max.z, b.p_max.z)};
+    r.p_min = lux_vec3{host_fminf(a.p_min.x, b.p_min.x), host_fminf(a.p_min.y, b.p_min.y),
+                       host_fminf(a.p_min.z, b.p_min.z)};
+    r.p_max = lux_vec3{host_fmaxf(a.p_max.x, b.p_max.x), host_fmaxf(a.p_max.y, b.p_max.y),
+                       host_fmaxf(a.p_max.z, b.p_max.z)};
     return r;
+}
+__device__ __forceinline__ void box_expand(lux_aabb& b, const lux_vec3& v) {
+    b.p_min = lux_vec3{host_fminf(b.p_min.x, v.x), host_fminf(b.p_min.y, v.y), host_fminf(b.p_min.z, v.z)};
+    b.p_max = lux_vec3{host_fmaxf(b.p_max.x, v.x), host_fmaxf(b.p_max.y, v.y), host_fmaxf(b.p_max.z, v.z)};
 }
 __device__ __forceinline__ lux_aabb tri_box(const lux_triangle& t) {
     // triangle_bounds: expand an empty box by v0, v1, v2 in order
     lux_aabb b = box_empty();
-    b.p_min = lux_vec3{fminf(b.p_min.x, t.v0.x), fminf(b.p_min.y, t.v0.y), fminf(b.p_min.z, t.v0.z)};
-    b.p_max = lux_vec3{fmaxf(b.p_max.x, t.v0.x), fmaxf(b.p_max.y, t.v0.y), fmaxf(b.p_max.z, t.v0.z)};
-    b.p_min = lux_vec3{fminf(b.p_min.x, t.v1.x), fminf(b.p_min.y, t.v1.y), fminf(b.p_min.z, t.v1.z)};
-    b.p_max = lux_vec3{fmaxf(b.p_max.x, t.v1.x), fmaxf(b.p_max.y, t.v1.y), fmaxf(b.p_max.z, t.v1.z)};
-    b.p_min = lux_vec3{fminf(b.p_min.x, t.v2.x), fminf(b.p_min.y, t.v2.y), fminf(b.p_min.z, t.v2.z)};
-    b.p_max = lux_vec3{fmaxf(b.p_max.x, t.v2.x), fmaxf(b.p_max.y, t.v2.y), fmaxf(b.p_max.z, t.v2.z)};
+    box_expand(b, t.v0);
+    box_expand(b, t.v1);
+    box_expand(b, t.v2);
     return b;
 }
 

@@ -52,12 +52,14 @@ inline int div_up(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 //   left/right/parent[n-1]        Karras radix tree (bit 31 flags as in the ABI)
 //   first/last[n-1]               leaf range of each internal node
 //   leaf_parent[n]
-//   frontier[<=2n], counter[1]    nodes whose range fits one leaf block but
-//                                 whose parent's does not
 //   rtop_int[n-1], rtop_leaf[n]   right steps (| depth << 16) from a frontier
 //                                 node to the root
-//   arrive[n-1]                   arrival counters above the frontier (parity)
-//   slots[2(n-1)]                 child boxes handed from first to second arriver
+//   edge_boxes[2n]                per leaf block: union from a left-child
+//                                 frontier node's first leaf to the block end
+//                                 at [2 first], from the block start to a
+//                                 right-child frontier node's last leaf at
+//                                 [2 last + 1] (other entries unused)
+//   block_boxes[n / 1024]         union of each leaf block
 //   pre_internal[n-1], pre_leaf[n] node -> position in the stackless array
 //   nodes[2n-1]                   32-byte stackless nodes, right-first preorder
 //   sort scratch: hist[4*256], status[tiles*256] (epoch-tagged look-back words),
@@ -77,11 +79,10 @@ struct BvhDev {
     uint32_t* first = nullptr;
     uint32_t* last = nullptr;
     uint32_t* leaf_parent = nullptr;
-    uint32_t* frontier = nullptr;
     uint32_t* rtop_int = nullptr;
     uint32_t* rtop_leaf = nullptr;
-    uint32_t* arrive = nullptr;
-    lux_aabb* slots = nullptr;
+    lux_aabb* edge_boxes = nullptr;
+    lux_aabb* block_boxes = nullptr;
     int32_t* pre_internal = nullptr;
     int32_t* pre_leaf = nullptr;
     lux_bvh_node* nodes = nullptr;
@@ -98,7 +99,7 @@ struct BvhDev {
     uint32_t* tickets = nullptr;
     uint8_t* partials = nullptr;
     DevInfo* info = nullptr;
-    // counter[0]: overlap pair counter of host-array queries; counter[1]: frontier size
+    // counter[0]: overlap pair counter of host-array queries
     int32_t* counter = nullptr;
     uint32_t epoch = 0;
     int status_words = 0;

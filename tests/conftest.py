@@ -21,7 +21,7 @@ def load_golden(name):
 
 
 LBVH_CASES = ["soup2000", "clustered1500", "identical64", "flat300", "sparse_ids777", "tiny1", "tiny2",
-              "tiny3", "tiny7", "rounding4"]
+              "tiny3", "tiny7", "rounding4", "signed_zero4000"]
 
 
 @pytest.fixture(scope="session")

@@ -179,8 +179,11 @@ def main():
     if only:
         if "rounding4" in only:
             rounding()
+        if "signed_zero4000" in only:
+            signed_zero()
         return
     rounding()
+    signed_zero()
     soup = W.soup(2000, seed=1)
     reference_case("soup2000", soup, mixed_rays(soup, 4096, 100))
 
@@ -212,6 +215,27 @@ def main():
     dense = squeeze(W.soup(2000, seed=31, edge=(0.002, 0.02)), 0.15)
     q = squeeze(W.soup(300, seed=32, edge=(0.002, 0.02)), 0.15)
     overlap_case("dense2000", W.triangle_boxes(dense), W.triangle_boxes(q))
+
+
+def signed_zero_scene(n=4000, seed=41):
+    """Triangles around the origin, 40 % of them flattened onto a coordinate
+    plane with every vertex's coordinate on that axis set to +0 or -0 at random:
+    leaf and internal boxes then meet +0 against -0 in min and max, where the
+    host's fminf/fmaxf keep their second operand. n spans four leaf blocks of
+    1024, so nodes above the block level see such ties too."""
+    rng = np.random.default_rng(seed)
+    t = (W.soup(n, seed=seed, edge=(0.01, 0.05)) - np.float32(0.5)).reshape(n, 3, 3)
+    flat = np.flatnonzero(rng.random(n) < 0.4)
+    axis = rng.integers(0, 3, len(flat))
+    signs = np.where(rng.random((len(flat), 3)) < 0.5, np.float32(-0.0), np.float32(0.0))
+    for k, (i, a) in enumerate(zip(flat, axis)):
+        t[i, :, a] = signs[k]
+    return np.ascontiguousarray(t.reshape(n, 9), np.float32)
+
+
+def signed_zero():
+    tris = signed_zero_scene()
+    reference_case("signed_zero4000", tris, mixed_rays(tris, 2048, 800))
 
 
 def rounding():

@@ -347,7 +347,7 @@ def test_input_and_device_arrays(gpu, case):
     assert empty.device_arrays() == (None, None, None)
 
 
-@pytest.mark.parametrize("case", ["soup2000", "clustered1500", "flat300", "tiny1", "rounding4"])
+@pytest.mark.parametrize("case", ["soup2000", "clustered1500", "flat300", "tiny1", "rounding4", "signed_zero4000"])
 def test_build_from_device_triangles(gpu, case):
     """lux_hip_bvh_build_triangles_device (box i = triangle_bounds of triangle i,
     id i, computed on the device) builds the tree of lux_hip_bvh_build with the
