@@ -8,7 +8,7 @@ uniform in [0,1]^3 with edge lengths ~U(0.001, 0.01), FP32:
      the sphere)
   3. any-hit (occlusion) of the same 2^22 rays
   4. LBVH build of 100 000 query boxes + their cross overlap query against the
-     1M-triangle tree
+     1M-triangle tree, and the self overlap query of the 1M-triangle tree
   5. refit of the 1M tree from vertex-jittered triangles (sigma = 1e-3)
 All inputs are resident in HBM before the timed region. `value` is the rays
 traced per second over the whole step (closest + any, i.e. 2 * 2^22 rays per
@@ -585,6 +585,7 @@ def main():
     d_hits, d_occ = lbvh.DeviceArray(8 * R), lbvh.DeviceArray(R)
     pair_cap = 1 << 22
     d_pairs, d_found = lbvh.DeviceArray(8 * pair_cap), lbvh.DeviceArray(4)
+    d_spairs, d_sfound = lbvh.DeviceArray(8 * pair_cap), lbvh.DeviceArray(4)
     bvh, qbvh = lbvh.GpuBvh(), lbvh.GpuBvh()
 
     def build():
@@ -596,9 +597,16 @@ def main():
     def anyhit():
         call("lux_hip_trace_any_device", bvh.handle, d_tris.ptr, N, d_rays.ptr, R, d_occ.ptr, None)
 
+    def overlap_cross():
+        call("lux_hip_overlap_cross_device", bvh.handle, qbvh.handle, pair_cap, d_pairs.ptr, d_found.ptr, None)
+
+    def overlap_self():
+        call("lux_hip_overlap_self_device", bvh.handle, pair_cap, d_spairs.ptr, d_sfound.ptr, None)
+
     def overlap():
         qbvh.build_device(d_qboxes, d_qids, Q)
-        call("lux_hip_overlap_cross_device", bvh.handle, qbvh.handle, pair_cap, d_pairs.ptr, d_found.ptr, None)
+        overlap_cross()
+        overlap_self()
 
     def refit():
         bvh.refit_triangles_device(d_jit, N)
@@ -630,12 +638,23 @@ def main():
     occ = d_occ.to_numpy(np.uint8, R)
     assert np.array_equal(occ.astype(bool), hits[:, 0] >= 0), "any-hit disagrees with closest-hit"
     found = int(d_found.to_numpy(np.int32, 1)[0])
+    self_found = int(d_sfound.to_numpy(np.int32, 1)[0])
 
     # per-stage wall time (same work, stage by stage, synchronised)
     stage_ms = {}
     for name, f in stages:
         if name == "closest" or name == "any" or name == "overlap":
             build()
+        lbvh.synchronize()
+        s0 = time.perf_counter()
+        for _ in range(args.steps):
+            f()
+        lbvh.synchronize()
+        stage_ms[name] = round((time.perf_counter() - s0) * 1e3 / args.steps, 4)
+    for name, f in (("overlap_cross", overlap_cross), ("overlap_self", overlap_self)):
+        build()
+        qbvh.build_device(d_qboxes, d_qids, Q)
+        f()
         lbvh.synchronize()
         s0 = time.perf_counter()
         for _ in range(args.steps):
@@ -672,8 +691,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"LBVH microbench: {N}-triangle soup (U[0,1]^3, edges U(0.001,0.01)), build + "
-                         f"closest-hit and any-hit of {R} rays + cross overlap of {Q} query boxes + refit "
-                         f"after sigma=1e-3 vertex jitter"),
+                         f"closest-hit and any-hit of {R} rays + cross overlap of {Q} query boxes + self overlap "
+                         f"of the tree + refit after sigma=1e-3 vertex jitter"),
             "triangles": N,
             "rays_per_step": rays_per_step,
             "queries": Q,
@@ -704,15 +723,18 @@ def main():
             "refit_hbm_frac": round(144 * N / (stage_ms["refit"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 5),
             "closest_Mrays_per_s": round(R / stage_ms["closest"] / 1e3, 2),
             "any_Mrays_per_s": round(R / stage_ms["any"] / 1e3, 2),
-            "overlap_Mqueries_per_s": round(Q / stage_ms["overlap"] / 1e3, 2),
-            "overlap_pairs": found,
+            "overlap_cross_Mqueries_per_s": round(Q / stage_ms["overlap_cross"] / 1e3, 2),
+            "overlap_self_Mqueries_per_s": round(N / stage_ms["overlap_self"] / 1e3, 2),
+            "overlap_cross_pairs": found,
+            "overlap_self_pairs": self_found,
             "hit_fraction": round(float((hits[:, 0] >= 0).mean()), 4),
         },
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(inp, min(args.cpu_sample_rays, R))
     _log(f"lbvh: {out['value']} Mrays/s")
-    del d_boxes, d_ids, d_tris, d_jit, d_rays, d_qboxes, d_qids, d_hits, d_occ, d_pairs, d_found, bvh, qbvh
+    del d_boxes, d_ids, d_tris, d_jit, d_rays, d_qboxes, d_qids, d_hits, d_occ, d_pairs, d_found
+    del d_spairs, d_sfound, bvh, qbvh
 
     if args.two_level == "on":
         out["two_level"] = bench_two_level(args)

@@ -261,73 +261,210 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const int4* __restrict__ src,
         dst[i] = src[i];
 }
 
-// Overlap: one query box per lane walks the tree; every loop step is one node
-// visit per live lane, and pairs found in that step are appended with a single
-// wave-aggregated atomic. Queries run in the query tree's Morton order for
-// coherent walks. The counter keeps counting past max_pairs so the caller gets
-// the exact total (src/bvh/bvh_overlap.cu:209-214).
-__global__ __launch_bounds__(kBlock) void k_overlap(const lux_bvh_node* __restrict__ nodes,
-                                                    int node_count,
-                                                    const lux_aabb* __restrict__ qboxes,
-                                                    const int32_t* __restrict__ qids,
-                                                    const uint32_t* __restrict__ qorder, int nq,
-                                                    int self_mode, lux_pair* __restrict__ pairs,
-                                                    int max_pairs, int32_t* counter, DevInfo* info) {
-    int q = blockIdx.x * kBlock + threadIdx.x;
-    bool active = q < nq;
-    lux_aabb qb = box_empty();
-    int qid = -1;
-    if (active) {
-        uint32_t src = qorder[q];
-        qb = qboxes[src];
-        qid = qids[src];
-    }
-    int k = active ? 0 : -1;
-    const uint32_t lane = lane_id();
-    for (;;) {
-        bool live = uint32_t(k) < uint32_t(node_count);
-        if (ballot64(live) == 0) break;
-        bool emit = false;
-        int other = -1;
-        if (live) {
-            float4 a, b;
-            load_node(nodes, k, a, b);
-            int prim = __float_as_int(b.z);
-            int esc = __float_as_int(b.w);
-            int next;
-            if (!box_overlap(a.x, a.y, a.z, a.w, b.x, b.y, qb)) {
-                next = esc;
-            } else if (prim >= 0) {
-                emit = self_mode ? prim > qid : true;
-                other = prim;
-                next = esc;
-            } else {
-                next = k + 1;
-            }
-            if (next >= 0 && next <= k) {
-                set_error(info, kErrWalkOrder);
-                next = -1;
-            }
-            k = next;
+// Overlap, level by level. A workgroup takes kOvQueries query boxes (in the
+// query tree's Morton order) and keeps in LDS the (query, sibling pair) items
+// of one tree level: each step tests both children of every item with one
+// 64-byte pair load (two items per lane in flight), collects the overlapping
+// leaves and lists the entered internal children for the next level. A query's
+// dependent chain is thus the tree depth, and the loads of a level are
+// independent. An item whose child finds the next list full walks that child's
+// subtree at once in preorder. Pairs collect in LDS and go out with one atomic
+// per workgroup at the end (directly, one wave-wide atomic per step, once the
+// LDS buffer is full). Every leaf whose box overlaps the query box is reached
+// (an ancestor's box contains its descendants'), so the pair set is that of
+// the per-query walk of src/bvh/bvh_overlap.cu:25-251, and the counter keeps
+// counting past max_pairs so the caller gets the exact total (:209-214).
+constexpr int kOvThreads = 256;
+constexpr int kOvQueries = 128;
+constexpr int kOvCap = 2048;
+constexpr int kOvPairBuf = 2048;
+
+struct OvShared {
+    lux_aabb qb[kOvQueries];
+    int32_t qid[kOvQueries];
+    uint32_t item[2][kOvCap];  // pair slot (children at slot, slot + 1)
+    uint8_t iq[2][kOvCap];     // query within the workgroup
+    lux_pair out[kOvPairBuf];
+    uint32_t count[2];
+    uint32_t nout;
+    int32_t base;
+};
+
+__device__ __forceinline__ void emit_global(bool emit, int a, int b, lux_pair* __restrict__ pairs, int max_pairs,
+                                            int32_t* counter) {
+    const uint64_t m = ballot64(emit);
+    if (!m) return;
+    const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+    int base = 0;
+    if (lane_id() == leader)
+        base = __hip_atomic_fetch_add(counter, int(__popcll(m)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base = __shfl(base, int(leader), 64);
+    if (emit) {
+        const int idx = base + int(__popcll(m & lanemask_lt()));
+        if (idx < max_pairs) {
+            lux_pair pr;
+            pr.a = a;
+            pr.b = b;
+            pairs[idx] = pr;
         }
-        uint64_t m = ballot64(emit);
-        if (m) {
-            uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
-            int base = 0;
-            if (lane == leader)
-                base = __hip_atomic_fetch_add(counter, int(__popcll(m)), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-            base = __shfl(base, int(leader), 64);
-            if (emit) {
-                int idx = base + int(__popcll(m & lanemask_lt()));
-                if (idx < max_pairs) {
-                    lux_pair pr;
-                    pr.a = self_mode ? qid : other;
-                    pr.b = self_mode ? other : qid;
-                    pairs[idx] = pr;
+    }
+}
+
+// One pair found by this lane (emit) goes to the workgroup's LDS buffer, or
+// out directly once the buffer is full.
+__device__ __forceinline__ void emit_pair(OvShared& sh, bool emit, int a, int b, lux_pair* __restrict__ pairs,
+                                          int max_pairs, int32_t* counter) {
+    bool spill = false;
+    if (emit) {
+        const uint32_t at = __hip_atomic_fetch_add(&sh.nout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (at < uint32_t(kOvPairBuf)) {
+            lux_pair pr;
+            pr.a = a;
+            pr.b = b;
+            sh.out[at] = pr;
+        } else {
+            spill = true;
+        }
+    }
+    if (ballot64(spill)) emit_global(spill, a, b, pairs, max_pairs, counter);
+}
+
+__device__ __forceinline__ bool node_overlap(const float4& a, const float4& b, const lux_aabb& q) {
+    return box_overlap(a.x, a.y, a.z, a.w, b.x, b.y, q);
+}
+
+// Preorder walk of the subtree at `slot` (an internal node already known to
+// overlap), ending at the node's escape.
+__device__ void overlap_subtree(OvShared& sh, const lux_bvh_node* __restrict__ nodes, uint32_t nc, uint32_t slot,
+                                uint32_t stop, const lux_aabb& qb, int qid, int self_mode,
+                                lux_pair* __restrict__ pairs, int max_pairs, int32_t* counter, DevInfo* info) {
+    uint32_t k = slot;
+    for (uint32_t guard = 0; guard <= nc; ++guard) {
+        float4 a, b;
+        load_node32(nodes, k, a, b);
+        const int prim = __float_as_int(b.z);
+        const bool hit = node_overlap(a, b, qb);
+        emit_pair(sh, hit && prim >= 0 && (!self_mode || prim > qid), self_mode ? qid : prim,
+                  self_mode ? prim : qid, pairs, max_pairs, counter);
+        k = (hit && prim < 0) ? uint32_t(~prim) : __float_as_uint(b.w);
+        if (k == stop || k >= nc) return;
+    }
+    set_error(info, kErrWalkOrder);
+}
+
+// Both children of one item: leaves to the pair buffer, entered internal
+// children to the next level's list.
+__device__ __forceinline__ void overlap_item(OvShared& sh, int nxt, bool valid, uint32_t c, int qi,
+                                             const float4& a0, const float4& b0, const float4& a1, const float4& b1,
+                                             const lux_bvh_node* __restrict__ nodes, uint32_t nc, int self_mode,
+                                             lux_pair* __restrict__ pairs, int max_pairs, int32_t* counter,
+                                             DevInfo* info) {
+    const lux_aabb qb = sh.qb[qi];
+    const int qid = sh.qid[qi];
+    const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
+    const bool h0 = valid && node_overlap(a0, b0, qb), h1 = valid && node_overlap(a1, b1, qb);
+    emit_pair(sh, h0 && p0 >= 0 && (!self_mode || p0 > qid), self_mode ? qid : p0, self_mode ? p0 : qid, pairs,
+              max_pairs, counter);
+    emit_pair(sh, h1 && p1 >= 0 && (!self_mode || p1 > qid), self_mode ? qid : p1, self_mode ? p1 : qid, pairs,
+              max_pairs, counter);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int p = j ? p1 : p0;
+        if (!(j ? h1 : h0) || p >= 0) continue;
+        const uint32_t child = uint32_t(~p);
+        if (child + 1 >= nc) {
+            set_error(info, kErrWalkOrder);
+            continue;
+        }
+        const uint32_t at = __hip_atomic_fetch_add(&sh.count[nxt], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (at < uint32_t(kOvCap)) {
+            sh.item[nxt][at] = child;
+            sh.iq[nxt][at] = uint8_t(qi);
+        } else {
+            overlap_subtree(sh, nodes, nc, c + uint32_t(j), __float_as_uint(j ? b1.w : b0.w), qb, qid, self_mode,
+                            pairs, max_pairs, counter, info);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kOvThreads) void k_overlap(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                                        const lux_aabb* __restrict__ qboxes,
+                                                        const int32_t* __restrict__ qids,
+                                                        const uint32_t* __restrict__ qorder, int nq, int self_mode,
+                                                        lux_pair* __restrict__ pairs, int max_pairs, int32_t* counter,
+                                                        DevInfo* info) {
+    __shared__ OvShared sh;
+    const uint32_t nc = uint32_t(node_count);
+    const int q0 = blockIdx.x * kOvQueries;
+    const int nql = min(kOvQueries, nq - q0);
+    const int tid = threadIdx.x;
+    if (tid < 2) sh.count[tid] = 0;
+    if (tid == 0) sh.nout = 0;
+    __syncthreads();
+
+    // the root: slot 0, alone
+    {
+        float4 a, b;
+        load_node32(nodes, 0, a, b);
+        const int prim = __float_as_int(b.z);
+        bool emit = false;
+        int qid = -1;
+        if (tid < nql) {
+            const uint32_t src = qorder[q0 + tid];
+            const lux_aabb qb = qboxes[src];
+            qid = qids[src];
+            sh.qb[tid] = qb;
+            sh.qid[tid] = qid;
+            if (node_overlap(a, b, qb)) {
+                if (prim >= 0) {
+                    emit = !self_mode || prim > qid;
+                } else if (uint32_t(~prim) + 1 < nc) {
+                    const uint32_t at = __hip_atomic_fetch_add(&sh.count[0], 1u, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    sh.item[0][at] = uint32_t(~prim);  // at < kOvQueries <= kOvCap
+                    sh.iq[0][at] = uint8_t(tid);
+                } else {
+                    set_error(info, kErrWalkOrder);
                 }
             }
         }
+        emit_pair(sh, emit, self_mode ? qid : prim, self_mode ? prim : qid, pairs, max_pairs, counter);
+    }
+    __syncthreads();
+
+    for (int cur = 0;; cur ^= 1) {
+        const uint32_t count = min(sh.count[cur], uint32_t(kOvCap));
+        if (count == 0) break;
+        __syncthreads();  // every thread has read the count
+        const int nxt = cur ^ 1;
+        if (tid == 0) sh.count[nxt] = 0;
+        __syncthreads();
+        for (uint32_t it = uint32_t(tid); it < count; it += 2 * kOvThreads) {
+            const uint32_t it2 = it + kOvThreads;
+            const bool two = it2 < count;
+            const uint32_t c0 = sh.item[cur][it], c1 = two ? sh.item[cur][it2] : c0;
+            const int q0i = sh.iq[cur][it], q1i = two ? sh.iq[cur][it2] : q0i;
+            float4 a00, b00, a01, b01, a10, b10, a11, b11;
+            load_node32(nodes, c0, a00, b00);
+            load_node32(nodes, c0 + 1, a01, b01);
+            load_node32(nodes, c1, a10, b10);
+            load_node32(nodes, c1 + 1, a11, b11);
+            overlap_item(sh, nxt, true, c0, q0i, a00, b00, a01, b01, nodes, nc, self_mode, pairs, max_pairs, counter,
+                         info);
+            overlap_item(sh, nxt, two, c1, q1i, a10, b10, a11, b11, nodes, nc, self_mode, pairs, max_pairs, counter,
+                         info);
+        }
+        __syncthreads();  // the next list is complete
+    }
+
+    // flush the buffered pairs: one atomic for the workgroup
+    const uint32_t nout = min(sh.nout, uint32_t(kOvPairBuf));
+    if (tid == 0)
+        sh.base = nout ? __hip_atomic_fetch_add(counter, int(nout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    __syncthreads();
+    for (uint32_t i = uint32_t(tid); i < nout; i += kOvThreads) {
+        const int64_t idx = int64_t(sh.base) + int64_t(i);
+        if (idx < max_pairs) pairs[idx] = sh.out[i];
     }
 }
 
@@ -391,10 +528,11 @@ void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, in
                    lux_pair* pairs, int32_t* counter, hipStream_t s) {
     LUX_LAUNCH("overlap_zero", k_zero_counter, 1, 1, 0, s, counter);
     if (tree.n <= 0 || queries.n <= 0 || max_pairs <= 0) return;
-    LUX_LAUNCH(self_mode ? "overlap_self" : "overlap_cross", k_overlap, div_up(queries.n, kBlock), kBlock,
-               0, s, (const lux_bvh_node*)tree.nodes, 2 * tree.n - 1, (const lux_aabb*)queries.boxes,
-               (const int32_t*)queries.ids, (const uint32_t*)queries.vals[0], queries.n,
-               self_mode ? 1 : 0, pairs, max_pairs, counter, tree.info);
+    const int slots = tree.n > 1 ? 2 * tree.n : 1;  // sibling-pair array
+    LUX_LAUNCH(self_mode ? "overlap_self" : "overlap_cross", k_overlap, div_up(queries.n, kOvQueries), kOvThreads,
+               0, s, (const lux_bvh_node*)tree.pairs, slots, (const lux_aabb*)queries.boxes,
+               (const int32_t*)queries.ids, (const uint32_t*)queries.vals[0], queries.n, self_mode ? 1 : 0, pairs,
+               max_pairs, counter, tree.info);
 }
 
 }  // namespace lux
