@@ -16,6 +16,7 @@ struct lux_hip_two_level {
     lux::TwoLevelDev dev;
     bool built = false;
     hipStream_t last_stream = nullptr;
+    lux::RaySortDev order;  // ray-order scratch of the query calls
 };
 
 struct lux_hip_bvh {
@@ -796,6 +797,7 @@ int lux_hip_two_level_destroy(lux_hip_two_level* s) {
         if (!s) return int(LUX_OK);
         if (g_initialized) (void)hipStreamSynchronize(s->last_stream ? s->last_stream : g_stream);
         two_level_release(s->dev);
+        raysort_release(s->order);
         delete s;
         return int(LUX_OK);
     });
@@ -845,7 +847,8 @@ int lux_hip_two_level_intersect(lux_hip_two_level* s, const lux_ray* rays, const
             hip_check(hipMemcpyAsync(di.p, ignore, sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream),
                       "H2D ignore");
         s->last_stream = g_stream;
-        two_level_closest(two_level_view(s->dev), dr.p, ignore ? di.p : nullptr, n, dh.p, g_stream);
+        two_level_closest(two_level_view(s->dev), two_level_order(s->order, s->dev, dr.p, n, g_stream), dr.p,
+                          ignore ? di.p : nullptr, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(hits, dh.p, sizeof(lux_scene_hit) * n, hipMemcpyDeviceToHost, g_stream),
                   "D2H hits");
         sync_two_level(s);
@@ -861,7 +864,9 @@ int lux_hip_two_level_intersect_device(lux_hip_two_level* s, const lux_ray* d_ra
         if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
         if (n > 0 && (!d_rays || !d_hits)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/hit arrays");
         if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
-        two_level_closest(two_level_view(s->dev), d_rays, d_ignore, n, d_hits, use_stream(s, stream));
+        const hipStream_t st = use_stream(s, stream);
+        two_level_closest(two_level_view(s->dev), two_level_order(s->order, s->dev, d_rays, n, st), d_rays, d_ignore,
+                          n, d_hits, st);
         return int(LUX_OK);
     });
 }
@@ -884,7 +889,8 @@ int lux_hip_two_level_occluded(lux_hip_two_level* s, const lux_ray* rays, const 
             hip_check(hipMemcpyAsync(di.p, ignore, sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream),
                       "H2D ignore");
         s->last_stream = g_stream;
-        two_level_occluded(two_level_view(s->dev), dr.p, ignore ? di.p : nullptr, n, dh.p, g_stream);
+        two_level_occluded(two_level_view(s->dev), two_level_order(s->order, s->dev, dr.p, n, g_stream), dr.p,
+                           ignore ? di.p : nullptr, n, dh.p, g_stream);
         hip_check(hipMemcpyAsync(occluded, dh.p, size_t(n), hipMemcpyDeviceToHost, g_stream), "D2H occluded");
         sync_two_level(s);
         return int(LUX_OK);
@@ -899,7 +905,9 @@ int lux_hip_two_level_occluded_device(lux_hip_two_level* s, const lux_ray* d_ray
         if (n < 0) throw Error(LUX_ERR_INVALID_ARGUMENT, "negative ray count");
         if (n > 0 && (!d_rays || !d_occluded)) throw Error(LUX_ERR_INVALID_ARGUMENT, "null ray/output arrays");
         if (!s->built) throw Error(LUX_ERR_INVALID_ARGUMENT, "scene has not been built");
-        two_level_occluded(two_level_view(s->dev), d_rays, d_ignore, n, d_occluded, use_stream(s, stream));
+        const hipStream_t st = use_stream(s, stream);
+        two_level_occluded(two_level_view(s->dev), two_level_order(s->order, s->dev, d_rays, n, st), d_rays,
+                           d_ignore, n, d_occluded, st);
         return int(LUX_OK);
     });
 }

@@ -26,12 +26,20 @@ __device__ __forceinline__ lux_scene_hit scene_hit(const TwoLevelHit& h) {
 
 // Closest hit near child first at both levels; a ray the walk cannot settle
 // (stack overflow or the check of ordered.hpp) takes the preorder walk.
-__global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, const lux_ray* __restrict__ rays,
+// `order`: the rays in ray order (a permutation of [0, n)), or null for input
+// order; a hit lands at its ray's own index.
+__global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, const uint32_t* __restrict__ order,
+                                                              const lux_ray* __restrict__ rays,
                                                               const int32_t* __restrict__ ignore, int n,
                                                               lux_scene_hit* __restrict__ hits) {
-    const int i = int(xcd_block()) * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const int j = int(xcd_block()) * kBlock + threadIdx.x;
+    if (j >= n) return;
     __shared__ uint32_t stack[kOrderedStack * kBlock];
+    const int i = order ? int(order[j]) : j;
+    if (uint32_t(i) >= uint32_t(n)) {  // only if the sort itself reported an error
+        set_error(s.info, kErrSpinLimit);
+        return;
+    }
     const lux_ray r = rays[i];
     const int ign = ignore ? ignore[i] : -1;
     TwoLevelHit h;
@@ -41,11 +49,17 @@ __global__ __launch_bounds__(kBlock) void k_two_level_closest(TwoLevelView s, co
     hits[i] = scene_hit(h);
 }
 
-__global__ __launch_bounds__(kBlock) void k_two_level_occluded(TwoLevelView s, const lux_ray* __restrict__ rays,
+__global__ __launch_bounds__(kBlock) void k_two_level_occluded(TwoLevelView s, const uint32_t* __restrict__ order,
+                                                               const lux_ray* __restrict__ rays,
                                                                const int32_t* __restrict__ ignore, int n,
                                                                uint8_t* __restrict__ occluded) {
-    const int i = int(xcd_block()) * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const int j = int(xcd_block()) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const int i = order ? int(order[j]) : j;
+    if (uint32_t(i) >= uint32_t(n)) {
+        set_error(s.info, kErrSpinLimit);
+        return;
+    }
     const lux_ray r = rays[i];
     occluded[i] =
         two_level_walk<true>(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ignore ? ignore[i] : -1).prim >= 0
@@ -305,18 +319,23 @@ TwoLevelView two_level_view(const TwoLevelDev& s) {
     return v;
 }
 
-void two_level_closest(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
-                       lux_scene_hit* hits, hipStream_t stream) {
-    if (n <= 0) return;
-    LUX_LAUNCH("two_level_closest", k_two_level_closest, div_up(n, kBlock), kBlock, 0, stream, v, rays, ignore, n,
-               hits);
+const uint32_t* two_level_order(RaySortDev& r, const TwoLevelDev& s, const lux_ray* rays, int n, hipStream_t stream) {
+    if (n < kOrderMinRays || s.tlas.n < 1) return nullptr;
+    return ray_order(r, s.tlas, rays, n, stream);  // origins quantised in the TLAS root box
 }
 
-void two_level_occluded(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
-                        uint8_t* occluded, hipStream_t stream) {
+void two_level_closest(const TwoLevelView& v, const uint32_t* order, const lux_ray* rays, const int32_t* ignore,
+                       int n, lux_scene_hit* hits, hipStream_t stream) {
     if (n <= 0) return;
-    LUX_LAUNCH("two_level_occluded", k_two_level_occluded, div_up(n, kBlock), kBlock, 0, stream, v, rays, ignore, n,
-               occluded);
+    LUX_LAUNCH("two_level_closest", k_two_level_closest, div_up(n, kBlock), kBlock, 0, stream, v, order, rays,
+               ignore, n, hits);
+}
+
+void two_level_occluded(const TwoLevelView& v, const uint32_t* order, const lux_ray* rays, const int32_t* ignore,
+                        int n, uint8_t* occluded, hipStream_t stream) {
+    if (n <= 0) return;
+    LUX_LAUNCH("two_level_occluded", k_two_level_occluded, div_up(n, kBlock), kBlock, 0, stream, v, order, rays,
+               ignore, n, occluded);
 }
 
 }  // namespace lux

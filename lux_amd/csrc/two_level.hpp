@@ -413,10 +413,12 @@ void two_level_release(TwoLevelDev& s);
 TwoLevelView two_level_view(const TwoLevelDev& s);
 
 // Closest hit (or occlusion) of n device rays; ignore (nullable) holds one scene
-// ordinal per ray that is skipped (-1: none).
-void two_level_closest(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
-                       lux_scene_hit* hits, hipStream_t stream);
-void two_level_occluded(const TwoLevelView& v, const lux_ray* rays, const int32_t* ignore, int n,
-                        uint8_t* occluded, hipStream_t stream);
+// ordinal per ray that is skipped (-1: none). order (nullable): the rays in
+// ray order, from two_level_order; results land at each ray's own index.
+const uint32_t* two_level_order(RaySortDev& r, const TwoLevelDev& s, const lux_ray* rays, int n, hipStream_t stream);
+void two_level_closest(const TwoLevelView& v, const uint32_t* order, const lux_ray* rays, const int32_t* ignore,
+                       int n, lux_scene_hit* hits, hipStream_t stream);
+void two_level_occluded(const TwoLevelView& v, const uint32_t* order, const lux_ray* rays, const int32_t* ignore,
+                        int n, uint8_t* occluded, hipStream_t stream);
 
 }  // namespace lux

@@ -147,8 +147,25 @@ def test_instanced_microbench_inputs_match_oracle(gpu):
     assert np.array_equal(h["primitive_id"][sample], r["prim"])
     assert np.array_equal(bits(h["t"][sample]), bits(r["t"]))
     assert np.array_equal(h["instance_id"][sample], r["instance"])
-    assert np.array_equal(s.occluded(inp["rays"]).astype(bool), h["primitive_id"] >= 0)
+    occ = s.occluded(inp["rays"])
+    assert np.array_equal(occ.astype(bool), h["primitive_id"] >= 0)
     assert (h["primitive_id"] >= 0).mean() > 0.2
+    # 2^18 rays are traced in ray order (octant, then origin); chunks below the
+    # ordering threshold run in input order and must give the same bits, also
+    # with a per-ray ignored primitive
+    rays = inp["rays"]
+    ign = np.where(h["primitive_id"] >= 0, h["primitive_id"], -1).astype(np.int32)
+    h_ign, occ_ign = s.intersect(rays, ignore=ign), s.occluded(rays, ignore=ign)
+    step = 60_000
+    parts = [(s.intersect(rays[k:k + step]), s.intersect(rays[k:k + step], ignore=ign[k:k + step]),
+              s.occluded(rays[k:k + step], ignore=ign[k:k + step])) for k in range(0, len(rays), step)]
+    for f in ("primitive_id", "instance_id", "local_primitive_id", "t", "u", "v"):
+        a = np.concatenate([p[0][f] for p in parts])
+        b = np.concatenate([p[1][f] for p in parts])
+        assert np.array_equal(a.view(np.uint32), np.ascontiguousarray(h[f]).view(np.uint32)), f
+        assert np.array_equal(b.view(np.uint32), np.ascontiguousarray(h_ign[f]).view(np.uint32)), f
+    assert np.array_equal(np.concatenate([p[2] for p in parts]), occ_ign)
+    assert not np.array_equal(h_ign["primitive_id"], h["primitive_id"])
     s.close()
 
 
