@@ -20,10 +20,15 @@ import time
 from . import PACKAGE_DIR
 
 TUNING_FILE = PACKAGE_DIR / "tuning.json"
+# The first tuning on gfx950/256CU chose the largest capacity and the smallest
+# grid of {2^22..2^25} x {1024, 2048, 4096} x {128, 256, 512} (block 128 was
+# last everywhere), so the grid now reaches past both edges of that choice. A
+# capacity above the calibration mix's ~38 M paths runs it in one batch, so
+# 2^26 is the last capacity that can differ.
 CANDIDATES = {
-    "capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
-    "grid": [1024, 2048, 4096],
-    "block": [128, 256, 512],
+    "capacity": [1 << 23, 1 << 24, 1 << 25, 1 << 26],
+    "grid": [512, 768, 1024, 2048],
+    "block": [256, 512],
 }
 _applied = False
 
