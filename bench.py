@@ -511,7 +511,9 @@ def bench_c5(args, c4_out, scs, jobs, dicts):
     """Served stream: Poisson arrivals at a fraction of the measured batched
     capacity, scenes drawn from the C4 mix, 30% of requests re-render a scene
     after per-vertex N(0, 1e-3^2) jitter (refit, no rebuild), 16-spp slices,
-    200 GB HBM budget."""
+    200 GB HBM budget with the scenes resident; then the same stream on cold
+    scenes (the server uploads and builds them, and drops idle ones) under a
+    budget that holds a quarter of the stream's scene bytes."""
     from lux_amd import render
 
     capacity = c4_out["batched"]["renders_per_s"]
@@ -534,6 +536,9 @@ def bench_c5(args, c4_out, scs, jobs, dicts):
         else:
             verts.append(None)
         seen.add(k)
+    # the stream's first-time vertices: the warm run below moves the C4 scenes
+    used = sorted(set(int(k) for k in pick))
+    cold = {k: render.Scene(dicts[k], deferred=True) for k in used}
     sv = render.Server(hbm_budget=200 << 30, slice_spp=16)
     req_scenes = [scs[int(k)] for k in pick]
     req_jobs = [jobs[int(k)] for k in pick]
@@ -542,6 +547,23 @@ def bench_c5(args, c4_out, scs, jobs, dicts):
     sv.close()
     lat_ms = lat * 1e3
     paths = sum(j["width"] * j["height"] * j["spp"] for j in req_jobs)
+    # the same stream on cold scenes (created deferred: the server uploads and
+    # builds each on admission) under a budget that holds a quarter of them
+    scene_bytes = sum(cold[k].info()[2] for k in used)
+    budget = int(st["peak_bytes"] - 0.75 * scene_bytes)
+    sv = render.Server(hbm_budget=budget, slice_spp=16)
+    _, clat, cwall = sv.run([cold[int(k)] for k in pick], req_jobs, arrivals, vertices=verts, want_films=False)
+    cst = sv.stats()
+    sv.close()
+    for sc in cold.values():
+        sc.close()
+    clat_ms = clat * 1e3
+    cold_out = {"hbm_budget_GB": round(budget / 1e9, 3), "scene_GB": round(scene_bytes / 1e9, 3),
+                "renders_per_s": round(n / cwall, 4), "wall_s": round(cwall, 3),
+                "p50_latency_ms": round(float(np.percentile(clat_ms, 50)), 2),
+                "p99_latency_ms": round(float(np.percentile(clat_ms, 99)), 2),
+                "uploads": int(cst["uploads"]), "evictions": int(cst["evictions"]), "refits": int(cst["refits"]),
+                "prepare_s": round(cst["prepare_seconds"], 3), "peak_hbm_GB": round(cst["peak_bytes"] / 1e9, 3)}
     out = {"workload": (f"{n} requests, Poisson arrivals at {rate:.3f}/s ({args.c5_load:.0%} of the measured "
                         f"batched capacity {capacity:.3f} renders/s), scenes from the C4 mix, {rerender} re-renders "
                         f"after sigma=1e-3 vertex jitter (refit), 16-spp slices, 200 GB HBM budget"),
@@ -551,8 +573,11 @@ def bench_c5(args, c4_out, scs, jobs, dicts):
            "mean_latency_ms": round(float(lat_ms.mean()), 2),
            "refits": int(st["refits"]), "batches": int(st["batches"]), "paths": int(paths),
            "peak_hbm_GB": round(st["peak_bytes"] / 1e9, 3),
-           "Mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2)}
-    _log(f"c5: {wall:.2f} s, p50 {out['p50_latency_ms']} ms p99 {out['p99_latency_ms']} ms")
+           "Mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
+           "cold_scenes_tight_budget": cold_out}
+    _log(f"c5: {wall:.2f} s, p50 {out['p50_latency_ms']} ms p99 {out['p99_latency_ms']} ms; cold/tight: "
+         f"p50 {cold_out['p50_latency_ms']} ms p99 {cold_out['p99_latency_ms']} ms, {cold_out['uploads']} uploads, "
+         f"{cold_out['evictions']} evictions")
     return out
 
 

@@ -315,8 +315,17 @@ typedef struct lux_hip_scene lux_hip_scene;
 
 /* PreparedScene(scene) + prepare_gpu() (src/scene/prepared_scene.cu:19-23,
  * :61-79): light tables, camera, device arrays and the LBVH over the
- * triangles. Synchronous. */
+ * triangles. Synchronous. The description is copied and kept, so a server may
+ * drop the device state of an idle scene and rebuild it later (the same
+ * state: built from these vertices, refit to the latest update). */
 int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out);
+/* PreparedScene(scene) without prepare_gpu(): the description is checked and
+ * copied; the device state is built on first use (a direct render, query or
+ * inspection call, or a server admitting a request for it). */
+int lux_hip_scene_create_deferred(const lux_scene_desc* desc, lux_hip_scene** out);
+/* PreparedScene::has_gpu (src/scene/prepared_scene.cu:81-83): 1 when the
+ * scene's device state exists. */
+int lux_hip_scene_resident(lux_hip_scene* scene, int32_t* resident);
 /* LUX_ERR_INVALID_ARGUMENT while a server request for the scene (submitted
  * and not yet completed) exists. */
 int lux_hip_scene_destroy(lux_hip_scene* scene);
@@ -423,6 +432,9 @@ typedef struct lux_serve_stats {
     uint64_t extension_rays, shadow_rays;
     uint64_t shared_batches;      /* batches that held paths of two or more requests */
     uint64_t max_batch_requests;  /* most requests with paths in one batch */
+    uint64_t uploads;             /* scenes the server made resident (upload + build) */
+    uint64_t evictions;           /* idle scenes the server dropped to fit the budget */
+    double prepare_seconds;       /* worker-thread time of uploads and refits */
 } lux_serve_stats;
 
 typedef struct lux_hip_server lux_hip_server;

@@ -99,6 +99,8 @@ _SIGNATURES = {
     "lux_hip_probe_end": [POINTER(c_double), POINTER(c_int64)],
     "lux_hip_trace_stats_device": [_P, _P, c_int32, _P, c_int32, _P],
     "lux_hip_scene_create": [_P, POINTER(c_void_p)],
+    "lux_hip_scene_create_deferred": [_P, POINTER(c_void_p)],
+    "lux_hip_scene_resident": [_P, POINTER(c_int32)],
     "lux_hip_scene_destroy": [_P],
     "lux_hip_scene_camera": [_P, _P],
     "lux_hip_scene_set_camera": [_P, _P],

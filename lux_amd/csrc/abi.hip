@@ -1092,6 +1092,23 @@ void scene_check(lux_hip_scene* sc) {
     }
 }
 
+// A scene whose device state was dropped (or never built: created deferred)
+// is rebuilt from its kept description before any call that reads it.
+void ensure_resident(lux_hip_scene* sc) {
+    if (sc->dev.resident) return;
+    if (!sc->dev.source) throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "scene has no device state");
+    if (sc->dev.server_jobs > 0)
+        throw lux::Error(LUX_ERR_INVALID_ARGUMENT, "scene is being prepared for server requests: wait for them");
+    lux::pt::SceneDev fresh;
+    try {
+        lux::pt::scene_restore(fresh, *sc->dev.source, g_stream);
+    } catch (...) {
+        lux::pt::scene_release(fresh);
+        throw;
+    }
+    lux::pt::scene_adopt(sc->dev, std::move(fresh));
+}
+
 lux::pt::Renderer& scene_renderer(lux_hip_scene* sc) {
     if (!sc->renderer) sc->renderer = std::make_unique<lux::pt::Renderer>();
     sc->renderer->set_tuning(g_tuning);
@@ -1105,6 +1122,7 @@ lux::pt::Renderer& batch_renderer() {
 }
 
 void render_one(lux_hip_scene* sc, const lux_render_settings* s, float* d_film, hipStream_t stream) {
+    ensure_resident(sc);
     lux::pt::Renderer& r = scene_renderer(sc);
     lux::pt::Request rq;
     rq.sv = lux::pt::scene_view(sc->dev);
@@ -1127,6 +1145,7 @@ void check_batch_args(lux_hip_scene* const* scenes, const lux_render_settings* s
 
 void render_batch(lux_hip_scene* const* scenes, const lux_render_settings* settings, float* const* d_films,
                   int32_t n, int mode, hipStream_t stream) {
+    for (int32_t i = 0; i < n; ++i) ensure_resident(scenes[i]);
     lux::pt::Renderer& r = batch_renderer();
     std::vector<lux::pt::Request> reqs(n);
     std::vector<lux::pt::Request*> ptrs(n);
@@ -1153,13 +1172,39 @@ int lux_hip_scene_create(const lux_scene_desc* desc, lux_hip_scene** out) {
         *out = nullptr;
         auto* sc = new lux_hip_scene();
         try {
-            pt::scene_upload(sc->dev, *desc, g_stream);
+            pt::scene_create(sc->dev, *desc, true, g_stream);
         } catch (...) {
             pt::scene_release(sc->dev);
             delete sc;
             throw;
         }
         *out = sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_create_deferred(const lux_scene_desc* desc, lux_hip_scene** out) {
+    return guarded([&] {
+        require_init();
+        if (!desc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto* sc = new lux_hip_scene();
+        try {
+            pt::scene_create(sc->dev, *desc, false, g_stream);
+        } catch (...) {
+            pt::scene_release(sc->dev);
+            delete sc;
+            throw;
+        }
+        *out = sc;
+        return int(LUX_OK);
+    });
+}
+
+int lux_hip_scene_resident(lux_hip_scene* sc, int32_t* resident) {
+    return guarded([&] {
+        if (!sc || !resident) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        *resident = sc->dev.resident ? 1 : 0;
         return int(LUX_OK);
     });
 }
@@ -1196,6 +1241,7 @@ int lux_hip_scene_triangles(lux_hip_scene* sc, float* corners, float* normals, i
     return guarded([&] {
         require_init();
         if (!sc) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        ensure_resident(sc);
         const pt::SceneDev& d = sc->dev;
         const size_t T = size_t(std::max(d.tri_count, 0));
         hip_check(hipStreamSynchronize(g_stream), "synchronize");
@@ -1322,6 +1368,12 @@ int lux_hip_scene_update_vertices(lux_hip_scene* sc, const lux_vec3* vertices, i
         if (sc->dev.server_jobs > 0)
             throw Error(LUX_ERR_INVALID_ARGUMENT,
                         "scene has unfinished server requests: submit the update with lux_hip_server_submit");
+        if (!sc->dev.resident) {  // kept for the rebuild: the refit happens there
+            if (vertex_count != sc->dev.vertex_count)
+                throw Error(LUX_ERR_MISMATCH, "vertex update must keep the vertex count");
+            sc->dev.source->current.assign(vertices, vertices + vertex_count);
+            return int(LUX_OK);
+        }
         pt::scene_update_vertices(sc->dev, vertices, vertex_count, g_stream);
         scene_check(sc);
         return int(LUX_OK);
@@ -1360,6 +1412,7 @@ int lux_hip_scene_lights(lux_hip_scene* sc, int32_t* count, int32_t* prim, float
                          float* pmf, int32_t* light_of_prim) {
     return guarded([&] {
         if (!sc || !count) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        ensure_resident(sc);
         const auto& L = sc->dev.host_lights;
         *count = int32_t(L.size());
         for (size_t i = 0; i < L.size(); ++i) {
@@ -1380,6 +1433,7 @@ int lux_hip_scene_light_bvh(lux_hip_scene* sc, int32_t* node_count, float* bound
                             int32_t* leaf_of_light, int32_t* root) {
     return guarded([&] {
         if (!sc || !node_count) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
+        ensure_resident(sc);
         const pt::SceneDev& d = sc->dev;
         *node_count = int32_t(d.host_light_nodes.size());
         if (root) *root = d.light_node_count > 0 ? d.light_root : -1;
@@ -1408,7 +1462,9 @@ int lux_hip_scene_bvh(lux_hip_scene* sc, lux_hip_bvh** out) {
     return guarded([&] {
         if (!sc || !out) throw Error(LUX_ERR_INVALID_ARGUMENT, "null argument");
         if (sc->dev.instanced) throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: no single-level LBVH");
-        // The alias shares the scene's device buffers; it must not be destroyed.
+        ensure_resident(sc);
+        // The alias shares the scene's device buffers; it must not be destroyed
+        // (and holds them only while the scene is resident: a server may drop it).
         sc->bvh_alias.dev = sc->dev.bvh;
         sc->bvh_alias.valid = sc->dev.bvh.n > 0;
         sc->bvh_alias.host_ids_known = false;
@@ -1582,6 +1638,9 @@ int lux_hip_server_stats(lux_hip_server* sv, lux_serve_stats* out) {
         out->peak_bytes = st.peak_bytes;
         out->shared_batches = st.shared_batches;
         out->max_batch_requests = st.max_batch_requests;
+        out->uploads = st.uploads;
+        out->evictions = st.evictions;
+        out->prepare_seconds = st.prepare_seconds;
         sv->server->ray_counts(&out->extension_rays, &out->shadow_rays);
         return int(LUX_OK);
     });

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "lux_runtime.hpp"
@@ -32,6 +33,20 @@ namespace pt {
 // the instance's material override. Traversal uses the two-level structure
 // (tl: a BLAS per mesh over its object-space triangles, a TLAS over the
 // instances' world bounds) instead of bvh.
+// Host copy of a one-mesh scene description (lux_scene_desc with its arrays,
+// light images included).
+struct SceneSource {
+    std::vector<lux_vec3> vertices;  // as first uploaded (the build)
+    std::vector<uint32_t> indices;
+    std::vector<lux_vec3> normals;   // empty: none
+    std::vector<int32_t> material_ids;
+    std::vector<lux_material> materials;
+    float camera[8] = {};
+    std::vector<lux_light> lights;   // image_pixels point into `texels`
+    std::vector<std::vector<lux_vec3>> texels;
+    std::vector<lux_vec3> current;   // latest vertex update (empty: none)
+};
+
 struct SceneDev {
     BvhDev bvh;
     lux_triangle* soup = nullptr;
@@ -87,6 +102,12 @@ struct SceneDev {
     // admitted, unfinished server requests: they hold this scene's device
     // pointers, so the instances may not move under them
     int server_jobs = 0;
+    // One-mesh scenes keep the description they were made from (and their
+    // latest vertex update), so the device state can be dropped and rebuilt:
+    // rebuilt = uploaded from the original vertices, then refit to the latest
+    // ones, which is the state the scene had (same topology, same boxes).
+    std::shared_ptr<SceneSource> source;
+    bool resident = false;  // the device arrays exist
 };
 
 // POD view of one scene handed to kernels.
@@ -238,6 +259,17 @@ private:
 };
 
 void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream);
+// A one-mesh scene with its description kept: uploaded now (upload = true) or
+// on first use (scene_restore), after the checks scene_upload makes.
+void scene_create(SceneDev& s, const lux_scene_desc& desc, bool upload, hipStream_t stream);
+// Builds the device state of a kept description into `fresh` (synchronous on
+// `stream`); scene_adopt then moves it into the scene, which keeps its camera,
+// source and server count.
+void scene_restore(SceneDev& fresh, const SceneSource& src, hipStream_t stream);
+void scene_adopt(SceneDev& s, SceneDev&& fresh);
+// Detaches the device state of a scene that has a source (the returned object
+// owns it: scene_release frees it); the scene stays valid, not resident.
+SceneDev scene_detach(SceneDev& s);
 // Replaces vertex positions (same count), regathers the soup on the device,
 // refits the BVH and rebuilds the light table; the topology is kept.
 void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream);

@@ -537,16 +537,17 @@ void prepare_scene_lights(SceneDev& s, const lux_light* lights, int count, hipSt
 }
 
 size_t bvh_bytes(int n) {
-    // boxes, ids, 2x keys/vals, 5 tree links, leaf parents, arrivals, 2 slots,
-    // 2 layout maps, 2 node records, 2 frontier entries, 2 climb words, 2
-    // sibling-pair records with their two slot maps
-    return size_t(std::max(n, 1)) * (24 + 4 + 16 + 20 + 4 + 4 + 48 + 8 + 64 + 8 + 8 + 64 + 16) +
-           (size_t(n) / 2048 + 1) * 2048;
+    // boxes, ids, 2x keys/vals, 5 tree links, leaf parents, 2 edge boxes, 2
+    // layout maps, 2 node records, 2 climb words, 2 sibling-pair records with
+    // their two slot maps; block boxes and sort state
+    return size_t(std::max(n, 1)) * (24 + 4 + 16 + 20 + 4 + 48 + 8 + 64 + 8 + 64 + 16) +
+           (size_t(n) / 1024 + 1) * 24 + (size_t(n) / 2048 + 1) * 2048;
 }
 }  // namespace
 
-void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
-    int nt = desc.triangle_count, nv = desc.vertex_count, nm = desc.material_count;
+namespace {
+void check_desc(const lux_scene_desc& desc) {
+    const int nt = desc.triangle_count, nv = desc.vertex_count, nm = desc.material_count;
     if (nt <= 0 || nv <= 0 || nm <= 0 || !desc.vertices || !desc.indices || !desc.material_ids || !desc.materials)
         throw Error(LUX_ERR_INVALID_ARGUMENT, "scene needs vertices, indices, material ids and materials");
     for (int t = 0; t < nt; ++t) {
@@ -556,6 +557,24 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
         if (desc.material_ids[t] < 0 || desc.material_ids[t] >= nm)
             throw Error(LUX_ERR_INVALID_ARGUMENT, "material id out of range");
     }
+    for (int m = 0; m < nm; ++m)
+        if (desc.materials[m].type < 0 || desc.materials[m].type >= kMaterialTypes)
+            throw Error(LUX_ERR_INVALID_ARGUMENT, "unknown material type");
+    if (desc.light_count < 0 || (desc.light_count > 0 && !desc.lights))
+        throw Error(LUX_ERR_INVALID_ARGUMENT, "bad scene light array");
+}
+
+// HBM of a one-mesh scene: triangle soup, indices, material ids, light of each
+// triangle, vertices (and normals), materials, light records, BVH.
+size_t scene_bytes(int nt, int nv, bool normals, int nm, size_t lights) {
+    return size_t(nt) * (36 + 12 + 4 + 4) + size_t(nv) * (12 + (normals ? 12 : 0)) + sizeof(DevMaterial) * nm +
+           sizeof(DevLight) * std::max<size_t>(lights, 1) + bvh_bytes(nt);
+}
+}  // namespace
+
+void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
+    check_desc(desc);
+    int nt = desc.triangle_count, nv = desc.vertex_count, nm = desc.material_count;
     std::vector<float> normals;
     if (desc.normals) {
         normals.resize(3 * size_t(nv));
@@ -619,9 +638,93 @@ void scene_upload(SceneDev& s, const lux_scene_desc& desc, hipStream_t stream) {
     LUX_LAUNCH("pt_gather_soup", k_gather_soup, std::min(div_up(nt, 256), 4096), 256, 0, stream, s.vertices,
                s.indices, nt, nv, s.soup, s.bvh.info);
     hip_check(hipStreamSynchronize(stream), "scene upload");
-    s.device_bytes = size_t(nt) * (36 + 12 + 4 + 4) + size_t(nv) * (12 + (desc.normals ? 12 : 0)) +
-                     sizeof(DevMaterial) * nm + sizeof(DevLight) * std::max<size_t>(s.host_lights.size(), 1) +
-                     bvh_bytes(nt);
+    s.device_bytes = scene_bytes(nt, nv, desc.normals != nullptr, nm, s.host_lights.size());
+    s.resident = true;
+}
+
+void scene_create(SceneDev& s, const lux_scene_desc& desc, bool upload, hipStream_t stream) {
+    check_desc(desc);
+    auto src = std::make_shared<SceneSource>();
+    const size_t nt = size_t(desc.triangle_count), nv = size_t(desc.vertex_count);
+    src->vertices.assign(desc.vertices, desc.vertices + nv);
+    src->indices.assign(desc.indices, desc.indices + 3 * nt);
+    if (desc.normals) src->normals.assign(desc.normals, desc.normals + nv);
+    src->material_ids.assign(desc.material_ids, desc.material_ids + nt);
+    src->materials.assign(desc.materials, desc.materials + desc.material_count);
+    std::memcpy(src->camera, desc.camera, sizeof(src->camera));
+    src->lights.assign(desc.lights, desc.lights + desc.light_count);
+    src->texels.resize(src->lights.size());
+    for (size_t i = 0; i < src->lights.size(); ++i) {
+        lux_light& L = src->lights[i];
+        if (L.image_pixels && L.image_width > 0 && L.image_height > 0) {
+            src->texels[i].assign(L.image_pixels, L.image_pixels + size_t(L.image_width) * size_t(L.image_height));
+            L.image_pixels = src->texels[i].data();
+        }
+    }
+    if (upload) {
+        scene_upload(s, desc, stream);
+    } else {
+        scene_release(s);
+        s.tri_count = int(nt);
+        s.vertex_count = int(nv);
+        s.mat_count = desc.material_count;
+        s.camera = host::make_camera(desc.camera);
+        size_t emitters = 0;
+        for (size_t t = 0; t < nt; ++t) {
+            const lux_material& m = desc.materials[desc.material_ids[t]];
+            emitters += fmaxf(m.emission.x, fmaxf(m.emission.y, m.emission.z)) > 0 ? 1 : 0;
+        }
+        s.device_bytes = scene_bytes(int(nt), int(nv), desc.normals != nullptr, desc.material_count,
+                                     emitters + size_t(desc.light_count));
+        s.resident = false;
+    }
+    s.source = std::move(src);
+}
+
+void scene_restore(SceneDev& fresh, const SceneSource& src, hipStream_t stream) {
+    lux_scene_desc d{};
+    d.vertices = src.vertices.data();
+    d.vertex_count = int32_t(src.vertices.size());
+    d.indices = src.indices.data();
+    d.normals = src.normals.empty() ? nullptr : src.normals.data();
+    d.material_ids = src.material_ids.data();
+    d.triangle_count = int32_t(src.material_ids.size());
+    d.materials = src.materials.data();
+    d.material_count = int32_t(src.materials.size());
+    std::memcpy(d.camera, src.camera, sizeof(d.camera));
+    d.lights = src.lights.empty() ? nullptr : src.lights.data();
+    d.light_count = int32_t(src.lights.size());
+    scene_upload(fresh, d, stream);
+    if (!src.current.empty()) scene_update_vertices(fresh, src.current.data(), int(src.current.size()), stream);
+}
+
+void scene_adopt(SceneDev& s, SceneDev&& fresh) {
+    const DevCamera camera = s.camera;  // as the caller last set it
+    std::shared_ptr<SceneSource> src = s.source;
+    const int jobs = s.server_jobs;
+    s = std::move(fresh);
+    fresh = SceneDev{};  // the device arrays belong to `s` now
+    s.camera = camera;
+    s.source = std::move(src);
+    s.server_jobs = jobs;
+    s.resident = true;
+}
+
+SceneDev scene_detach(SceneDev& s) {
+    if (!s.source || s.instanced) throw Error(LUX_ERR_INVALID_ARGUMENT, "only a kept one-mesh scene can be dropped");
+    SceneDev dead = std::move(s);
+    s = SceneDev{};
+    s.source = dead.source;
+    s.server_jobs = dead.server_jobs;
+    s.camera = dead.camera;
+    s.tri_count = dead.tri_count;
+    s.vertex_count = dead.vertex_count;
+    s.mat_count = dead.mat_count;
+    s.device_bytes = dead.device_bytes;
+    s.opaque = dead.opaque;
+    s.resident = false;
+    dead.source.reset();
+    return dead;
 }
 
 void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_count, hipStream_t stream) {
@@ -644,6 +747,7 @@ void scene_update_vertices(SceneDev& s, const lux_vec3* vertices, int vertex_cou
     upload_lights(s, stream);
     // the host arrays above are pageable: the copies complete before return
     hip_check(hipStreamSynchronize(stream), "scene update");
+    if (s.source) s.source->current.assign(vertices, vertices + vertex_count);
 }
 
 // ---- instanced scenes ------------------------------------------------------------------
@@ -845,6 +949,7 @@ void scene_upload_instanced(SceneDev& s, const lux_instanced_scene_desc& d, hipS
     }
     n.device_bytes = n.tl.device_bytes + size_t(n.tri_count) * (36 + 12 + (d.normals ? 36 : 0) + 4 + 4) +
                      sizeof(DevMaterial) * nmat + sizeof(DevLight) * std::max<size_t>(n.host_lights.size(), 1);
+    n.resident = true;
     scene_release(s);
     s = std::move(n);
 }
@@ -871,6 +976,7 @@ void scene_set_instances(SceneDev& s, const lux_instance* instances, int count, 
     flatten_instances(s, inst.data(), count, stream);
     s.device_bytes = s.tl.device_bytes + size_t(s.tri_count) * (36 + 12 + (s.normals ? 36 : 0) + 4 + 4) +
                      sizeof(DevMaterial) * s.mat_count + sizeof(DevLight) * std::max<size_t>(s.host_lights.size(), 1);
+    s.resident = true;
 }
 
 void scene_release(SceneDev& s) {

@@ -13,13 +13,26 @@ Server::Server(const ServeConfig& cfg, const Tuning& tuning, hipStream_t stream)
     if (cfg.slice_spp < 0 || cfg.max_inflight < 1 || cfg.max_inflight > 64)
         throw Error(LUX_ERR_INVALID_ARGUMENT, "server config out of range");
     engine_.set_tuning(tuning);
+    hip_check(hipGetDevice(&device_), "hipGetDevice");
     hip_check(hipStreamCreateWithFlags(&update_stream_, hipStreamNonBlocking), "hipStreamCreate");
     engine_.reset_ray_counts(stream_);
+    worker_ = std::thread([this] { worker_loop(); });
 }
 
 Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (worker_.joinable()) worker_.join();  // finishes the queued tasks first
+    for (auto& t : tasks_) {
+        if (t->restore) scene_release(t->fresh);  // never adopted
+        scene_release(t->dead);
+    }
     (void)hipStreamSynchronize(stream_);
     for (Job* j : active_) --j->scene->server_jobs;  // admitted, never retired
+    for (Job* j : waiting_) --j->scene->server_jobs;  // admitted, scene work never finished
     for (auto& j : pending_) --j->scene->server_jobs;  // submitted, never admitted
     for (auto& f : inflight_) (void)hipEventDestroy(f.ev);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
@@ -35,7 +48,7 @@ double Server::now() const {
 
 uint64_t Server::submit(SceneDev* scene, const lux_render_settings& rs, const lux_vec3* vertices, int vertex_count,
                         double arrival) {
-    if (!scene || !scene->soup) throw Error(LUX_ERR_INVALID_ARGUMENT, "request needs a scene");
+    if (!scene || (!scene->soup && !scene->source)) throw Error(LUX_ERR_INVALID_ARGUMENT, "request needs a scene");
     if (rs.mode != LUX_RENDER_WAVEFRONT) throw Error(LUX_ERR_INVALID_ARGUMENT, "the server renders wavefront only");
     if (vertices && scene->instanced)
         throw Error(LUX_ERR_INVALID_ARGUMENT, "instanced scene: vertex updates do not apply");
@@ -81,57 +94,180 @@ void Server::film_free(float* p, size_t bytes) {
 }
 
 size_t Server::footprint_in_use() const {
-    return resident_scene_bytes_ + films_in_use_ + films_pooled_ + engine_.planned_bytes();
+    return resident_scene_bytes_ + freeing_bytes_ + films_in_use_ + films_pooled_ + engine_.planned_bytes();
+}
+
+void Server::post(const std::shared_ptr<Task>& t) {
+    tasks_.push_back(t);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        queue_.push_back(t);
+    }
+    cv_.notify_one();
+}
+
+void Server::worker_loop() {
+    (void)hipSetDevice(device_);
+    for (;;) {
+        std::shared_ptr<Task> t;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+            if (queue_.empty()) return;  // stop_ and nothing left
+            t = queue_.front();
+            queue_.pop_front();
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        try {
+            if (t->restore) {
+                scene_restore(t->fresh, *t->source, update_stream_);
+                if (!t->vertices.empty()) {
+                    scene_update_vertices(t->fresh, t->vertices.data(), int(t->vertices.size()), update_stream_);
+                    t->source->current = t->vertices;
+                }
+            } else if (!t->vertices.empty()) {
+                // no admitted request renders the scene: no queued batch reads it
+                scene_update_vertices(*t->scene, t->vertices.data(), int(t->vertices.size()), update_stream_);
+            }
+            scene_release(t->dead);  // hipFree waits for the device: off the scheduling thread
+        } catch (...) {
+            t->error = std::current_exception();
+        }
+        t->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        t->done.store(true, std::memory_order_release);
+    }
+}
+
+// Adopts finished uploads, releases the budget of finished frees and lets the
+// requests that waited on a scene's work into the batches.
+void Server::finish_tasks() {
+    for (auto it = tasks_.begin(); it != tasks_.end();) {
+        Task* t = it->get();
+        if (!t->done.load(std::memory_order_acquire)) {
+            ++it;
+            continue;
+        }
+        if (t->error) {
+            std::exception_ptr e = t->error;
+            if (t->scene) preparing_.erase(t->scene);
+            tasks_.erase(it);
+            std::rethrow_exception(e);
+        }
+        freeing_bytes_ -= t->dead_bytes;
+        if (t->scene) {
+            if (t->restore) scene_adopt(*t->scene, std::move(t->fresh));
+            preparing_.erase(t->scene);
+            stats_.prepare_seconds += t->seconds;
+            for (auto w = waiting_.begin(); w != waiting_.end();) {
+                if ((*w)->scene == t->scene) {
+                    activate(*w);
+                    w = waiting_.erase(w);
+                } else {
+                    ++w;
+                }
+            }
+        }
+        it = tasks_.erase(it);
+    }
+}
+
+// Drops idle resident scenes (kept description, no admitted request, no work
+// running), least recently used first, until `need` more bytes fit.
+void Server::evict_for(size_t need, const SceneDev* keep) {
+    std::vector<std::pair<uint64_t, SceneDev*>> idle;
+    for (auto& kv : resident_) {
+        SceneDev* sc = kv.first;
+        if (sc == keep || !sc->source || sc->instanced || !sc->resident || scene_users_[sc] > 0 ||
+            preparing_.count(sc))
+            continue;
+        idle.emplace_back(last_use_[sc], sc);
+    }
+    std::sort(idle.begin(), idle.end());
+    for (auto& c : idle) {
+        if (footprint_in_use() + need <= cfg_.hbm_budget) break;
+        SceneDev* sc = c.second;
+        const size_t bytes = resident_[sc];
+        auto t = std::make_shared<Task>();
+        t->dead = scene_detach(*sc);
+        t->dead_bytes = bytes;
+        resident_.erase(sc);
+        resident_scene_bytes_ -= bytes;
+        freeing_bytes_ += bytes;
+        ++stats_.evictions;
+        post(t);
+    }
+}
+
+void Server::activate(Job* j) {
+    j->rq.sv = scene_view(*j->scene);
+    j->rq.rs = j->rs;
+    j->rq.film = j->film;
+    j->rq.total = request_paths(j->rs);
+    j->rq.issued = 0;
+    j->rq.last_batch = -1;
+    j->rq.tag = j->id;
+    active_.push_back(j);
+    issue_.push_back(j);
 }
 
 void Server::admit() {
+    finish_tasks();
     std::vector<const SceneDev*> blocked;  // scenes with an earlier request still waiting
     auto is_blocked = [&](const SceneDev* s) { return std::find(blocked.begin(), blocked.end(), s) != blocked.end(); };
     for (auto it = pending_.begin(); it != pending_.end();) {
         Job* j = it->get();
-        if (is_blocked(j->scene)) {
+        SceneDev* sc = j->scene;
+        if (is_blocked(sc) || preparing_.count(sc)) {  // arrival order within a scene
+            blocked.push_back(sc);
             ++it;
             continue;
         }
-        if (j->update && scene_users_[j->scene] > 0) {  // wait until the scene is idle
-            blocked.push_back(j->scene);
+        if (j->update && scene_users_[sc] > 0) {  // wait until the scene is idle
+            blocked.push_back(sc);
             ++it;
             continue;
         }
-        size_t extra = j->film_bytes + (resident_.count(j->scene) ? 0 : j->scene->device_bytes);
+        const bool cold = !sc->resident;
+        const size_t scene_extra = (cold || !resident_.count(sc)) ? sc->device_bytes : 0;
+        const size_t extra = j->film_bytes + scene_extra;
         if (footprint_in_use() + extra > cfg_.hbm_budget) {
-            // give pooled films back before refusing
+            // give pooled films back, then drop idle scenes, before refusing
             while (!film_pool_.empty() && footprint_in_use() + extra > cfg_.hbm_budget) {
                 auto f = film_pool_.begin();
                 (void)hipFree(f->second);
                 films_pooled_ -= f->first;
                 film_pool_.erase(f);
             }
+            if (footprint_in_use() + extra > cfg_.hbm_budget) evict_for(extra, sc);
             if (footprint_in_use() + extra > cfg_.hbm_budget) break;  // arrival order: later ones wait too
         }
-        if (!resident_.count(j->scene)) {
-            resident_[j->scene] = j->scene->device_bytes;
-            resident_scene_bytes_ += j->scene->device_bytes;
-        }
-        if (j->update) {
-            // the scene has no unfinished request, so no queued batch reads it
-            scene_update_vertices(*j->scene, j->vertices.data(), int(j->vertices.size()), update_stream_);
-            j->vertices.clear();
-            j->vertices.shrink_to_fit();
-            ++stats_.refits;
+        if (!resident_.count(sc)) {
+            resident_[sc] = sc->device_bytes;
+            resident_scene_bytes_ += sc->device_bytes;
         }
         j->film = film_alloc(j->film_bytes);
-        j->rq.sv = scene_view(*j->scene);
-        j->rq.rs = j->rs;
-        j->rq.film = j->film;
-        j->rq.total = request_paths(j->rs);
-        j->rq.issued = 0;
-        j->rq.last_batch = -1;
-        j->rq.tag = j->id;
         j->admitted = now();
-        ++scene_users_[j->scene];
-        active_.push_back(j);
-        issue_.push_back(j);
+        ++scene_users_[sc];
+        last_use_[sc] = ++tick_;
+        if (cold || j->update) {
+            auto t = std::make_shared<Task>();
+            t->scene = sc;
+            t->restore = cold;
+            if (cold) {
+                t->source = sc->source;
+                ++stats_.uploads;
+            }
+            if (j->update) {
+                t->vertices = std::move(j->vertices);
+                j->vertices = std::vector<lux_vec3>();
+                ++stats_.refits;
+            }
+            preparing_[sc] = t.get();
+            waiting_.push_back(j);
+            post(t);
+        } else {
+            activate(j);
+        }
         jobs_[j->id] = std::move(*it);
         it = pending_.erase(it);
         ++stats_.admitted;
@@ -201,6 +337,7 @@ void Server::retire(bool block, std::vector<uint64_t>* done) {
                 j->done = true;
                 j->finish = t;
                 --scene_users_[j->scene];
+                last_use_[j->scene] = ++tick_;
                 --j->scene->server_jobs;
                 ++stats_.completed;
                 if (done) done->push_back(j->id);
@@ -214,7 +351,7 @@ void Server::retire(bool block, std::vector<uint64_t>* done) {
 
 void Server::progress(bool block, std::vector<uint64_t>* done) {
     retire(false, done);
-    admit();
+    admit();  // also takes in finished scene work
     bool enqueued = false;
     while (enqueue_one()) enqueued = true;
     if (block && !enqueued && !inflight_.empty()) {
@@ -267,6 +404,8 @@ void run_trace(Server& sv, int n, SceneDev* const* scenes, const lux_render_sett
             if (next < n) {
                 const double wait = arrival[next] - (sv.now() - t0);
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 2e-4)));
+            } else if (sv.preparing()) {
+                std::this_thread::sleep_for(std::chrono::duration<double>(5e-5));  // scene work on the worker
             } else if (sv.busy()) {
                 // nothing on the GPU, no arrival left, and the queue cannot be admitted
                 std::vector<uint64_t> again;

@@ -2,20 +2,33 @@
 //
 // Requests (scene, render settings, optional new vertex positions) arrive at
 // any time. They are admitted in arrival order while their HBM footprint fits
-// the budget (scenes stay resident once seen; films come from a pool), and
-// every admitted request's paths join the shared wavefront loop in slices of
-// `slice_spp` samples, round-robin, so requests that arrive while others are
-// on the GPU start within a batch or two. A request that moves a scene's
-// vertices waits until no admitted request still renders that scene, then the
-// scene is refit on a side stream before the request's first batch. A request
-// completes when the batch holding its last path has finished; its film is
-// then exactly the film of the same render run alone.
+// the budget (films come from a pool), and every admitted request's paths join
+// the shared wavefront loop in slices of `slice_spp` samples, round-robin, so
+// requests that arrive while others are on the GPU start within a batch or two.
+//
+// The server owns scene residency (PreparedScene::prepare_gpu,
+// src/scene/prepared_scene.cu:61-79, on demand): a request for a scene whose
+// device state is absent (created deferred, or dropped earlier) admits with
+// the scene's bytes counted and has it uploaded and built; when the budget
+// binds, idle scenes (no admitted request) are dropped, least recently used
+// first, and rebuilt from their kept description when asked for again. A
+// request that moves a scene's vertices waits until no admitted request still
+// renders that scene, then the scene is refit. Uploads, refits and the frees
+// of dropped scenes run on a worker thread and its own stream, never on the
+// scheduling thread; the request's paths enter batches once that work is
+// done. A request completes when the batch holding its last path has
+// finished; its film is then exactly the film of the same render run alone.
 #pragma once
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -36,6 +49,8 @@ struct ServeStats {
     uint64_t resident_bytes = 0, peak_bytes = 0;
     uint64_t shared_batches = 0;      // batches holding paths of two or more requests
     uint64_t max_batch_requests = 0;  // most requests with paths in one batch
+    uint64_t uploads = 0, evictions = 0;  // scenes made resident / dropped by the server
+    double prepare_seconds = 0;           // worker time of uploads and refits
 };
 
 class Server {
@@ -57,8 +72,10 @@ public:
     // new batch could be enqueued. Appends newly completed ids to `done`.
     void progress(bool block, std::vector<uint64_t>* done);
 
-    bool busy() const { return !pending_.empty() || !active_.empty(); }
+    bool busy() const { return !pending_.empty() || !active_.empty() || !waiting_.empty(); }
     bool has_inflight() const { return !inflight_.empty(); }
+    // Uploads, refits or frees still running on the worker.
+    bool preparing() const { return !tasks_.empty(); }
 
     // Completed request: film sums to `host_film` (nullable), times, film freed.
     void take_result(uint64_t id, float* host_film, double* arrival, double* admitted, double* finish);
@@ -80,8 +97,28 @@ private:
         bool done = false;
     };
 
+    // Work of the worker thread: a scene made resident (restore) and/or refit
+    // (vertices), or the device state of a dropped scene freed (dead).
+    struct Task {
+        SceneDev* scene = nullptr;
+        std::shared_ptr<SceneSource> source;  // restore from this
+        bool restore = false;
+        std::vector<lux_vec3> vertices;  // refit to these
+        SceneDev fresh;                  // restore target, adopted on completion
+        SceneDev dead;
+        size_t dead_bytes = 0;
+        std::atomic<bool> done{false};
+        std::exception_ptr error;
+        double seconds = 0;
+    };
+
     void retire(bool block, std::vector<uint64_t>* done);
     void admit();
+    void activate(Job* j);
+    void finish_tasks();
+    void evict_for(size_t need, const SceneDev* keep);
+    void post(const std::shared_ptr<Task>& t);
+    void worker_loop();
     bool enqueue_one();
     float* film_alloc(size_t bytes);
     void film_free(float* p, size_t bytes);
@@ -97,9 +134,21 @@ private:
     std::vector<Job*> active_;                     // admitted, not yet complete
     std::deque<Job*> issue_;                       // admitted with paths left to place
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs_;  // every job not yet taken
-    std::unordered_map<const SceneDev*, int> scene_users_;       // active jobs per scene
-    std::unordered_map<const SceneDev*, size_t> resident_;       // scenes seen (bytes)
+    std::unordered_map<const SceneDev*, int> scene_users_;       // admitted, unfinished jobs per scene
+    std::unordered_map<SceneDev*, size_t> resident_;             // scenes resident for the server (bytes)
     size_t resident_scene_bytes_ = 0;
+    size_t freeing_bytes_ = 0;                                   // dropped scenes not yet freed
+    std::unordered_map<const SceneDev*, uint64_t> last_use_;     // admission tick
+    uint64_t tick_ = 0;
+    std::vector<Job*> waiting_;                                  // admitted, scene work running
+    std::vector<std::shared_ptr<Task>> tasks_;                   // posted, not yet finished
+    std::unordered_map<const SceneDev*, Task*> preparing_;       // scene -> its running task
+    int device_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<Task>> queue_;
+    bool stop_ = false;
+    std::thread worker_;
     size_t films_in_use_ = 0, films_pooled_ = 0;
     std::multimap<size_t, float*> film_pool_;
     struct Inflight {

@@ -117,7 +117,9 @@ def settings(width, height, spp, depth, seed=0, frame=0, sample_offset=0, batch_
 class Scene:
     instanced = False
 
-    def __init__(self, scene):
+    def __init__(self, scene, deferred=False):
+        """deferred: keep the description only; the device state is built on
+        first use (a render, an inspection call, or a server request)."""
         lbvh.init()
         _register()
         from . import autotune
@@ -152,7 +154,8 @@ class Scene:
         desc.lights = ctypes.addressof(arr) if n else None
         desc.light_count = n
         h = c_void_p()
-        call("lux_hip_scene_create", ctypes.byref(desc), ctypes.byref(h))
+        call("lux_hip_scene_create_deferred" if deferred else "lux_hip_scene_create", ctypes.byref(desc),
+             ctypes.byref(h))
         self.handle = h
         self.triangle_count = len(self._m)
         if "camera_vectors" in scene:
@@ -283,6 +286,12 @@ class Scene:
         call("lux_hip_scene_info", self.handle, ctypes.byref(t), ctypes.byref(v), ctypes.byref(b))
         return t.value, v.value, b.value
 
+    def resident(self):
+        """True while the scene's device state exists (a server may drop it)."""
+        r = c_int32()
+        call("lux_hip_scene_resident", self.handle, ctypes.byref(r))
+        return bool(r.value)
+
     def update_vertices(self, vertices):
         """Move the vertices (same count): device regather + LBVH refit + light table."""
         v = np.ascontiguousarray(vertices, np.float32)
@@ -354,7 +363,8 @@ class ServeConfig(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, c_uint64) for n in ("submitted", "admitted", "completed", "refits", "batches", "paths",
                                         "resident_bytes", "peak_bytes", "extension_rays", "shadow_rays",
-                                        "shared_batches", "max_batch_requests")]
+                                        "shared_batches", "max_batch_requests", "uploads", "evictions")] + \
+               [("prepare_seconds", ctypes.c_double)]
 
 
 class Server:

@@ -165,3 +165,72 @@ def test_scene_guarded_while_requests_are_unfinished(rd):
     sc.update_vertices(jitter(dicts[1]["vertices"], 5))
     sv.close()
     sc.close()
+
+
+def test_deferred_scene_builds_on_first_use(rd):
+    """A deferred scene has no device state until its first render, then renders
+    what the same scene created resident renders; a vertex update made while it
+    is not resident is applied by the rebuild (build from the first vertices,
+    refit to the update), as on a resident scene."""
+    print("case: deferred scene", flush=True)
+    d = scene_dicts()[1]
+    a, b = rd.Scene(d), rd.Scene(d, deferred=True)
+    assert a.resident() and not b.resident() and b.info()[2] > 0
+    fa = a.render(32, 32, 8, 5, seed=3)
+    fb = b.render(32, 32, 8, 5, seed=3)
+    assert b.resident() and np.array_equal(bits(fa), bits(fb))
+    c = rd.Scene(d, deferred=True)
+    v = jitter(d["vertices"], 55)
+    c.update_vertices(v)
+    a.update_vertices(v)
+    assert not c.resident()
+    assert np.array_equal(bits(a.render(32, 32, 8, 5, seed=4)), bits(c.render(32, 32, 8, 5, seed=4)))
+    for s in (a, b, c):
+        s.close()
+
+
+def test_server_owns_residency_under_tight_budget(rd):
+    """Deferred scenes: the server uploads a scene when it admits a request for
+    it and, under a budget that holds about two and a half scenes beside its
+    fixed footprint, drops idle scenes least recently used first and rebuilds
+    them when they are asked for again; films equal the standalone renders."""
+    print("case: server residency, tight budget", flush=True)
+    from lux_amd import scenes
+
+    dicts = [scenes.displaced_icosphere(freq=12 + 2 * k, seed=0x4C5600 + k, width=24, height=24) for k in range(5)]
+    scs = [rd.Scene(d, deferred=True) for d in dicts]
+    assert not any(sc.resident() for sc in scs)
+    sizes = [sc.info()[2] for sc in scs]
+
+    def job(i):
+        return dict(width=24, height=24, spp=6, depth=4, seed=i)
+
+    # the server's fixed footprint (engine buffers, one film): one request alone
+    probe = rd.Server(hbm_budget=1 << 40, slice_spp=8)
+    probe.run([scs[0]], [job(0)], [0.0])
+    fixed = probe.stats()["peak_bytes"] - sizes[0]
+    probe.close()
+    assert scs[0].resident()
+    budget = fixed + int(2.5 * max(sizes))
+    order = [0, 1, 2, 3, 4, 0, 2, 4, 1, 3, 0]
+    moves = {5: 301, 8: 302}  # request index -> jitter seed of a vertex update before it
+    verts = [jitter(dicts[order[i]]["vertices"], moves[i]) if i in moves else None for i in range(len(order))]
+    sv = rd.Server(hbm_budget=budget, slice_spp=8)
+    films, lat, wall = sv.run([scs[k] for k in order], [job(i) for i in range(len(order))],
+                              [0.002 * i for i in range(len(order))], vertices=verts)
+    st = sv.stats()
+    sv.close()
+    print(f"residency: uploads={st['uploads']} evictions={st['evictions']} refits={st['refits']} "
+          f"peak={st['peak_bytes']} budget={budget} prepare_s={st['prepare_seconds']:.4f}", flush=True)
+    assert st["completed"] == len(order) and st["refits"] == len(moves)
+    assert st["evictions"] >= 2 and st["uploads"] >= 6
+    assert st["peak_bytes"] <= budget
+    alone = [rd.Scene(d) for d in dicts]
+    for i, k in enumerate(order):
+        if verts[i] is not None:
+            alone[k].update_vertices(verts[i])
+        j = job(i)
+        a = alone[k].render(j["width"], j["height"], j["spp"], j["depth"], seed=j["seed"])
+        assert np.array_equal(bits(a), bits(films[i])), i
+    for s in alone + scs:
+        s.close()
