@@ -14,11 +14,16 @@ All inputs are resident in HBM before the timed region. `value` is the rays
 traced per second over the whole step (closest + any, i.e. 2 * 2^22 rays per
 step, with build, overlap and refit time included).
 
-Prints ONE JSON line. `roofline` is for the dominant kernel (closest-hit),
-timed with HIP events on its own stream inside the timed steps. `cpu_baseline`
-times the reference's own host traversal (intersect_cpu_bvh_tree over its host
-LBVH, compiled from the LUX sources into oracle/_ref) on a bounded ray sample,
-single thread, like the reference renderer.
+Prints ONE JSON line. `roofline` is for the step's longest kernel (any hit,
+k_trace_any), timed with HIP events on its own stream inside the timed steps:
+its HBM bound (compulsory bytes per ray over the kernel time, and the bytes the
+counters see below L2), and beside it its VALU bound from the committed
+counter summary (profiles/counters.json): how much of the kernel's time the
+vector ALUs issue, and how many lanes an issued instruction carries. The
+closest-hit kernel's figures follow under `closest_hit`. `cpu_baseline` times
+the CPU restatement of the reference's host traversal (intersect_cpu_bvh_tree
+over its host LBVH; bit-identical to the reference on the golden vectors) on a
+bounded ray sample, single thread, like the reference renderer.
 """
 
 import argparse
@@ -37,6 +42,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mrays/s; batched renders/s + p50/p99 latency vs plain GPU path at equal RMSE"
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+COUNTERS_FILE = ROOT / "profiles" / "counters.json"
 
 
 def parse():
@@ -160,7 +166,7 @@ def bench_c3(args):
         # rows from 128: the lit top of the sphere (the rows below the equator
         # see no light)
         out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], args.cpu_render_rows,
-                                                  128)
+                                                  128, sc=sc)
     sc.close()
     return out
 
@@ -295,7 +301,26 @@ def bench_bunny(args):
     return out
 
 
-def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows, r0):
+def sample_divergence(sc, pt, W, H, spp, depth, rows, n=4096, seed=0x4C55):
+    """Per-sample agreement of the GPU path radiance with the CPU restatement's
+    trace of the same path: n random (pixel, sample) pairs in `rows`, each GPU
+    radiance from a 1-spp render at that sample index. Paths that take another
+    bounce (device cosf/sinf/powf round a few inputs differently from the host
+    libm) show up as samples off by more than 1e-4 relative."""
+    rng = np.random.default_rng(seed)
+    px = rng.integers(rows[0] * W, rows[1] * W, n).astype(np.int32)
+    sm = rng.integers(0, spp, n).astype(np.int32)
+    ref = pt.samples(W, H, depth, px, sm)
+    got = np.zeros_like(ref)
+    for k in np.unique(sm):
+        one = sc.render(W, H, 1, depth, seed=0, sample_offset=int(k)).reshape(-1, 3)
+        sel = sm == k
+        got[sel] = one[px[sel]]
+    close = np.all(np.abs(got - ref) <= 1e-4 * np.maximum(np.abs(ref), 1e-2), axis=1)
+    return {"samples": n, "within_1e-4": round(float(close.mean()), 5), "diverged": int((~close).sum())}
+
+
+def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows, r0, sc=None):
     """A band of `nrows` rows from row r0 on the CPU, by the CPU restatement of
     the reference path tracer (bit-identical to the reference's trace_path on
     the golden renders): one thread (`value`, as the reference renders) and all
@@ -323,6 +348,8 @@ def cpu_render_baseline(scene, W, H, spp, depth, gpu_film, nrows, r0):
     out.update(value=round(paths / t_port / 1e6, 4), all_threads_Mpaths_per_s=round(paths / t_all / 1e6, 4),
                all_threads=threads,
                gpu_vs_cpu_band_relrmse=round(rel_rmse(gpu_film[band] / spp, port[band] / spp), 8))
+    if sc is not None:
+        out["sample_divergence"] = sample_divergence(sc, pt, W, H, spp, depth, rows)
     return out
 
 
@@ -410,6 +437,18 @@ def bench_c4(args, prepared=None):
         out["equal_rmse"] = (f"films bit-identical to the plain path ({same}/{len(scs)} scenes), so the "
                              "per-image relRMSE vs the CPU restatement is equal")
         _log(f"c4 plain: {psec:.3f} s")
+    if not args.no_cpu_baseline:
+        import oracle
+
+        div = []
+        for i in (0, 21, 42, 63):
+            if i < len(scs):
+                j = jobs[i]
+                pt = oracle.OraclePathTracer(dicts[i])
+                div.append(dict(scene=i, **sample_divergence(scs[i], pt, j["width"], j["height"], j["spp"],
+                                                             j["depth"], (0, j["height"]), n=1024, seed=0x4C56 + i)))
+        out["sample_divergence"] = div
+        _log(f"c4 sample divergence: {[d['diverged'] for d in div]} of {[d['samples'] for d in div]}")
     for f in d_films:
         f.free()
     return out, scs, jobs, dicts
@@ -647,7 +686,7 @@ def main():
     lbvh.synchronize()
     bvh.info()  # surfaces any device-side check failure before timing
 
-    with lbvh.Probe("trace_closest") as probe:
+    with lbvh.Probe("trace_any") as probe:
         lbvh.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -658,12 +697,22 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     kernel_ms = probe.total_ms / max(probe.launches, 1)
 
+
     # sanity of the last step's results (parity itself is the test suite's job)
     hits = d_hits.to_numpy(np.int32, (R, 2))
     occ = d_occ.to_numpy(np.uint8, R)
     assert np.array_equal(occ.astype(bool), hits[:, 0] >= 0), "any-hit disagrees with closest-hit"
     found = int(d_found.to_numpy(np.int32, 1)[0])
     self_found = int(d_sfound.to_numpy(np.int32, 1)[0])
+
+    # the closest-hit kernel, timed the same way over as many launches (on a
+    # fresh build: the step ends with a refit to jittered triangles)
+    build()
+    with lbvh.Probe("trace_closest") as cprobe:
+        for _ in range(args.steps):
+            closest()
+        lbvh.synchronize()
+    closest_ms = cprobe.total_ms / max(cprobe.launches, 1)
 
     # per-stage wall time (same work, stage by stage, synchronised)
     stage_ms = {}
@@ -690,15 +739,31 @@ def main():
     stats = (ctypes.c_uint64 * 6)()
     call("lux_hip_trace_stats_device", bvh.handle, d_tris.ptr, N, d_rays.ptr, R, stats)
 
-    algo_bytes = 40 * R  # per ray: o, d, t_min, t_max in (32 B) + {id, t} out (8 B)
+    # any hit: o, d, t_min, t_max in (32 B) + a 4-byte flag out per ray
+    algo_bytes = 36 * R
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
-    # closest-hit walk (binary right-first array): 32-B nodes + 36-B triangles
-    touched = 32 * stats[0] + 32 * stats[1] + 36 * stats[2]
-    traffic = None
-    if TRAFFIC_FILE.exists():
-        t = json.loads(TRAFFIC_FILE.read_text()).get("trace_closest")
-        if t and t.get("rays") == R and t.get("triangles") == N:
-            traffic = t.get("bytes_per_launch")
+    # touched by the walks: 32-B nodes + 36-B triangles (stats[3..5] any hit,
+    # stats[0..2] closest hit)
+    touched = 32 * stats[3] + 32 * stats[4] + 36 * stats[5]
+    c_algo = 40 * R  # closest hit: ray in (32 B) + {id, t} out (8 B)
+    c_touched = 32 * stats[0] + 32 * stats[1] + 36 * stats[2]
+
+    def profiled(table, key, fields):
+        """A committed profile entry of this workload (same rays and triangles)."""
+        path = TRAFFIC_FILE if table == "traffic" else COUNTERS_FILE
+        if not path.exists():
+            return None
+        e = json.loads(path.read_text()).get(key)
+        if not e or e.get("rays") != R or e.get("triangles") != N:
+            return None
+        return {f: e.get(f) for f in fields} if fields else e
+
+    t_any = profiled("traffic", "trace_any", ["bytes_per_launch"])
+    t_closest = profiled("traffic", "trace_closest", ["bytes_per_launch"])
+    valu_fields = ["valu_busy_frac", "lane_utilization", "valu_insts_per_wave", "waves_per_simd", "l2_hit_rate",
+                   "kernel_us", "source"]
+    v_any = profiled("counters", "k_trace_any", valu_fields)
+    v_closest = profiled("counters", "k_trace_closest", valu_fields)
 
     rays_per_step = 2 * R
     out = {
@@ -724,19 +789,33 @@ def main():
             "parallelism": "single-gpu",
         },
         "roofline": {
-            "kernel": "trace_closest",
+            "kernel": "trace_any",
             "bound": "hbm",
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 6),
-            "traffic": traffic,
+            "traffic": t_any["bytes_per_launch"] if t_any else None,
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": algo_bytes,
             "touched_bytes_per_launch": int(touched),
             "touched_GBps": round(touched / (kernel_ms * 1e-3) / 1e9, 2),
-            "node_visits_per_ray": round(stats[0] / R, 2),
-            "triangle_tests_per_ray": round(stats[2] / R, 3),
+            "node_visits_per_ray": round(stats[3] / R, 2),
+            "triangle_tests_per_ray": round(stats[5] / R, 3),
+            # the VALU bound beside it: issue-busy fraction of the vector ALUs over
+            # the kernel and the share of lanes an issued instruction carries
+            # (rocprofv3 SQ counters of this kernel on this workload)
+            "valu": v_any,
+            "closest_hit": {
+                "kernel_ms": round(closest_ms, 4),
+                "achieved_GBps": round(c_algo / (closest_ms * 1e-3) / 1e9, 3),
+                "frac": round(c_algo / (closest_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 6),
+                "traffic": t_closest["bytes_per_launch"] if t_closest else None,
+                "touched_bytes_per_launch": int(c_touched),
+                "node_visits_per_ray": round(stats[0] / R, 2),
+                "triangle_tests_per_ray": round(stats[2] / R, 3),
+                "valu": v_closest,
+            },
         },
         "stages_ms": stage_ms,
         "stage_rates": {
