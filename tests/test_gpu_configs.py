@@ -27,9 +27,9 @@ pytestmark = pytest.mark.gpu
 # GPU's cosf/sinf/powf round differently from the host's, so a small share of
 # paths take another bounce. Measured on MI355X: relRMSE 3.3e-3, no pixel off
 # by more than 1e-3, all 4096 sampled radiances within 1e-4.
-RELRMSE_C3 = 1e-2
-FRAC_PIXELS_OFF_C3 = 0.01   # pixels with |delta| > 1e-3 (spp-averaged)
-SAMPLES_CLOSE_C3 = 0.99     # (pixel, sample) radiances within 1e-4 relative
+RELRMSE_C3 = 5e-3
+FRAC_PIXELS_OFF_C3 = 0.002  # pixels with |delta| > 1e-3 (spp-averaged)
+SAMPLES_CLOSE_C3 = 0.999    # (pixel, sample) radiances within 1e-4 relative
 
 
 def bits(a):
