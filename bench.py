@@ -176,8 +176,9 @@ def bench_instanced(args):
     triangles each, 5 242 880 instanced triangles) over a ground plane,
     1024x1024, 16 spp, depth 8, seed 0; the same geometry flattened into one
     mesh (the instanced scene's own world-space shading view) rendered through
-    the one-level path beside it. CPU baseline: the reference's trace_path on a
-    band of rows, one thread."""
+    the one-level path beside it. CPU baseline: the CPU restatement of the
+    reference's trace_path (two-level walk; bit-identical to the reference on
+    the golden instanced renders) on a band of rows, one thread."""
     from lux_amd import lbvh, render, scenes
 
     W = H = 1024
@@ -244,7 +245,8 @@ def bench_bunny(args):
     256x256, 64 spp, depth 4, seed 0): the Stanford bunny (69 451 triangles)
     imported from the OBJ fixture, placed by its instance over a floor under a
     quad light. Reported: OBJ import time, scene upload/build, render rate; CPU
-    baseline: the reference's trace_path on a band of rows, one thread."""
+    baseline: the CPU restatement of the reference's trace_path (bit-identical
+    to the reference's bunny render) on a band of rows, one thread."""
     import lzma
     import tempfile
 

@@ -737,6 +737,11 @@ def _pt_lib():
         L.orc_pt_create.restype = c_void_p
         L.orc_pt_create.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32,
                                     c_void_p]
+        L.orc_pt_create_instanced.restype = c_void_p
+        L.orc_pt_create_instanced.argtypes = [c_void_p] * 6 + [c_int32] + [c_void_p] * 4 + [c_int32, c_void_p,
+                                                                                              c_int32, c_void_p]
+        L.orc_pt_triangle_count.restype = c_int32
+        L.orc_pt_triangle_count.argtypes = [c_void_p]
         L.orc_pt_free.argtypes = [c_void_p]
         L.orc_pt_camera.argtypes = [c_void_p, c_void_p]
         L.orc_pt_lights.restype = c_int32
@@ -750,13 +755,29 @@ def _pt_lib():
 
 
 class OraclePathTracer:
-    """The CPU restatement of the reference path tracer on a scene dict."""
+    """The CPU restatement of the reference path tracer on a scene dict (an
+    instanced one when the dict has "inst_mesh", as RefPathTracer takes it;
+    "inst_inv" is used where given, else the inverse_affine of "inst_m")."""
 
     def __init__(self, scene):
         L = _pt_lib()
         self._keep = _scene_args(scene)
         v, i, n, uv, m, mats, cam = self._keep
-        self.h = L.orc_pt_create(_p(v), len(v), _p(i), _p(n), _p(m), len(m), _p(mats), len(mats), _p(cam))
+        if "inst_mesh" in scene:
+            vo = np.ascontiguousarray(scene["vertex_offsets"], np.int32)
+            to = np.ascontiguousarray(scene["triangle_offsets"], np.int32)
+            im = np.ascontiguousarray(scene["inst_mesh"], np.int32)
+            mm = np.ascontiguousarray(scene["inst_m"], np.float32).reshape(-1, 16)
+            inv = scene.get("inst_inv")
+            inv = None if inv is None else np.ascontiguousarray(inv, np.float32).reshape(-1, 16)
+            ov = scene.get("inst_override")
+            ov = np.full(len(im), -1, np.int32) if ov is None else np.ascontiguousarray(ov, np.int32)
+            self._inst = (vo, to, im, mm, inv, ov)
+            self.h = L.orc_pt_create_instanced(_p(v), _p(vo), _p(i), _p(to), _p(n), _p(m), len(vo) - 1, _p(im),
+                                               _p(mm), _p(inv), _p(ov), len(im), _p(mats), len(mats), _p(cam))
+        else:
+            self.h = L.orc_pt_create(_p(v), len(v), _p(i), _p(n), _p(m), len(m), _p(mats), len(mats), _p(cam))
+        self.ordinals = L.orc_pt_triangle_count(self.h)
 
     def __del__(self):
         try:
@@ -772,7 +793,7 @@ class OraclePathTracer:
     def lights(self):
         L = _pt_lib()
         n = L.orc_pt_lights(self.h, None, None, None, None, None, None, None, None)
-        t = len(self._keep[4])
+        t = self.ordinals
         out = {"prim": np.zeros(n, np.int32), "area": np.zeros(n, np.float32), "cdf": np.zeros(n, np.float32),
                "prob": np.zeros(n, np.float32), "alias": np.zeros(n, np.int32), "pmf": np.zeros(n, np.float32),
                "primitive_light_index": np.zeros(t, np.int32), "total_weight": np.zeros(1, np.float32)}

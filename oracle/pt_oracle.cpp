@@ -2,7 +2,9 @@
 // (src/integrator/cpu_path_tracer.cpp: trace_path, estimate_direct_light,
 // trace_shadow) with the power light sampler, the six BSDFs, the PCG32 sampler
 // stream and the pinhole camera, for a scene of one mesh under an identity
-// instance.
+// instance (the one-level walk) and for instanced scenes (meshes placed by
+// affine instances: the two-level walk of two_level_oracle.cpp, shading on the
+// world-space triangle view).
 //
 // TEST INFRASTRUCTURE ONLY: the checker for the HIP wavefront path tracer, and
 // the CPU baseline of bench.py. Same rules as lbvh_oracle.cpp. Arithmetic keeps
@@ -11,9 +13,22 @@
 // same machine; tests/golden holds renders and per-sample radiance written by
 // the reference to pin that.
 
+#include <cstring>
 #include <thread>
 
 #include "oracle_core.hpp"
+
+// The two-level walk of two_level_oracle.cpp (same library).
+extern "C" {
+struct orc_tl;
+orc_tl* orc_tl_build(const float* tris, const int32_t* tri_off, int32_t mesh_count, const int32_t* inst_mesh,
+                     const float* inst_inv, const float* inst_world_bounds, int32_t instance_count);
+void orc_tl_free(orc_tl* h);
+int32_t orc_tl_trace(const orc_tl* h, const float* rays, const int32_t* ignore, int32_t n, int32_t* prim,
+                     float* t_out, float* u_out, float* v_out, int32_t* inst_out, int32_t* local_out);
+void orc_inverse_affine(const float* m16, float* out16);
+void orc_transform_bounds(const float* m16, const float* box6, float* out6);
+}
 
 namespace pt {
 
@@ -458,8 +473,12 @@ struct Scene {
     std::vector<AliasEntry> alias;
     float total_weight = 0;
     std::vector<int> light_of_prim;
-    orc::Tree* bvh = nullptr;
-    ~Scene() { delete bvh; }
+    orc::Tree* bvh = nullptr;  // one-mesh scenes
+    orc_tl* tl = nullptr;      // instanced scenes: tris/n0/n1/n2/mat are the world-space view per ordinal
+    ~Scene() {
+        delete bvh;
+        if (tl) orc_tl_free(tl);
+    }
 };
 
 inline V tri_v(const orc::Tri& t, int k) {
@@ -470,15 +489,38 @@ inline V tri_v(const orc::Tri& t, int k) {
 inline V tri_normal(const orc::Tri& t) { return normalize(cross(tri_v(t, 1) - tri_v(t, 0), tri_v(t, 2) - tri_v(t, 0))); }
 inline float tri_area(const orc::Tri& t) { return 0.5f * length(cross(tri_v(t, 1) - tri_v(t, 0), tri_v(t, 2) - tri_v(t, 0))); }
 
-// transform_instance_normal under the identity transform (src/scene/scene_data.h:262-268,
-// src/core/mat4.cuh:209-229): adjugate product, then normalised twice.
-inline V instance_normal(const V& n) {
+// transform_normal (src/core/mat4.cuh:209-230): the adjugate of the upper 3x3
+// applied to n, then normalised; transform_instance_normal
+// (src/scene/scene_data.h:262-268) normalises once more and maps a zero normal
+// (before or after) to zero.
+using M4 = float[4][4];
+constexpr M4 kIdentity = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+inline V transform_normal(const M4& m, const V& n) {
+    float a00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    float a01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    float a02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    float a10 = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+    float a11 = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+    float a12 = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+    float a20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    float a21 = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+    float a22 = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    return normalize(V(a00 * n.x + a01 * n.y + a02 * n.z, a10 * n.x + a11 * n.y + a12 * n.z,
+                       a20 * n.x + a21 * n.y + a22 * n.z));
+}
+inline V instance_normal(const M4& m, const V& n) {
     if (length2(n) == 0) return V(0);
-    V t(1.0f * n.x + 0.0f * n.y + 0.0f * n.z, 0.0f * n.x + 1.0f * n.y + 0.0f * n.z,
-        0.0f * n.x + 0.0f * n.y + 1.0f * n.z);
-    V once = normalize(t);
-    if (length2(once) == 0) return V(0);
-    return normalize(once);
+    V t = transform_normal(m, n);
+    if (length2(t) == 0) return V(0);
+    return normalize(t);
+}
+// transform_point (src/core/mat4.cuh:173-186): divides by w unless w == 1.
+inline orc::V3 transform_point(const M4& m, float px, float py, float pz) {
+    float x = m[0][0] * px + m[0][1] * py + m[0][2] * pz + m[0][3];
+    float y = m[1][0] * px + m[1][1] * py + m[1][2] * pz + m[1][3];
+    float z = m[2][0] * px + m[2][1] * py + m[2][2] * pz + m[2][3];
+    float w = m[3][0] * px + m[3][1] * py + m[3][2] * pz + m[3][3];
+    return w == 1.0f ? orc::v3(x, y, z) : orc::v3(x / w, y / w, z / w);
 }
 
 // make_perspective_camera with lens radius 0 (src/camera/camera.h:58-109)
@@ -575,8 +617,17 @@ struct Hit {
 
 // Closest hit through the host LBVH (intersect_scene_bvh / _skip_ref with the
 // identity instance): the host stack walk, skipping `ignore` at its leaf.
+inline bool finish_hit(const Scene& s, const V& o, const V& d, int id, float th, float uh, float vh, Hit& h);
+
 inline bool intersect(const Scene& s, const V& o, const V& d, float t_min, float t_max, int ignore, Hit& h) {
     orc::Ray r{orc::v3(o.x, o.y, o.z), orc::v3(d.x, d.y, d.z), t_min, t_max, 0};
+    if (s.tl) {  // the two-level walk (scene_intersection.h:147-240), hit finished on the world view
+        int32_t prim = -1, inst = -1, local = -1;
+        float tt = 0, uu = 0, vv = 0;
+        orc_tl_trace(s.tl, reinterpret_cast<const float*>(&r), &ignore, 1, &prim, &tt, &uu, &vv, &inst, &local);
+        if (prim < 0) return false;
+        return finish_hit(s, o, d, prim, tt, uu, vv, h);
+    }
     const orc::Tree& T = *s.bvh;
     if (T.n <= 0) return false;
     int ni = T.n - 1;
@@ -606,6 +657,10 @@ inline bool intersect(const Scene& s, const V& o, const V& d, float t_min, float
         stack[sp++] = orc::host_index(T.right[k], ni);
     }
     if (id < 0) return false;
+    return finish_hit(s, o, d, id, th, uh, vh, h);
+}
+
+inline bool finish_hit(const Scene& s, const V& o, const V& d, int id, float th, float uh, float vh, Hit& h) {
     h.prim = id;
     h.t = th;
     h.u = uh;
@@ -823,10 +878,8 @@ extern "C" {
 
 struct orc_pt;
 
-orc_pt* orc_pt_create(const float* vertices, int32_t nv, const uint32_t* indices, const float* normals,
-                      const int32_t* mat_ids, int32_t nt, const float* materials, int32_t nm, const float* camera) {
-    (void)nv;
-    pt::Scene* s = new pt::Scene();
+// Materials as 11 floats each: type, albedo rgb, emission rgb, two_sided, roughness, metallic, eta.
+static void load_materials(pt::Scene& s, const float* materials, int32_t nm) {
     for (int m = 0; m < nm; ++m) {
         const float* p = materials + 11 * m;
         pt::Material mat;
@@ -837,8 +890,15 @@ orc_pt* orc_pt_create(const float* vertices, int32_t nv, const uint32_t* indices
         mat.roughness = p[8];
         mat.metallic = p[9];
         mat.eta = p[10];
-        s->mats.push_back(mat);
+        s.mats.push_back(mat);
     }
+}
+
+orc_pt* orc_pt_create(const float* vertices, int32_t nv, const uint32_t* indices, const float* normals,
+                      const int32_t* mat_ids, int32_t nt, const float* materials, int32_t nm, const float* camera) {
+    (void)nv;
+    pt::Scene* s = new pt::Scene();
+    load_materials(*s, materials, nm);
     s->tris.resize(nt);
     s->n0.resize(nt);
     s->n1.resize(nt);
@@ -847,7 +907,7 @@ orc_pt* orc_pt_create(const float* vertices, int32_t nv, const uint32_t* indices
     auto vert = [&](uint32_t k) { return orc::v3(vertices[3 * k], vertices[3 * k + 1], vertices[3 * k + 2]); };
     auto nrm = [&](uint32_t k) {
         if (!normals) return pt::V(0);
-        return pt::instance_normal(pt::V(normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]));
+        return pt::instance_normal(pt::kIdentity, pt::V(normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]));
     };
     std::vector<orc::Box> boxes(nt);
     std::vector<int32_t> ids(nt);
@@ -872,6 +932,82 @@ orc_pt* orc_pt_create(const float* vertices, int32_t nv, const uint32_t* indices
     pt::build_lights(*s);
     return reinterpret_cast<orc_pt*>(s);
 }
+
+// An instanced scene (add_scene_mesh per mesh, add_scene_instance per instance,
+// src/scene/scene_data.h): meshes concatenated (mesh m owns vertices
+// [vertex_offsets[m], vertex_offsets[m+1]) and triangles [triangle_offsets[m],
+// triangle_offsets[m+1]), indices mesh-local), instance i places mesh
+// inst_mesh[i] by the row-major object-to-world inst_m[i] (inverse inst_inv[i],
+// or inverse_affine of inst_m[i] when inst_inv is null) with material override
+// inst_override[i] (-1 = the mesh's own). Shading, lights and the light
+// sampler see scene_triangle_view per ordinal (src/scene/scene_data.h:350-371):
+// world corners, transform_instance_normal corner normals, the material after
+// the override; rays take the two-level walk (BLAS per mesh, TLAS over
+// transform_bounds of each mesh's box, src/scene/prepared_scene_cpu.cpp:27-42).
+orc_pt* orc_pt_create_instanced(const float* vertices, const int32_t* vertex_offsets, const uint32_t* indices,
+                                const int32_t* triangle_offsets, const float* normals, const int32_t* mat_ids,
+                                int32_t mesh_count, const int32_t* inst_mesh, const float* inst_m,
+                                const float* inst_inv, const int32_t* inst_override, int32_t instance_count,
+                                const float* materials, int32_t nm, const float* camera) {
+    pt::Scene* s = new pt::Scene();
+    load_materials(*s, materials, nm);
+    auto vtx = [&](int mesh, uint32_t k) { return vertices + 3 * (size_t(vertex_offsets[mesh]) + k); };
+    // object-space soups and mesh boxes
+    int total = triangle_offsets[mesh_count];
+    std::vector<orc::Tri> obj(total);
+    std::vector<orc::Box> mesh_box(mesh_count, orc::empty_box());
+    for (int m = 0; m < mesh_count; ++m)
+        for (int t = triangle_offsets[m]; t < triangle_offsets[m + 1]; ++t) {
+            orc::V3 c[3];
+            for (int k = 0; k < 3; ++k) {
+                const float* p = vtx(m, indices[3 * t + k]);
+                c[k] = orc::v3(p[0], p[1], p[2]);
+                mesh_box[m].lo = orc::vmin(mesh_box[m].lo, c[k]);
+                mesh_box[m].hi = orc::vmax(mesh_box[m].hi, c[k]);
+            }
+            obj[t] = orc::Tri{c[0], c[1], c[2]};
+        }
+    std::vector<float> inv(16 * size_t(instance_count)), wb(6 * size_t(instance_count));
+    for (int i = 0; i < instance_count; ++i) {
+        const float* m16 = inst_m + 16 * i;
+        if (inst_inv)
+            std::memcpy(&inv[16 * i], inst_inv + 16 * i, 16 * sizeof(float));
+        else
+            orc_inverse_affine(m16, &inv[16 * i]);
+        int mesh = inst_mesh[i];
+        orc::Box b = mesh >= 0 && mesh < mesh_count ? mesh_box[mesh] : orc::empty_box();
+        orc_transform_bounds(m16, reinterpret_cast<const float*>(&b), &wb[6 * i]);
+        if (mesh < 0 || mesh >= mesh_count) continue;
+        pt::M4 M;
+        std::memcpy(M, m16, sizeof(M));
+        for (int t = triangle_offsets[mesh]; t < triangle_offsets[mesh + 1]; ++t) {
+            orc::V3 c[3];
+            pt::V n[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = indices[3 * t + k];
+                const float* p = vtx(mesh, vi);
+                c[k] = pt::transform_point(M, p[0], p[1], p[2]);
+                if (normals) {
+                    const float* q = normals + 3 * (size_t(vertex_offsets[mesh]) + vi);
+                    n[k] = pt::instance_normal(M, pt::V(q[0], q[1], q[2]));
+                }
+            }
+            s->tris.push_back(orc::Tri{c[0], c[1], c[2]});
+            s->n0.push_back(n[0]);
+            s->n1.push_back(n[1]);
+            s->n2.push_back(n[2]);
+            s->mat.push_back(inst_override[i] >= 0 ? inst_override[i] : mat_ids[t]);
+        }
+    }
+    s->tl = orc_tl_build(reinterpret_cast<const float*>(obj.data()), triangle_offsets, mesh_count, inst_mesh,
+                         inv.data(), wb.data(), instance_count);
+    s->cam = pt::make_camera(pt::V(camera[0], camera[1], camera[2]), pt::V(camera[3], camera[4], camera[5]),
+                             camera[6], camera[7]);
+    pt::build_lights(*s);
+    return reinterpret_cast<orc_pt*>(s);
+}
+
+int32_t orc_pt_triangle_count(const orc_pt* h) { return int32_t(reinterpret_cast<const pt::Scene*>(h)->tris.size()); }
 
 void orc_pt_free(orc_pt* h) { delete reinterpret_cast<pt::Scene*>(h); }
 

@@ -5,12 +5,21 @@ float64: world corners are object_to_world applied to the mesh vertices; world
 normals are the inverse-transpose normals, flipped for a mirroring instance
 (the reference's adjugate transform_normal equals det(M) * M^-T); materials
 follow the instance override; area lights are the emissive ordinals in order
-with their world-space areas."""
+with their world-space areas. Then the CPU restatement of the path tracer on
+those instanced scenes and on the reference's bunny scene (rebuilt from the OBJ
+fixture): light tables, films and per-sample radiance bit-identical to the
+reference's own render."""
+
+import lzma
 
 import numpy as np
 import pytest
 
-from conftest import load_golden
+import oracle
+from conftest import GOLDEN, load_golden
+
+SCENE_KEYS = ("vertices", "indices", "normals", "uvs", "material_ids", "materials", "camera", "vertex_offsets",
+              "triangle_offsets", "inst_mesh", "inst_m", "inst_override")
 
 CASES = ["pt_instanced", "pt_instanced_opaque"]
 
@@ -81,3 +90,38 @@ def test_area_lights_are_emissive_ordinals_with_world_area(case):
     # an instanced emitter under a non-uniform scale is among them
     inst_of = np.array([i for i, *_ in ordinals(g)])
     assert len(set(inst_of[exp])) >= 2
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def check_port(o, g, ordinals):
+    assert o.ordinals == ordinals
+    assert np.array_equal(bits(o.camera()), bits(g["camera_vectors"]))
+    for k, v in o.lights().items():
+        assert np.array_equal(np.asarray(v).view(np.uint32), np.asarray(g["light_" + k]).view(np.uint32)), k
+    w, h, spp, depth, seed = (int(g[k]) for k in ("width", "height", "spp", "depth", "seed"))
+    frame = int(g["frame"])
+    assert np.array_equal(bits(o.render(w, h, spp, depth, seed=seed, frame=frame, threads=4)), bits(g["film_sum"]))
+    s = o.samples(w, h, depth, g["sample_pixels"], g["sample_index"], seed=seed, frame=frame)
+    assert np.array_equal(bits(s), bits(g["sample_radiance"]))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_port_renders_instanced_scene_like_reference(case):
+    g = load_golden(case)
+    check_port(oracle.OraclePathTracer({k: g[k] for k in SCENE_KEYS}), g, len(g["tri_material"]))
+
+
+def test_port_renders_bunny_scene_like_reference(tmp_path):
+    """The bunny's instance carries the OBJ importer's world_to_object, which
+    the restatement takes as the instance inverse."""
+    from lux_amd import scenes
+
+    g = load_golden("pt_bunny")
+    path = tmp_path / "stanford-bunny.obj"
+    path.write_bytes(lzma.decompress((GOLDEN / "obj" / "stanford-bunny.obj.xz").read_bytes()))
+    d = scenes.bunny(path, int(g["width"]), int(g["height"]))
+    assert "inst_inv" in d
+    check_port(oracle.OraclePathTracer(d), g, int(g["ordinals"]))
