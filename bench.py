@@ -233,7 +233,8 @@ def bench_instanced(args):
             "value": round(paths / t_port / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
             "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
             "host_cpu": _cpu_name(),
-            "gpu_vs_cpu_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6)}
+            "gpu_vs_cpu_band_relrmse": round(render.rel_rmse(gpu_band / spp, band / spp), 6),
+            "sample_divergence": sample_divergence(sc, port, W, H, spp, depth, (r0, r0 + nrows))}
     sc.close()
     fsc.close()
     film.free()
@@ -297,7 +298,8 @@ def bench_bunny(args):
             "sample": f"rows {r0}..{r0 + nrows - 1} of the {W}x{H} image at {spp} spp ({paths} paths)",
             "host_cpu": _cpu_name(),
             "gpu_vs_cpu_band_relrmse": round(
-                render.rel_rmse(gpu[r0:r0 + nrows].reshape(-1, 3) / spp, band / spp), 8)}
+                render.rel_rmse(gpu[r0:r0 + nrows].reshape(-1, 3) / spp, band / spp), 8),
+            "sample_divergence": sample_divergence(sc, port, W, H, spp, depth, (r0, r0 + nrows))}
     sc.close()
     film.free()
     return out
