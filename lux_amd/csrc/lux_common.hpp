@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lux_hip.h"
 
 namespace lux {
@@ -170,16 +172,34 @@ __device__ __forceinline__ bool slab_hit(const SlabRay& r, float t_max, float lo
     return !miss && !(t_near > t_far);
 }
 
+// Slab-test modes, a compile-time choice made once per wave (slab_mode):
+// kSlabParallel  some lane has an axis-parallel direction: slab_hit's branches;
+// kSlabLane      no parallel axis, direction signs read per lane;
+// 0..7           no parallel axis and every lane of the wave in octant m
+//                (bit 0: d.x < 0, bit 1: d.y < 0, bit 2: d.z < 0), so the
+//                near/far planes are fixed and the per-lane selects vanish.
+// All modes perform the same float operations on the same operands.
+constexpr int kSlabParallel = -2;
+constexpr int kSlabLane = -1;
+
+__device__ __forceinline__ int ray_octant(const SlabRay& r) {
+    return int(r.neg_x) | int(r.neg_y) << 1 | int(r.neg_z) << 2;
+}
+
 // slab_hit for a ray with no axis parallel to the box faces (|d| >= 1e-12 on
 // every axis): the same operations and result without the per-axis branches.
+template <int kMode = kSlabLane>
 __device__ __forceinline__ bool slab_hit_np(const SlabRay& r, float t_max, float lox, float loy, float loz, float hix,
                                             float hiy, float hiz) {
-    const float ax = ((r.neg_x ? hix : lox) - r.o.x) * r.inv.x;
-    const float cx = ((r.neg_x ? lox : hix) - r.o.x) * r.inv.x;
-    const float ay = ((r.neg_y ? hiy : loy) - r.o.y) * r.inv.y;
-    const float cy = ((r.neg_y ? loy : hiy) - r.o.y) * r.inv.y;
-    const float az = ((r.neg_z ? hiz : loz) - r.o.z) * r.inv.z;
-    const float cz = ((r.neg_z ? loz : hiz) - r.o.z) * r.inv.z;
+    const bool nx = kMode < 0 ? r.neg_x : (kMode & 1) != 0;
+    const bool ny = kMode < 0 ? r.neg_y : (kMode & 2) != 0;
+    const bool nz = kMode < 0 ? r.neg_z : (kMode & 4) != 0;
+    const float ax = ((nx ? hix : lox) - r.o.x) * r.inv.x;
+    const float cx = ((nx ? lox : hix) - r.o.x) * r.inv.x;
+    const float ay = ((ny ? hiy : loy) - r.o.y) * r.inv.y;
+    const float cy = ((ny ? loy : hiy) - r.o.y) * r.inv.y;
+    const float az = ((nz ? hiz : loz) - r.o.z) * r.inv.z;
+    const float cz = ((nz ? loz : hiz) - r.o.z) * r.inv.z;
     const float t_near = fmaxf(fmaxf(fmaxf(r.t_min, ax), ay), az);
     const float t_far = fminf(fminf(fminf(t_max, cx), cy), cz);
     return !(t_near > t_far);
@@ -189,6 +209,39 @@ __device__ __forceinline__ bool slab_hit_np(const SlabRay& r, float t_max, float
 // take the branch-free slab test when none has (a wave-uniform choice).
 __device__ __forceinline__ bool wave_any_parallel(const SlabRay& r) {
     return __ballot(r.par_x || r.par_y || r.par_z) != 0;
+}
+
+template <int kMode>
+__device__ __forceinline__ bool slab_pass(const SlabRay& r, float t_max, float lox, float loy, float loz, float hix,
+                                          float hiy, float hiz) {
+    if constexpr (kMode == kSlabParallel)
+        return slab_hit(r, t_max, lox, loy, loz, hix, hiy, hiz);
+    else
+        return slab_hit_np<kMode>(r, t_max, lox, loy, loz, hix, hiy, hiz);
+}
+
+template <int kMode>
+using SlabMode = std::integral_constant<int, kMode>;
+
+// Calls f(SlabMode<m>{}) with the wave's slab-test mode (wave-uniform: every
+// active lane takes the same call). Rays traced in octant order fill most waves
+// from one octant.
+template <typename F>
+__device__ __forceinline__ auto with_slab_mode(const SlabRay& r, F&& f) {
+    if (wave_any_parallel(r)) return f(SlabMode<kSlabParallel>{});
+    const int oct = ray_octant(r);
+    const int oct0 = __builtin_amdgcn_readfirstlane(oct);
+    if (__ballot(oct != oct0) != 0) return f(SlabMode<kSlabLane>{});
+    switch (oct0) {
+        case 0: return f(SlabMode<0>{});
+        case 1: return f(SlabMode<1>{});
+        case 2: return f(SlabMode<2>{});
+        case 3: return f(SlabMode<3>{});
+        case 4: return f(SlabMode<4>{});
+        case 5: return f(SlabMode<5>{});
+        case 6: return f(SlabMode<6>{});
+        default: return f(SlabMode<7>{});
+    }
 }
 
 // Moller-Trumbore, operation order of src/geometry/triangle.h:33-52, without

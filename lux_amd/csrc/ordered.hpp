@@ -139,6 +139,31 @@ __device__ __forceinline__ bool slab_near(const SlabRay& r, float t_bound, float
     return geo && !(tn > t_bound);
 }
 
+// slab_near in a wave-uniform slab mode (lux_common.hpp): kSlabParallel is
+// slab_near<true>; the other modes are slab_near<false> with the direction
+// signs read per lane (kSlabLane) or fixed by the wave's octant.
+template <int kMode>
+__device__ __forceinline__ bool slab_near(SlabMode<kMode>, const SlabRay& r, float t_bound, float lox, float loy,
+                                          float loz, float hix, float hiy, float hiz, float& t_near) {
+    if constexpr (kMode == kSlabParallel) {
+        return slab_near<true>(r, t_bound, lox, loy, loz, hix, hiy, hiz, t_near);
+    } else {
+        const bool nx = kMode < 0 ? r.neg_x : (kMode & 1) != 0;
+        const bool ny = kMode < 0 ? r.neg_y : (kMode & 2) != 0;
+        const bool nz = kMode < 0 ? r.neg_z : (kMode & 4) != 0;
+        const float ax = ((nx ? hix : lox) - r.o.x) * r.inv.x;
+        const float cx = ((nx ? lox : hix) - r.o.x) * r.inv.x;
+        const float ay = ((ny ? hiy : loy) - r.o.y) * r.inv.y;
+        const float cy = ((ny ? loy : hiy) - r.o.y) * r.inv.y;
+        const float az = ((nz ? hiz : loz) - r.o.z) * r.inv.z;
+        const float cz = ((nz ? loz : hiz) - r.o.z) * r.inv.z;
+        const float tn = fmaxf(fmaxf(fmaxf(r.t_min, ax), ay), az);
+        const float tf = fminf(fminf(cx, cy), cz);
+        t_near = tn;
+        return !(tn > tf) && !(tn > t_bound);
+    }
+}
+
 // Walks the pair array from the root, entering boxes up to best.t_cull.
 // leaf(slot, prim, tn) tests one leaf whose box passed (tn = its box's entry)
 // and may lower best.t_max (it holds the tie rule above); it returns true to
@@ -146,15 +171,15 @@ __device__ __forceinline__ bool slab_near(const SlabRay& r, float t_bound, float
 // words apart. Returns false when the stack overflowed (the walk stopped
 // early); *bad is set on a malformed array (child slot out of range, a pair
 // entered twice).
-template <bool kParallel, typename Leaf>
-__device__ __forceinline__ bool ordered_walk(const lux_bvh_node* __restrict__ nodes, uint32_t node_count,
-                                             const SlabRay& sr, OrderedBest& best, uint32_t* stack, int stride,
-                                             uint32_t& visits, bool& bad, Leaf&& leaf) {
+template <int kMode, typename Leaf>
+__device__ __forceinline__ bool ordered_walk(SlabMode<kMode> mode, const lux_bvh_node* __restrict__ nodes,
+                                             uint32_t node_count, const SlabRay& sr, OrderedBest& best,
+                                             uint32_t* stack, int stride, uint32_t& visits, bool& bad, Leaf&& leaf) {
     float4 a, b;
     load_node32(nodes, 0, a, b);
     ++visits;
     float tn;
-    if (!slab_near<kParallel>(sr, best.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)) return true;
+    if (!slab_near(mode, sr, best.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)) return true;
     const int root = __float_as_int(b.z);
     if (root >= 0) {
         (void)leaf(0u, root, tn);
@@ -173,8 +198,8 @@ __device__ __forceinline__ bool ordered_walk(const lux_bvh_node* __restrict__ no
         load_node32(nodes, c + 1, a1, b1);
         visits += 2;
         float tn0, tn1;
-        const bool h0 = slab_near<kParallel>(sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
-        const bool h1 = slab_near<kParallel>(sr, best.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
+        const bool h0 = slab_near(mode, sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
+        const bool h1 = slab_near(mode, sr, best.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
         const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
         if (h0 && p0 >= 0 && leaf(c, p0, tn0)) return true;
         if (h1 && p1 >= 0 && leaf(c + 1, p1, tn1)) return true;

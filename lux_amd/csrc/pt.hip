@@ -156,8 +156,8 @@ __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O,
         }
         return false;
     };
-    const bool done = ordered_walk<kParallel>(sv.nodes, uint32_t(sv.node_count), sr, ob, stack, stride, visits, bad,
-                                              leaf);
+    const bool done = ordered_walk(SlabMode<(kParallel ? kSlabParallel : kSlabLane)>{}, sv.nodes,
+                                   uint32_t(sv.node_count), sr, ob, stack, stride, visits, bad, leaf);
     if (bad) set_error(sv.info, kErrWalkOrder);
     if (!done || (best >= 0 && ob.ambiguous()))
         return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
