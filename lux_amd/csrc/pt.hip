@@ -355,11 +355,11 @@ __device__ __forceinline__ void shadow_ray(const BatchDesc* __restrict__ desc, c
 // the counters the following shade launch appends to (zero_a, zero_b; never
 // the ones read here) and adds the queue sizes to the ray totals.
 // One-level variants: at least 8 waves per SIMD (64 VGPRs, 65 uncapped). C3
-// measured 0.079 s at 8 waves against 0.096 s at 7. Two-level variants keep
-// their natural 81-100 VGPRs (4-5 waves): capped at 8 waves they spill 116-168
-// bytes per lane and the instanced render measured 94.8 ms against 92.1 ms.
+// measured 0.079 s at 8 waves against 0.096 s at 7. Two-level variants: 6
+// waves (80 VGPRs, 86 uncapped, 28-40 bytes per lane spilled); the instanced
+// field measured 91.6 ms at 6 waves, 96.7 ms at 7 and 113 ms at the natural 5.
 template <int B, bool kAnyShadow, bool kTwoLevel>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kTwoLevel ? 1 : 8))) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kTwoLevel ? 6 : 8))) void k_trace(const BatchDesc* __restrict__ desc, Paths ps, Queues qs, int ext_in,
                                                   int sh_in, int zero_a, int zero_b) {
     const uint32_t n_ext = ext_in >= 0 ? qs.counts[ext_in] : 0u;
     const uint32_t n_sh = sh_in >= 0 ? qs.counts[sh_in] : 0u;

@@ -196,20 +196,22 @@ constexpr uint32_t kInstanceItem = 0x80000000u;
 __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s, const f3& WO, const f3& WD,
                                                           float t_min, float t_max0, int ignore, uint32_t* stack,
                                                           int stride, TwoLevelHit& best) {
+    // Per-lane state is kept small (the path tracer's trace kernel holds this
+    // walk at 80 VGPRs): the world slab ray is recomputed on leaving an
+    // instance, a mesh's triangle and tie arrays are read at its leaf tests,
+    // and the winner's instance, local id, u and v are derived once at the end.
     best = TwoLevelHit{};
     OrderedBest ob(t_max0);
     float inst_tn = -INFINITY;  // entry of the current instance's TLAS leaf box
     const uint32_t tlas_nc = uint32_t(s.tlas_node_count);
     if (tlas_nc == 0u) return true;
-    const SlabRay wsr = make_slab_ray(WO, WD, t_min);
-    SlabRay sr = wsr;
+    SlabRay sr = make_slab_ray(WO, WD, t_min);
     f3 O = WO, D = WD;
     const lux_bvh_node* nodes = s.tlas_nodes;
     uint32_t nc = tlas_nc;
     bool in_blas = false;
-    const lux_triangle* tris = nullptr;
-    const uint32_t* blas_pre = nullptr;
-    int tcount = 0, first = -1, inst_cur = -1, ignore_local = -1;
+    int cur_mesh = -1, first = -1, ignore_local = -1;
+    int best_prim = -1;
     uint32_t inst_slot = 0, best_inst_slot = 0, best_slot = 0;
     int sp = 0, sp0 = 0;
     uint32_t steps = s.step_limit;
@@ -230,28 +232,27 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
     // leaf or that of the instance's TLAS leaf.
     auto leaf = [&](uint32_t slot, int prim, float tn) {
         if (prim == ignore_local) return;  // should_skip_primitive
-        if (prim >= tcount) {
+        const DevMesh& m = s.meshes[cur_mesh];
+        if (prim >= m.tri_count) {
             set_error(s.info, kErrPrimRange);
             return;
         }
-        const float* p = reinterpret_cast<const float*>(tris + prim);
+        const float* p = reinterpret_cast<const float*>(m.tris + prim);
         const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
         float t, u, v;
         if (!tri_uvt(O, D, v0, v1, v2, t, u, v) || !(t > t_min)) return;
         ob.note(t, fmaxf(tn, inst_tn));
-        const bool take =
-            t < ob.t_max ||
-            (best.prim >= 0 && t == ob.t_max &&
-             (inst_slot != best_inst_slot ? s.tlas_slot_pre[inst_slot] < s.tlas_slot_pre[best_inst_slot]
-                                          : blas_pre[slot] < blas_pre[best_slot]));
+        bool take = t < ob.t_max;
+        if (!take && best_prim >= 0 && t == ob.t_max) {  // exact tie: the smaller (TLAS, BLAS) preorder position
+            if (inst_slot != best_inst_slot) {
+                take = s.tlas_slot_pre[inst_slot] < s.tlas_slot_pre[best_inst_slot];
+            } else {
+                take = m.slot_pre[slot] < m.slot_pre[best_slot];
+            }
+        }
         if (!take) return;
         ob.take(t);
-        best.prim = first + prim;
-        best.instance = inst_cur;
-        best.local = prim;
-        best.t = t;
-        best.u = u;
-        best.v = v;
+        best_prim = first + prim;
         best_inst_slot = inst_slot;
         best_slot = slot;
     };
@@ -262,7 +263,8 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         }
         bool descend = false;
         if (item & kInstanceItem) {
-            // enter the instance of TLAS leaf slot `ls`, if it has a BLAS
+            // enter the instance of TLAS leaf slot `ls`, if it has a BLAS (an
+            // instance item is only taken in the TLAS, so sr is the world ray)
             const uint32_t ls = item & ~kInstanceItem;
             if (ls >= tlas_nc) {
                 set_error(s.info, kErrWalkOrder);
@@ -275,6 +277,13 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             const bool ok = in && in->mesh >= 0 && in->mesh < s.mesh_count;
             const DevMesh mesh = ok ? s.meshes[in->mesh] : DevMesh{};
             if (ok && mesh.node_count > 0 && in->first >= 0 && uint32_t(mesh.node_count) <= kOffset32Nodes) {
+                {
+                    // entry of this instance's TLAS leaf box (tested when its
+                    // item was taken or pushed)
+                    const bool wpar = wave_any_parallel(sr);
+                    (void)(wpar ? slab_near<true>(sr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn)
+                                : slab_near<false>(sr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn));
+                }
                 float inv[16];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -284,24 +293,14 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
                     inv[4 * i + 2] = row.z;
                     inv[4 * i + 3] = row.w;
                 }
-                {
-                    // entry of this instance's TLAS leaf box (tested when its
-                    // item was taken or pushed)
-                    const bool wpar = wave_any_parallel(wsr);
-                    (void)(wpar ? slab_near<true>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn)
-                                : slab_near<false>(wsr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn));
-                }
                 in_blas = true;
                 sp0 = sp;
                 nodes = mesh.nodes;
                 nc = uint32_t(mesh.node_count);
-                tris = mesh.tris;
-                blas_pre = mesh.slot_pre;
-                tcount = mesh.tri_count;
+                cur_mesh = in->mesh;
                 first = in->first;
-                inst_cur = prim;
                 inst_slot = ls;
-                ignore_local = (ignore >= first && ignore - first < tcount) ? ignore - first : -1;
+                ignore_local = (ignore >= first && ignore - first < mesh.tri_count) ? ignore - first : -1;
                 O = xform_point(inv, WO);
                 D = xform_vector(inv, WD);
                 sr = make_slab_ray(O, D, t_min);
@@ -368,14 +367,42 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             in_blas = false;
             nodes = s.tlas_nodes;
             nc = tlas_nc;
-            sr = wsr;
+            sr = make_slab_ray(WO, WD, t_min);
             O = WO;
             D = WD;
         }
-        if (sp == 0) return !(best.prim >= 0 && ob.ambiguous());
+        if (sp == 0) break;
         --sp;
         item = stack[sp * stride];
     }
+    if (best_prim < 0) return true;
+    if (ob.ambiguous()) return false;
+    // the winner's instance, local id, u and v: the same operations on the
+    // same operands as the leaf test that took it
+    float4 a, b;
+    load_node32(s.tlas_nodes, best_inst_slot, a, b);
+    const int inst = __float_as_int(b.z);
+    const DevInstance& in = s.instances[inst];
+    const DevMesh& m = s.meshes[in.mesh];
+    float inv[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float4 row = reinterpret_cast<const float4*>(in.inv)[i];
+        inv[4 * i] = row.x;
+        inv[4 * i + 1] = row.y;
+        inv[4 * i + 2] = row.z;
+        inv[4 * i + 3] = row.w;
+    }
+    const int local = best_prim - in.first;
+    const float* p = reinterpret_cast<const float*>(m.tris + local);
+    float t;
+    (void)tri_uvt(xform_point(inv, WO), xform_vector(inv, WD), make3(p[0], p[1], p[2]), make3(p[3], p[4], p[5]),
+                  make3(p[6], p[7], p[8]), t, best.u, best.v);
+    best.prim = best_prim;
+    best.instance = inst;
+    best.local = local;
+    best.t = ob.t_max;
+    return true;
 }
 
 
