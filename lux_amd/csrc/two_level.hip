@@ -146,7 +146,11 @@ void upload_instances(TwoLevelDev& s, const lux_instance* instances, int count, 
         const int64_t tri = s.mesh_tri_off[in.mesh_id + 1] - s.mesh_tri_off[in.mesh_id];
         if (cursor + tri > INT32_MAX) throw Error(LUX_ERR_INVALID_ARGUMENT, "more than 2^31 scene primitives");
         r.first = int32_t(cursor);
-        r.pad[0] = r.pad[1] = 0;
+        r.nodes = s.blas[in.mesh_id].pairs;
+        r.node_count = tri > 1 ? int32_t(2 * tri) : int32_t(tri);
+        r.tri_base = s.mesh_tri_off[in.mesh_id];
+        r.tri_count = int32_t(tri);
+        r.pad = 0;
         cursor += tri;
         if (box_is_empty(in.world_bounds)) continue;
         boxes.push_back(in.world_bounds);
@@ -311,6 +315,7 @@ TwoLevelView two_level_view(const TwoLevelDev& s) {
     v.tlas_slot_pre = s.tlas.slot_pre;
     v.instances = s.instances;
     v.meshes = s.meshes;
+    v.soup = s.soup;
     v.info = s.tlas.info;
     v.tlas_node_count = s.tlas.n > 1 ? 2 * s.tlas.n : s.tlas.n;
     v.instance_count = s.instance_count;

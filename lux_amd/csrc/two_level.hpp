@@ -21,15 +21,22 @@ struct DevMesh {
     int32_t tri_count;
 };
 
-// One instance: Transform::inv (row-major 4x4), its mesh and the scene ordinal
-// of its local triangle 0 (GpuSceneInstance::first_primitive, src/scene/gpu_scene.h:149-172).
+// One instance: Transform::inv (row-major 4x4), the scene ordinal of its local
+// triangle 0 (GpuSceneInstance::first_primitive, src/scene/gpu_scene.h:149-172)
+// and, copied from its mesh so that entering the instance is one dependent load,
+// the mesh's BLAS (sibling-pair slots; 0 for an empty or invalid mesh) and the
+// offset of its triangles in the scene's soup.
 struct DevInstance {
     float inv[16];
-    int32_t mesh;
+    const lux_bvh_node* nodes;
+    int32_t node_count;
+    int32_t tri_base;
+    int32_t tri_count;
     int32_t first;
-    int32_t pad[2];
+    int32_t mesh;
+    int32_t pad;
 };
-static_assert(sizeof(DevInstance) == 80, "instance record");
+static_assert(sizeof(DevInstance) == 96, "instance record");
 
 // What the walk kernels take by value.
 struct TwoLevelView {
@@ -37,6 +44,7 @@ struct TwoLevelView {
     const uint32_t* tlas_slot_pre;
     const DevInstance* instances;
     const DevMesh* meshes;
+    const lux_triangle* soup;  // every mesh's object-space triangles, mesh by mesh
     DevInfo* info;
     int32_t tlas_node_count, instance_count, mesh_count;
     uint32_t step_limit;  // TLAS slots + the BLAS slots of every instance: no walk takes more steps
@@ -125,9 +133,7 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
         if (!in_blas) {  // instance leaf: enter its BLAS if it has one
             if (prim >= s.instance_count) continue;
             const DevInstance& in = s.instances[prim];
-            if (in.mesh < 0 || in.mesh >= s.mesh_count) continue;
-            const DevMesh mesh = s.meshes[in.mesh];
-            if (mesh.node_count <= 0 || in.first < 0) continue;
+            if (in.node_count <= 0 || in.first < 0) continue;
             float inv[16];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -139,10 +145,10 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
             }
             resume = esc;
             in_blas = true;
-            nodes = mesh.nodes;
-            nc = uint32_t(mesh.node_count);
-            tris = mesh.tris;
-            tcount = mesh.tri_count;
+            nodes = in.nodes;
+            nc = uint32_t(in.node_count);
+            tris = s.soup + in.tri_base;
+            tcount = in.tri_count;
             first = in.first;
             inst_cur = prim;
             ignore_local = (ignore >= first && ignore - first < tcount) ? ignore - first : -1;
@@ -210,7 +216,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
     const lux_bvh_node* nodes = s.tlas_nodes;
     uint32_t nc = tlas_nc;
     bool in_blas = false;
-    int cur_mesh = -1, first = -1, ignore_local = -1;
+    int tri_base = 0, first = -1, ignore_local = -1;
     int best_prim = -1;
     uint32_t inst_slot = 0, best_inst_slot = 0, best_slot = 0;
     int sp = 0, sp0 = 0;
@@ -230,14 +236,14 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
     // Tests BLAS leaf `slot` (local triangle prim, box entry tn) of the current
     // instance. The largest entry on the triangle's path is that of its BLAS
     // leaf or that of the instance's TLAS leaf.
+    // A BLAS of nc slots has (nc + 1) / 2 leaves (nc = 1: a single leaf).
     auto leaf = [&](uint32_t slot, int prim, float tn) {
         if (prim == ignore_local) return;  // should_skip_primitive
-        const DevMesh& m = s.meshes[cur_mesh];
-        if (prim >= m.tri_count) {
+        if (uint32_t(prim) >= (nc + 1u) / 2u) {
             set_error(s.info, kErrPrimRange);
             return;
         }
-        const float* p = reinterpret_cast<const float*>(m.tris + prim);
+        const float* p = reinterpret_cast<const float*>(s.soup + tri_base + prim);
         const f3 v0 = make3(p[0], p[1], p[2]), v1 = make3(p[3], p[4], p[5]), v2 = make3(p[6], p[7], p[8]);
         float t, u, v;
         if (!tri_uvt(O, D, v0, v1, v2, t, u, v) || !(t > t_min)) return;
@@ -246,8 +252,11 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         if (!take && best_prim >= 0 && t == ob.t_max) {  // exact tie: the smaller (TLAS, BLAS) preorder position
             if (inst_slot != best_inst_slot) {
                 take = s.tlas_slot_pre[inst_slot] < s.tlas_slot_pre[best_inst_slot];
-            } else {
-                take = m.slot_pre[slot] < m.slot_pre[best_slot];
+            } else {  // same instance: its mesh's BLAS preorder positions
+                float4 ta, tb;
+                load_node32(s.tlas_nodes, inst_slot, ta, tb);
+                const uint32_t* pre = s.meshes[s.instances[__float_as_int(tb.z)].mesh].slot_pre;
+                take = pre[slot] < pre[best_slot];
             }
         }
         if (!take) return;
@@ -274,9 +283,8 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             load_node32(s.tlas_nodes, ls, a, b);
             const int prim = __float_as_int(b.z);
             const DevInstance* in = (prim >= 0 && prim < s.instance_count) ? &s.instances[prim] : nullptr;
-            const bool ok = in && in->mesh >= 0 && in->mesh < s.mesh_count;
-            const DevMesh mesh = ok ? s.meshes[in->mesh] : DevMesh{};
-            if (ok && mesh.node_count > 0 && in->first >= 0 && uint32_t(mesh.node_count) <= kOffset32Nodes) {
+            const bool ok = in && in->node_count > 0 && in->first >= 0;
+            if (ok && uint32_t(in->node_count) <= kOffset32Nodes) {
                 {
                     // entry of this instance's TLAS leaf box (tested when its
                     // item was taken or pushed)
@@ -295,12 +303,12 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
                 }
                 in_blas = true;
                 sp0 = sp;
-                nodes = mesh.nodes;
-                nc = uint32_t(mesh.node_count);
-                cur_mesh = in->mesh;
+                nodes = in->nodes;
+                nc = uint32_t(in->node_count);
+                tri_base = in->tri_base;
                 first = in->first;
                 inst_slot = ls;
-                ignore_local = (ignore >= first && ignore - first < mesh.tri_count) ? ignore - first : -1;
+                ignore_local = (ignore >= first && ignore - first < in->tri_count) ? ignore - first : -1;
                 O = xform_point(inv, WO);
                 D = xform_vector(inv, WD);
                 sr = make_slab_ray(O, D, t_min);
@@ -315,7 +323,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
                     item = ~uint32_t(root);
                     descend = true;
                 }
-            } else if (ok && mesh.node_count > 0 && in->first >= 0) {
+            } else if (ok) {
                 return false;  // a BLAS beyond 32-bit node offsets: the preorder walk takes the ray
             }
         } else {
@@ -383,7 +391,6 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
     load_node32(s.tlas_nodes, best_inst_slot, a, b);
     const int inst = __float_as_int(b.z);
     const DevInstance& in = s.instances[inst];
-    const DevMesh& m = s.meshes[in.mesh];
     float inv[16];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -394,7 +401,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         inv[4 * i + 3] = row.w;
     }
     const int local = best_prim - in.first;
-    const float* p = reinterpret_cast<const float*>(m.tris + local);
+    const float* p = reinterpret_cast<const float*>(s.soup + in.tri_base + local);
     float t;
     (void)tri_uvt(xform_point(inv, WO), xform_vector(inv, WD), make3(p[0], p[1], p[2]), make3(p[3], p[4], p[5]),
                   make3(p[6], p[7], p[8]), t, best.u, best.v);
