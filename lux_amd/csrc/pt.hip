@@ -235,7 +235,7 @@ template <bool kTwoLevel>
 __device__ __forceinline__ bool scene_occluded(const SceneView& sv, const vec& o, const vec& d, float t_min,
                                                float t_max, int ignore) {
     if constexpr (kTwoLevel)
-        return two_level_walk<true>(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore).prim >= 0;
+        return two_level_any(sv.tl, make3(o.x, o.y, o.z), make3(d.x, d.y, d.z), t_min, t_max, ignore);
     else
         return occluded(sv, o, d, t_min, t_max, ignore);
 }

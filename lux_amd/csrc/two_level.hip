@@ -61,10 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_two_level_occluded(TwoLevelView s, c
         return;
     }
     const lux_ray r = rays[i];
-    occluded[i] =
-        two_level_walk<true>(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ignore ? ignore[i] : -1).prim >= 0
-            ? 1
-            : 0;
+    occluded[i] = two_level_any(s, ld3(r.origin), ld3(r.direction), r.t_min, r.t_max, ignore ? ignore[i] : -1) ? 1 : 0;
 }
 
 template <typename T>

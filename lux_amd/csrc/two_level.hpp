@@ -180,6 +180,107 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
     return best;
 }
 
+// Occlusion of the same two-level scene: two_level_walk<true>'s per-lane
+// sequence (the same box tests, instance entries and triangle tests in the same
+// order; t_max never changes, and the walk stops at the first accepted
+// triangle), with the expensive steps gathered across the wave. A lane whose
+// walk reaches a passed leaf (an instance in the TLAS, a triangle in a BLAS)
+// holds it and stops; once a quarter of the wave's working lanes hold one (or
+// none can step on), the wave enters the held instances and tests the held
+// triangles together.
+__device__ __forceinline__ bool two_level_any(const TwoLevelView& s, const f3& WO, const f3& WD, float t_min,
+                                              float t_max, int ignore) {
+    const SlabRay wsr = make_slab_ray(WO, WD, t_min);
+    SlabRay sr = wsr;
+    f3 O = WO, D = WD;
+    const lux_bvh_node* nodes = s.tlas_nodes;
+    uint32_t nc = uint32_t(s.tlas_node_count);
+    int tri_base = 0, tcount = 0, ignore_local = -1;
+    bool in_blas = false, found = false;
+    uint32_t resume = 0, k = 0;
+    uint32_t steps = s.step_limit;
+    int held = -1;
+    for (;;) {
+        // one step of the walk, for a lane that holds nothing and is not done
+        if (held < 0 && !found) {
+            if (k >= nc && in_blas) {  // back to the TLAS at the instance leaf's escape
+                in_blas = false;
+                nodes = s.tlas_nodes;
+                nc = uint32_t(s.tlas_node_count);
+                k = resume;
+                sr = wsr;
+                O = WO;
+                D = WD;
+            }
+            if (k < nc) {
+                if (steps-- == 0u) {
+                    set_error(s.info, kErrWalkOrder);
+                    k = nc;
+                    in_blas = false;
+                } else {
+                    float4 a, b;
+                    load_node(nodes, int(k), a, b);
+                    const int prim = __float_as_int(b.z);
+                    const bool pass = wave_any_parallel(sr) ? slab_hit(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y)
+                                                            : slab_hit_np(sr, t_max, a.x, a.y, a.z, a.w, b.x, b.y);
+                    held = (pass && prim >= 0) ? prim : -1;
+                    k = (pass && prim < 0) ? uint32_t(~prim) : __float_as_uint(b.w);
+                }
+            }
+        }
+        const bool working = held < 0 && !found && (k < nc || in_blas);
+        const uint64_t want = __ballot(held >= 0);
+        const uint64_t more = __ballot(working);
+        if (!more) {
+            if (!want) break;
+        } else if (__popcll(want) * 4 < __popcll(want | more)) {
+            continue;
+        }
+        if (held < 0) continue;
+        if (!in_blas) {  // instance leaf: enter its BLAS if it has one (k holds the leaf's escape)
+            const int inst = held;
+            held = -1;
+            if (inst >= s.instance_count) continue;
+            const DevInstance& in = s.instances[inst];
+            if (in.node_count <= 0 || in.first < 0) continue;
+            float inv[16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float4 row = reinterpret_cast<const float4*>(in.inv)[i];
+                inv[4 * i] = row.x;
+                inv[4 * i + 1] = row.y;
+                inv[4 * i + 2] = row.z;
+                inv[4 * i + 3] = row.w;
+            }
+            resume = k;
+            in_blas = true;
+            nodes = in.nodes;
+            nc = uint32_t(in.node_count);
+            tri_base = in.tri_base;
+            tcount = in.tri_count;
+            ignore_local = (ignore >= in.first && ignore - in.first < tcount) ? ignore - in.first : -1;
+            O = xform_point(inv, WO);
+            D = xform_vector(inv, WD);
+            sr = make_slab_ray(O, D, t_min);
+            k = 0;
+            continue;
+        }
+        const int prim = held;
+        held = -1;
+        if (prim == ignore_local) continue;  // should_skip_primitive
+        if (prim >= tcount) {
+            set_error(s.info, kErrPrimRange);
+            continue;
+        }
+        const float* p = reinterpret_cast<const float*>(s.soup + tri_base + prim);
+        float t, u, v;
+        if (tri_hit(O, D, t_min, t_max, make3(p[0], p[1], p[2]), make3(p[3], p[4], p[5]), make3(p[6], p[7], p[8]), t,
+                    u, v))
+            found = true;
+    }
+    return found;
+}
+
 // Closest hit of the same two-level scene, near child first at both levels
 // (ordered.hpp). A TLAS step tests both children of a node; each child that
 // passes is an internal node (its pair slot) or an instance leaf (its slot
