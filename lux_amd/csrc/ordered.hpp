@@ -220,4 +220,92 @@ __device__ __forceinline__ bool ordered_walk(SlabMode<kMode> mode, const lux_bvh
     }
 }
 
+// ordered_walk for closest hit (leaf never ends the walk) with the triangle
+// tests gathered across the wave: a lane whose step passes one or both leaf
+// children holds them and does not step again until they are tested (its next
+// slab tests depend on the t_cull they may lower); the wave tests the held
+// leaves together once a quarter of its working lanes hold some, or none can
+// step on. Each lane's sequence of box tests, leaf tests and stack operations
+// is that of ordered_walk.
+template <int kMode, typename Leaf>
+__device__ __forceinline__ bool ordered_walk_gathered(SlabMode<kMode> mode, const lux_bvh_node* __restrict__ nodes,
+                                                      uint32_t node_count, const SlabRay& sr, OrderedBest& best,
+                                                      uint32_t* stack, int stride, uint32_t& visits, bool& bad,
+                                                      Leaf&& leaf) {
+    float4 a, b;
+    load_node32(nodes, 0, a, b);
+    ++visits;
+    float tn;
+    if (!slab_near(mode, sr, best.t_cull, a.x, a.y, a.z, a.w, b.x, b.y, tn)) return true;
+    const int root = __float_as_int(b.z);
+    if (root >= 0) {
+        (void)leaf(0u, root, tn);
+        return true;
+    }
+    uint32_t c = ~uint32_t(root);
+    int sp = 0;
+    int guard = int(node_count / 2) + 1;  // each internal node's pair is entered once
+    bool done = false, overflow = false;
+    uint32_t hc = 0;            // pair slot of the held leaves
+    int hp0 = -1, hp1 = -1;     // held leaf children (primitive ids), -1: none
+    float htn0 = 0, htn1 = 0;   // their box entries
+    for (;;) {
+        if (!done & (hp0 < 0) & (hp1 < 0)) {
+            if (c + 1 >= node_count || --guard < 0) {
+                bad = true;
+                done = true;
+            } else {
+                float4 a0, b0, a1, b1;
+                load_node32(nodes, c, a0, b0);
+                load_node32(nodes, c + 1, a1, b1);
+                visits += 2;
+                float tn0, tn1;
+                const bool h0 = slab_near(mode, sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
+                const bool h1 = slab_near(mode, sr, best.t_cull, a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, tn1);
+                const int p0 = __float_as_int(b0.z), p1 = __float_as_int(b1.z);
+                hc = c;
+                hp0 = (h0 & (p0 >= 0)) ? p0 : -1;
+                hp1 = (h1 & (p1 >= 0)) ? p1 : -1;
+                htn0 = tn0;
+                htn1 = tn1;
+                const bool i0 = h0 & (p0 < 0), i1 = h1 & (p1 < 0);
+                if (i0 & i1) {
+                    if (sp == kOrderedStack) {
+                        overflow = true;
+                        done = true;
+                    } else {
+                        const bool far0 = tn1 < tn0;
+                        stack[sp * stride] = far0 ? ~uint32_t(p0) : ~uint32_t(p1);
+                        ++sp;
+                        c = far0 ? ~uint32_t(p1) : ~uint32_t(p0);
+                    }
+                } else if (i0 | i1) {
+                    c = ~uint32_t(i0 ? p0 : p1);
+                } else if (sp == 0) {
+                    done = true;
+                } else {
+                    --sp;
+                    c = stack[sp * stride];
+                }
+            }
+        }
+        const bool holds = (hp0 >= 0) | (hp1 >= 0);
+        const uint64_t want = __ballot(holds);
+        const uint64_t more = __ballot(!done & !holds);
+        if (!more) {
+            if (!want) break;
+        } else if (__popcll(want) * 4 < __popcll(want | more)) {
+            continue;
+        }
+        // one copy of the leaf test, run for the first then the second child
+#pragma unroll 1
+        for (int j = 0; j < 2; ++j) {
+            const int pj = j ? hp1 : hp0;
+            if (pj >= 0) (void)leaf(hc + uint32_t(j), pj, j ? htn1 : htn0);
+        }
+        hp0 = hp1 = -1;
+    }
+    return !overflow;
+}
+
 }  // namespace lux

@@ -213,8 +213,8 @@ __device__ __forceinline__ int walk_ordered(const lux_bvh_node* __restrict__ nod
         }
         return false;
     };
-    const bool done =
-        ordered_walk(SlabMode<kMode>{}, nodes, uint32_t(node_count), sr, ob, stack, kBlock, visits, bad, leaf);
+    const bool done = ordered_walk_gathered(SlabMode<kMode>{}, nodes, uint32_t(node_count), sr, ob, stack, kBlock,
+                                            visits, bad, leaf);
     if (bad_prim) set_error(info, kErrPrimRange);
     if (bad) set_error(info, kErrWalkOrder);
     if (!done || (best >= 0 && ob.ambiguous()))
