@@ -127,6 +127,74 @@ void sync_and_check(const BvhDev& d, hipStream_t stream, const char* what) {
     if (info.error != 0) throw Error(LUX_ERR_DEVICE_CHECK, std::string("device check failed in ") + what);
 }
 
+// The culling copy of the TLAS (TwoLevelDev::tlas_cull): the same slots, each
+// leaf box the instance's world bounds intersected with the world box of its
+// mesh's bounding sphere, each internal box the union of its two children.
+// Every box still holds the geometry below it, so only the culling changes.
+void build_tlas_cull(TwoLevelDev& s, const lux_instance* instances, hipStream_t stream) {
+    const int n = s.tlas.n;
+    const int slots = n > 1 ? 2 * n : n;
+    if (s.tlas_cull) hip_check(hipFree(s.tlas_cull), "hipFree culling TLAS");
+    s.tlas_cull = nullptr;
+    if (slots <= 0) return;
+    std::vector<lux_bvh_node> nodes(static_cast<size_t>(slots));
+    hip_check(hipMemcpyAsync(nodes.data(), s.tlas.pairs, sizeof(lux_bvh_node) * slots, hipMemcpyDeviceToHost, stream),
+              "D2H TLAS");
+    hip_check(hipStreamSynchronize(stream), "synchronize TLAS download");
+    auto leaf_box = [&](int inst) {
+        const lux_instance& in = instances[inst];
+        lux_aabb b = in.world_bounds;
+        const float* sph = &s.mesh_sphere[4 * size_t(in.mesh_id)];
+        if (!(sph[3] >= 0)) return b;  // empty mesh
+        const float* m = in.object_to_world;
+        double c[3], h[3];
+        for (int r = 0; r < 3; ++r) {
+            c[r] = double(m[4 * r]) * sph[0] + double(m[4 * r + 1]) * sph[1] + double(m[4 * r + 2]) * sph[2] +
+                   double(m[4 * r + 3]);
+            h[r] = double(sph[3]) * std::sqrt(double(m[4 * r]) * m[4 * r] + double(m[4 * r + 1]) * m[4 * r + 1] +
+                                              double(m[4 * r + 2]) * m[4 * r + 2]);
+            h[r] = h[r] * (1.0 + 1e-5) + 1e-6 * std::fabs(c[r]) + 1e-30;  // outward margin for float rounding
+        }
+        const float lo[3] = {float(c[0] - h[0]), float(c[1] - h[1]), float(c[2] - h[2])};
+        const float hi[3] = {float(c[0] + h[0]), float(c[1] + h[1]), float(c[2] + h[2])};
+        b.p_min = lux_vec3{std::max(b.p_min.x, lo[0]), std::max(b.p_min.y, lo[1]), std::max(b.p_min.z, lo[2])};
+        b.p_max = lux_vec3{std::min(b.p_max.x, hi[0]), std::min(b.p_max.y, hi[1]), std::min(b.p_max.z, hi[2])};
+        return b;
+    };
+    // post-order over the pair slots from the root: a leaf takes its
+    // instance's box, an internal node the union of its children's
+    std::vector<int> todo{0};
+    std::vector<uint8_t> expanded(static_cast<size_t>(slots), 0);
+    while (!todo.empty()) {
+        const int k = todo.back();
+        lux_bvh_node& nd = nodes[size_t(k)];
+        if (nd.primitive_id >= 0) {
+            if (nd.primitive_id < s.instance_count) nd.bounds = leaf_box(nd.primitive_id);
+            todo.pop_back();
+            continue;
+        }
+        const int c = ~nd.primitive_id;
+        if (c < 0 || c + 1 >= slots) throw Error(LUX_ERR_DEVICE_CHECK, "malformed TLAS slot array");
+        if (!expanded[size_t(k)]) {
+            expanded[size_t(k)] = 1;
+            todo.push_back(c);
+            todo.push_back(c + 1);
+            continue;
+        }
+        const lux_aabb& l = nodes[size_t(c)].bounds;
+        const lux_aabb& r = nodes[size_t(c) + 1].bounds;
+        nd.bounds.p_min = lux_vec3{std::min(l.p_min.x, r.p_min.x), std::min(l.p_min.y, r.p_min.y),
+                                   std::min(l.p_min.z, r.p_min.z)};
+        nd.bounds.p_max = lux_vec3{std::max(l.p_max.x, r.p_max.x), std::max(l.p_max.y, r.p_max.y),
+                                   std::max(l.p_max.z, r.p_max.z)};
+        todo.pop_back();
+    }
+    s.tlas_cull = dalloc_n<lux_bvh_node>(size_t(slots));
+    hip_check(hipMemcpyAsync(s.tlas_cull, nodes.data(), sizeof(lux_bvh_node) * slots, hipMemcpyHostToDevice, stream),
+              "H2D culling TLAS");
+    hip_check(hipStreamSynchronize(stream), "synchronize culling TLAS upload");
+}
+
 // TLAS over the non-empty world bounds (ids = instance ids) and the instance
 // records; first ordinals follow the instances in order (each adds its mesh's
 // triangle count).
@@ -180,6 +248,7 @@ void upload_instances(TwoLevelDev& s, const lux_instance* instances, int count, 
         limit += tri > 1 ? 2u * uint64_t(tri) : uint64_t(tri);
     }
     s.step_limit = uint32_t(std::min<uint64_t>(limit + 1, 0xffffffffu));
+    build_tlas_cull(s, instances, stream);
 }
 
 size_t bvh_device_bytes(const BvhDev& d) {
@@ -249,7 +318,7 @@ lux_aabb transform_bounds(const float* m, const lux_aabb& b) {
 void two_level_release(TwoLevelDev& s) {
     for (BvhDev& b : s.blas) bvh_release(b);
     bvh_release(s.tlas);
-    void* ptrs[] = {s.soup, s.meshes, s.instances};
+    void* ptrs[] = {s.soup, s.meshes, s.instances, s.tlas_cull};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s = TwoLevelDev{};
@@ -288,6 +357,33 @@ void two_level_build(TwoLevelDev& s, const lux_triangle* tris, const int32_t* of
         meshes[m].tri_count = n;
         s.device_bytes += bvh_device_bytes(s.blas[m]);
     }
+    // each mesh's bounding sphere: the center of its box, the largest vertex
+    // distance from it (in double, padded), radius -1 for an empty mesh
+    s.mesh_sphere.assign(4 * size_t(mesh_count), 0.0f);
+    for (int m = 0; m < mesh_count; ++m) {
+        float* sph = &s.mesh_sphere[4 * size_t(m)];
+        if (offsets[m + 1] == offsets[m]) {
+            sph[3] = -1.0f;
+            continue;
+        }
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int t = offsets[m]; t < offsets[m + 1]; ++t)
+            for (const lux_vec3& v : {tris[t].v0, tris[t].v1, tris[t].v2}) {
+                const float q[3] = {v.x, v.y, v.z};
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], q[a]);
+                    hi[a] = std::max(hi[a], q[a]);
+                }
+            }
+        double r2 = 0;
+        for (int a = 0; a < 3; ++a) sph[a] = 0.5f * (lo[a] + hi[a]);
+        for (int t = offsets[m]; t < offsets[m + 1]; ++t)
+            for (const lux_vec3& v : {tris[t].v0, tris[t].v1, tris[t].v2}) {
+                const double dx = double(v.x) - sph[0], dy = double(v.y) - sph[1], dz = double(v.z) - sph[2];
+                r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+            }
+        sph[3] = float(std::sqrt(r2) * (1.0 + 1e-6)) + 1e-30f;
+    }
     s.meshes = dalloc_n<DevMesh>(meshes.size());
     if (!meshes.empty())
         hip_check(hipMemcpyAsync(s.meshes, meshes.data(), sizeof(DevMesh) * meshes.size(), hipMemcpyHostToDevice,
@@ -295,7 +391,8 @@ void two_level_build(TwoLevelDev& s, const lux_triangle* tris, const int32_t* of
                   "H2D meshes");
     upload_instances(s, instances, instance_count, stream);
     s.device_bytes += sizeof(lux_triangle) * size_t(total) + sizeof(DevMesh) * meshes.size() +
-                      sizeof(DevInstance) * size_t(instance_count) + bvh_device_bytes(s.tlas);
+                      sizeof(DevInstance) * size_t(instance_count) + bvh_device_bytes(s.tlas) +
+                      sizeof(lux_bvh_node) * 2 * size_t(std::max(instance_count, 1));
 }
 
 void two_level_set_instances(TwoLevelDev& s, const lux_instance* instances, int instance_count,
@@ -309,6 +406,7 @@ void two_level_set_instances(TwoLevelDev& s, const lux_instance* instances, int 
 TwoLevelView two_level_view(const TwoLevelDev& s) {
     TwoLevelView v;
     v.tlas_nodes = s.tlas.pairs;
+    v.tlas_cull = s.tlas_cull ? s.tlas_cull : s.tlas.pairs;
     v.tlas_slot_pre = s.tlas.slot_pre;
     v.instances = s.instances;
     v.meshes = s.meshes;

@@ -40,7 +40,8 @@ static_assert(sizeof(DevInstance) == 96, "instance record");
 
 // What the walk kernels take by value.
 struct TwoLevelView {
-    const lux_bvh_node* tlas_nodes;
+    const lux_bvh_node* tlas_nodes;  // the TLAS (sibling-pair copy) with the reference's boxes
+    const lux_bvh_node* tlas_cull;   // the same slots with tighter culling boxes (see TwoLevelDev)
     const uint32_t* tlas_slot_pre;
     const DevInstance* instances;
     const DevMesh* meshes;
@@ -314,7 +315,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
     if (tlas_nc == 0u) return true;
     SlabRay sr = make_slab_ray(WO, WD, t_min);
     f3 O = WO, D = WD;
-    const lux_bvh_node* nodes = s.tlas_nodes;
+    const lux_bvh_node* nodes = s.tlas_cull;  // TLAS steps cull with the tighter boxes
     uint32_t nc = tlas_nc;
     bool in_blas = false;
     int tri_base = 0, first = -1, ignore_local = -1;
@@ -387,8 +388,9 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
             const bool ok = in && in->node_count > 0 && in->first >= 0;
             if (ok && uint32_t(in->node_count) <= kOffset32Nodes) {
                 {
-                    // entry of this instance's TLAS leaf box (tested when its
-                    // item was taken or pushed)
+                    // entry of this instance's TLAS leaf box as the reference
+                    // tests it (its own box, not the culling box): the exactness
+                    // check weighs hits against the reference walk's entries
                     const bool wpar = wave_any_parallel(sr);
                     (void)(wpar ? slab_near<true>(sr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn)
                                 : slab_near<false>(sr, INFINITY, a.x, a.y, a.z, a.w, b.x, b.y, inst_tn));
@@ -474,7 +476,7 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
         // pop: leave the instance once its part of the stack is used up
         if (in_blas && sp == sp0) {
             in_blas = false;
-            nodes = s.tlas_nodes;
+            nodes = s.tlas_cull;
             nc = tlas_nc;
             sr = make_slab_ray(WO, WD, t_min);
             O = WO;
@@ -518,6 +520,13 @@ __device__ __forceinline__ bool two_level_closest_ordered(const TwoLevelView& s,
 struct TwoLevelDev {
     std::vector<BvhDev> blas;  // per mesh (n = 0 for an empty mesh)
     BvhDev tlas;               // over the non-empty instance world bounds, ids = instance ids
+    // The TLAS's slots again, each leaf box narrowed to its instance's world
+    // bounds intersected with the world box of the mesh's bounding sphere
+    // (center M c, half-extent r |row i of M| on axis i: the exact box of the
+    // transformed sphere), internal boxes the unions of their children. Only
+    // the near-child-first closest walk culls with them.
+    lux_bvh_node* tlas_cull = nullptr;
+    std::vector<float> mesh_sphere;  // per mesh: center xyz, radius (padded)
     lux_triangle* soup = nullptr;
     DevMesh* meshes = nullptr;
     DevInstance* instances = nullptr;
