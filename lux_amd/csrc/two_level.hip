@@ -129,8 +129,9 @@ void sync_and_check(const BvhDev& d, hipStream_t stream, const char* what) {
 
 // The culling copy of the TLAS (TwoLevelDev::tlas_cull): the same slots, each
 // leaf box the instance's world bounds intersected with the world box of its
-// mesh's bounding sphere, each internal box the union of its two children.
-// Every box still holds the geometry below it, so only the culling changes.
+// mesh's bounding sphere (both widened as below), each internal box the union
+// of its two children. Every box still holds every world-space ray point that
+// can hit the geometry below it, so only the culling changes.
 void build_tlas_cull(TwoLevelDev& s, const lux_instance* instances, hipStream_t stream) {
     const int n = s.tlas.n;
     const int slots = n > 1 ? 2 * n : n;
@@ -141,24 +142,55 @@ void build_tlas_cull(TwoLevelDev& s, const lux_instance* instances, hipStream_t 
     hip_check(hipMemcpyAsync(nodes.data(), s.tlas.pairs, sizeof(lux_bvh_node) * slots, hipMemcpyDeviceToHost, stream),
               "D2H TLAS");
     hip_check(hipStreamSynchronize(stream), "synchronize TLAS download");
+    // A leaf box must hold every world point w(t) of a ray whose object-space
+    // image p(t) = inv w(t) lies in the mesh: M p(t) = (M inv) w(t) lies in the
+    // transformed mesh, and M inv is the identity only to the scene
+    // validation's 1e-4 per entry, so w(t) may sit up to 3 e |w| + e_t off it
+    // (e, e_t: the largest deviation of M inv from the identity in the 3x3 and
+    // translation parts). Both boxes are widened by that, plus a margin for
+    // float rounding, before they are intersected.
     auto leaf_box = [&](int inst) {
         const lux_instance& in = instances[inst];
-        lux_aabb b = in.world_bounds;
-        const float* sph = &s.mesh_sphere[4 * size_t(in.mesh_id)];
-        if (!(sph[3] >= 0)) return b;  // empty mesh
         const float* m = in.object_to_world;
-        double c[3], h[3];
-        for (int r = 0; r < 3; ++r) {
-            c[r] = double(m[4 * r]) * sph[0] + double(m[4 * r + 1]) * sph[1] + double(m[4 * r + 2]) * sph[2] +
-                   double(m[4 * r + 3]);
-            h[r] = double(sph[3]) * std::sqrt(double(m[4 * r]) * m[4 * r] + double(m[4 * r + 1]) * m[4 * r + 1] +
-                                              double(m[4 * r + 2]) * m[4 * r + 2]);
-            h[r] = h[r] * (1.0 + 1e-5) + 1e-6 * std::fabs(c[r]) + 1e-30;  // outward margin for float rounding
+        const float* v = in.world_to_object;
+        double e3 = 0, et = 0;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) {
+                double x = c == 3 ? double(m[4 * r + 3]) : 0.0;
+                for (int k = 0; k < 3; ++k) x += double(m[4 * r + k]) * v[4 * k + c];
+                if (c == 3) {
+                    et = std::max(et, std::fabs(x));
+                } else {
+                    e3 = std::max(e3, std::fabs(x - (r == c ? 1.0 : 0.0)));
+                }
+            }
+        const lux_aabb& ref = in.world_bounds;
+        const double R = std::max({std::fabs(double(ref.p_min.x)), std::fabs(double(ref.p_min.y)),
+                                   std::fabs(double(ref.p_min.z)), std::fabs(double(ref.p_max.x)),
+                                   std::fabs(double(ref.p_max.y)), std::fabs(double(ref.p_max.z))});
+        const double pad = 3.0 * e3 * R + et + 1e-5 * (R + 1e-30);
+        double lo[3] = {ref.p_min.x - pad, ref.p_min.y - pad, ref.p_min.z - pad};
+        double hi[3] = {ref.p_max.x + pad, ref.p_max.y + pad, ref.p_max.z + pad};
+        const float* sph = &s.mesh_sphere[4 * size_t(in.mesh_id)];
+        if (sph[3] >= 0) {
+            for (int r = 0; r < 3; ++r) {
+                const double c = double(m[4 * r]) * sph[0] + double(m[4 * r + 1]) * sph[1] +
+                                 double(m[4 * r + 2]) * sph[2] + double(m[4 * r + 3]);
+                const double h = double(sph[3]) * std::sqrt(double(m[4 * r]) * m[4 * r] +
+                                                            double(m[4 * r + 1]) * m[4 * r + 1] +
+                                                            double(m[4 * r + 2]) * m[4 * r + 2]);
+                lo[r] = std::max(lo[r], c - h - pad);
+                hi[r] = std::min(hi[r], c + h + pad);
+            }
         }
-        const float lo[3] = {float(c[0] - h[0]), float(c[1] - h[1]), float(c[2] - h[2])};
-        const float hi[3] = {float(c[0] + h[0]), float(c[1] + h[1]), float(c[2] + h[2])};
-        b.p_min = lux_vec3{std::max(b.p_min.x, lo[0]), std::max(b.p_min.y, lo[1]), std::max(b.p_min.z, lo[2])};
-        b.p_max = lux_vec3{std::min(b.p_max.x, hi[0]), std::min(b.p_max.y, hi[1]), std::min(b.p_max.z, hi[2])};
+        lux_aabb b;
+        b.p_min = lux_vec3{float(lo[0]), float(lo[1]), float(lo[2])};
+        b.p_max = lux_vec3{float(hi[0]), float(hi[1]), float(hi[2])};
+        // rounding to float must not move a face inward
+        b.p_min = lux_vec3{std::nextafter(b.p_min.x, -INFINITY), std::nextafter(b.p_min.y, -INFINITY),
+                           std::nextafter(b.p_min.z, -INFINITY)};
+        b.p_max = lux_vec3{std::nextafter(b.p_max.x, INFINITY), std::nextafter(b.p_max.y, INFINITY),
+                           std::nextafter(b.p_max.z, INFINITY)};
         return b;
     };
     // post-order over the pair slots from the root: a leaf takes its

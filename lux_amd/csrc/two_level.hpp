@@ -181,10 +181,12 @@ __device__ __forceinline__ TwoLevelHit two_level_walk(const TwoLevelView& s, con
     return best;
 }
 
-// Occlusion of the same two-level scene: two_level_walk<true>'s per-lane
-// sequence (the same box tests, instance entries and triangle tests in the same
-// order; t_max never changes, and the walk stops at the first accepted
-// triangle), with the expensive steps gathered across the wave. A lane whose
+// Occlusion of the same two-level scene: two_level_walk<true>'s walk with the
+// TLAS steps culled by tlas_cull. t_max never changes and the answer is whether
+// any triangle is hit in (t_min, t_max): an instance whose culling box the
+// segment misses holds no such hit (the box holds every ray point that can
+// reach its mesh), and every other instance is walked as the reference walks
+// it. The expensive steps are gathered across the wave. A lane whose
 // walk reaches a passed leaf (an instance in the TLAS, a triangle in a BLAS)
 // holds it and stops; once a quarter of the wave's working lanes hold one (or
 // none can step on), the wave enters the held instances and tests the held
@@ -194,7 +196,7 @@ __device__ __forceinline__ bool two_level_any(const TwoLevelView& s, const f3& W
     const SlabRay wsr = make_slab_ray(WO, WD, t_min);
     SlabRay sr = wsr;
     f3 O = WO, D = WD;
-    const lux_bvh_node* nodes = s.tlas_nodes;
+    const lux_bvh_node* nodes = s.tlas_cull;
     uint32_t nc = uint32_t(s.tlas_node_count);
     int tri_base = 0, tcount = 0, ignore_local = -1;
     bool in_blas = false, found = false;
@@ -206,7 +208,7 @@ __device__ __forceinline__ bool two_level_any(const TwoLevelView& s, const f3& W
         if (held < 0 && !found) {
             if (k >= nc && in_blas) {  // back to the TLAS at the instance leaf's escape
                 in_blas = false;
-                nodes = s.tlas_nodes;
+                nodes = s.tlas_cull;
                 nc = uint32_t(s.tlas_node_count);
                 k = resume;
                 sr = wsr;
@@ -523,8 +525,10 @@ struct TwoLevelDev {
     // The TLAS's slots again, each leaf box narrowed to its instance's world
     // bounds intersected with the world box of the mesh's bounding sphere
     // (center M c, half-extent r |row i of M| on axis i: the exact box of the
-    // transformed sphere), internal boxes the unions of their children. Only
-    // the near-child-first closest walk culls with them.
+    // transformed sphere), both widened by what the instance's M inv may
+    // deviate from the identity; internal boxes the unions of their children.
+    // The near-child-first closest walk and the occlusion walk cull TLAS steps
+    // with them.
     lux_bvh_node* tlas_cull = nullptr;
     std::vector<float> mesh_sphere;  // per mesh: center xyz, radius (padded)
     lux_triangle* soup = nullptr;
