@@ -242,3 +242,59 @@ def test_two_level_rounding_case_matches_reference(gpu):
     assert np.array_equal(h["primitive_id"], g["hit_ids"])
     assert np.array_equal(bits(h["t"]), bits(g["hit_t"]))
     s.close()
+
+
+def test_culling_boxes_keep_hits_of_round_meshes_and_loose_inverses(gpu):
+    """Round meshes (geodesic spheres, whose culling boxes are well inside the
+    reference's transform_bounds boxes) under random rotations and scales, with
+    each instance inverse perturbed by up to 1e-5 per entry (still inside the
+    scene validation's 1e-4), and rays aimed at points just inside and just
+    outside the spheres' silhouettes: closest hits (ids and t) equal the oracle's
+    walk over the reference boxes with the same inverses, and occlusion equals
+    closest-found, through the ray-ordered path (2^17 rays)."""
+    from lux_amd import scenes
+
+    print_case("two-level culling boxes, round meshes")
+    rng = np.random.default_rng(11)
+    meshes = []
+    for freq in (12, 7):
+        v, i = scenes.geodesic_sphere(freq)
+        meshes.append((np.asarray(v, np.float32) * np.float32(0.9), np.asarray(i, np.uint32)))
+    tris = np.concatenate([v[i].reshape(-1, 9) for v, i in meshes]).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum([len(i) for _, i in meshes])]).astype(np.int32)
+    inst_mesh, inst_m = W.instances_random(48, len(meshes), seed=12, extent=2.5, scale=(0.3, 0.8))
+    inv = np.stack([TL.inverse_affine(m).reshape(4, 4) for m in inst_m]).astype(np.float32)
+    inv[:, :3, :] += rng.uniform(-1e-5, 1e-5, (len(inv), 3, 4)).astype(np.float32)
+    for m, iv in zip(inst_m, inv):
+        assert oracle.instance_transform_valid(m, iv)
+    rec = TL.make_instances(inst_mesh, inst_m, tris=tris, mesh_offsets=off, inverses=inv)
+    n = 1 << 17
+    # targets on the spheres' silhouettes as seen from each origin, pushed in or out by 1e-4 .. 1e-2
+    k = rng.integers(0, len(inst_m), n)
+    centers = np.stack([m.reshape(4, 4)[:3, 3] for m in inst_m])[k].astype(np.float64)
+    radius = 0.9 * np.stack([np.linalg.svd(m.reshape(4, 4)[:3, :3], compute_uv=False).max() for m in inst_m])[k]
+    o = rng.uniform(-4, 4, (n, 3))
+    to_c = centers - o
+    dist = np.linalg.norm(to_c, axis=1, keepdims=True)
+    side = rng.standard_normal((n, 3))
+    side -= (side * to_c).sum(1, keepdims=True) / dist ** 2 * to_c
+    side /= np.linalg.norm(side, axis=1, keepdims=True)
+    scale = 1 + rng.choice([-1, 1], n) * 10 ** rng.uniform(-4, -2, n)
+    target = centers + side * (radius * scale)[:, None]
+    d = target - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 9), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3:6] = d
+    rays[:, 7] = np.inf
+    s = TL.TwoLevelScene(tris, off, rec)
+    h = s.intersect(rays)
+    r = oracle.OracleTwoLevel(tris, off, inst_mesh, rec["world_to_object"], rec["world_bounds"]).trace(rays)
+    assert r["overflows"] == 0
+    assert np.array_equal(h["primitive_id"], r["prim"])
+    assert np.array_equal(bits(h["t"]), bits(r["t"]))
+    assert np.array_equal(h["instance_id"], r["instance"])
+    hit = h["primitive_id"] >= 0
+    assert 0.2 < hit.mean() < 0.95
+    assert np.array_equal(s.occluded(rays).astype(bool), hit)
+    s.close()
