@@ -127,7 +127,7 @@ __device__ __forceinline__ int closest_walk(const SceneView& sv, const f3& O, co
 // (entries `stride` words apart), exact-t ties kept by preorder position. A
 // ray whose stack overflows, or whose walk the check of ordered.hpp flags,
 // takes closest_walk.
-template <bool kParallel>
+template <int kMode>
 __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O, const f3& D, const SlabRay& sr,
                                                float t_min, float t_max0, int ignore, float& t_hit, float& u_hit,
                                                float& v_hit, uint32_t* stack, int stride) {
@@ -156,11 +156,11 @@ __device__ __forceinline__ int closest_ordered(const SceneView& sv, const f3& O,
         }
         return false;
     };
-    const bool done = ordered_walk(SlabMode<(kParallel ? kSlabParallel : kSlabLane)>{}, sv.nodes,
-                                   uint32_t(sv.node_count), sr, ob, stack, stride, visits, bad, leaf);
+    const bool done = ordered_walk(SlabMode<kMode>{}, sv.nodes, uint32_t(sv.node_count), sr, ob, stack, stride,
+                                   visits, bad, leaf);
     if (bad) set_error(sv.info, kErrWalkOrder);
     if (!done || (best >= 0 && ob.ambiguous()))
-        return closest_walk<kParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
+        return closest_walk<kMode == kSlabParallel, false>(sv, O, D, sr, t_min, t_max0, ignore, t_hit, u_hit, v_hit);
     t_hit = ob.t_max;
     if (best >= 0) {
         // u, v of the winner, recomputed once rather than carried through the
@@ -181,13 +181,12 @@ __device__ __forceinline__ int closest(const SceneView& sv, const vec& o, const 
     const SlabRay sr = make_slab_ray(O, D, t_min);
     const bool ordered =
         stack && !sv.preorder && sv.node_count > 1 && uint32_t(sv.node_count) <= kOffset32Nodes;
-    if (wave_any_parallel(sr)) {
-        if (ordered)
-            return closest_ordered<true>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
-        return closest_walk<true, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
-    }
     if (ordered)
-        return closest_ordered<false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit, stack, stride);
+        return with_slab_mode(sr, [&](auto mode) {
+            return closest_ordered<decltype(mode)::value>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit,
+                                                          stack, stride);
+        });
+    if (wave_any_parallel(sr)) return closest_walk<true, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
     return closest_walk<false, false>(sv, O, D, sr, t_min, t_max, ignore, t_hit, u_hit, v_hit);
 }
 
