@@ -16,8 +16,9 @@ step, with build, overlap and refit time included).
 
 Prints ONE JSON line. `roofline` is for the step's longest kernel (any hit,
 k_trace_any), timed with HIP events on its own stream inside the timed steps:
-its HBM bound (compulsory bytes per ray over the kernel time, and the bytes the
-counters see below L2), and beside it its VALU bound from the committed
+its HBM bound (compulsory bytes per ray over the kernel time, against the HBM
+peak and against the bandwidth the plain copy kernel attains on the same box,
+and the bytes the counters see below L2), and beside it its VALU bound from the committed
 counter summary (profiles/counters.json): how much of the kernel's time the
 vector ALUs issue, and how many lanes an issued instruction carries. The
 closest-hit kernel's figures follow under `closest_hit`. `cpu_baseline` times
@@ -718,6 +719,21 @@ def main():
         lbvh.synchronize()
     closest_ms = cprobe.total_ms / max(cprobe.launches, 1)
 
+    # attainable HBM bandwidth on this box: the plain copy kernel over 1 GiB
+    # (1 GiB read + 1 GiB written per launch), timed the same way
+    from lux_amd.abi import call as _call
+
+    copy_bytes = 1 << 30
+    c_src, c_dst = lbvh.DeviceArray(copy_bytes), lbvh.DeviceArray(copy_bytes)
+    _call("lux_hip_copy_device", c_dst.ptr, c_src.ptr, ctypes.c_size_t(copy_bytes), None)
+    with lbvh.Probe("copy16") as kprobe:
+        for _ in range(5):
+            _call("lux_hip_copy_device", c_dst.ptr, c_src.ptr, ctypes.c_size_t(copy_bytes), None)
+        lbvh.synchronize()
+    attainable = 2 * copy_bytes / (kprobe.total_ms / max(kprobe.launches, 1) * 1e-3) / 1e9
+    c_src.free()
+    c_dst.free()
+
     # per-stage wall time (same work, stage by stage, synchronised)
     stage_ms = {}
     for name, f in stages:
@@ -799,6 +815,9 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 6),
+            # the copy kernel's measured bandwidth, and achieved against it
+            "attainable_GBps": round(attainable, 1),
+            "frac_of_attainable": round(achieved / attainable, 6),
             "traffic": t_any["bytes_per_launch"] if t_any else None,
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": algo_bytes,
