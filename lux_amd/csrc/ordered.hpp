@@ -220,13 +220,12 @@ __device__ __forceinline__ bool ordered_walk(SlabMode<kMode> mode, const lux_bvh
     }
 }
 
-// ordered_walk for closest hit (leaf never ends the walk) with the triangle
-// tests gathered across the wave: a lane whose step passes one or both leaf
-// children holds them and does not step again until they are tested (its next
-// slab tests depend on the t_cull they may lower); the wave tests the held
-// leaves together once a quarter of its working lanes hold some, or none can
-// step on. Each lane's sequence of box tests, leaf tests and stack operations
-// is that of ordered_walk.
+// ordered_walk with the triangle tests gathered across the wave: a lane whose
+// step passes one or both leaf children holds them and does not step again
+// until they are tested (its next slab tests depend on the t_cull they may
+// lower); the wave tests the held leaves together once a quarter of its
+// working lanes hold some, or none can step on. Each lane's sequence of box
+// tests, leaf tests and stack operations is that of ordered_walk.
 template <int kMode, typename Leaf>
 __device__ __forceinline__ bool ordered_walk_gathered(SlabMode<kMode> mode, const lux_bvh_node* __restrict__ nodes,
                                                       uint32_t node_count, const SlabRay& sr, OrderedBest& best,
@@ -246,6 +245,7 @@ __device__ __forceinline__ bool ordered_walk_gathered(SlabMode<kMode> mode, cons
     int sp = 0;
     int guard = int(node_count / 2) + 1;  // each internal node's pair is entered once
     bool done = false, overflow = false;
+    bool ended = false;         // a leaf ended the walk (its later held leaves are not tested)
     uint32_t hc = 0;            // pair slot of the held leaves
     int hp0 = -1, hp1 = -1;     // held leaf children (primitive ids), -1: none
     float htn0 = 0, htn1 = 0;   // their box entries
@@ -297,11 +297,12 @@ __device__ __forceinline__ bool ordered_walk_gathered(SlabMode<kMode> mode, cons
         } else if (__popcll(want) * 4 < __popcll(want | more)) {
             continue;
         }
-        // one copy of the leaf test, run for the first then the second child
+        // one copy of the leaf test, run for the first then the second child; a
+        // leaf that returns true ends the walk (an any-hit query)
 #pragma unroll 1
         for (int j = 0; j < 2; ++j) {
             const int pj = j ? hp1 : hp0;
-            if (pj >= 0) (void)leaf(hc + uint32_t(j), pj, j ? htn1 : htn0);
+            if (pj >= 0 && !ended && leaf(hc + uint32_t(j), pj, j ? htn1 : htn0)) ended = done = true;
         }
         hp0 = hp1 = -1;
     }

@@ -1,7 +1,7 @@
 // query.hip -- closest-hit, any-hit and AABB-overlap queries over the
-// right-first preorder stackless LBVH. Closest hit takes the near-child-first
-// walk of ordered.hpp, which returns what the preorder walk below returns;
-// any hit, and rays whose ordered stack overflows, take the preorder walk.
+// right-first preorder stackless LBVH. Closest and any hit take the
+// near-child-first walk of ordered.hpp, which returns what the preorder walk
+// below returns; rays whose ordered stack overflows take the preorder walk.
 //
 // Walk rule (src/bvh/bvh_device.cuh:59-87): test node i; on a miss or after a
 // leaf jump to nodes[i].escape, on an internal hit step to i+1. With the nodes in
@@ -163,6 +163,46 @@ __device__ __forceinline__ int walk_any_deferred(const lux_bvh_node* __restrict_
     return found;
 }
 
+// Any hit by the near-child-first walk with gathered triangle tests. With
+// t_max fixed, a box passes or not whatever the visit order, so the walk meets
+// the same boxes and triangles as the preorder walk until one triangle is hit;
+// near child first meets one sooner. A lane whose stack overflows takes the
+// preorder walk.
+template <int kMode>
+__device__ __forceinline__ int walk_any_ordered(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                                const lux_triangle* __restrict__ tris, int tri_count,
+                                                const RayIn& ray, const SlabRay& sr, float& t_best, uint32_t* stack,
+                                                DevInfo* info, uint32_t& visits, uint32_t& tests) {
+    const float t_max = __builtin_canonicalizef(ray.t_max);
+    OrderedBest ob(t_max);  // t_cull stays t_max: no hit is taken before the walk ends
+    int found = -1;
+    float t_found = t_max;
+    bool bad = false, bad_prim = false;
+    auto leaf = [&](uint32_t, int prim, float) {
+        if (prim >= tri_count) {
+            bad_prim = true;
+            return false;
+        }
+        f3 v0, v1, v2;
+        load_tri(tris, prim, v0, v1, v2);
+        ++tests;
+        float t, u, v;
+        if (tri_hit(ray.o, ray.d, ray.t_min, t_max, v0, v1, v2, t, u, v)) {
+            found = prim;
+            t_found = t;
+            return true;
+        }
+        return false;
+    };
+    const bool done = ordered_walk_gathered(SlabMode<kMode>{}, nodes, uint32_t(node_count), sr, ob, stack, kBlock,
+                                            visits, bad, leaf);
+    if (bad_prim) set_error(info, kErrPrimRange);
+    if (bad) set_error(info, kErrWalkOrder);
+    if (!done) return walk_any_deferred<kMode>(nodes, node_count, tris, tri_count, ray, sr, t_best, info, visits, tests);
+    t_best = t_found;
+    return found;
+}
+
 template <bool kAny>
 __device__ __forceinline__ int walk(const lux_bvh_node* __restrict__ nodes, int node_count,
                                     const lux_triangle* __restrict__ tris, int tri_count, const RayIn& ray,
@@ -242,6 +282,22 @@ __device__ __forceinline__ int trace_one(const lux_bvh_node* __restrict__ nodes,
     return walk<kAny>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
 }
 
+// One ray's any hit: the near-child-first walk on this lane's LDS stack when
+// the array allows it, otherwise the preorder walk.
+__device__ __forceinline__ int trace_any_one(const lux_bvh_node* __restrict__ nodes, int node_count,
+                                             const lux_triangle* __restrict__ tris, int tri_count, const RayIn& r,
+                                             float& t, DevInfo* info, uint32_t& visits, uint32_t& tests,
+                                             uint32_t* stack) {
+    if (stack && uint32_t(node_count) <= kOffset32Nodes && node_count > 1) {
+        const SlabRay sr = make_slab_ray(r.o, r.d, r.t_min);
+        return with_slab_mode(sr, [&](auto mode) {
+            return walk_any_ordered<decltype(mode)::value>(nodes, node_count, tris, tri_count, r, sr, t, stack, info,
+                                                           visits, tests);
+        });
+    }
+    return trace_one<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
+}
+
 // Closest hit, one ray per lane in `order` (a permutation of [0, n), or input
 // order); the hit lands at the ray's own index.
 template <bool kOrdered>
@@ -275,6 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __rest
                                                       const lux_ray* __restrict__ rays, int n,
                                                       const uint32_t* __restrict__ order,
                                                       uint8_t* __restrict__ occluded, DevInfo* info) {
+    __shared__ uint32_t stack[kOrderedStack * kBlock];
     const int i = int(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
     const int ray = order ? int(order[i]) : i;  // order is a permutation of [0, n)
@@ -285,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(const lux_bvh_node* __rest
     RayIn r = load_ray(rays, ray);
     float t;
     uint32_t visits = 0, tests = 0;
-    int id = trace_one<true>(nodes, node_count, tris, tri_count, r, t, info, visits, tests);
+    const int id = trace_any_one(nodes, node_count, tris, tri_count, r, t, info, visits, tests, stack + threadIdx.x);
     occluded[ray] = id >= 0 ? 1 : 0;
 }
 
@@ -308,7 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_stats(const lux_bvh_node* __re
         float t;
         trace_one<false>(nodes, node_count, tris, tri_count, r, t, info, v0, t0, slot_pre,
                          kOrdered ? stack + threadIdx.x : nullptr);
-        trace_one<true>(nodes, node_count, tris, tri_count, r, t, info, v1, t1);
+        trace_any_one(nodes, node_count, tris, tri_count, r, t, info, v1, t1,
+                      kOrdered ? stack + threadIdx.x : nullptr);
     }
     unsigned long long c[6] = {v0, 0, t0, v1, 0, t1};
 #pragma unroll
@@ -572,9 +630,10 @@ void trace_any(const BvhDev& d, RaySortDev* order, const lux_triangle* tris, int
     if (n <= 0) return;
     int node_count = d.n > 1 ? 2 * d.n : (d.n > 0 ? 1 : 0);  // sibling-pair slots
     const uint32_t* ord = maybe_order(d, order, rays, n, s);
-    // Any hit stays on the preorder walk: the ordered walk measured 5.84 ms
-    // against 5.40 ms here (it stops at the first hit, so near-first order
-    // saves little and its pair steps cost more).
+    // Any hit walks near child first with its triangle tests gathered across
+    // the wave: 4.52 ms against 4.66 ms for the gathered preorder walk on 2^22
+    // random rays over 2^20 triangles (ungathered, near child first had lost:
+    // 5.84 against 5.40 ms).
     LUX_LAUNCH("trace_any", k_trace_any, div_up(n, kBlock), kBlock, 0, s, (const lux_bvh_node*)d.pairs, node_count,
                tris, tri_count, rays, n, ord, occluded, d.info);
 }
