@@ -58,8 +58,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
-    ap.add_argument("--render", default="c3,instanced,bunny,c4,c5",
-                    help="render workloads to add to the line: any of c3,instanced,bunny,c4,c5 (comma-separated) "
+    ap.add_argument("--render", default="c1,c3,instanced,bunny,c4,c5",
+                    help="render workloads to add to the line: any of c1,c3,instanced,bunny,c4,c5 (comma-separated) "
                          "or none")
     ap.add_argument("--two-level", default="on", choices=["on", "off"],
                     help="add the instanced (two-level TLAS/BLAS) workload to the line")
@@ -128,6 +128,47 @@ def _render_rate(ext, sh, pixels, seconds):
     return {"Mrays_per_s": round((ext + sh) / seconds / 1e6, 2), "extension_rays": int(ext), "shadow_rays": int(sh),
             "algorithmic_GB": round(algo / 1e9, 3), "achieved_GBps": round(gbps, 2),
             "hbm_frac": round(gbps / HBM_PEAK_GBPS, 5)}
+
+
+def bench_c1(args):
+    """The CLI's Cornell render (`-scene cornell -size 256 256 -spp 64 -depth 4
+    -light-sampler power`, seed 0): GPU wavefront and plain renders (films
+    bit-identical), and the CPU restatement of the reference's trace_path on a
+    band of 16 rows, one thread and all host threads, with the whole image's
+    CPU time projected from the one-thread rate."""
+    from lux_amd import lbvh, render, scenes
+
+    W = H = 256
+    spp, depth = 64, 4
+    d = scenes.cornell(W, H)
+    sc = render.Scene(d)
+    film = lbvh.DeviceArray(W * H * 12)
+    out = {"workload": f"Cornell box ({sc.triangle_count} triangles), {W}x{H}, {spp} spp, depth {depth}"}
+    films = {}
+    for name, mode in (("wavefront", render.WAVEFRONT), ("plain", render.PLAIN)):
+        sc.render_device(film, W, H, 1, depth, mode=mode)
+        lbvh.synchronize()
+        best = None
+        for _ in range(3):
+            s0 = time.perf_counter()
+            sc.render_device(film, W, H, spp, depth, mode=mode)
+            lbvh.synchronize()
+            sec = time.perf_counter() - s0
+            best = sec if best is None else min(best, sec)
+        ext, sh = sc.rays()
+        films[name] = film.to_numpy(np.float32, (H, W, 3))
+        out[name] = {"render_s": round(best, 5), "Mpaths_per_s": round(W * H * spp / best / 1e6, 2),
+                     **_render_rate(ext, sh, W * H, best)}
+    out["speedup_vs_plain"] = round(out["plain"]["render_s"] / out["wavefront"]["render_s"], 3)
+    out["films_bit_identical"] = bool(np.array_equal(films["wavefront"].view(np.uint32),
+                                                     films["plain"].view(np.uint32)))
+    _log(f"c1: {out['wavefront']['render_s'] * 1e3:.2f} ms wavefront, {out['plain']['render_s'] * 1e3:.2f} ms plain")
+    if args.cpu_render_rows > 0:
+        out["cpu_baseline"] = cpu_render_baseline(d, W, H, spp, depth, films["wavefront"], 16, 120, sc=sc)
+        out["cpu_baseline"]["projected_image_cpu_s"] = round(W * H * spp / (out["cpu_baseline"]["value"] * 1e6), 2)
+    sc.close()
+    film.free()
+    return out
 
 
 def bench_c3(args):
@@ -454,6 +495,33 @@ def bench_c4(args, prepared=None):
                                                              j["depth"], (0, j["height"]), n=1024, seed=0x4C56 + i)))
         out["sample_divergence"] = div
         _log(f"c4 sample divergence: {[d['diverged'] for d in div]} of {[d['samples'] for d in div]}")
+        # CPU baseline: the restatement on a band of each sampled scene (about
+        # 2^16 paths per scene, one thread), and the whole mix's CPU time
+        # projected from that rate
+        cpu_paths, cpu_s, bands = 0, 0.0, []
+        for i in (0, 21, 42, 63):
+            if i >= len(scs):
+                continue
+            j = jobs[i]
+            W_, H_, spp = j["width"], j["height"], j["spp"]
+            nrows = max(1, min(H_, (1 << 16) // (W_ * spp)))
+            r0 = H_ // 2 - nrows // 2
+            pt = oracle.OraclePathTracer(dicts[i])
+            s0 = time.perf_counter()
+            band = pt.render(W_, H_, spp, j["depth"], rows=(r0, r0 + nrows), threads=1)[r0:r0 + nrows]
+            dt = time.perf_counter() - s0
+            cpu_paths += W_ * nrows * spp
+            cpu_s += dt
+            gpu = d_films[i].to_numpy(np.float32, (H_, W_, 3))[r0:r0 + nrows]
+            bands.append({"scene": i, "rows": f"{r0}..{r0 + nrows - 1}", "paths": W_ * nrows * spp,
+                          "gpu_vs_cpu_band_relrmse": round(render.rel_rmse(gpu / spp, band / spp), 8)})
+        rate = cpu_paths / cpu_s
+        out["cpu_baseline"] = {
+            "value": round(rate / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "sample": f"a band of rows of scenes 0, 21, 42, 63 at their own spp ({cpu_paths} paths)",
+            "host_cpu": _cpu_name(), "projected_mix_cpu_s": round(paths / rate, 1),
+            "gpu_speedup_over_projected_cpu": round(paths / rate / sec, 1), "bands": bands}
+        _log(f"c4 cpu baseline: {rate / 1e6:.3f} Mpaths/s, mix projected {paths / rate:.0f} s")
     for f in d_films:
         f.free()
     return out, scs, jobs, dicts
@@ -875,6 +943,8 @@ def main():
         render.set_tuning(use_graph=0)
     if wanted:
         out["render"] = {}
+    if "c1" in wanted:
+        out["render"]["c1_cornell"] = bench_c1(args)
     if "c3" in wanted:
         out["render"]["c3_single_scene"] = bench_c3(args)
     if "instanced" in wanted:
