@@ -19,10 +19,16 @@
 namespace lux {
 namespace {
 
-// 128 threads (2 waves) per block: the closest-hit kernel measured 4.95 ms
-// against 5.11 ms at 256 and 5.82 ms at 512 on 2^22 random rays (its LDS stack
-// is 8 KiB per block at 128); any hit is unchanged.
-constexpr int kBlock = 128;
+// 64 threads (one wave) per block for the ray queries: with the gathered
+// near-child-first walks the microbench step measured 822 Mrays/s at 64
+// against 802 at 128 and 791 at 256 (a workgroup's resources are freed only
+// when its last wave ends, so a finished wave's slot waits for its siblings;
+// the LDS stack is 4 KiB per block at 64).
+constexpr int kBlock = 64;
+// The plain copy: 128-thread blocks, at most 4096 of them (4.8 TB/s on 1 GiB;
+// other shapes from 2048 to 16384 blocks of 256 or 512 threads measured
+// 4.3-5.0 TB/s).
+constexpr int kCopyBlock = 128;
 
 struct RayIn {
     f3 o, d;
@@ -383,9 +389,9 @@ __global__ void k_zero_u64(unsigned long long* p, int count) {
 
 __global__ void k_zero_counter(int32_t* counter) { *counter = 0; }
 
-__global__ __launch_bounds__(kBlock) void k_copy16(const int4* __restrict__ src, int4* __restrict__ dst,
-                                                   size_t count) {
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < count; i += size_t(gridDim.x) * kBlock)
+__global__ __launch_bounds__(kCopyBlock) void k_copy16(const int4* __restrict__ src, int4* __restrict__ dst,
+                                                       size_t count) {
+    for (size_t i = size_t(blockIdx.x) * kCopyBlock + threadIdx.x; i < count; i += size_t(gridDim.x) * kCopyBlock)
         dst[i] = src[i];
 }
 
@@ -649,8 +655,9 @@ void trace_stats(const BvhDev& d, const lux_triangle* tris, int tri_count, const
 void copy_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
     size_t count = bytes / 16;
     if (count == 0) return;
-    int grid = int(std::min<size_t>((count + kBlock - 1) / kBlock, 4096));
-    LUX_LAUNCH("copy16", k_copy16, grid, kBlock, 0, s, static_cast<const int4*>(src), static_cast<int4*>(dst), count);
+    int grid = int(std::min<size_t>((count + kCopyBlock - 1) / kCopyBlock, 4096));
+    LUX_LAUNCH("copy16", k_copy16, grid, kCopyBlock, 0, s, static_cast<const int4*>(src), static_cast<int4*>(dst),
+               count);
 }
 
 void overlap_query(const BvhDev& tree, const BvhDev& queries, bool self_mode, int max_pairs,
