@@ -408,10 +408,15 @@ __global__ __launch_bounds__(kCopyBlock) void k_copy16(const int4* __restrict__ 
 // (an ancestor's box contains its descendants'), so the pair set is that of
 // the per-query walk of src/bvh/bvh_overlap.cu:25-251, and the counter keeps
 // counting past max_pairs so the caller gets the exact total (:209-214).
+// LDS per workgroup: the two item lists (kOvCap entries of 5 bytes each) and
+// the pair buffer (8 bytes an entry) set how many workgroups share a CU. On the
+// 1M-triangle self query: lists 2048 / buffer 2048 (40 KiB) 334 us, 2048 / 512
+// 290 us, 1024 / 1024 236 us, 1024 / 512 (17 KiB) 222 us, and smaller sizes
+// the same 221-222 us.
 constexpr int kOvThreads = 256;
 constexpr int kOvQueries = 128;
-constexpr int kOvCap = 2048;
-constexpr int kOvPairBuf = 2048;
+constexpr int kOvCap = 1024;
+constexpr int kOvPairBuf = 512;
 
 struct OvShared {
     lux_aabb qb[kOvQueries];
