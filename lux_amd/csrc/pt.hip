@@ -1210,38 +1210,80 @@ void Renderer::ray_counts(hipStream_t stream, uint64_t* extension, uint64_t* sha
 }
 
 namespace {
+// Grid of the wavefront kernels: the tuned grid, and for k_shade at most the
+// blocks the device holds at once. k_shade walks its queue in rounds of
+// gridDim.x * B paths with block barriers, so blocks beyond that number run as
+// a second, partial round behind the first (at block 512 it holds 128 VGPRs,
+// 2 blocks per CU: 512 blocks on 256 CUs against the tuned 768 took C3 from
+// 0.0730 to 0.0713 s and the 64-scene mix from 1.243 to 1.215 s, same films).
+struct LaunchGrid {
+    int grid;
+    int shade;
+};
+
+template <int B, int kLight>
+int shade_blocks_per_cu() {
+    int per = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_shade<B, kLight>), B,
+                                                           0),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    return per;
+}
+
+template <int kLight>
+int shade_blocks_per_cu_b(int block) {
+    switch (block) {
+        case 128: return shade_blocks_per_cu<128, kLight>();
+        case 512: return shade_blocks_per_cu<512, kLight>();
+        default: return shade_blocks_per_cu<256, kLight>();
+    }
+}
+
+int shade_resident_blocks(int block, int light) {
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    int per = 0;
+    switch (light) {
+        case kLightUniform: per = shade_blocks_per_cu_b<kLightUniform>(block); break;
+        case kLightBvh: per = shade_blocks_per_cu_b<kLightBvh>(block); break;
+        default: per = shade_blocks_per_cu_b<kLightPower>(block); break;
+    }
+    return std::max(1, per * cus);
+}
+
 // The wavefront launch sequence of one batch (captured or launched directly).
 // Raw launches (no probe events) when captured.
 template <int B, bool kProbe, bool kOpaque, bool kTwoLevel, int kLight>
-void wavefront_sequence(const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+void wavefront_sequence(const PathBuf& buf, int depth, LaunchGrid grid, hipStream_t stream) {
     const Paths ps = paths_of(buf);
     const Queues qs = queues_of(buf);
     const BatchDesc* d = buf.desc;
-#define PT_LAUNCH(NAME, K, ...)                                                      \
+#define PT_LAUNCH(NAME, G, K, ...)                                                   \
     do {                                                                             \
         if (kProbe) {                                                                \
-            LUX_LAUNCH(NAME, K, grid, B, 0, stream, __VA_ARGS__);                    \
+            LUX_LAUNCH(NAME, K, G, B, 0, stream, __VA_ARGS__);                       \
         } else {                                                                     \
-            hipLaunchKernelGGL(K, dim3(grid), dim3(B), 0, stream, __VA_ARGS__);      \
+            hipLaunchKernelGGL(K, dim3(G), dim3(B), 0, stream, __VA_ARGS__);         \
             hip_check(hipGetLastError(), NAME);                                      \
         }                                                                            \
     } while (0)
-    PT_LAUNCH("pt_generate", k_generate<B>, d, ps, qs);
+    PT_LAUNCH("pt_generate", grid.grid, k_generate<B>, d, ps, qs);
     for (int b = 0; b <= depth; ++b) {
         const int in = b & 1;
-        PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, in, b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1,
-                  2 + (b & 1));
-        PT_LAUNCH("pt_shade", (k_shade<B, kLight>), d, ps, qs, in, b, -1);
+        PT_LAUNCH("pt_trace", grid.grid, (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, in,
+                  b > 0 ? 2 + ((b - 1) & 1) : -1, in ^ 1, 2 + (b & 1));
+        PT_LAUNCH("pt_shade", grid.shade, (k_shade<B, kLight>), d, ps, qs, in, b, -1);
     }
-    PT_LAUNCH("pt_trace", (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
-    PT_LAUNCH("pt_flush", k_flush<B>, d, ps);
+    PT_LAUNCH("pt_trace", grid.grid, (k_trace<B, kOpaque, kTwoLevel>), d, ps, qs, -1, 2 + (depth & 1), -1, -1);
+    PT_LAUNCH("pt_flush", grid.grid, k_flush<B>, d, ps);
 #undef PT_LAUNCH
 }
 }  // namespace
 
 namespace {
 template <bool kProbe, bool kOpaque, bool kTwoLevel, int kLight>
-void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+void wavefront_sequence_b(int block, const PathBuf& buf, int depth, LaunchGrid grid, hipStream_t stream) {
     switch (block) {
         case 128: wavefront_sequence<128, kProbe, kOpaque, kTwoLevel, kLight>(buf, depth, grid, stream); break;
         case 512: wavefront_sequence<512, kProbe, kOpaque, kTwoLevel, kLight>(buf, depth, grid, stream); break;
@@ -1250,7 +1292,7 @@ void wavefront_sequence_b(int block, const PathBuf& buf, int depth, int grid, hi
 }
 
 template <bool kProbe, bool kOpaque, bool kTwoLevel>
-void wavefront_sequence_l(int light, int block, const PathBuf& buf, int depth, int grid, hipStream_t stream) {
+void wavefront_sequence_l(int light, int block, const PathBuf& buf, int depth, LaunchGrid grid, hipStream_t stream) {
     switch (light) {
         case kLightUniform:
             wavefront_sequence_b<kProbe, kOpaque, kTwoLevel, kLightUniform>(block, buf, depth, grid, stream);
@@ -1269,7 +1311,7 @@ void wavefront_sequence_l(int light, int block, const PathBuf& buf, int depth, i
 // selection are template choices made here, per batch.
 template <bool kProbe>
 void wavefront_sequence_o(bool opaque, bool two_level, int light, int block, const PathBuf& buf, int depth,
-                          int grid, hipStream_t stream) {
+                          LaunchGrid grid, hipStream_t stream) {
     if (two_level) {
         if (opaque)
             wavefront_sequence_l<kProbe, true, true>(light, block, buf, depth, grid, stream);
@@ -1286,12 +1328,18 @@ void wavefront_sequence_o(bool opaque, bool two_level, int light, int block, con
 
 void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, int light, hipStream_t stream) {
     const int block = tune_.block;
-    const int grid = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
+    const int tuned = std::max(1, std::min(tune_.grid, div_up(buf_.capacity, block)));
+    if (shade_resident_block_ != block || shade_resident_light_ != light) {  // outside any capture
+        shade_resident_ = shade_resident_blocks(block, light);
+        shade_resident_block_ = block;
+        shade_resident_light_ = light;
+    }
+    const LaunchGrid grid{tuned, std::min(tuned, shade_resident_)};
     if (!tune_.use_graph) {
         wavefront_sequence_o<true>(opaque, two_level, light, block, buf_, depth, grid, stream);
         return;
     }
-    if (!graph_ || graph_depth_ != depth || graph_grid_ != grid || graph_capacity_ != buf_.capacity ||
+    if (!graph_ || graph_depth_ != depth || graph_grid_ != tuned || graph_capacity_ != buf_.capacity ||
         graph_block_ != block || graph_opaque_ != int(opaque) || graph_two_level_ != int(two_level) ||
         graph_light_ != light) {
         release_graph();
@@ -1309,7 +1357,7 @@ void Renderer::launch_wavefront(int depth, bool opaque, bool two_level, int ligh
         (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
         graph_depth_ = depth;
-        graph_grid_ = grid;
+        graph_grid_ = tuned;
         graph_capacity_ = buf_.capacity;
         graph_block_ = block;
         graph_opaque_ = int(opaque);

@@ -255,6 +255,8 @@ private:
     hipGraphExec_t graph_ = nullptr;
     int graph_depth_ = -1, graph_grid_ = -1, graph_capacity_ = -1, graph_block_ = -1, graph_opaque_ = -1,
         graph_two_level_ = -1, graph_light_ = -1;
+    // k_shade blocks resident on the device for (block, light), queried outside any capture
+    int shade_resident_ = 1, shade_resident_block_ = -1, shade_resident_light_ = -1;
     int64_t batch_counter_ = 0;
 };
 
