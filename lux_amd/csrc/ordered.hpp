@@ -193,14 +193,16 @@ __device__ __forceinline__ bool ordered_walk(SlabMode<kMode> mode, const lux_bvh
     uint32_t c = ~uint32_t(root);
     int sp = 0;
     int guard = int(node_count / 2) + 1;  // each internal node's pair is entered once
+    const uint32_t last_pair = node_count - 1;  // a pair slot c needs c + 1 < node_count
+    const lux_bvh_node* const nodes1 = nodes + 1;  // both nodes of a pair load from one offset
     for (;;) {
-        if (c + 1 >= node_count || --guard < 0) {
+        if (c >= last_pair || --guard < 0) {
             bad = true;
             return true;
         }
         float4 a0, b0, a1, b1;
         load_node32(nodes, c, a0, b0);
-        load_node32(nodes, c + 1, a1, b1);
+        load_node32(nodes1, c, a1, b1);
         visits += 2;
         float tn0, tn1;
         const bool h0 = slab_near(mode, sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
@@ -249,20 +251,22 @@ __device__ __forceinline__ bool ordered_walk_gathered(SlabMode<kMode> mode, cons
     uint32_t c = ~uint32_t(root);
     int sp = 0;
     int guard = int(node_count / 2) + 1;  // each internal node's pair is entered once
+    const uint32_t last_pair = node_count - 1;     // a pair slot c needs c + 1 < node_count
+    const lux_bvh_node* const nodes1 = nodes + 1;  // both nodes of a pair load from one offset
     bool done = false, overflow = false;
     bool ended = false;         // a leaf ended the walk (its later held leaves are not tested)
     uint32_t hc = 0;            // pair slot of the held leaves
     int hp0 = -1, hp1 = -1;     // held leaf children (primitive ids), -1: none
     float htn0 = 0, htn1 = 0;   // their box entries
     for (;;) {
-        if (!done & (hp0 < 0) & (hp1 < 0)) {
-            if (c + 1 >= node_count || --guard < 0) {
+        if (!done & ((hp0 & hp1) < 0)) {  // holds no leaf (both words negative); && measured 5 % slower
+            if (c >= last_pair || --guard < 0) {
                 bad = true;
                 done = true;
             } else {
                 float4 a0, b0, a1, b1;
                 load_node32(nodes, c, a0, b0);
-                load_node32(nodes, c + 1, a1, b1);
+                load_node32(nodes1, c, a1, b1);
                 visits += 2;
                 float tn0, tn1;
                 const bool h0 = slab_near(mode, sr, best.t_cull, a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, tn0);
